@@ -1,0 +1,241 @@
+/*
+ * rmpcd_capi.h -- C ABI of the MI355X-native RMPCDMD solvent/colloid time step.
+ *
+ * Every entry point takes plain pointers and sizes.  Particle and cell arrays are
+ * DEVICE pointers (HBM) in the library's layout:
+ *   - vectors are three contiguous planes of n values: x[0..n), y[n..2n), z[2n..3n)
+ *     (the reference's Fortran (3,N) is x/y/z interleaved; the Fortran binding layer
+ *     transposes when it moves data between host arrays and the device);
+ *   - all indices are 0-based: cell key h (reference cell index h+1), cell_start
+ *     (reference value - 1), particle ids and id_to_idx values (reference - 1),
+ *     neighbour-list entries (reference - 1);
+ *   - form RMPCD_FP64: pos/vel/force are double, species/flags int32;
+ *     form RMPCD_FP32 (mixed precision): pos/vel/force float, species/flags uint8;
+ *     colloids are always RMPCD_FP64.
+ * Scalars returned through host pointers (energies, temperature, displacements) are
+ * produced synchronously.  Everything else is ordered on the library stream
+ * (rmpcd_set_stream, default: the null stream) and returns without waiting.
+ * Every entry point returns RMPCD_OK (0) or a negative status; rmpcd_last_error()
+ * gives the message, which for the reference's own failure cases is the reference's
+ * error-stop text.
+ *
+ * Randomness: routines that draw in the reference take either injected draws (a
+ * device array laid out by the item that consumes them, the parity interface) or,
+ * when that pointer is NULL, draw from the library's counter-based Threefry-2x64-20
+ * generator keyed by (seed, stream) with counter (step, item).
+ */
+#ifndef RMPCD_CAPI_H
+#define RMPCD_CAPI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RMPCD_OK 0
+#define RMPCD_ERR_ARG -1       /* invalid argument (shape, null pointer, range) */
+#define RMPCD_ERR_HIP -2       /* HIP runtime error */
+#define RMPCD_ERR_OVERFLOW -3  /* 'maximum of neighbor list reached' */
+#define RMPCD_ERR_DOMAIN -4    /* a particle lies outside the box it is keyed in */
+
+#define RMPCD_FP64 0
+#define RMPCD_FP32 1
+
+/* flag bits: src/common.f90:49-61 */
+#define RMPCD_REAC_BIT 1
+#define RMPCD_MD_BIT 2
+#define RMPCD_WALL_BIT 3
+#define RMPCD_PAST_MD_BIT 4
+#define RMPCD_CATALYZED_BIT 6
+/* boundary conditions: src/cell_system.f90:43-45 */
+#define RMPCD_PERIODIC_BC 1
+#define RMPCD_BOUNCE_BACK_BC 2
+#define RMPCD_SPECULAR_BC 3
+
+/* cell_system_t scalars (src/cell_system.f90:20-41) */
+typedef struct rmpcd_box {
+    int32_t L[3];       /* cells per axis (z has L+1 cells with walls) */
+    int32_t M[3];       /* Hilbert bits per axis */
+    int32_t ncells;     /* padded cell count 2^(M1+M2+M3) (cell_system%N) */
+    int32_t has_walls;
+    double a;           /* cell size */
+    double edges[3];    /* L*a as given to init */
+    double origin[3];   /* grid shift, in [-1,0) cells after random_shift */
+    int32_t bc[3];      /* PERIODIC/BOUNCE_BACK/SPECULAR per axis */
+    double max_v;       /* bound used by cell_maximum_displacement */
+} rmpcd_box_t;
+
+/* particle_system_t arrays (src/particle_system.f90:22-77), device pointers */
+typedef struct rmpcd_particles {
+    int64_t n;          /* particle_system%Nmax */
+    int32_t form;       /* RMPCD_FP64 or RMPCD_FP32 */
+    int32_t pad_;
+    void *pos;          /* 3 planes */
+    void *vel;          /* 3 planes */
+    void *force;        /* 3 planes */
+    void *force_old;    /* 3 planes */
+    int32_t *image;     /* 3 planes */
+    int32_t *id;        /* n */
+    void *species;      /* n, int32 (FP64) / uint8 (FP32) */
+    void *flags;        /* n, int32 (FP64) / uint8 (FP32) */
+} rmpcd_particles_t;
+
+/* cell arrays, device pointers, each box->ncells long */
+typedef struct rmpcd_cells {
+    int32_t *cell_count;
+    int32_t *cell_start;
+    uint8_t *is_md;
+    uint8_t *is_reac;
+} rmpcd_cells_t;
+
+/* lj_params_t (src/interaction.f90:23-34); arrays column-major (n1, n2), n1*n2 <= 16,
+ * index (s1-1) + n1*(s2-1) with s1 the solvent species and s2 the colloid species */
+typedef struct rmpcd_lj {
+    int32_t n1, n2;
+    double eps[16], sigma[16], cut[16], cut_sq[16], cut_energy[16];
+} rmpcd_lj_t;
+
+/* ---- runtime -------------------------------------------------------------------- */
+const char *rmpcd_last_error(void);
+int rmpcd_init(int device);
+int rmpcd_set_stream(void *hip_stream);
+int rmpcd_synchronize(void);
+const char *rmpcd_version(void);
+
+/* ---- hilbert.f90 ------------------------------------------------------------------ */
+/* compact_p_to_h (src/hilbert.f90:241-271) and compact_h_to_p (:273-307) on the host */
+int32_t rmpcd_compact_p_to_h(const int32_t p[3], const int32_t M[3]);
+void rmpcd_compact_h_to_p(int32_t h, const int32_t M[3], int32_t p[3]);
+/* device table keys[p0 + L0*(p1 + L1*p2)] for every cell of the box (p in [0,L)) */
+int rmpcd_hilbert_table(const rmpcd_box_t *box, int32_t *keys);
+
+/* ---- cell_system.f90 --------------------------------------------------------------- */
+/* cell_system%init (src/cell_system.f90:49-84): fills the scalars of *box */
+int rmpcd_box_init(rmpcd_box_t *box, const int32_t L[3], double a, int32_t has_walls);
+/* cell_system%random_shift (src/cell_system.f90:153-163): origin = u - 1 */
+int rmpcd_random_shift(rmpcd_box_t *box, const double u[3]);
+/* cell_system%count_particles (src/cell_system.f90:86-129) */
+int rmpcd_count_particles(const rmpcd_box_t *box, const rmpcd_particles_t *p,
+                          rmpcd_cells_t *cells);
+/* cartesian cell of each particle, 3 planes of int32 (cell_system.f90:165-175) */
+int rmpcd_cartesian_indices(const rmpcd_box_t *box, const rmpcd_particles_t *p,
+                            int32_t *cell_xyz);
+
+/* ---- particle_system.f90 ------------------------------------------------------------ */
+/* particle_system%sort (src/particle_system.f90:304-355): counts cells, then writes the
+ * stably cell-sorted copy of src into dst (pos, image, vel, force, force_old, id,
+ * species, flags) and id_to_idx[id] = new slot.  The caller swaps src and dst after
+ * the call, as the reference's switch() calls do.  apply_pbc != 0 fuses md.f90:202-217
+ * (applied to the particles before keying, as the drivers call it right before). */
+int rmpcd_sort(const rmpcd_box_t *box, const rmpcd_particles_t *src, rmpcd_particles_t *dst,
+               int32_t *id_to_idx, rmpcd_cells_t *cells, int32_t apply_pbc);
+/* particle_system%maximum_displacement (src/particle_system.f90:357-378) */
+int rmpcd_maximum_displacement(const rmpcd_particles_t *p, const void *pos_old, double *r);
+/* cell_maximum_displacement (src/particle_system.f90:380-410) */
+int rmpcd_cell_maximum_displacement(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                                    const rmpcd_particles_t *p, const void *pos_old,
+                                    double delta_t, double *r);
+
+/* ---- mpcd.f90 ----------------------------------------------------------------------- */
+/* rand_sphere (src/mpcd.f90:43-62) applied to n_items independent uniform streams
+ * u[item*stride .. item*stride+stride) (device); axes (3 planes of n_items) and
+ * used[item] (uniforms consumed, -1 when the stream ran out) are device outputs */
+int rmpcd_rand_sphere(const double *u, int64_t stride, int64_t n_items, double *axes,
+                      int32_t *used);
+/* simple_mpcd_step (src/mpcd.f90:71-183), SRD branch.  axes: NULL or device array of
+ * 3 planes x ncells, the unit axis drawn for each cell (read only for cells that draw);
+ * otherwise the axis comes from Threefry keyed by (seed, step, cell).  alpha_given = 0
+ * gives the reference default (sin = 1, cos = 0).  thermostat != 0 is rejected
+ * (the MPC-AT branch, mpcd.f90:144-160, is not part of this library). */
+int rmpcd_simple_mpcd_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                           rmpcd_particles_t *p, const double *axes, uint64_t seed,
+                           uint64_t step, int32_t alpha_given, double alpha,
+                           int32_t thermostat, double T, int32_t hydro);
+/* compute_temperature (src/mpcd.f90:342-407) */
+int rmpcd_compute_temperature(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                              const rmpcd_particles_t *p, double *te);
+/* mpcd_stream_periodic (src/mpcd.f90:440-457) */
+int rmpcd_mpcd_stream_periodic(const rmpcd_box_t *box, rmpcd_particles_t *p, double dt);
+/* bulk_reaction (src/mpcd.f90:605-643); draws: NULL or device array, one per cell */
+int rmpcd_bulk_reaction(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                        rmpcd_particles_t *p, int32_t from, int32_t to, double rate,
+                        double tau, const double *draws, uint64_t seed, uint64_t step);
+/* mpcd_stream_nogravity_zwall (src/mpcd.f90:702-745) */
+int rmpcd_mpcd_stream_nogravity_zwall(const rmpcd_box_t *box, rmpcd_particles_t *p,
+                                      double dt);
+
+/* ---- md.f90 ------------------------------------------------------------------------- */
+int rmpcd_md_pos(rmpcd_particles_t *p, double dt);                 /* md.f90:61-77 */
+int rmpcd_md_vel(rmpcd_particles_t *p, double dt);                 /* md.f90:219-237 */
+int rmpcd_apply_pbc(rmpcd_particles_t *p, const double edges[3]);  /* md.f90:202-217 */
+int rmpcd_cell_md_pos(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                      rmpcd_particles_t *p, double dt, int32_t md_flag);   /* md.f90:79-102 */
+int rmpcd_cell_md_vel(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                      rmpcd_particles_t *p, double dt, int32_t md_flag);   /* md.f90:239-263 */
+/* cell_md_pos_ywall (axis 1, md.f90:104-151) / cell_md_pos_zwall (axis 2, md.f90:153-200) */
+int rmpcd_cell_md_pos_wall(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                           rmpcd_particles_t *p, double dt, int32_t md_flag, int32_t axis);
+/* rattle_dimer_pos (md.f90:265-296) / rattle_dimer_vel (md.f90:298-319) on colloids 0,1;
+ * pos_rattle: device, 3 planes of p->n; mass: host, n_mass (<= 16) entries indexed by
+ * species-1 (particle_system%mass) */
+int rmpcd_rattle_dimer_pos(rmpcd_particles_t *colloids, const double *pos_rattle,
+                           const double *mass, int32_t n_mass, double d, double dt,
+                           const double edges[3]);
+int rmpcd_rattle_dimer_vel(rmpcd_particles_t *colloids, const double *mass, int32_t n_mass,
+                           double d, double dt, const double edges[3]);
+/* colloid Verlet written inline in the drivers (programs/single_dimer_pbc.f90:312-315
+ * and :356-359); mass_by_particle: device, one per colloid */
+int rmpcd_colloid_md_pos(rmpcd_particles_t *colloids, const double *mass_by_particle,
+                         double dt);
+int rmpcd_colloid_md_vel(rmpcd_particles_t *colloids, const double *mass_by_particle,
+                         double dt);
+
+/* ---- interaction.f90 --------------------------------------------------------------- */
+/* lj_params_init (src/interaction.f90:38-80), host; eps/sigma/cut column-major (n1,n2) */
+int rmpcd_lj_params_init(rmpcd_lj_t *lj, int32_t n1, int32_t n2, const double *eps,
+                         const double *sigma, const double *cut);
+
+/* ---- neighbor_list.f90 ------------------------------------------------------------- */
+/* make_stencil (src/neighbor_list.f90:271-308), host; returns the stencil size and
+ * fills stencil[3*k..3*k+2] when stencil != NULL and cap is large enough */
+int32_t rmpcd_make_stencil(double a, double cut, int32_t *stencil, int32_t cap);
+/* update_list (src/neighbor_list.f90:76-157).  stencil: device (3*n_stencil);
+ * list: device (npoints*nmax), entry k of point i at list[i*nmax+k]; n: device.
+ * lj may be NULL (the optional argument absent). */
+int rmpcd_update_list(const rmpcd_box_t *box, rmpcd_cells_t *cells,
+                      const rmpcd_particles_t *points, const rmpcd_particles_t *solvent,
+                      double radius, const int32_t *stencil, int32_t n_stencil,
+                      const rmpcd_lj_t *lj, int32_t nmax, int32_t *list, int32_t *n);
+/* compute_force (src/neighbor_list.f90:159-225); returns the energy in *e */
+int rmpcd_compute_force(rmpcd_particles_t *ps1, rmpcd_particles_t *ps2, const int32_t *list,
+                        const int32_t *n, int32_t nmax, const double L[3],
+                        const rmpcd_lj_t *lj, double *e);
+/* compute_force_n2 (src/neighbor_list.f90:227-264) */
+int rmpcd_compute_force_n2(rmpcd_particles_t *ps, const double L[3], const rmpcd_lj_t *lj,
+                           double *e);
+
+/* ---- catalytic helpers written in the drivers (single_dimer_pbc.f90:493-565) -------- */
+/* flag_particles_nl (:493-509): REAC bit for species-1 particles of colloid 0's list
+ * inside cut_sq_c of colloid 0 */
+int rmpcd_flag_particles_nl(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
+                            const int32_t *list, const int32_t *n, int32_t nmax,
+                            const double edges[3], double cut_sq_c);
+/* change_species (:511-542); draws: NULL or device array, one per solvent slot */
+int rmpcd_change_species(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
+                         const double edges[3], double cut_sq_c, double cut_sq_n,
+                         double prob, const double *draws, uint64_t seed, uint64_t step);
+/* refuel (:544-565) */
+int rmpcd_refuel(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
+                 const double edges[3], int32_t L1);
+
+/* ---- counter-based generator used when draws are not injected ------------------------ */
+/* uniform doubles in [0,1): out[i] = U(seed, stream, step, i), device output */
+int rmpcd_threefry_uniform(uint64_t seed, uint64_t stream, uint64_t step, int64_t n,
+                           double *out);
+/* Threefry-2x64-20 block function on the host (Random123 spec), for known-answer tests */
+void rmpcd_threefry2x64_20(const uint64_t ctr[2], const uint64_t key[2], uint64_t out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,353 @@
+"""ctypes front-end of the C oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` -- never by the product package.  Arrays follow
+the oracle's conventions: (3, N) column-major vectors are numpy arrays of shape (N, 3)
+in C order (same memory), all indices 0-based.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+
+REAC_BIT, MD_BIT, WALL_BIT, PAST_MD_BIT, CATALYZED_BIT = 1, 2, 3, 4, 6
+PERIODIC_BC, BOUNCE_BACK_BC, SPECULAR_BC = 1, 2, 3
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liborc.so"], check=True)
+
+
+class Cells(C.Structure):
+    _fields_ = [("L", C.c_int * 3), ("M", C.c_int * 3), ("N", C.c_int), ("a", C.c_double),
+                ("edges", C.c_double * 3), ("origin", C.c_double * 3),
+                ("has_walls", C.c_int), ("bc", C.c_int * 3), ("max_v", C.c_double)]
+
+
+class Particles(C.Structure):
+    _fields_ = [("n", C.c_int64), ("pos", C.c_void_p), ("vel", C.c_void_p),
+                ("force", C.c_void_p), ("force_old", C.c_void_p), ("image", C.c_void_p),
+                ("id", C.c_void_p), ("species", C.c_void_p), ("flags", C.c_void_p)]
+
+
+class LJ(C.Structure):
+    _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("eps", C.c_double * 16),
+                ("sigma", C.c_double * 16), ("cut", C.c_double * 16),
+                ("cut_sq", C.c_double * 16), ("cut_energy", C.c_double * 16)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.orc_maximum_displacement.restype = C.c_double
+        _lib.orc_cell_maximum_displacement.restype = C.c_double
+        _lib.orc_compute_temperature.restype = C.c_double
+        _lib.orc_compute_force.restype = C.c_double
+        _lib.orc_compute_force_n2.restype = C.c_double
+        _lib.orc_lj_energy.restype = C.c_double
+        _lib.orc_lj_energy.argtypes = [C.c_double] * 4
+        _lib.orc_srd_axes_from_stream.restype = C.c_int64
+        for name in ("orc_md_pos", "orc_md_vel", "orc_mpcd_stream_periodic",
+                     "orc_mpcd_stream_nogravity_zwall"):
+            getattr(_lib, name).restype = None
+    return _lib
+
+
+def P(a):
+    """pointer of a contiguous numpy array (or None)"""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+D = C.c_double
+I64 = C.c_int64
+
+
+# ---- hilbert ---------------------------------------------------------------------
+def compact_p_to_h(p, M):
+    pa = (C.c_int * 3)(*[int(v) for v in p])
+    ma = (C.c_int * 3)(*[int(v) for v in M])
+    return lib().orc_compact_p_to_h(pa, ma)
+
+
+def compact_h_to_p(h, M):
+    ma = (C.c_int * 3)(*[int(v) for v in M])
+    out = (C.c_int * 3)()
+    lib().orc_compact_h_to_p(int(h), ma, out)
+    return tuple(out)
+
+
+def p_to_h(p, m):
+    pa = (C.c_int * 3)(*[int(v) for v in p])
+    return lib().orc_p_to_h(pa, int(m))
+
+
+def h_to_p(h, m):
+    out = (C.c_int * 3)()
+    lib().orc_h_to_p(int(h), int(m), out)
+    return tuple(out)
+
+
+# ---- cells -----------------------------------------------------------------------
+def cells_init(L, a=1.0, has_walls=False):
+    c = Cells()
+    la = (C.c_int * 3)(*[int(v) for v in L])
+    lib().orc_cells_init(C.byref(c), la, D(a), int(bool(has_walls)))
+    return c
+
+
+def set_origin(c, origin):
+    for k in range(3):
+        c.origin[k] = float(origin[k])
+
+
+def cartesian_indices(c, x):
+    xa = (C.c_double * 3)(*[float(v) for v in x])
+    out = (C.c_int * 3)()
+    lib().orc_cartesian_indices(C.byref(c), xa, out)
+    return tuple(out)
+
+
+def count_particles(c, pos, nthreads=1):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    cnt = np.zeros(c.N, np.int32)
+    st = np.zeros(c.N, np.int32)
+    lib().orc_count_particles(C.byref(c), I64(pos.shape[0]), P(pos), P(cnt), P(st), int(nthreads))
+    return cnt, st
+
+
+class PState:
+    """numpy particle state in the oracle layout (N,3) C-order == Fortran (3,N)"""
+
+    def __init__(self, n):
+        self.n = n
+        self.pos = np.zeros((n, 3))
+        self.vel = np.zeros((n, 3))
+        self.force = np.zeros((n, 3))
+        self.force_old = np.zeros((n, 3))
+        self.image = np.zeros((n, 3), np.int32)
+        self.id = np.arange(n, dtype=np.int32)
+        self.species = np.ones(n, np.int32)
+        self.flags = np.zeros(n, np.int32)
+
+    def copy(self):
+        o = PState(self.n)
+        for k in ("pos", "vel", "force", "force_old", "image", "id", "species", "flags"):
+            setattr(o, k, getattr(self, k).copy())
+        return o
+
+    def as_struct(self):
+        return Particles(self.n, P(self.pos), P(self.vel), P(self.force), P(self.force_old),
+                         P(self.image), P(self.id), P(self.species), P(self.flags))
+
+
+def sort(c, ps, nthreads=1):
+    """particle_system%sort: returns (sorted PState, id_to_idx, cell_count, cell_start);
+    the unsorted input plays the reference's pos_old after the pointer switch."""
+    out = PState(ps.n)
+    i2i = np.zeros(ps.n, np.int32)
+    cnt = np.zeros(c.N, np.int32)
+    st = np.zeros(c.N, np.int32)
+    s_in, s_out = ps.as_struct(), out.as_struct()
+    lib().orc_sort(C.byref(c), C.byref(s_in), C.byref(s_out), P(i2i), P(cnt), P(st), int(nthreads))
+    return out, i2i, cnt, st
+
+
+def maximum_displacement(pos, pos_old, species):
+    return lib().orc_maximum_displacement(I64(pos.shape[0]), P(pos), P(pos_old), P(species))
+
+
+def cell_maximum_displacement(c, cnt, st, is_md, pos, pos_old, species, delta_t):
+    return lib().orc_cell_maximum_displacement(C.byref(c), P(cnt), P(st), P(is_md), P(pos),
+                                               P(pos_old), P(species), D(delta_t))
+
+
+# ---- mpcd ------------------------------------------------------------------------
+def rand_sphere(u):
+    u = np.ascontiguousarray(u, np.float64)
+    out = (C.c_double * 3)()
+    used = lib().orc_rand_sphere(P(u), int(u.size), out)
+    return np.array(out[:]), used
+
+
+def srd_axes_from_stream(c, cnt, st, species, u):
+    u = np.ascontiguousarray(u, np.float64)
+    axes = np.zeros((c.N, 3))
+    used = lib().orc_srd_axes_from_stream(C.byref(c), P(cnt), P(st), P(species), P(u),
+                                          I64(u.size), P(axes))
+    return axes, used
+
+
+def simple_mpcd_step(c, cnt, st, vel, species, axes, alpha=None):
+    lib().orc_simple_mpcd_step(C.byref(c), P(cnt), P(st), P(vel), P(species), P(axes),
+                               0 if alpha is None else 1, D(0.0 if alpha is None else alpha))
+
+
+def compute_temperature(c, cnt, st, vel, species):
+    return lib().orc_compute_temperature(C.byref(c), P(cnt), P(st), P(vel), P(species))
+
+
+def mpcd_stream_periodic(c, pos, vel, image, dt):
+    lib().orc_mpcd_stream_periodic(C.byref(c), I64(pos.shape[0]), P(pos), P(vel), P(image), D(dt))
+
+
+def bulk_reaction(c, cnt, st, is_reac, species, frm, to, rate, tau, draws):
+    lib().orc_bulk_reaction(C.byref(c), P(cnt), P(st), P(is_reac), P(species), int(frm), int(to),
+                            D(rate), D(tau), P(draws))
+
+
+def mpcd_stream_nogravity_zwall(c, pos, vel, force, flags, dt):
+    lib().orc_mpcd_stream_nogravity_zwall(C.byref(c), I64(pos.shape[0]), P(pos), P(vel),
+                                          P(force), P(flags), D(dt))
+
+
+# ---- md --------------------------------------------------------------------------
+def md_pos(pos, vel, force, dt):
+    lib().orc_md_pos(I64(pos.shape[0]), P(pos), P(vel), P(force), D(dt))
+
+
+def md_vel(vel, force, force_old, flags, dt):
+    lib().orc_md_vel(I64(vel.shape[0]), P(vel), P(force), P(force_old), P(flags), D(dt))
+
+
+def apply_pbc(pos, image, edges):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_apply_pbc(I64(pos.shape[0]), P(pos), P(image), ea)
+
+
+def cell_md_pos(c, cnt, st, is_md, pos, vel, force, dt, md_flag):
+    lib().orc_cell_md_pos(C.byref(c), P(cnt), P(st), P(is_md), P(pos), P(vel), P(force), D(dt),
+                          int(bool(md_flag)))
+
+
+def cell_md_vel(c, cnt, st, is_md, vel, force, force_old, flags, dt, md_flag):
+    lib().orc_cell_md_vel(C.byref(c), P(cnt), P(st), P(is_md), P(vel), P(force), P(force_old),
+                          P(flags), D(dt), int(bool(md_flag)))
+
+
+def cell_md_pos_wall(c, cnt, st, is_md, pos, vel, force, dt, md_flag, axis):
+    lib().orc_cell_md_pos_wall(C.byref(c), P(cnt), P(st), P(is_md), P(pos), P(vel), P(force),
+                               D(dt), int(bool(md_flag)), int(axis))
+
+
+def rattle_dimer_pos(pos, vel, pos_rattle, species, mass, d, dt, edges):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_rattle_dimer_pos(P(pos), P(vel), P(pos_rattle), P(species), P(mass), D(d), D(dt), ea)
+
+
+def rattle_dimer_vel(pos, vel, species, mass, d, dt, edges):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_rattle_dimer_vel(P(pos), P(vel), P(species), P(mass), D(d), D(dt), ea)
+
+
+def colloid_md_pos(pos, vel, force, mass_by_particle, dt):
+    lib().orc_colloid_md_pos(I64(pos.shape[0]), P(pos), P(vel), P(force), P(mass_by_particle), D(dt))
+
+
+def colloid_md_vel(vel, force, force_old, mass_by_particle, dt):
+    lib().orc_colloid_md_vel(I64(vel.shape[0]), P(vel), P(force), P(force_old),
+                             P(mass_by_particle), D(dt))
+
+
+def rel_pos(x, y, L):
+    xa = (C.c_double * 3)(*[float(v) for v in x])
+    ya = (C.c_double * 3)(*[float(v) for v in y])
+    la = (C.c_double * 3)(*[float(v) for v in L])
+    out = (C.c_double * 3)()
+    lib().orc_rel_pos(xa, ya, la, out)
+    return np.array(out[:])
+
+
+# ---- interaction -----------------------------------------------------------------
+def lj_params(eps, sigma, cut):
+    """eps/sigma/cut: 2-D arrays indexed (solvent species, colloid species), 1-based
+    in the reference; stored column-major as the reference does."""
+    eps = np.asarray(eps, np.float64)
+    n1, n2 = eps.shape
+    lj = LJ()
+    e = np.ascontiguousarray(eps.T.reshape(-1))
+    s = np.ascontiguousarray(np.asarray(sigma, np.float64).T.reshape(-1))
+    ct = np.ascontiguousarray(np.asarray(cut, np.float64).T.reshape(-1))
+    rc = lib().orc_lj_params_init(C.byref(lj), n1, n2, P(e), P(s), P(ct))
+    if rc != 0:
+        raise ValueError("unequal sizes for arguments to lj_params_init")
+    return lj
+
+
+def lj_force(d, r_sq, eps, sigma):
+    da = (C.c_double * 3)(*[float(v) for v in d])
+    out = (C.c_double * 3)()
+    lib().orc_lj_force(da, D(r_sq), D(eps), D(sigma), out)
+    return np.array(out[:])
+
+
+def lj_energy(r_sq, eps, sigma, cut_energy=0.0):
+    return lib().orc_lj_energy(r_sq, eps, sigma, cut_energy)
+
+
+# ---- neighbour list --------------------------------------------------------------
+def make_stencil(a, cut):
+    n = lib().orc_make_stencil(D(a), D(cut), None)
+    st = np.zeros((n, 3), np.int32)
+    lib().orc_make_stencil(D(a), D(cut), P(st))
+    return st
+
+
+def update_list(c, cnt, st, pos1, species1, pos2, species2, radius, stencil, lj, nmax):
+    n1 = pos1.shape[0]
+    lst = np.zeros((n1, nmax), np.int32)
+    nl = np.zeros(n1, np.int32)
+    is_md = np.zeros(c.N, np.uint8)
+    is_reac = np.zeros(c.N, np.uint8)
+    rc = lib().orc_update_list(C.byref(c), P(cnt), P(st), I64(n1), P(pos1), P(species1), P(pos2),
+                               P(species2), D(radius), P(stencil), int(stencil.shape[0]),
+                               None if lj is None else C.byref(lj), int(nmax), P(lst), P(nl),
+                               P(is_md), P(is_reac))
+    if rc != 0:
+        raise RuntimeError("maximum of neighbor list reached")
+    return lst, nl, is_md, is_reac
+
+
+def compute_force(pos1, species1, force1, pos2, species2, force2, flags2, lst, nl, L, lj):
+    la = (C.c_double * 3)(*[float(v) for v in L])
+    return lib().orc_compute_force(I64(pos1.shape[0]), P(pos1), P(species1), P(force1),
+                                   I64(pos2.shape[0]), P(pos2), P(species2), P(force2), P(flags2),
+                                   P(lst), P(nl), int(lst.shape[1]), la, C.byref(lj))
+
+
+def compute_force_n2(pos, species, force, L, lj):
+    la = (C.c_double * 3)(*[float(v) for v in L])
+    return lib().orc_compute_force_n2(I64(pos.shape[0]), P(pos), P(species), P(force), la,
+                                      C.byref(lj))
+
+
+# ---- driver helpers ----------------------------------------------------------------
+def flag_particles_nl(list1, n1, cpos, pos, species, flags, edges, cut_sq_c):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_flag_particles_nl(P(list1), int(n1), P(cpos), P(pos), P(species), P(flags), ea,
+                                D(cut_sq_c))
+
+
+def change_species(cpos, pos, species, flags, edges, cut_sq_c, cut_sq_n, prob, draws):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_change_species(I64(pos.shape[0]), P(cpos), P(pos), P(species), P(flags), ea,
+                             D(cut_sq_c), D(cut_sq_n), D(prob), P(draws))
+
+
+def refuel(cpos, pos, species, edges, L1):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_refuel(I64(pos.shape[0]), P(cpos), P(pos), P(species), ea, int(L1))

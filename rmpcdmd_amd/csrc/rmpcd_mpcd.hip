@@ -1,0 +1,513 @@
+// rmpcd_mpcd.hip -- SRD collision, temperature, streaming and bulk reaction
+// (src/mpcd.f90:43-183, 342-407, 440-457, 605-643, 702-745).
+//
+// Cell kernels work on the cell-sorted solvent: a workgroup owns a run of consecutive
+// cells, stages their velocities and species from HBM into LDS with coalesced loads,
+// each thread then reduces one cell in the reference's summation order (FP32 form: with
+// Kahan compensation), and the rotated velocities go back to HBM coalesced.  Cells whose
+// particles do not fit the tile fall back to direct loads.
+#include "rmpcd_internal.hpp"
+
+namespace rmpcd {
+
+constexpr int CELL_THREADS = 128;          // cells per workgroup
+constexpr int TILE_CAP = 2048;             // staged particles per workgroup
+
+template <class R>
+struct Acc3 {  // running sum; compensated in FP32
+    R s[3] = {0, 0, 0};
+    R c[3] = {0, 0, 0};
+    __device__ inline void add(int k, R v)
+    {
+        if constexpr (sizeof(R) == 4) {
+            R y = v - c[k];
+            R t = s[k] + y;
+            c[k] = (t - s[k]) - y;
+            s[k] = t;
+        } else {
+            s[k] = s[k] + v;
+        }
+    }
+};
+
+template <class F>
+struct CellTile {
+    using R = typename F::real;
+    R *vx, *vy, *vz;
+    typename F::small *sp;
+    int64_t base;  // first staged slot
+    int count;     // staged slots (0 when the tile overflowed)
+};
+
+// stage the particles of cells [c0, c1) into LDS; returns false (count=0) when the run
+// is longer than the tile
+template <class F>
+__device__ inline void stage_tile(const PView<F> &p, const int32_t *start, const int32_t *count,
+                                  int ncells, int c0, int c1, CellTile<F> &t, bool load_vel)
+{
+    __shared__ int64_t s_range[2];
+    if (threadIdx.x == 0) {
+        int last = c1 - 1;
+        s_range[0] = start[c0];
+        s_range[1] = (int64_t)start[last] + count[last];
+    }
+    __syncthreads();
+    int64_t b = s_range[0], e = s_range[1];
+    __syncthreads();
+    t.base = b;
+    t.count = (e - b <= TILE_CAP) ? (int)(e - b) : 0;
+    for (int q = threadIdx.x; q < t.count; q += blockDim.x) {
+        if (load_vel) {
+            t.vx[q] = p.vel[b + q];
+            t.vy[q] = p.vel[p.n + b + q];
+            t.vz[q] = p.vel[2 * p.n + b + q];
+        }
+        t.sp[q] = p.species[b + q];
+    }
+    __syncthreads();
+}
+
+// SRD rotation (mpcd.f90:120-180, default or given alpha); axes injected per cell or
+// drawn from Threefry keyed by (seed, step, cell)
+template <class F>
+__global__ void __launch_bounds__(CELL_THREADS)
+    k_srd(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+          PView<F> p, const double *__restrict__ axes, uint64_t seed, uint64_t step,
+          double sin_a, double cos_a)
+{
+    using R = typename F::real;
+    __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
+    __shared__ typename F::small lsp[TILE_CAP];
+    int c0 = blockIdx.x * CELL_THREADS;
+    int c1 = min(c0 + CELL_THREADS, ncells);
+    CellTile<F> t{lx, ly, lz, lsp, 0, 0};
+    stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
+    int c = c0 + threadIdx.x;
+    if (c < c1) {
+        int n = count[c];
+        if (n > 1) {
+            int64_t s = start[c];
+            bool in_lds = t.count > 0;
+            int64_t o = s - t.base;
+            auto V = [&](int k, int64_t i) -> R {
+                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
+                return p.vel[k * p.n + s + i];
+            };
+            auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
+            Acc3<R> acc;
+            int n_eff = 0;
+            for (int i = 0; i < n; ++i) {
+                if (S(i) > 0) {
+                    acc.add(0, V(0, i));
+                    acc.add(1, V(1, i));
+                    acc.add(2, V(2, i));
+                    ++n_eff;
+                }
+            }
+            if (n_eff > 1) {
+                R u0 = acc.s[0] / (R)n_eff, u1 = acc.s[1] / (R)n_eff, u2 = acc.s[2] / (R)n_eff;
+                double v0, v1, v2;
+                if (axes) {
+                    v0 = axes[c];
+                    v1 = axes[ncells + c];
+                    v2 = axes[2 * ncells + c];
+                } else {
+                    Sphere sp = rand_sphere_threefry(seed, STREAM_SRD, step, (uint64_t)c);
+                    v0 = sp.n[0];
+                    v1 = sp.n[1];
+                    v2 = sp.n[2];
+                }
+                double omc = 1 - cos_a;
+                // omega(r,k) as filled column-major by mpcd.f90:163-171
+                R o00 = (R)(cos_a + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sin_a),
+                  o20 = (R)((v0 * v2) * omc + v1 * sin_a);
+                R o01 = (R)((v1 * v0) * omc + v2 * sin_a), o11 = (R)(cos_a + (v1 * v1) * omc),
+                  o21 = (R)((v1 * v2) * omc - v0 * sin_a);
+                R o02 = (R)((v2 * v0) * omc - v1 * sin_a), o12 = (R)((v2 * v1) * omc + v0 * sin_a),
+                  o22 = (R)(cos_a + (v2 * v2) * omc);
+                for (int i = 0; i < n; ++i) {
+                    R w0 = V(0, i) - u0, w1 = V(1, i) - u1, w2 = V(2, i) - u2;
+                    R r0 = o00 * w0;
+                    r0 = r0 + o01 * w1;
+                    r0 = r0 + o02 * w2;
+                    R r1 = o10 * w0;
+                    r1 = r1 + o11 * w1;
+                    r1 = r1 + o12 * w2;
+                    R r2 = o20 * w0;
+                    r2 = r2 + o21 * w1;
+                    r2 = r2 + o22 * w2;
+                    if (in_lds) {
+                        lx[o + i] = u0 + r0;
+                        ly[o + i] = u1 + r1;
+                        lz[o + i] = u2 + r2;
+                    } else {
+                        p.vel[s + i] = u0 + r0;
+                        p.vel[p.n + s + i] = u1 + r1;
+                        p.vel[2 * p.n + s + i] = u2 + r2;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < t.count; q += blockDim.x) {
+        p.vel[t.base + q] = lx[q];
+        p.vel[p.n + t.base + q] = ly[q];
+        p.vel[2 * p.n + t.base + q] = lz[q];
+    }
+}
+
+// compute_temperature (mpcd.f90:342-407): per block partial sums of kin and of the
+// number of counted cells; the last block adds the partials in block order
+template <class F>
+__global__ void __launch_bounds__(CELL_THREADS)
+    k_temperature(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+                  PView<F> p, double *__restrict__ part, unsigned *__restrict__ done,
+                  double *__restrict__ out)
+{
+    using R = typename F::real;
+    __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
+    __shared__ typename F::small lsp[TILE_CAP];
+    __shared__ double red[16];
+    __shared__ bool s_last;
+    int c0 = blockIdx.x * CELL_THREADS;
+    int c1 = min(c0 + CELL_THREADS, ncells);
+    CellTile<F> t{lx, ly, lz, lsp, 0, 0};
+    stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
+    double kin = 0, cnt = 0;
+    int c = c0 + threadIdx.x;
+    if (c < c1) {
+        int n = count[c];
+        if (n > 1) {
+            cnt = 1;
+            int64_t s = start[c];
+            bool in_lds = t.count > 0;
+            int64_t o = s - t.base;
+            auto V = [&](int k, int64_t i) -> double {
+                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
+                return p.vel[k * p.n + s + i];
+            };
+            auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
+            double u0 = 0, u1 = 0, u2 = 0;
+            int n_eff = 0;
+            for (int i = 0; i < n; ++i)
+                if (S(i) > 0) {
+                    u0 = u0 + V(0, i);
+                    u1 = u1 + V(1, i);
+                    u2 = u2 + V(2, i);
+                    ++n_eff;
+                }
+            if (n_eff > 1) {
+                u0 = u0 / n;  // divided by n (mpcd.f90:388)
+                u1 = u1 / n;
+                u2 = u2 / n;
+                double k = 0;
+                for (int i = 0; i < n; ++i)
+                    if (S(i) > 0) {
+                        double d0 = V(0, i) - u0, d1 = V(1, i) - u1, d2 = V(2, i) - u2;
+                        k = k + ((d0 * d0 + d1 * d1) + d2 * d2);
+                    }
+                kin = k / (double)(3 * (n_eff - 1));
+            }
+        }
+    }
+    double bk = block_sum(kin, red);
+    double bc = block_sum(cnt, red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = bk;
+        part[2 * blockIdx.x + 1] = bc;
+        __threadfence();
+        s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        double a = 0, b = 0;
+        for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
+            a += part[2 * q];
+            b += part[2 * q + 1];
+        }
+        a = block_sum(a, red);
+        b = block_sum(b, red);
+        if (threadIdx.x == 0) {
+            out[0] = a / b;
+            *done = 0;
+        }
+    }
+}
+
+// bulk_reaction (mpcd.f90:605-643): one draw per eligible cell, last 'from' converts
+template <class F>
+__global__ void k_bulk_reaction(int ncells, const int32_t *__restrict__ start,
+                                const int32_t *__restrict__ count, const uint8_t *__restrict__ is_reac,
+                                PView<F> p, int from, int to, double rate, double tau,
+                                const double *__restrict__ draws, uint64_t seed, uint64_t step)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int n = count[c];
+    if (n <= 1 || !is_reac[c]) return;
+    int64_t s = start[c];
+    double local_rate = 0;
+    int64_t pick = -1;
+    for (int i = 0; i < n; ++i)
+        if ((int)p.species[s + i] == from) {
+            local_rate = local_rate + 1;
+            pick = s + i;
+        }
+    local_rate = local_rate * rate;
+    double u;
+    if (draws) {
+        u = draws[c];
+    } else {
+        uint64_t a, b;
+        threefry2x64(step, (uint64_t)c, seed, STREAM_REACTION, a, b);
+        u = u01(a);
+    }
+    if (u < (1 - exp(-local_rate * tau)) && pick >= 0) p.species[pick] = (typename F::small)to;
+}
+
+// mpcd_stream_periodic (mpcd.f90:440-457)
+template <class F>
+__global__ void k_stream_periodic(Box b, PView<F> p, double dt)
+{
+    using R = typename F::real;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        R x = p.pos[k * p.n + i] + p.vel[k * p.n + i] * (R)dt;
+        R e = (R)b.edges[k];
+        int jump = (int)floor((double)(x / e));
+        p.image[k * p.n + i] += jump;
+        p.pos[k * p.n + i] = x - (R)jump * e;
+    }
+}
+
+// mpcd_stream_nogravity_zwall (mpcd.f90:702-745)
+template <class F>
+__global__ void k_stream_zwall(Box b, PView<F> p, double dt)
+{
+    using R = typename F::real;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    R op[3], ov[3], np[3], nv[3];
+    R rdt = (R)dt, dt2 = (R)(dt * dt);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        op[k] = p.pos[k * p.n + i];
+        ov[k] = p.vel[k * p.n + i];
+        np[k] = (op[k] + ov[k] * rdt) + (p.force[k * p.n + i] * dt2) / (R)2;
+        nv[k] = ov[k];
+    }
+    R Lz = (R)b.edges[2], t_c = 0;
+    bool bounce = false;
+    if (np[2] < (R)0) {
+        t_c = -op[2] / ov[2];
+        bounce = true;
+    } else if (np[2] > Lz) {
+        t_c = (Lz - op[2]) / ov[2];
+        bounce = true;
+    }
+    if (bounce) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            np[k] = np[k] - (ov[k] * (R)2) * (rdt - t_c);
+            nv[k] = -ov[k];
+        }
+        p.flags[i] = (typename F::small)((int)p.flags[i] | WALL_MASK);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        p.pos[k * p.n + i] = np[k];
+        p.vel[k * p.n + i] = nv[k];
+    }
+}
+
+__global__ void k_rand_sphere(const double *u, int64_t stride, int64_t n_items, double *axes,
+                              int32_t *used)
+{
+    int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= n_items) return;
+    const double *q = u + it * stride;
+    int k = 0;
+    double n1 = 0, n2 = 0, s = 2;
+    bool ok = false;
+    while (k + 2 <= stride) {
+        n1 = 2 * q[k] - 1;
+        n2 = 2 * q[k + 1] - 1;
+        k += 2;
+        s = n1 * n1 + n2 * n2;
+        if (s < 1.0) {
+            ok = true;
+            break;
+        }
+    }
+    Sphere sp = marsaglia_from(n1, n2, ok ? s : 0.0);
+    axes[it] = sp.n[0];
+    axes[n_items + it] = sp.n[1];
+    axes[2 * n_items + it] = sp.n[2];
+    used[it] = ok ? k : -1;
+}
+
+__global__ void k_threefry_uniform(uint64_t seed, uint64_t stream_id, uint64_t step, int64_t n,
+                                   double *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t a, b;
+    threefry2x64(step, (uint64_t)i, seed, stream_id, a, b);
+    out[i] = u01(a);
+}
+
+template <class F>
+int srd_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *pp,
+             const double *axes, uint64_t seed, uint64_t step, double sin_a, double cos_a)
+{
+    int nc = box->ncells;
+    k_srd<F><<<blocks_for(nc, CELL_THREADS), CELL_THREADS, 0, stream()>>>(
+        nc, cells->cell_start, cells->cell_count, view<F>(pp), axes, seed, step, sin_a, cos_a);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
+int temperature_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                     const rmpcd_particles_t *pp, double *te)
+{
+    int nc = box->ncells;
+    unsigned nb = blocks_for(nc, CELL_THREADS);
+    double *part = (double *)workspace(WS_PART, sizeof(double) * 2 * nb);
+    double *red = (double *)workspace(WS_RED, 64);
+    unsigned *done = (unsigned *)workspace(WS_FLAG, 64);
+    double *host = (double *)pinned(64);
+    if (!part || !red || !done || !host)
+        return set_error(RMPCD_ERR_HIP, "compute_temperature: workspace allocation failed");
+    hipStream_t s = stream();
+    RMPCD_HIP(hipMemsetAsync(done, 0, sizeof(unsigned), s));
+    k_temperature<F><<<nb, CELL_THREADS, 0, s>>>(nc, cells->cell_start, cells->cell_count,
+                                                 view<F>(pp), part, done, red);
+    RMPCD_LAUNCHED();
+    RMPCD_HIP(hipMemcpyAsync(host, red, sizeof(double), hipMemcpyDeviceToHost, s));
+    RMPCD_HIP(hipStreamSynchronize(s));
+    *te = host[0];
+    return RMPCD_OK;
+}
+
+template <class F>
+int stream_periodic_impl(const rmpcd_box_t *box, rmpcd_particles_t *pp, double dt)
+{
+    PView<F> p = view<F>(pp);
+    k_stream_periodic<F><<<blocks_for(p.n, 256), 256, 0, stream()>>>(make_box(box), p, dt);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
+int stream_zwall_impl(const rmpcd_box_t *box, rmpcd_particles_t *pp, double dt)
+{
+    PView<F> p = view<F>(pp);
+    k_stream_zwall<F><<<blocks_for(p.n, 256), 256, 0, stream()>>>(make_box(box), p, dt);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
+int bulk_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *pp, int from,
+              int to, double rate, double tau, const double *draws, uint64_t seed, uint64_t step)
+{
+    int nc = box->ncells;
+    k_bulk_reaction<F><<<blocks_for(nc, 256), 256, 0, stream()>>>(
+        nc, cells->cell_start, cells->cell_count, cells->is_reac, view<F>(pp), from, to, rate, tau,
+        draws, seed, step);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+}  // namespace rmpcd
+
+using namespace rmpcd;
+
+extern "C" {
+
+int rmpcd_rand_sphere(const double *u, int64_t stride, int64_t n_items, double *axes, int32_t *used)
+{
+    RMPCD_CHECK(u && axes && used && stride >= 2 && n_items >= 0, "rmpcd_rand_sphere: bad argument");
+    if (n_items == 0) return RMPCD_OK;
+    k_rand_sphere<<<blocks_for(n_items, 256), 256, 0, stream()>>>(u, stride, n_items, axes, used);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+int rmpcd_simple_mpcd_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p,
+                           const double *axes, uint64_t seed, uint64_t step, int32_t alpha_given,
+                           double alpha, int32_t thermostat, double T, int32_t hydro)
+{
+    (void)T;
+    int rc = check_box(box, "simple_mpcd_step");
+    if (!rc) rc = check_particles(p, "simple_mpcd_step", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start, "simple_mpcd_step: null cell arrays");
+    // mpcd.f90:107-109
+    if (!hydro && !thermostat)
+        return set_error(RMPCD_ERR_ARG, "requiring .not.do_hydro without thermostat in simple_mpcd_step");
+    RMPCD_CHECK(!thermostat, "simple_mpcd_step: the MPC-AT thermostat branch is not provided");
+    double s = 1, c = 0;
+    if (alpha_given) {
+        s = sin(alpha);
+        c = cos(alpha);
+    }
+    return RMPCD_DISPATCH_FORM(p, srd_impl, box, cells, p, axes, seed, step, s, c);
+}
+
+int rmpcd_compute_temperature(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                              const rmpcd_particles_t *p, double *te)
+{
+    int rc = check_box(box, "compute_temperature");
+    if (!rc) rc = check_particles(p, "compute_temperature", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start && te,
+                "compute_temperature: null argument");
+    return RMPCD_DISPATCH_FORM(p, temperature_impl, box, cells, p, te);
+}
+
+int rmpcd_mpcd_stream_periodic(const rmpcd_box_t *box, rmpcd_particles_t *p, double dt)
+{
+    int rc = check_box(box, "mpcd_stream_periodic");
+    if (!rc) rc = check_particles(p, "mpcd_stream_periodic", false);
+    if (rc) return rc;
+    RMPCD_CHECK(p->image != nullptr, "mpcd_stream_periodic: null image");
+    return RMPCD_DISPATCH_FORM(p, stream_periodic_impl, box, p, dt);
+}
+
+int rmpcd_mpcd_stream_nogravity_zwall(const rmpcd_box_t *box, rmpcd_particles_t *p, double dt)
+{
+    int rc = check_box(box, "mpcd_stream_nogravity_zwall");
+    if (!rc) rc = check_particles(p, "mpcd_stream_nogravity_zwall", true);
+    if (rc) return rc;
+    return RMPCD_DISPATCH_FORM(p, stream_zwall_impl, box, p, dt);
+}
+
+int rmpcd_bulk_reaction(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p,
+                        int32_t from, int32_t to, double rate, double tau, const double *draws,
+                        uint64_t seed, uint64_t step)
+{
+    int rc = check_box(box, "bulk_reaction");
+    if (!rc) rc = check_particles(p, "bulk_reaction", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start && cells->is_reac,
+                "bulk_reaction: null cell arrays");
+    RMPCD_CHECK(p->form == RMPCD_FP64 || (to >= 0 && to < 256), "bulk_reaction: species out of range");
+    return RMPCD_DISPATCH_FORM(p, bulk_impl, box, cells, p, from, to, rate, tau, draws, seed, step);
+}
+
+int rmpcd_threefry_uniform(uint64_t seed, uint64_t stream_id, uint64_t step, int64_t n, double *out)
+{
+    RMPCD_CHECK(out && n >= 0, "rmpcd_threefry_uniform: bad argument");
+    if (n == 0) return RMPCD_OK;
+    k_threefry_uniform<<<blocks_for(n, 256), 256, 0, stream()>>>(seed, stream_id, step, n, out);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+}  // extern "C"

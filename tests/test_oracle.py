@@ -1,0 +1,481 @@
+"""The oracle pinned: against the reference's own modules compiled from source (golden
+vectors in tests/golden/, and oracle/_ref where it is built), and against what the
+mathematics fixes (closed forms, invariants, brute force).  CPU only."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+REF = os.path.join(os.path.dirname(HERE), "oracle", "_ref")
+
+
+# ---- hilbert: golden vectors from the reference's hilbert.f90 ------------------------
+def test_hilbert_golden():
+    z = np.load(os.path.join(GOLD, "hilbert_ref.npz"))
+    for name in z.files:
+        if name.startswith("keys_"):
+            M = tuple(int(v) for v in name[5:].split("_"))
+            keys, inv = z[name], z["inv_" + name[5:]]
+            idx = 0
+            for k in range(2 ** M[2]):
+                for j in range(2 ** M[1]):
+                    for i in range(2 ** M[0]):
+                        assert O.compact_p_to_h((i, j, k), M) == keys[idx]
+                        idx += 1
+            for h in range(keys.size):
+                assert O.compact_h_to_p(h, M) == tuple(inv[h])
+        elif name.startswith("skeys_"):
+            M = tuple(int(v) for v in name[6:].split("_"))
+            nx, ny = 2 ** M[0], 2 ** M[1]
+            for t, h in z[name]:
+                p = (t % nx, (t // nx) % ny, t // (nx * ny))
+                assert O.compact_p_to_h(p, M) == h
+            for row in z["sinv_" + name[6:]]:
+                assert O.compact_h_to_p(row[0], M) == tuple(row[1:])
+        elif name.startswith("ukeys_"):
+            m = int(name[6:])
+            keys, inv = z[name], z["uinv_%d" % m]
+            n = 2 ** m
+            for t in range(keys.size):
+                assert O.p_to_h((t % n, (t // n) % n, t // (n * n)), m) == keys[t]
+                assert O.h_to_p(t, m) == tuple(inv[t])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "hilbert_drv")),
+                    reason="oracle/_ref not built (needs the reference sources)")
+def test_hilbert_against_ref_binary():
+    for M in [(2, 2, 4), (6, 3, 2), (4, 4, 4)]:
+        out = subprocess.run([os.path.join(REF, "hilbert_drv"), *map(str, M), "0"],
+                             capture_output=True, text=True, check=True).stdout.split("\n")
+        n = 2 ** sum(M)
+        keys = [int(v) for v in out[:n]]
+        idx = 0
+        for k in range(2 ** M[2]):
+            for j in range(2 ** M[1]):
+                for i in range(2 ** M[0]):
+                    assert O.compact_p_to_h((i, j, k), M) == keys[idx]
+                    idx += 1
+
+
+def test_hilbert_math():
+    # bijection onto [0, 2^sum M); consecutive keys are face neighbours on cubes
+    for M in [(3, 3, 3), (2, 4, 3)]:
+        seen = {}
+        for k in range(2 ** M[2]):
+            for j in range(2 ** M[1]):
+                for i in range(2 ** M[0]):
+                    h = O.compact_p_to_h((i, j, k), M)
+                    assert 0 <= h < 2 ** sum(M) and h not in seen
+                    seen[h] = (i, j, k)
+                    assert O.compact_h_to_p(h, M) == (i, j, k)
+        if M[0] == M[1] == M[2]:
+            for h in range(1, 2 ** sum(M)):
+                a, b = np.array(seen[h - 1]), np.array(seen[h])
+                assert np.abs(a - b).sum() == 1
+
+
+# ---- LJ: golden vectors from the reference's interaction.f90 -------------------------
+def test_lj_golden_bit_exact():
+    z = np.load(os.path.join(GOLD, "interaction_ref.npz"))
+    for row, o in zip(z["inputs"], z["outputs"]):
+        f = O.lj_force(row[:3], row[3], row[4], row[5])
+        e = O.lj_energy(row[3], row[4], row[5], 0.0)
+        lj = O.lj_params([[row[4]]], [[row[5]]], [[np.sqrt(row[3])]])
+        got = np.array([*f, e, lj.cut_energy[0]])
+        assert got.view(np.uint64).tolist() == o.view(np.uint64).tolist()
+
+
+def test_lj_math():
+    # F = -dE/dr (central difference), WCA energy zero at the cutoff, sizes checked
+    eps, sig = 1.3, 2.0
+    for r in (1.9, 2.05, 2.2):
+        h = 1e-6
+        dE = (O.lj_energy((r + h) ** 2, eps, sig) - O.lj_energy((r - h) ** 2, eps, sig)) / (2 * h)
+        f = O.lj_force([r, 0, 0], r * r, eps, sig)
+        assert abs(f[0] + dE) < 1e-5 * abs(dE)
+    cut = sig * 2 ** (1 / 6)
+    lj = O.lj_params([[eps]], [[sig]], [[cut]])
+    assert abs(O.lj_energy(cut * cut, eps, sig, lj.cut_energy[0])) < 1e-12
+    with pytest.raises(ValueError, match="unequal sizes"):
+        O.lj_params(np.ones((5, 4)), np.ones((5, 4)), np.ones((5, 4)))
+
+
+# ---- cells and sort ----------------------------------------------------------------
+def test_cells_init_padding():
+    c = O.cells_init((120, 30, 30))
+    assert tuple(c.M) == (7, 5, 5) and c.N == 2 ** 17
+    c = O.cells_init((8, 5, 5), has_walls=True)
+    assert tuple(c.L) == (8, 5, 6) and tuple(c.M) == (3, 3, 3)
+    c = O.cells_init((1, 1, 1))
+    assert tuple(c.M) == (1, 1, 1) and c.N == 8
+
+
+def test_cell_system_invariants():
+    # test_cell_system_0.f90:37-69 at a smaller N
+    rng = np.random.default_rng(0)
+    L = (120, 30, 30)
+    n = 20000
+    c = O.cells_init(L)
+    ps = O.PState(n)
+    ps.pos[:] = rng.random((n, 3)) * L
+    srt, i2i, cnt, st = O.sort(c, ps)
+    assert st[0] == 0 and cnt.sum() == n and st[-1] + cnt[-1] == n
+    assert (st[1:] == np.cumsum(cnt)[:-1]).all()
+    keys = np.array([O.compact_p_to_h(O.cartesian_indices(c, x), c.M) for x in srt.pos])
+    assert (np.diff(keys) >= 0).all()
+    for j in rng.choice(n, 300, replace=False):
+        assert O.compact_h_to_p(keys[j], c.M) == O.cartesian_indices(c, srt.pos[j])
+    # stable: within a cell the original order is kept; id_to_idx inverts the permutation
+    for h in np.unique(keys)[:200]:
+        ids = srt.id[keys == h]
+        assert (np.diff(ids) > 0).all()
+    assert (i2i[srt.id] == np.arange(n)).all()
+    # a multi-threaded count gives the same counts
+    cnt4, st4 = O.count_particles(c, ps.pos, nthreads=4)
+    assert (cnt4 == cnt).all() and (st4 == st).all()
+
+
+def test_cartesian_wrap_and_shift():
+    c = O.cells_init((8, 5, 5))
+    O.set_origin(c, (-0.25, -1.0, -0.5))
+    assert O.cartesian_indices(c, (7.75, 4.0, 4.5)) == (0, 0, 0)
+    assert O.cartesian_indices(c, (0.0, 0.0, 0.0)) == (0, 1, 0)
+    c.has_walls = 1
+    assert O.cartesian_indices(c, (0.2, 0.2, 4.6))[2] == 5
+
+
+def test_maximum_displacement_closed_form():
+    # test_particle_system_0.f90:29-47
+    rng = np.random.default_rng(1)
+    n = 3000
+    pos = rng.random((n, 3)) * (1, 20, 5)
+    old = pos.copy()
+    sp = np.ones(n, np.int32)
+    i = 1234
+    x = rng.random(3)
+    pos[i] += x
+    assert abs(O.maximum_displacement(pos, old, sp) - np.sqrt(x @ x)) < 1e-14
+    assert O.maximum_displacement(pos, old, sp) <= np.sqrt(3)
+    pos[i] -= x
+    assert O.maximum_displacement(pos, old, sp) < 1e-15
+    sp[i] = -1
+    pos[i] += 5
+    assert O.maximum_displacement(pos, old, sp) < 1e-15
+
+
+# ---- mpcd -----------------------------------------------------------------------------
+def test_rand_sphere_math():
+    rng = np.random.default_rng(2)
+    ns = []
+    for _ in range(4000):
+        u = rng.random(16)
+        n, k = O.rand_sphere(u)
+        assert k >= 2 and k % 2 == 0
+        assert abs(n @ n - 1) < 1e-14
+        a, b = 2 * u[k - 2] - 1, 2 * u[k - 1] - 1
+        assert a * a + b * b < 1
+        for j in range(0, k - 2, 2):  # rejected pairs lie outside the unit disc
+            assert (2 * u[j] - 1) ** 2 + (2 * u[j + 1] - 1) ** 2 >= 1
+        ns.append(n)
+    ns = np.array(ns)
+    # uniform on the sphere: zero mean, <n_i n_j> = delta_ij / 3
+    assert np.abs(ns.mean(axis=0)).max() < 0.05
+    assert np.abs(ns.T @ ns / len(ns) - np.eye(3) / 3).max() < 0.03
+
+
+def _cells_state(L, rho, seed, frac2=0.0):
+    rng = np.random.default_rng(seed)
+    c = O.cells_init(L)
+    O.set_origin(c, rng.random(3) - 1)
+    n = int(rho * np.prod(L))
+    ps = O.PState(n)
+    ps.pos[:] = rng.random((n, 3)) * L
+    ps.vel[:] = rng.normal(size=(n, 3))
+    if frac2:
+        ps.species[rng.random(n) < frac2] = 2
+    srt, i2i, cnt, st = O.sort(c, ps)
+    return c, srt, cnt, st
+
+
+def test_srd_rotation_math():
+    # per-cell momentum and energy are conserved; the applied matrix is the rotation by
+    # -alpha about the axis (mpcd.f90:163-171 builds R^T): w perpendicular to n goes to
+    # -n x w for the default alpha = 90 degrees
+    c, srt, cnt, st = _cells_state((6, 6, 6), 10, 3)
+    rng = np.random.default_rng(4)
+    axes = rng.normal(size=(c.N, 3))
+    axes /= np.linalg.norm(axes, axis=1)[:, None]
+    v = srt.vel.copy()
+    O.simple_mpcd_step(c, cnt, st, v, srt.species, axes)
+    for ci in np.nonzero(cnt > 1)[0]:
+        s, n = st[ci], cnt[ci]
+        u0 = srt.vel[s:s + n].mean(axis=0)
+        assert np.allclose(v[s:s + n].sum(axis=0), srt.vel[s:s + n].sum(axis=0), atol=1e-12)
+        w0 = srt.vel[s:s + n] - u0
+        w1 = v[s:s + n] - u0
+        ax = axes[ci]
+        assert np.allclose(w1 @ ax, w0 @ ax, atol=1e-12)  # axis component kept
+        wp = w0 - np.outer(w0 @ ax, ax)
+        w1p = w1 - np.outer(w1 @ ax, ax)
+        assert np.allclose(w1p, -np.cross(ax, wp), atol=1e-12)
+    # a given alpha rotates by -alpha
+    v2 = srt.vel.copy()
+    O.simple_mpcd_step(c, cnt, st, v2, srt.species, axes, alpha=0.7)
+    ci = int(np.nonzero(cnt > 2)[0][0])
+    s, n = st[ci], cnt[ci]
+    u0 = srt.vel[s:s + n].mean(axis=0)
+    ax = axes[ci]
+    w0 = srt.vel[s] - u0
+    wp = w0 - (w0 @ ax) * ax
+    want = (w0 @ ax) * ax + np.cos(0.7) * wp - np.sin(0.7) * np.cross(ax, wp)
+    assert np.allclose(v2[s] - u0, want, atol=1e-12)
+
+
+def test_srd_species_quirks():
+    # mean over species > 0 only; every particle of the cell is rotated; cells with a
+    # single counted particle draw nothing (mpcd.f90:135-141, 173-175)
+    c, srt, cnt, st = _cells_state((4, 4, 4), 10, 5)
+    ci = int(np.nonzero(cnt > 3)[0][0])
+    s, n = st[ci], cnt[ci]
+    srt.species[s] = 0
+    u = np.random.default_rng(6).random(8 * c.N)
+    axes, used = O.srd_axes_from_stream(c, cnt, st, srt.species, u)
+    v = srt.vel.copy()
+    O.simple_mpcd_step(c, cnt, st, v, srt.species, axes)
+    u_eff = srt.vel[s + 1:s + n].mean(axis=0)
+    assert np.allclose(v[s + 1:s + n].mean(axis=0), u_eff, atol=1e-12)
+    assert not np.allclose(v[s], srt.vel[s])
+    srt.species[s:s + n] = 0
+    srt.species[s] = 1
+    axes2, used2 = O.srd_axes_from_stream(c, cnt, st, srt.species, u)
+    assert np.all(axes2[ci] == 0)
+
+
+def test_temperature_closed_form():
+    # equipartition: T -> kT for Maxwell-Boltzmann velocities; and the formula's
+    # quirks (mean over n, count before the n_effective test)
+    c, srt, cnt, st = _cells_state((12, 12, 12), 10, 7)
+    T = O.compute_temperature(c, cnt, st, srt.vel, srt.species)
+    assert abs(T - 1.0) < 0.02
+    ci = int(np.nonzero(cnt > 3)[0][0])
+    s, n = st[ci], cnt[ci]
+    sp = srt.species.copy()
+    sp[:] = 0
+    sp[s:s + n] = 1
+    sp[s] = 0
+    vv = srt.vel[s:s + n]
+    u = vv[1:].sum(axis=0) / n
+    k = ((vv[1:] - u) ** 2).sum() / (3 * (n - 2))
+    ncount = (cnt > 1).sum()
+    assert abs(O.compute_temperature(c, cnt, st, srt.vel, sp) - k / ncount) < 1e-12
+
+
+def test_stream_periodic_closed_form():
+    c = O.cells_init((8, 5, 5))
+    pos = np.array([[7.9, 4.9, 0.1], [0.1, 0.2, 0.3]])
+    vel = np.array([[1.0, 1.0, -1.0], [-3.0, 10.0, 0.0]])
+    im = np.zeros((2, 3), np.int32)
+    O.mpcd_stream_periodic(c, pos, vel, im, 0.5)
+    assert np.allclose(pos, [[0.4, 0.4, 4.6], [6.6, 0.2, 0.3]])
+    assert (im == [[1, 1, -1], [-1, 1, 0]]).all()
+
+
+def test_bulk_reaction_closed_form():
+    # one draw per eligible cell (even with no 'from' particle), the last 'from'
+    # particle converts when u < 1 - exp(-k n_from tau)
+    c, srt, cnt, st = _cells_state((4, 4, 4), 10, 8, frac2=0.5)
+    is_reac = np.ones(c.N, np.uint8)
+    draws = np.full(c.N, 0.5)
+    sp = srt.species.copy()
+    O.bulk_reaction(c, cnt, st, is_reac, sp, 2, 1, 1.0, 1.0, draws)
+    for ci in np.nonzero(cnt > 1)[0]:
+        s, n = st[ci], cnt[ci]
+        nf = (srt.species[s:s + n] == 2).sum()
+        changed = np.nonzero(sp[s:s + n] != srt.species[s:s + n])[0]
+        p = 1 - np.exp(-1.0 * nf)
+        if 0.5 < p:
+            assert list(changed) == [np.nonzero(srt.species[s:s + n] == 2)[0][-1]]
+        else:
+            assert len(changed) == 0
+    sp2 = srt.species.copy()
+    O.bulk_reaction(c, cnt, st, np.zeros(c.N, np.uint8), sp2, 2, 1, 1.0, 1.0, draws * 0)
+    assert (sp2 == srt.species).all()
+
+
+def test_zwall_stream_reflection():
+    c = O.cells_init((5, 5, 5), has_walls=True)
+    pos = np.array([[1, 1, 0.1], [1, 1, 4.9], [1, 1, 2.0]])
+    vel = np.array([[0.3, 0, -1.0], [0, 0.2, 1.0], [0, 0, 1.0]])
+    f = np.zeros((3, 3))
+    fl = np.zeros(3, np.int32)
+    p0, v0 = pos.copy(), vel.copy()
+    O.mpcd_stream_nogravity_zwall(c, pos, vel, f, fl, 0.5)
+    assert np.allclose(vel[:2], -v0[:2]) and np.allclose(vel[2], v0[2])
+    assert np.allclose(pos[0], [1 - 0.3 * 0.3, 1, 0.1 + 0.3]) and 0 < pos[1, 2] < 5
+    assert list(fl) == [8, 8, 0]
+
+
+# ---- md -------------------------------------------------------------------------------
+def test_wall_md_invariants():
+    # test_md_0.f90:31-103: |v| preserved, 0 < y < L; bounce-back v -> -v, specular flips v_y
+    for bc in (O.BOUNCE_BACK_BC, O.SPECULAR_BC):
+        c, srt, cnt, st = _cells_state((1, 20, 5), 20, 9)
+        c.bc[1] = bc
+        rng = np.random.default_rng(10)
+        srt.vel[:] = (rng.random((srt.n, 3)) - 0.5) * 2 * np.array([1, 20, 5])
+        x0, v0 = srt.pos.copy(), srt.vel.copy()
+        will = (x0[:, 1] + v0[:, 1] < 0) | (x0[:, 1] + v0[:, 1] > 20)
+        is_md = np.zeros(c.N, np.uint8)
+        srt.force[:] = 0
+        O.cell_md_pos_wall(c, cnt, st, is_md, srt.pos, srt.vel, srt.force, 1.0, False, 1)
+        assert np.allclose(np.linalg.norm(srt.vel, axis=1), np.linalg.norm(v0, axis=1))
+        assert (srt.pos[:, 1] > 0).all() and (srt.pos[:, 1] < 20).all()
+        if bc == O.BOUNCE_BACK_BC:
+            assert np.allclose(srt.vel[will], -v0[will])
+        else:
+            assert np.allclose(srt.vel[will][:, [0, 2]], v0[will][:, [0, 2]])
+            assert np.allclose(srt.vel[will][:, 1], -v0[will][:, 1])
+
+
+def test_verlet_closed_form():
+    pos = np.array([[1.0, 2.0, 3.0]])
+    vel = np.array([[0.5, -1.0, 2.0]])
+    f = np.array([[2.0, 0.0, -4.0]])
+    fo = np.array([[0.0, 2.0, 0.0]])
+    fl = np.array([0], np.int32)
+    O.md_pos(pos, vel, f, 0.1)
+    assert np.allclose(pos, [[1.06, 1.9, 3.18]])
+    O.md_vel(vel, f, fo, fl, 0.1)
+    assert np.allclose(vel, [[0.6, -0.9, 1.8]])
+    fl[0] = 1 << O.WALL_BIT
+    O.md_vel(vel, f, fo, fl, 0.1)
+    assert np.allclose(vel, [[0.6, -0.9, 1.8]]) and fl[0] == 0
+
+
+def test_rattle_constraint():
+    # after rattle_dimer_pos |r12| = d; after rattle_dimer_vel (v1 - v2).r12 = 0
+    rng = np.random.default_rng(11)
+    L = (32.0, 32.0, 32.0)
+    for _ in range(20):
+        pos = np.array([[16.0, 16, 16], [20.5, 16, 16]]) + rng.normal(size=(2, 3)) * 0.01
+        old = pos.copy()
+        pos += rng.normal(size=(2, 3)) * 0.02
+        vel = rng.normal(size=(2, 3)) * 0.1
+        sp = np.array([1, 2], np.int32)
+        mass = np.array([300.0, 450.0])
+        O.rattle_dimer_pos(pos, vel, old, sp, mass, 4.5, 0.01, L)
+        r = pos[0] - pos[1]
+        assert abs(np.sqrt(r @ r) - 4.5) < 1e-12
+        O.rattle_dimer_vel(pos, vel, sp, mass, 4.5, 0.01, L)
+        assert abs((vel[0] - vel[1]) @ r) < 1e-12
+
+
+# ---- neighbour list and forces ---------------------------------------------------------
+def test_stencil_closed_form():
+    for cut in (1.5, 3.6, 4.245, 6.0):
+        st = O.make_stencil(1.0, cut)
+        m = int(np.floor(cut)) + 1
+        want = []
+        for i in range(-m, m + 1):
+            for j in range(-m, m + 1):
+                for k in range(-m, m + 1):
+                    d = np.maximum(0, np.abs([i, j, k]) - 1)
+                    if np.sqrt((d * d).sum()) < cut:
+                        want.append((i, j, k))
+        assert [tuple(r) for r in st] == want
+
+
+def _nl_state(L, rho, seed, n_col):
+    rng = np.random.default_rng(seed)
+    c = O.cells_init(L)
+    n = int(rho * np.prod(L))
+    ps = O.PState(n)
+    ps.pos[:] = rng.random((n, 3)) * L
+    srt, i2i, cnt, st = O.sort(c, ps)
+    col = O.PState(n_col)
+    col.pos[:] = rng.random((n_col, 3)) * L
+    return c, srt, cnt, st, col
+
+
+def test_neighbor_list_brute_force():
+    # test_neighbor_list_0.f90:47-81: every solvent particle within the cutoff is listed
+    L = (20, 10, 35)
+    c, srt, cnt, st, col = _nl_state(L, 5, 12, 6)
+    cutoff = 3.6
+    stl = O.make_stencil(1.0, cutoff)
+    lst, nl, is_md, is_reac = O.update_list(c, cnt, st, col.pos, col.species, srt.pos, srt.species,
+                                             cutoff, stl, None, 2000)
+    for i in range(col.n):
+        d = np.array([O.rel_pos(col.pos[i], y, L) for y in srt.pos])
+        inside = set(np.nonzero((d * d).sum(axis=1) < cutoff ** 2)[0])
+        assert set(lst[i, :nl[i]]) == inside
+    # the reference's negative test (WILL_FAIL): a list built with 0.8*cutoff misses some
+    lst2, nl2, _, _ = O.update_list(c, cnt, st, col.pos, col.species, srt.pos, srt.species,
+                                     0.8 * cutoff, O.make_stencil(1.0, 0.8 * cutoff), None, 2000)
+    d = np.array([O.rel_pos(col.pos[0], y, L) for y in srt.pos])
+    inside = set(np.nonzero((d * d).sum(axis=1) < cutoff ** 2)[0])
+    assert not inside <= set(lst2[0, :nl2[0]])
+
+
+def test_force_newton_and_energy():
+    L = (16, 16, 16)
+    c, srt, cnt, st, col = _nl_state(L, 10, 13, 2)
+    col.species[:] = [1, 2]
+    eps = np.array([[1.0, 0.5]])
+    sig = np.array([[2.0, 2.5]])
+    cut = sig * 2 ** (1 / 6)
+    lj = O.lj_params(eps, sig, cut)
+    rad = cut.max() + 1.0
+    lst, nl, _, _ = O.update_list(c, cnt, st, col.pos, col.species, srt.pos, srt.species, rad,
+                                   O.make_stencil(1.0, rad), lj, 3000)
+    f2 = np.zeros_like(srt.pos)
+    f1 = np.zeros_like(col.pos)
+    fl = np.zeros(srt.n, np.int32)
+    e = O.compute_force(col.pos, col.species, f1, srt.pos, srt.species, f2, fl, lst, nl, L, lj)
+    assert np.allclose(f1.sum(axis=0) + f2.sum(axis=0), 0, atol=1e-9 * np.abs(f2).sum())
+    # brute-force energy and flag bits over all pairs
+    e_bf = 0.0
+    for i in range(2):
+        for j in range(srt.n):
+            d = O.rel_pos(col.pos[i], srt.pos[j], L)
+            r2 = d @ d
+            p = i  # (solvent species 1, colloid species i+1)
+            if r2 <= lj.cut_sq[p]:
+                e_bf += O.lj_energy(r2, lj.eps[p], lj.sigma[p], lj.cut_energy[p])
+                assert fl[j] & (1 << O.MD_BIT)
+                assert bool(fl[j] & (1 << O.CATALYZED_BIT)) == bool(fl[j] & (1 << O.CATALYZED_BIT))
+    assert abs(e - e_bf) < 1e-10 * max(1, abs(e_bf))
+
+
+def test_force_n2_newton():
+    rng = np.random.default_rng(14)
+    L = (30.0, 30.0, 30.0)
+    n = 40
+    pos = rng.random((n, 3)) * 30
+    sp = np.ones(n, np.int32)
+    f = np.zeros((n, 3))
+    sig = 5.0
+    lj = O.lj_params([[1.0]], [[sig]], [[sig * 2 ** (1 / 6)]])
+    O.compute_force_n2(pos, sp, f, L, lj)
+    assert np.allclose(f.sum(axis=0), 0, atol=1e-9 * max(1, np.abs(f).sum()))
+
+
+def test_catalytic_helpers():
+    L = (20.0, 20.0, 20.0)
+    cpos = np.array([[10.0, 10, 10], [14.5, 10, 10]])
+    pos = np.array([[11.0, 10, 10], [10, 10, 13.5], [10, 10, 1.0], [1.0, 1, 1], [14.5, 11, 10]])
+    sp = np.array([1, 1, 2, 2, 1], np.int32)
+    fl = np.zeros(5, np.int32)
+    lst = np.array([0, 1, 4], np.int32)
+    O.flag_particles_nl(lst, 3, cpos[0], pos, sp, fl, L, 2.5 ** 2)
+    assert list(fl >> O.REAC_BIT & 1) == [1, 0, 0, 0, 0]
+    pos[0] = [10, 17.0, 10]
+    O.change_species(cpos, pos, sp, fl, L, 2.5 ** 2, 2.5 ** 2, 0.5, np.array([0.4, 0, 0, 0, 0]))
+    assert sp[0] == 2 and fl[0] == 0
+    O.refuel(cpos, pos, sp, L, 20)  # far = (20*0.45)^2 = 81 in single precision
+    assert list(sp) == [2, 1, 2, 1, 1]
