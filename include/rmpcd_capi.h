@@ -228,6 +228,52 @@ int rmpcd_change_species(rmpcd_particles_t *solvent, const rmpcd_particles_t *co
 int rmpcd_refuel(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
                  const double edges[3], int32_t L1);
 
+/* ---- the single-dimer nanomotor program (programs/single_dimer_pbc.f90) -------------- */
+/* The time loop of the driver on device-resident state: set-up (:183-296), the MD step
+ * (:306-383) and the collision cycle (:386-443), with the catalytic helpers (:493-565).
+ * mode RMPCD_SIM_REFERENCE issues the reference's call sequence through this ABI and
+ * reads the skin test on the host every step; RMPCD_SIM_FUSED runs the same step as
+ * fused device-side kernels, one hipGraph per collision cycle. */
+#define RMPCD_SIM_REFERENCE 0
+#define RMPCD_SIM_FUSED 1
+typedef struct rmpcd_dimer_config {
+    int32_t L[3];
+    int32_t rho;          /* solvent per cell */
+    int32_t n_md;         /* N_MD: MD steps per collision, dt = tau / N_MD */
+    int32_t form;         /* RMPCD_FP64 or RMPCD_FP32 for the solvent */
+    int32_t mode;         /* RMPCD_SIM_REFERENCE or RMPCD_SIM_FUSED */
+    int32_t bulk_rmpcd;   /* bulk B->A reaction instead of refuel */
+    double T, tau;
+    double sigma_C, sigma_N, d;
+    double epsilon_C[2];  /* C sphere with solvent species 1, 2 */
+    double epsilon_N[2];  /* N sphere with solvent species 1, 2 */
+    double epsilon_CC, epsilon_NC, epsilon_NN;
+    double prob;          /* A->B probability on leaving the catalytic sphere */
+    double bulk_rate;
+    double skin;          /* 0: the driver's value, 2 */
+    int32_t nmax_list;    /* 0: max(int(300 sigma^3), 1.3x the expected shell count) */
+    int32_t pad_;
+    uint64_t seed;
+} rmpcd_dimer_config_t;
+typedef struct rmpcd_dimer_stats {
+    int64_t n_solvent, md_steps, cycles, n_extra_sorting;
+    double e1, e2;        /* last compute_force / compute_force_n2 energies */
+    double dt;
+    double mass[2];
+    int32_t nmax, pad_;
+    int32_t *list, *nlist;     /* device: the colloids' neighbour lists */
+    void *pos_old;             /* device: solvent positions at the last sort */
+    double *colloid_pos_old;   /* device */
+} rmpcd_dimer_stats_t;
+typedef struct rmpcd_dimer rmpcd_dimer_t;
+int rmpcd_dimer_create(const rmpcd_dimer_config_t *cfg, rmpcd_dimer_t **sim);
+/* n_cycles collision cycles of N_MD steps each (single_dimer_pbc.f90:303-445) */
+int rmpcd_dimer_run(rmpcd_dimer_t *sim, int32_t n_cycles);
+/* views of the current device state (valid until the next run) */
+int rmpcd_dimer_state(rmpcd_dimer_t *sim, rmpcd_particles_t *solvent, rmpcd_particles_t *colloids,
+                      rmpcd_box_t *box, rmpcd_cells_t *cells, rmpcd_dimer_stats_t *stats);
+int rmpcd_dimer_destroy(rmpcd_dimer_t *sim);
+
 /* ---- counter-based generator used when draws are not injected ------------------------ */
 /* uniform doubles in [0,1): out[i] = U(seed, stream, step, i), device output */
 int rmpcd_threefry_uniform(uint64_t seed, uint64_t stream, uint64_t step, int64_t n,

@@ -51,6 +51,27 @@ class LJ(C.Structure):
                 ("cut_sq", C.c_double * 16), ("cut_energy", C.c_double * 16)]
 
 
+SIM_REFERENCE, SIM_FUSED = 0, 1
+
+
+class DimerConfig(C.Structure):
+    _fields_ = [("L", C.c_int32 * 3), ("rho", C.c_int32), ("n_md", C.c_int32), ("form", C.c_int32),
+                ("mode", C.c_int32), ("bulk_rmpcd", C.c_int32), ("T", C.c_double), ("tau", C.c_double),
+                ("sigma_C", C.c_double), ("sigma_N", C.c_double), ("d", C.c_double),
+                ("epsilon_C", C.c_double * 2), ("epsilon_N", C.c_double * 2),
+                ("epsilon_CC", C.c_double), ("epsilon_NC", C.c_double), ("epsilon_NN", C.c_double),
+                ("prob", C.c_double), ("bulk_rate", C.c_double), ("skin", C.c_double),
+                ("nmax_list", C.c_int32), ("pad_", C.c_int32), ("seed", C.c_uint64)]
+
+
+class DimerStats(C.Structure):
+    _fields_ = [("n_solvent", C.c_int64), ("md_steps", C.c_int64), ("cycles", C.c_int64),
+                ("n_extra_sorting", C.c_int64), ("e1", C.c_double), ("e2", C.c_double),
+                ("dt", C.c_double), ("mass", C.c_double * 2), ("nmax", C.c_int32), ("pad_", C.c_int32),
+                ("list", C.c_void_p), ("nlist", C.c_void_p), ("pos_old", C.c_void_p),
+                ("colloid_pos_old", C.c_void_p)]
+
+
 _P = C.c_void_p
 _D = C.c_double
 _I = C.c_int32
@@ -103,6 +124,10 @@ SIGNATURES = {
     "rmpcd_flag_particles_nl": [_PARTP, _PARTP, _P, _P, _I, _D3, _D],
     "rmpcd_change_species": [_PARTP, _PARTP, _D3, _D, _D, _D, _P, _U64, _U64],
     "rmpcd_refuel": [_PARTP, _PARTP, _D3, _I],
+    "rmpcd_dimer_create": [C.POINTER(DimerConfig), C.POINTER(C.c_void_p)],
+    "rmpcd_dimer_run": [_P, _I],
+    "rmpcd_dimer_state": [_P, _PARTP, _PARTP, _BOXP, _CELLP, C.POINTER(DimerStats)],
+    "rmpcd_dimer_destroy": [_P],
     "rmpcd_threefry_uniform": [_U64, _U64, _U64, _I64, _P],
     "rmpcd_threefry2x64_20": [C.POINTER(_U64), C.POINTER(_U64), C.POINTER(_U64)],
 }

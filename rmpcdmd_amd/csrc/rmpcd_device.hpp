@@ -239,6 +239,7 @@ enum : uint64_t {
     STREAM_UNIFORM = 4,
     STREAM_SHIFT = 5,
     STREAM_INIT = 6,
+    STREAM_PLACE = 7,
 };
 
 // Marsaglia's method on a stream of uniform pairs (src/mpcd.f90:43-62)

@@ -1,0 +1,65 @@
+"""The single-dimer nanomotor program (programs/single_dimer_pbc.f90) as a
+device-resident simulation behind the C ABI (rmpcd_dimer_*)."""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import capi
+from .capi import FP32, FP64, SIM_FUSED, SIM_REFERENCE
+
+# the dimer of the nanomotor experiments: sigma = 2, d = 4.5, tau = 0.5, N_MD = 50
+DIMER_DEFAULTS = dict(L=(48, 48, 48), rho=10, n_md=50, T=1.0, tau=0.5, sigma_C=2.0, sigma_N=2.0,
+                      d=4.5, epsilon_C=(1.0, 1.0), epsilon_N=(1.0, 1.0), epsilon_CC=1.0,
+                      epsilon_NC=1.0, epsilon_NN=1.0, prob=1.0, bulk_rmpcd=0, bulk_rate=0.01,
+                      skin=2.0, nmax_list=0, seed=20261202, form=FP32, mode=SIM_FUSED)
+
+
+def make_config(**kw):
+    p = dict(DIMER_DEFAULTS)
+    unknown = set(kw) - set(p)
+    if unknown:
+        raise TypeError("unknown dimer parameters: %s" % sorted(unknown))
+    p.update(kw)
+    c = capi.DimerConfig()
+    c.L = (C.c_int32 * 3)(*p["L"])
+    for k in ("rho", "n_md", "form", "mode", "bulk_rmpcd", "nmax_list"):
+        setattr(c, k, int(p[k]))
+    for k in ("T", "tau", "sigma_C", "sigma_N", "d", "epsilon_CC", "epsilon_NC", "epsilon_NN", "prob",
+              "bulk_rate", "skin"):
+        setattr(c, k, float(p[k]))
+    c.epsilon_C = (C.c_double * 2)(*p["epsilon_C"])
+    c.epsilon_N = (C.c_double * 2)(*p["epsilon_N"])
+    c.seed = int(p["seed"])
+    return c
+
+
+class SingleDimer:
+    def __init__(self, **kw):
+        self.config = make_config(**kw)
+        self._h = C.c_void_p()
+        capi.call("rmpcd_dimer_create", C.byref(self.config), C.byref(self._h))
+
+    def run(self, n_cycles):
+        capi.call("rmpcd_dimer_run", self._h, int(n_cycles))
+
+    def state(self):
+        sol, col = capi.Particles(), capi.Particles()
+        box, cells, st = capi.Box(), capi.Cells(), capi.DimerStats()
+        capi.call("rmpcd_dimer_state", self._h, C.byref(sol), C.byref(col), C.byref(box), C.byref(cells),
+                  C.byref(st))
+        return sol, col, box, cells, st
+
+    @property
+    def n_solvent(self):
+        return self.state()[4].n_solvent
+
+    def close(self):
+        if self._h:
+            capi.lib().rmpcd_dimer_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
