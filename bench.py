@@ -31,7 +31,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=4, help="collision cycles timed")
     ap.add_argument("--warmup", type=int, default=1, help="untimed collision cycles")
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--mode", choices=["fused", "reference"], default="reference")
+    ap.add_argument("--mode", choices=["fused", "reference"], default="fused")
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--n-md", type=int, default=50)
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")

@@ -26,6 +26,7 @@
  */
 #ifndef RMPCD_CAPI_H
 #define RMPCD_CAPI_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -101,6 +102,11 @@ int rmpcd_init(int device);
 int rmpcd_set_stream(void *hip_stream);
 int rmpcd_synchronize(void);
 const char *rmpcd_version(void);
+/* device memory for callers without another allocator (the Fortran binding layer);
+ * rmpcd_memcpy is synchronous, kind 0: host->device, 1: device->host, 2: device->device */
+int rmpcd_malloc(void **ptr, size_t bytes);
+int rmpcd_free(void *ptr);
+int rmpcd_memcpy(void *dst, const void *src, size_t bytes, int32_t kind);
 
 /* ---- hilbert.f90 ------------------------------------------------------------------ */
 /* compact_p_to_h (src/hilbert.f90:241-271) and compact_h_to_p (:273-307) on the host */
@@ -252,11 +258,13 @@ typedef struct rmpcd_dimer_config {
     double bulk_rate;
     double skin;          /* 0: the driver's value, 2 */
     int32_t nmax_list;    /* 0: max(int(300 sigma^3), 1.3x the expected shell count) */
-    int32_t pad_;
-    uint64_t seed;
+    int32_t n_replicas;   /* independent systems advanced together (0 or 1: one) */
+    uint64_t seed;        /* replica r uses seed + r */
+    const int32_t *replica_L;  /* host, 3 per replica; NULL: every replica uses L */
 } rmpcd_dimer_config_t;
 typedef struct rmpcd_dimer_stats {
-    int64_t n_solvent, md_steps, cycles, n_extra_sorting;
+    int64_t n_solvent, md_steps, cycles, n_extra_sorting;  /* of the replica asked for */
+    int64_t n_replicas, n_solvent_total;
     double e1, e2;        /* last compute_force / compute_force_n2 energies */
     double dt;
     double mass[2];
@@ -269,9 +277,10 @@ typedef struct rmpcd_dimer rmpcd_dimer_t;
 int rmpcd_dimer_create(const rmpcd_dimer_config_t *cfg, rmpcd_dimer_t **sim);
 /* n_cycles collision cycles of N_MD steps each (single_dimer_pbc.f90:303-445) */
 int rmpcd_dimer_run(rmpcd_dimer_t *sim, int32_t n_cycles);
-/* views of the current device state (valid until the next run) */
-int rmpcd_dimer_state(rmpcd_dimer_t *sim, rmpcd_particles_t *solvent, rmpcd_particles_t *colloids,
-                      rmpcd_box_t *box, rmpcd_cells_t *cells, rmpcd_dimer_stats_t *stats);
+/* views of replica r's current device state (valid until the next run) */
+int rmpcd_dimer_state(rmpcd_dimer_t *sim, int32_t replica, rmpcd_particles_t *solvent,
+                      rmpcd_particles_t *colloids, rmpcd_box_t *box, rmpcd_cells_t *cells,
+                      rmpcd_dimer_stats_t *stats);
 int rmpcd_dimer_destroy(rmpcd_dimer_t *sim);
 
 /* ---- counter-based generator used when draws are not injected ------------------------ */

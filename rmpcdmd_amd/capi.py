@@ -61,12 +61,14 @@ class DimerConfig(C.Structure):
                 ("epsilon_C", C.c_double * 2), ("epsilon_N", C.c_double * 2),
                 ("epsilon_CC", C.c_double), ("epsilon_NC", C.c_double), ("epsilon_NN", C.c_double),
                 ("prob", C.c_double), ("bulk_rate", C.c_double), ("skin", C.c_double),
-                ("nmax_list", C.c_int32), ("pad_", C.c_int32), ("seed", C.c_uint64)]
+                ("nmax_list", C.c_int32), ("n_replicas", C.c_int32), ("seed", C.c_uint64),
+                ("replica_L", C.c_void_p)]
 
 
 class DimerStats(C.Structure):
     _fields_ = [("n_solvent", C.c_int64), ("md_steps", C.c_int64), ("cycles", C.c_int64),
-                ("n_extra_sorting", C.c_int64), ("e1", C.c_double), ("e2", C.c_double),
+                ("n_extra_sorting", C.c_int64), ("n_replicas", C.c_int64),
+                ("n_solvent_total", C.c_int64), ("e1", C.c_double), ("e2", C.c_double),
                 ("dt", C.c_double), ("mass", C.c_double * 2), ("nmax", C.c_int32), ("pad_", C.c_int32),
                 ("list", C.c_void_p), ("nlist", C.c_void_p), ("pos_old", C.c_void_p),
                 ("colloid_pos_old", C.c_void_p)]
@@ -90,6 +92,9 @@ SIGNATURES = {
     "rmpcd_set_stream": [_P],
     "rmpcd_synchronize": [],
     "rmpcd_version": [],
+    "rmpcd_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
+    "rmpcd_free": [_P],
+    "rmpcd_memcpy": [_P, _P, C.c_size_t, _I],
     "rmpcd_compact_p_to_h": [C.POINTER(_I), C.POINTER(_I)],
     "rmpcd_compact_h_to_p": [_I, C.POINTER(_I), C.POINTER(_I)],
     "rmpcd_hilbert_table": [_BOXP, _P],
@@ -126,7 +131,7 @@ SIGNATURES = {
     "rmpcd_refuel": [_PARTP, _PARTP, _D3, _I],
     "rmpcd_dimer_create": [C.POINTER(DimerConfig), C.POINTER(C.c_void_p)],
     "rmpcd_dimer_run": [_P, _I],
-    "rmpcd_dimer_state": [_P, _PARTP, _PARTP, _BOXP, _CELLP, C.POINTER(DimerStats)],
+    "rmpcd_dimer_state": [_P, _I, _PARTP, _PARTP, _BOXP, _CELLP, C.POINTER(DimerStats)],
     "rmpcd_dimer_destroy": [_P],
     "rmpcd_threefry_uniform": [_U64, _U64, _U64, _I64, _P],
     "rmpcd_threefry2x64_20": [C.POINTER(_U64), C.POINTER(_U64), C.POINTER(_U64)],

@@ -206,6 +206,24 @@ __host__ __device__ inline double lj_energy(double r_sq, double eps, double sigm
     return (4.0 * eps) * (s6 * s6 - s6) - ecut;
 }
 
+// LJ parameters passed by value to kernels (lj_params_t without the cut radii)
+struct LJ {
+    int n1;
+    double eps[16], sigma[16], cut_sq[16], cut_energy[16];
+};
+inline LJ make_lj(const rmpcd_lj_t *l)
+{
+    LJ o;
+    o.n1 = l->n1;
+    for (int k = 0; k < 16; ++k) {
+        o.eps[k] = l->eps[k];
+        o.sigma[k] = l->sigma[k];
+        o.cut_sq[k] = l->cut_sq[k];
+        o.cut_energy[k] = l->cut_energy[k];
+    }
+    return o;
+}
+
 // ---------------------------------------------------------------------------------
 // Threefry-2x64-20 (Salmon, Moraes, Dror, Shaw, SC'11; Random123 rotation constants)
 __host__ __device__ inline uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
@@ -336,13 +354,36 @@ __device__ inline void atomic_max_nonneg(double *addr, double v)
     atomicMax((unsigned long long *)addr, (unsigned long long)__double_as_longlong(v));
 }
 
-// OR into a flag of either width
+// OR / AND into a flag of either width (a uint8 flag is updated through its aligned
+// 32-bit word; flag arrays are allocated in multiples of 256 bytes)
 __device__ inline void atomic_or_flag(int32_t *f, int m) { atomicOr(f, m); }
 __device__ inline void atomic_or_flag(uint8_t *f, int m)
 {
     uintptr_t a = (uintptr_t)f;
     unsigned int *w = (unsigned int *)(a & ~(uintptr_t)3);
     atomicOr(w, (unsigned int)(m & 0xff) << (8 * (a & 3)));
+}
+__device__ inline void atomic_and_flag(int32_t *f, int m) { atomicAnd(f, m); }
+__device__ inline void atomic_and_flag(uint8_t *f, int m)
+{
+    uintptr_t a = (uintptr_t)f;
+    unsigned int *w = (unsigned int *)(a & ~(uintptr_t)3);
+    unsigned int sh = 8 * (unsigned)(a & 3);
+    atomicAnd(w, ~((~(unsigned int)m & 0xffu) << sh));
+}
+
+// Hilbert key of stencil cell j around the cell of position x
+// (neighbor_list.f90:111-121: modulo(cell + stencil - 1, L) with the 1-based cell)
+__device__ inline int stencil_key_dev(const Box &b, const double *x, const int32_t *st, int j)
+{
+    int p0, p1, p2;
+    cell_of<double>(b, x[0], x[1], x[2], p0, p1, p2);
+    int a0 = (p0 + st[3 * j]) % b.L[0], a1 = (p1 + st[3 * j + 1]) % b.L[1],
+        a2 = (p2 + st[3 * j + 2]) % b.L[2];
+    a0 += a0 < 0 ? b.L[0] : 0;
+    a1 += a1 < 0 ? b.L[1] : 0;
+    a2 += a2 < 0 ? b.L[2] : 0;
+    return compact_key(a0, a1, a2, b.M[0], b.M[1], b.M[2]);
 }
 
 }  // namespace rmpcd

@@ -9,23 +9,6 @@
 
 namespace rmpcd {
 
-struct LJ {
-    int n1;
-    double eps[16], sigma[16], cut_sq[16], cut_energy[16];
-};
-static LJ make_lj(const rmpcd_lj_t *l)
-{
-    LJ o;
-    o.n1 = l->n1;
-    for (int k = 0; k < 16; ++k) {
-        o.eps[k] = l->eps[k];
-        o.sigma[k] = l->sigma[k];
-        o.cut_sq[k] = l->cut_sq[k];
-        o.cut_energy[k] = l->cut_energy[k];
-    }
-    return o;
-}
-
 constexpr int NL_THREADS = 256;
 
 // cell of a colloid and the Hilbert key of stencil cell j (neighbor_list.f90:111-121)

@@ -158,6 +158,31 @@ int rmpcd_synchronize(void)
     return RMPCD_OK;
 }
 
+int rmpcd_malloc(void **ptr, size_t bytes)
+{
+    RMPCD_CHECK(ptr != nullptr, "rmpcd_malloc: null argument");
+    *ptr = nullptr;
+    RMPCD_HIP(hipMalloc(ptr, bytes < 256 ? 256 : (bytes + 255) & ~(size_t)255));
+    return RMPCD_OK;
+}
+
+int rmpcd_free(void *ptr)
+{
+    if (ptr) RMPCD_HIP(hipFree(ptr));
+    return RMPCD_OK;
+}
+
+int rmpcd_memcpy(void *dst, const void *src, size_t bytes, int32_t kind)
+{
+    RMPCD_CHECK((dst && src) || bytes == 0, "rmpcd_memcpy: null pointer");
+    RMPCD_CHECK(kind >= 0 && kind <= 2, "rmpcd_memcpy: kind must be 0, 1 or 2");
+    if (bytes == 0) return RMPCD_OK;
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    RMPCD_HIP(hipDeviceSynchronize());
+    RMPCD_HIP(hipMemcpy(dst, src, bytes, k));
+    return RMPCD_OK;
+}
+
 int32_t rmpcd_compact_p_to_h(const int32_t p[3], const int32_t M[3])
 {
     return compact_key(p[0], p[1], p[2], M[0], M[1], M[2]);

@@ -1,29 +1,30 @@
 // rmpcd_sim.hip -- the single-dimer nanomotor time loop of
 // programs/single_dimer_pbc.f90 (set-up :183-296, MD step :306-383, collision cycle
-// :386-443, helpers :493-565) on device-resident state.
+// :386-443, helpers :493-565) on device-resident state, for one or many independent
+// replicas.
 //
 // Two execution modes share the state:
 //   RMPCD_SIM_REFERENCE  the reference's call sequence through the library's own C ABI
-//                        entry points, with the host reading the skin test every step
-//                        (the plain form);
-//   RMPCD_SIM_FUSED      the fused device-side step of rmpcd_fused.hip (no host round
-//                        trip per step, hipGraph-captured cycles).
-// Random numbers come from Threefry-2x64-20 keyed by (seed, stream); the counters are
-// listed next to each use so that a test can reproduce every draw on the host.
+//                        entry points, one replica after the other, with the host
+//                        reading the skin test every step (the plain form);
+//   RMPCD_SIM_FUSED      the fused device-side step of rmpcd_fused.hip: every replica in
+//                        each launch, the skin test decided on the device, one hipGraph
+//                        per collision cycle.
+// Random numbers come from Threefry-2x64-20 keyed by (seed + replica, stream); the
+// counters are given next to each use so a test can reproduce every draw.
 #include <math.h>
 #include <string.h>
 
 #include <new>
 #include <vector>
 
-#include "rmpcd_internal.hpp"
 #include "rmpcd_sim.hpp"
 
 namespace rmpcd {
 
 // ---- set-up kernels --------------------------------------------------------------
-// Maxwell-Boltzmann velocities: particle id draws threefry((0, id), (seed, INIT)) and
-// threefry((1, id), (seed, INIT)); Box-Muller on the first two pairs gives 3 normals
+// Maxwell-Boltzmann velocities: particle i draws threefry((0, i), (seed, INIT)) and
+// threefry((1, i), (seed, INIT)); Box-Muller on the two pairs gives three normals
 __device__ inline void normals3(uint64_t seed, uint64_t id, double out[3])
 {
     uint64_t a, b, c, d;
@@ -36,8 +37,9 @@ __device__ inline void normals3(uint64_t seed, uint64_t id, double out[3])
     out[2] = r2 * cos(2 * M_PI * u4);
 }
 
+// velocities and their per-block sums (summed in block order by k_remove_mean)
 template <class F>
-__global__ void k_init_velocities(PView<F> p, uint64_t seed, double sqrtT, double *__restrict__ sum)
+__global__ void k_init_velocities(PView<F> p, uint64_t seed, double sqrtT, double *__restrict__ part)
 {
     __shared__ double red[16];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -45,14 +47,25 @@ __global__ void k_init_velocities(PView<F> p, uint64_t seed, double sqrtT, doubl
     if (i < p.n) {
         normals3(seed, (uint64_t)i, v);
         for (int k = 0; k < 3; ++k) {
-            v[k] *= sqrtT;
-            p.vel[k * p.n + i] = (typename F::real)v[k];
+            p.vel[k * p.n + i] = (typename F::real)(v[k] * sqrtT);
             v[k] = (double)p.vel[k * p.n + i];
         }
     }
     for (int k = 0; k < 3; ++k) {
         double s = block_sum(v[k], red);
-        if (threadIdx.x == 0) atomicAdd(&sum[k], s);
+        if (threadIdx.x == 0) part[3 * blockIdx.x + k] = s;
+    }
+}
+
+__global__ void k_sum_blocks3(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    __shared__ double red[16];
+    double s[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int k = 0; k < 3; ++k) s[k] += part[3 * b + k];
+    for (int k = 0; k < 3; ++k) {
+        double t = block_sum(s[k], red);
+        if (threadIdx.x == 0) out[k] = t;
     }
 }
 
@@ -66,13 +79,16 @@ __global__ void k_remove_mean(PView<F> p, const double *__restrict__ sum)
 }
 
 // random_placement (particle_system.f90:250-302) with the colloids as obstacles;
-// attempt k of particle i uses threefry((k, i), (seed, PLACE)) and threefry((k, i+2^40), ...)
+// attempt k of particle i uses threefry((k, i), (seed, PLACE)) for x, y and
+// threefry((k, i + 2^40), (seed, PLACE)) for z
 template <class F>
-__global__ void k_place(PView<F> p, PView<F64> obst, int n_obst, double L0, double L1, double L2,
-                        LJCut cut, uint64_t seed, int32_t *__restrict__ fail)
+__global__ void k_place_random(PView<F> p, PView<F64> obst, int n_obst, double L0, double L1, double L2,
+                               int n1, double cs0, double cs1, double cs2, double cs3, uint64_t seed,
+                               int32_t *__restrict__ fail)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
+    double cut_sq[4] = {cs0, cs1, cs2, cs3};
     int s1 = (int)p.species[i];
     double x[3] = {0, 0, 0};
     bool ok = false;
@@ -90,7 +106,8 @@ __global__ void k_place(PView<F> p, PView<F64> obst, int n_obst, double L0, doub
             double d2 = min_image(x[2] - obst.pos[2 * obst.n + j], L2);
             double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
             int s2 = obst.species[j];
-            if (rsq < cut.cut_sq[(s1 - 1) + cut.n1 * (s2 - 1)]) ok = false;
+            int q = (s1 - 1) + n1 * (s2 - 1);
+            if (q >= 0 && q < 4 && rsq < cut_sq[q]) ok = false;
         }
     }
     if (!ok) atomicOr(fail, 1);
@@ -118,10 +135,9 @@ __global__ void k_fill_state(PView<F> p)
     p.flags[i] = (typename F::small)0;
 }
 
-// three uniforms of the grid shift of collision s: threefry((s, 0), (seed, SHIFT)) gives
-// u0, u1 and threefry((s, 1), ...) gives u2 (the reference draws three doubles in order,
-// cell_system.f90:159-161)
-void shift_draws(uint64_t seed, uint64_t s, double u[3])
+// the grid shift of collision s: threefry((s, 0), (seed, SHIFT)) gives u0, u1 and
+// threefry((s, 1), ...) gives u2, drawn in the order of cell_system.f90:159-161
+__host__ __device__ void shift_uniforms(uint64_t seed, uint64_t s, double u[3])
 {
     uint64_t a, b, c, d;
     threefry2x64(s, 0, seed, STREAM_SHIFT, a, b);
@@ -130,6 +146,7 @@ void shift_draws(uint64_t seed, uint64_t s, double u[3])
     u[1] = u01(b);
     u[2] = u01(c);
 }
+void shift_draws(uint64_t seed, uint64_t s, double u[3]) { shift_uniforms(seed, s, u); }
 
 // ---- host-side state management ---------------------------------------------------
 static size_t real_size(int form) { return form == RMPCD_FP64 ? 8 : 4; }
@@ -137,7 +154,11 @@ static size_t small_size(int form) { return form == RMPCD_FP64 ? 4 : 1; }
 
 int DimerSim::alloc(void **p, size_t bytes)
 {
-    if (hipMalloc(p, bytes) != hipSuccess) return set_error(RMPCD_ERR_HIP, "dimer: hipMalloc of %zu bytes failed", bytes);
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return set_error(RMPCD_ERR_HIP, "dimer: hipMalloc of %zu bytes failed", bytes);
+    }
     allocs.push_back(*p);
     if (hipMemset(*p, 0, bytes) != hipSuccess) return set_error(RMPCD_ERR_HIP, "dimer: hipMemset failed");
     return RMPCD_OK;
@@ -147,18 +168,39 @@ DimerSim::~DimerSim()
 {
     (void)hipDeviceSynchronize();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (fstream) (void)hipStreamDestroy(fstream);
     for (void *p : allocs) (void)hipFree(p);
 }
 
-int DimerSim::setup(const rmpcd_dimer_config_t &c)
+int DimerSim::setup(const rmpcd_dimer_config_t &c, const int32_t *replica_L)
 {
     cfg = c;
-    int rc = rmpcd_box_init(&box, cfg.L, 1.0, 0);
-    if (rc) return rc;
-    n = (int64_t)cfg.rho * cfg.L[0] * cfg.L[1] * cfg.L[2];
-    RMPCD_CHECK(n > 0 && n < ((int64_t)1 << 31), "dimer: solvent count out of range");
-    RMPCD_CHECK(cfg.n_md > 0 && cfg.tau > 0, "dimer: N_MD and tau must be positive");
+    R = cfg.n_replicas > 0 ? cfg.n_replicas : 1;
+    RMPCD_CHECK(R <= 4096, "dimer: at most 4096 replicas");
+    RMPCD_CHECK(cfg.n_md > 0 && cfg.tau > 0 && cfg.rho > 0, "dimer: N_MD, tau and rho must be positive");
     RMPCD_CHECK(cfg.form == RMPCD_FP64 || cfg.form == RMPCD_FP32, "dimer: unknown form");
+    RMPCD_CHECK(cfg.mode == RMPCD_SIM_REFERENCE || cfg.mode == RMPCD_SIM_FUSED, "dimer: unknown mode");
+    rep.resize(R);
+    ntot = nctot = luttot = nmax_rep = ncmax_rep = 0;
+    for (int r = 0; r < R; ++r) {
+        const int32_t *L = replica_L ? replica_L + 3 * r : cfg.L;
+        int rc = rmpcd_box_init(&rep[r].box, L, 1.0, 0);
+        if (rc) return rc;
+        int64_t n = (int64_t)cfg.rho * L[0] * L[1] * L[2];
+        RMPCD_CHECK(n > 0 && n < ((int64_t)1 << 31), "dimer: solvent count out of range");
+        rep[r].n = n;
+        rep[r].p_off = ntot;
+        rep[r].c_off = nctot;
+        rep[r].lut_off = luttot;
+        rep[r].seed = cfg.seed + (uint64_t)r;
+        ntot += n;
+        nctot += rep[r].box.ncells;
+        luttot += (int64_t)L[0] * L[1] * L[2];
+        nmax_rep = n > nmax_rep ? n : nmax_rep;
+        ncmax_rep = rep[r].box.ncells > ncmax_rep ? rep[r].box.ncells : ncmax_rep;
+    }
+    RMPCD_CHECK(ntot < ((int64_t)1 << 31), "dimer: total solvent count exceeds int32 indexing");
     dt = cfg.tau / cfg.n_md;
     skin = cfg.skin > 0 ? cfg.skin : 2.0;  // single_dimer_pbc.f90:274
     // interaction parameters (single_dimer_pbc.f90:153-181)
@@ -169,14 +211,13 @@ int DimerSim::setup(const rmpcd_dimer_config_t &c)
     for (int k = 0; k < 4; ++k) cut[k] = sig[k] * f6;
     max_cut = cut[0];
     for (int k = 1; k < 4; ++k) max_cut = cut[k] > max_cut ? cut[k] : max_cut;
-    rc = rmpcd_lj_params_init(&lj_sc, 2, 2, eps, sig, cut);
+    int rc = rmpcd_lj_params_init(&lj_sc, 2, 2, eps, sig, cut);
     if (rc) return rc;
     double ceps[4] = {cfg.epsilon_CC, cfg.epsilon_NC, cfg.epsilon_NC, cfg.epsilon_NN};
     double csig[4] = {2 * cfg.sigma_C, cfg.sigma_C + cfg.sigma_N, cfg.sigma_C + cfg.sigma_N, 2 * cfg.sigma_N};
     double ccut[4];
     for (int k = 0; k < 4; ++k) ccut[k] = csig[k] * f6;
-    rc = rmpcd_lj_params_init(&lj_cc, 2, 2, ceps, csig, ccut);
-    if (rc) return rc;
+    if ((rc = rmpcd_lj_params_init(&lj_cc, 2, 2, ceps, csig, ccut))) return rc;
     // default-REAL pi literal 3.14159265 (single_dimer_pbc.f90:180-181)
     const double pi_f = (double)3.14159265f;
     mass[0] = (((cfg.rho * (cfg.sigma_C * cfg.sigma_C * cfg.sigma_C)) * 4) * pi_f) / 3;
@@ -189,118 +230,123 @@ int DimerSim::setup(const rmpcd_dimer_config_t &c)
     int nmax_est = (int)(1.3 * cfg.rho * 4.0 / 3.0 * M_PI * radius * radius * radius) + 64;
     nmax = cfg.nmax_list > 0 ? cfg.nmax_list : (nmax_ref > nmax_est ? nmax_ref : nmax_est);
 
-    size_t R = real_size(cfg.form), S = small_size(cfg.form);
-    size_t nc = (size_t)box.ncells;
+    size_t RS = real_size(cfg.form), SS = small_size(cfg.form);
     for (int b = 0; b < 2; ++b) {
-        if ((rc = alloc(&sol_pos[b], 3 * n * R))) return rc;
-        if ((rc = alloc(&sol_vel[b], 3 * n * R))) return rc;
-        if ((rc = alloc((void **)&sol_image[b], 3 * n * 4))) return rc;
-        if ((rc = alloc((void **)&sol_id[b], n * 4))) return rc;
-        if ((rc = alloc(&sol_species[b], n * S))) return rc;
-        if ((rc = alloc(&sol_flags[b], n * S))) return rc;
+        if ((rc = alloc(&sol_pos[b], 3 * ntot * RS))) return rc;
+        if ((rc = alloc(&sol_vel[b], 3 * ntot * RS))) return rc;
+        if ((rc = alloc((void **)&sol_image[b], 3 * ntot * 4))) return rc;
+        if ((rc = alloc((void **)&sol_id[b], ntot * 4))) return rc;
+        if ((rc = alloc(&sol_species[b], ntot * SS))) return rc;
+        if ((rc = alloc(&sol_flags[b], ntot * SS))) return rc;
+        if ((rc = alloc((void **)&cell_count[b], nctot * 4))) return rc;
     }
     for (int b = 0; b < 4; ++b)
-        if ((rc = alloc(&sol_force[b], 3 * n * R))) return rc;
-    if ((rc = alloc((void **)&id_to_idx, n * 4))) return rc;
-    if ((rc = alloc((void **)&claim, n * 4))) return rc;
-    if ((rc = alloc((void **)&cells.cell_count, nc * 4))) return rc;
-    if ((rc = alloc((void **)&cells.cell_start, nc * 4))) return rc;
-    if ((rc = alloc((void **)&cells.is_md, nc))) return rc;
-    if ((rc = alloc((void **)&cells.is_reac, nc))) return rc;
-    if ((rc = alloc((void **)&col_pos, 3 * 2 * 8))) return rc;
-    if ((rc = alloc((void **)&col_pos_old, 3 * 2 * 8))) return rc;
-    if ((rc = alloc((void **)&col_pos_rattle, 3 * 2 * 8))) return rc;
-    if ((rc = alloc((void **)&col_vel, 3 * 2 * 8))) return rc;
-    if ((rc = alloc((void **)&col_force[0], 3 * 2 * 8))) return rc;
-    if ((rc = alloc((void **)&col_force[1], 3 * 2 * 8))) return rc;
-    if ((rc = alloc((void **)&col_image, 3 * 2 * 4))) return rc;
-    if ((rc = alloc((void **)&col_id, 2 * 4))) return rc;
-    if ((rc = alloc((void **)&col_species, 2 * 4))) return rc;
-    if ((rc = alloc((void **)&col_flags, 2 * 4))) return rc;
-    if ((rc = alloc((void **)&col_mass, 2 * 8))) return rc;
-    if ((rc = alloc((void **)&list, (size_t)2 * nmax * 4))) return rc;
-    if ((rc = alloc((void **)&nlist, 2 * 4))) return rc;
-    if ((rc = alloc((void **)&scratch, 4096))) return rc;
+        if ((rc = alloc(&sol_force[b], 3 * ntot * RS))) return rc;
+    if ((rc = alloc((void **)&id_to_idx, ntot * 4))) return rc;
+    if ((rc = alloc((void **)&cell_start, nctot * 4))) return rc;
+    if ((rc = alloc((void **)&is_md, nctot))) return rc;
+    if ((rc = alloc((void **)&is_reac, nctot))) return rc;
+    size_t c6 = 6 * (size_t)R * sizeof(double);
+    if ((rc = alloc((void **)&col_pos, c6))) return rc;
+    if ((rc = alloc((void **)&col_pos_old, c6))) return rc;
+    if ((rc = alloc((void **)&col_pos_rattle, c6))) return rc;
+    if ((rc = alloc((void **)&col_vel, c6))) return rc;
+    if ((rc = alloc((void **)&col_force[0], c6))) return rc;
+    if ((rc = alloc((void **)&col_force[1], c6))) return rc;
+    if ((rc = alloc((void **)&col_image, 6 * (size_t)R * 4))) return rc;
+    if ((rc = alloc((void **)&col_id, 2 * (size_t)R * 4))) return rc;
+    if ((rc = alloc((void **)&col_species, 2 * (size_t)R * 4))) return rc;
+    if ((rc = alloc((void **)&col_flags, 2 * (size_t)R * 4))) return rc;
+    if ((rc = alloc((void **)&col_mass, 2 * (size_t)R * 8))) return rc;
+    if ((rc = alloc((void **)&list, (size_t)2 * R * nmax * 4))) return rc;
+    if ((rc = alloc((void **)&nlist, 2 * (size_t)R * 4))) return rc;
+    if ((rc = alloc((void **)&scratch, 1 << 16))) return rc;
     // stencil (make_stencil(solvent_cells, max_cut+skin), single_dimer_pbc.f90:277)
-    n_stencil = rmpcd_make_stencil(box.a, radius, nullptr, 0);
+    n_stencil = rmpcd_make_stencil(1.0, radius, nullptr, 0);
     std::vector<int32_t> st(3 * (size_t)n_stencil);
-    rmpcd_make_stencil(box.a, radius, st.data(), n_stencil);
+    rmpcd_make_stencil(1.0, radius, st.data(), n_stencil);
     if ((rc = alloc((void **)&stencil, st.size() * 4))) return rc;
     RMPCD_HIP(hipMemcpy(stencil, st.data(), st.size() * 4, hipMemcpyHostToDevice));
-    return initialise();
+    for (int r = 0; r < R; ++r)
+        if ((rc = initialise(r))) return rc;
+    if (cfg.mode == RMPCD_SIM_FUSED) return fused_prepare(*this);
+    return RMPCD_OK;
 }
 
-rmpcd_particles_t DimerSim::solvent_view() const
+rmpcd_cells_t DimerSim::cells_view(int r) const
 {
-    rmpcd_particles_t p;
-    memset(&p, 0, sizeof(p));
-    p.n = n;
-    p.form = cfg.form;
-    p.pos = sol_pos[cur];
-    p.vel = sol_vel[cur];
-    p.force = sol_force[2 * cur + fpar];
-    p.force_old = sol_force[2 * cur + (1 - fpar)];
-    p.image = sol_image[cur];
-    p.id = sol_id[cur];
-    p.species = sol_species[cur];
-    p.flags = sol_flags[cur];
-    return p;
+    rmpcd_cells_t c;
+    c.cell_count = cell_count[rep[r].cnt] + rep[r].c_off;
+    c.cell_start = cell_start + rep[r].c_off;
+    c.is_md = is_md + rep[r].c_off;
+    c.is_reac = is_reac + rep[r].c_off;
+    return c;
 }
 
-rmpcd_particles_t DimerSim::solvent_alt() const
+static rmpcd_particles_t solvent_buffers(const DimerSim &s, int r, int b, int f)
 {
-    rmpcd_particles_t p = solvent_view();
-    int o = 1 - cur;
-    p.pos = sol_pos[o];
-    p.vel = sol_vel[o];
-    p.force = sol_force[2 * o + fpar];
-    p.force_old = sol_force[2 * o + (1 - fpar)];
-    p.image = sol_image[o];
-    p.id = sol_id[o];
-    p.species = sol_species[o];
-    p.flags = sol_flags[o];
-    return p;
+    const Replica &p = s.rep[r];
+    size_t RS = real_size(s.cfg.form), SS = small_size(s.cfg.form);
+    rmpcd_particles_t v;
+    memset(&v, 0, sizeof(v));
+    v.n = p.n;
+    v.form = s.cfg.form;
+    v.pos = (char *)s.sol_pos[b] + 3 * p.p_off * RS;
+    v.vel = (char *)s.sol_vel[b] + 3 * p.p_off * RS;
+    v.force = (char *)s.sol_force[2 * b + f] + 3 * p.p_off * RS;
+    v.force_old = (char *)s.sol_force[2 * b + (1 - f)] + 3 * p.p_off * RS;
+    v.image = s.sol_image[b] + 3 * p.p_off;
+    v.id = s.sol_id[b] + p.p_off;
+    v.species = (char *)s.sol_species[b] + p.p_off * SS;
+    v.flags = (char *)s.sol_flags[b] + p.p_off * SS;
+    return v;
 }
 
-rmpcd_particles_t DimerSim::colloid_view() const
+rmpcd_particles_t DimerSim::solvent_view(int r) const { return solvent_buffers(*this, r, rep[r].cur, rep[r].fpar); }
+rmpcd_particles_t DimerSim::solvent_alt(int r) const { return solvent_buffers(*this, r, 1 - rep[r].cur, rep[r].fpar); }
+
+rmpcd_particles_t DimerSim::colloid_view(int r) const
 {
     rmpcd_particles_t p;
     memset(&p, 0, sizeof(p));
     p.n = 2;
     p.form = RMPCD_FP64;
-    p.pos = col_pos;
-    p.vel = col_vel;
-    p.force = col_force[cfpar];
-    p.force_old = col_force[1 - cfpar];
-    p.image = col_image;
-    p.id = col_id;
-    p.species = col_species;
-    p.flags = col_flags;
+    p.pos = col_pos + 6 * r;
+    p.vel = col_vel + 6 * r;
+    p.force = col_force[rep[r].cfpar] + 6 * r;
+    p.force_old = col_force[1 - rep[r].cfpar] + 6 * r;
+    p.image = col_image + 6 * r;
+    p.id = col_id + 2 * r;
+    p.species = col_species + 2 * r;
+    p.flags = col_flags + 2 * r;
     return p;
 }
 
 template <class F>
-static int init_solvent(DimerSim &s)
+static int init_solvent(DimerSim &s, int r)
 {
     hipStream_t st = stream();
-    rmpcd_particles_t sv = s.solvent_view();
+    rmpcd_particles_t sv = s.solvent_view(r);
     PView<F> p = view<F>(&sv);
-    k_fill_state<F><<<blocks_for(s.n, 256), 256, 0, st>>>(p);
+    Replica &rp = s.rep[r];
+    k_fill_state<F><<<blocks_for(rp.n, 256), 256, 0, st>>>(p);
     RMPCD_LAUNCHED();
+    unsigned nb = blocks_for(rp.n, 256);
+    double *part = (double *)workspace(WS_PART, sizeof(double) * 3 * (nb + 1));
+    if (!part) return set_error(RMPCD_ERR_HIP, "dimer: workspace allocation failed");
     double *sum = (double *)s.scratch;
-    RMPCD_HIP(hipMemsetAsync(sum, 0, 4 * sizeof(double), st));
-    k_init_velocities<F><<<blocks_for(s.n, 256), 256, 0, st>>>(p, s.cfg.seed, sqrt(s.cfg.T), sum);
+    k_init_velocities<F><<<nb, 256, 0, st>>>(p, rp.seed, sqrt(s.cfg.T), part);
     RMPCD_LAUNCHED();
-    k_remove_mean<F><<<blocks_for(s.n, 256), 256, 0, st>>>(p, sum);
+    k_sum_blocks3<<<1, 256, 0, st>>>(part, (int)nb, sum);
     RMPCD_LAUNCHED();
-    rmpcd_particles_t cv = s.colloid_view();
-    LJCut cut;
-    cut.n1 = s.lj_sc.n1;
-    for (int k = 0; k < 16; ++k) cut.cut_sq[k] = s.lj_sc.cut_sq[k];
+    k_remove_mean<F><<<nb, 256, 0, st>>>(p, sum);
+    RMPCD_LAUNCHED();
+    rmpcd_particles_t cv = s.colloid_view(r);
     int32_t *fail = (int32_t *)(sum + 4);
     RMPCD_HIP(hipMemsetAsync(fail, 0, 4, st));
-    k_place<F><<<blocks_for(s.n, 256), 256, 0, st>>>(p, view<F64>(&cv), 2, s.box.edges[0], s.box.edges[1],
-                                                    s.box.edges[2], cut, s.cfg.seed, fail);
+    k_place_random<F><<<nb, 256, 0, st>>>(p, view<F64>(&cv), 2, rp.box.edges[0], rp.box.edges[1],
+                                          rp.box.edges[2], s.lj_sc.n1, s.lj_sc.cut_sq[0], s.lj_sc.cut_sq[1],
+                                          s.lj_sc.cut_sq[2], s.lj_sc.cut_sq[3], rp.seed, fail);
     RMPCD_LAUNCHED();
     int32_t hf = 0;
     RMPCD_HIP(hipMemcpyAsync(&hf, fail, 4, hipMemcpyDeviceToHost, st));
@@ -309,137 +355,148 @@ static int init_solvent(DimerSim &s)
     return RMPCD_OK;
 }
 
-int DimerSim::initialise()
+int DimerSim::initialise(int r)
 {
     hipStream_t st = stream();
-    // colloids (single_dimer_pbc.f90:197-200, 250-252)
-    double cp[6], zero6[6] = {0, 0, 0, 0, 0, 0};
+    Replica &rp = rep[r];
+    // colloids (single_dimer_pbc.f90:197-200, 250-252); pos_old starts at zero
+    double cp[6];
     for (int k = 0; k < 3; ++k) {
-        cp[2 * k] = box.edges[k] / 2.0;
-        cp[2 * k + 1] = box.edges[k] / 2.0;
+        cp[2 * k] = rp.box.edges[k] / 2.0;
+        cp[2 * k + 1] = rp.box.edges[k] / 2.0;
     }
     cp[1] = cp[1] + cfg.d;  // pos(1,2) += d
     int32_t sp[2] = {1, 2}, ids[2] = {0, 1};
-    RMPCD_HIP(hipMemcpy(col_pos, cp, sizeof(cp), hipMemcpyHostToDevice));
-    RMPCD_HIP(hipMemcpy(col_pos_old, zero6, sizeof(zero6), hipMemcpyHostToDevice));
-    RMPCD_HIP(hipMemcpy(col_species, sp, sizeof(sp), hipMemcpyHostToDevice));
-    RMPCD_HIP(hipMemcpy(col_id, ids, sizeof(ids), hipMemcpyHostToDevice));
+    RMPCD_HIP(hipMemcpy(col_pos + 6 * r, cp, sizeof(cp), hipMemcpyHostToDevice));
+    RMPCD_HIP(hipMemcpy(col_species + 2 * r, sp, sizeof(sp), hipMemcpyHostToDevice));
+    RMPCD_HIP(hipMemcpy(col_id + 2 * r, ids, sizeof(ids), hipMemcpyHostToDevice));
     // the drivers index colloids%mass by the particle (single_dimer_pbc.f90:314)
-    RMPCD_HIP(hipMemcpy(col_mass, mass, sizeof(mass), hipMemcpyHostToDevice));
-    int rc = cfg.form == RMPCD_FP64 ? init_solvent<F64>(*this) : init_solvent<F32>(*this);
+    RMPCD_HIP(hipMemcpy(col_mass + 2 * r, mass, sizeof(mass), hipMemcpyHostToDevice));
+    int rc = cfg.form == RMPCD_FP64 ? init_solvent<F64>(*this, r) : init_solvent<F32>(*this, r);
     if (rc) return rc;
     // sort, neighbour list, first forces (single_dimer_pbc.f90:270-296)
-    if ((rc = sort_solvent(false))) return rc;
-    if ((rc = build_list())) return rc;
-    rmpcd_particles_t sv = solvent_view(), cv = colloid_view();
-    if ((rc = rmpcd_compute_force(&cv, &sv, list, nlist, nmax, box.edges, &lj_sc, &e1))) return rc;
-    if ((rc = rmpcd_compute_force_n2(&cv, box.edges, &lj_cc, &e2))) return rc;
-    // force_old = force
-    RMPCD_HIP(hipMemcpyAsync(sv.force_old, sv.force, 3 * n * real_size(cfg.form), hipMemcpyDeviceToDevice, st));
+    if ((rc = sort_solvent(r, false))) return rc;
+    if ((rc = build_list(r))) return rc;
+    rmpcd_particles_t sv = solvent_view(r), cv = colloid_view(r);
+    if ((rc = rmpcd_compute_force(&cv, &sv, list + 2 * (size_t)r * nmax, nlist + 2 * r, nmax, rp.box.edges,
+                                  &lj_sc, &rp.e1)))
+        return rc;
+    if ((rc = rmpcd_compute_force_n2(&cv, rp.box.edges, &lj_cc, &rp.e2))) return rc;
+    // force_old = force (single_dimer_pbc.f90:295-296)
+    RMPCD_HIP(hipMemcpyAsync(sv.force_old, sv.force, 3 * rp.n * real_size(cfg.form), hipMemcpyDeviceToDevice, st));
     RMPCD_HIP(hipMemcpyAsync(cv.force_old, cv.force, 6 * sizeof(double), hipMemcpyDeviceToDevice, st));
     RMPCD_HIP(hipStreamSynchronize(st));
     return RMPCD_OK;
 }
 
-int DimerSim::sort_solvent(bool apply_pbc)
+int DimerSim::sort_solvent(int r, bool apply_pbc)
 {
-    rmpcd_particles_t src = solvent_view(), dst = solvent_alt();
-    int rc = rmpcd_sort(&box, &src, &dst, id_to_idx, &cells, apply_pbc ? 1 : 0);
+    rmpcd_particles_t src = solvent_view(r), dst = solvent_alt(r);
+    rmpcd_cells_t c = cells_view(r);
+    int rc = rmpcd_sort(&rep[r].box, &src, &dst, id_to_idx + rep[r].p_off, &c, apply_pbc ? 1 : 0);
     if (rc) return rc;
-    cur = 1 - cur;  // the switch() calls of particle_system.f90:346-353
+    rep[r].cur = 1 - rep[r].cur;  // the switch() calls of particle_system.f90:346-353
     return RMPCD_OK;
 }
 
-int DimerSim::build_list()
+int DimerSim::build_list(int r)
 {
-    rmpcd_particles_t sv = solvent_view(), cv = colloid_view();
-    return rmpcd_update_list(&box, &cells, &cv, &sv, radius, stencil, n_stencil, &lj_sc, nmax, list, nlist);
+    rmpcd_particles_t sv = solvent_view(r), cv = colloid_view(r);
+    rmpcd_cells_t c = cells_view(r);
+    return rmpcd_update_list(&rep[r].box, &c, &cv, &sv, radius, stencil, n_stencil, &lj_sc, nmax,
+                             list + 2 * (size_t)r * nmax, nlist + 2 * r);
 }
 
 // one MD step in the reference's call sequence (single_dimer_pbc.f90:306-383)
-int DimerSim::md_step_reference()
+int DimerSim::md_step_reference(int r)
 {
     hipStream_t st = stream();
-    size_t R = real_size(cfg.form);
+    Replica &rp = rep[r];
+    size_t RS = real_size(cfg.form);
     int rc;
-    rmpcd_particles_t sv = solvent_view(), cv = colloid_view();
+    int32_t *lst = list + 2 * (size_t)r * nmax, *nl = nlist + 2 * r;
+    double *cpos_old = col_pos_old + 6 * r, *cpos_rattle = col_pos_rattle + 6 * r;
+    rmpcd_particles_t sv = solvent_view(r), cv = colloid_view(r);
     if ((rc = rmpcd_md_pos(&sv, dt))) return rc;
-    RMPCD_HIP(hipMemcpyAsync(col_pos_rattle, col_pos, 6 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    if ((rc = rmpcd_colloid_md_pos(&cv, col_mass, dt))) return rc;
-    if ((rc = rmpcd_rattle_dimer_pos(&cv, col_pos_rattle, mass, 2, cfg.d, dt, box.edges))) return rc;
+    RMPCD_HIP(hipMemcpyAsync(cpos_rattle, cv.pos, 6 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if ((rc = rmpcd_colloid_md_pos(&cv, col_mass + 2 * r, dt))) return rc;
+    if ((rc = rmpcd_rattle_dimer_pos(&cv, cpos_rattle, mass, 2, cfg.d, dt, rp.box.edges))) return rc;
     double so_max, co_max;
-    if ((rc = rmpcd_maximum_displacement(&sv, sol_pos[1 - cur], &so_max))) return rc;
-    if ((rc = rmpcd_maximum_displacement(&cv, col_pos_old, &co_max))) return rc;
+    if ((rc = rmpcd_maximum_displacement(&sv, solvent_alt(r).pos, &so_max))) return rc;
+    if ((rc = rmpcd_maximum_displacement(&cv, cpos_old, &co_max))) return rc;
     if (co_max >= skin * 0.1 || so_max >= skin * 0.89) {
-        if ((rc = rmpcd_apply_pbc(&cv, box.edges))) return rc;
-        if ((rc = sort_solvent(true))) return rc;
-        if ((rc = build_list())) return rc;
-        sv = solvent_view();
-        RMPCD_HIP(hipMemcpyAsync(sol_pos[1 - cur], sv.pos, 3 * n * R, hipMemcpyDeviceToDevice, st));
-        RMPCD_HIP(hipMemcpyAsync(col_pos_old, col_pos, 6 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        ++n_extra_sorting;
+        if ((rc = rmpcd_apply_pbc(&cv, rp.box.edges))) return rc;
+        if ((rc = sort_solvent(r, true))) return rc;
+        if ((rc = build_list(r))) return rc;
+        sv = solvent_view(r);
+        RMPCD_HIP(hipMemcpyAsync(solvent_alt(r).pos, sv.pos, 3 * rp.n * RS, hipMemcpyDeviceToDevice, st));
+        RMPCD_HIP(hipMemcpyAsync(cpos_old, cv.pos, 6 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        ++rp.n_extra;
     }
     // switch(force, force_old) for both systems, then zero the new forces
-    fpar = 1 - fpar;
-    cfpar = 1 - cfpar;
-    sv = solvent_view();
-    cv = colloid_view();
-    RMPCD_HIP(hipMemsetAsync(sv.force, 0, 3 * n * R, st));
+    rp.fpar = 1 - rp.fpar;
+    rp.cfpar = 1 - rp.cfpar;
+    sv = solvent_view(r);
+    cv = colloid_view(r);
+    RMPCD_HIP(hipMemsetAsync(sv.force, 0, 3 * rp.n * RS, st));
     RMPCD_HIP(hipMemsetAsync(cv.force, 0, 6 * sizeof(double), st));
-    if ((rc = rmpcd_compute_force(&cv, &sv, list, nlist, nmax, box.edges, &lj_sc, &e1))) return rc;
-    if ((rc = rmpcd_compute_force_n2(&cv, box.edges, &lj_cc, &e2))) return rc;
+    if ((rc = rmpcd_compute_force(&cv, &sv, lst, nl, nmax, rp.box.edges, &lj_sc, &rp.e1))) return rc;
+    if ((rc = rmpcd_compute_force_n2(&cv, rp.box.edges, &lj_cc, &rp.e2))) return rc;
     if ((rc = rmpcd_md_vel(&sv, dt))) return rc;
-    if ((rc = rmpcd_colloid_md_vel(&cv, col_mass, dt))) return rc;
-    if ((rc = rmpcd_rattle_dimer_vel(&cv, mass, 2, cfg.d, dt, box.edges))) return rc;
-    if ((rc = rmpcd_flag_particles_nl(&sv, &cv, list, nlist, nmax, box.edges, lj_sc.cut_sq[0]))) return rc;
-    if ((rc = rmpcd_change_species(&sv, &cv, box.edges, lj_sc.cut_sq[0], lj_sc.cut_sq[2], cfg.prob, nullptr,
-                                   cfg.seed, (uint64_t)md_steps)))
+    if ((rc = rmpcd_colloid_md_vel(&cv, col_mass + 2 * r, dt))) return rc;
+    if ((rc = rmpcd_rattle_dimer_vel(&cv, mass, 2, cfg.d, dt, rp.box.edges))) return rc;
+    if ((rc = rmpcd_flag_particles_nl(&sv, &cv, lst, nl, nmax, rp.box.edges, lj_sc.cut_sq[0]))) return rc;
+    // change_species draws threefry((md_step, id), (seed, CHANGE))
+    if ((rc = rmpcd_change_species(&sv, &cv, rp.box.edges, lj_sc.cut_sq[0], lj_sc.cut_sq[2], cfg.prob, nullptr,
+                                   rp.seed, (uint64_t)rp.md_steps)))
         return rc;
-    ++md_steps;
+    ++rp.md_steps;
     return RMPCD_OK;
 }
 
 // the collision part of a cycle (single_dimer_pbc.f90:405-428)
-int DimerSim::collide_reference()
+int DimerSim::collide_reference(int r)
 {
     hipStream_t st = stream();
-    size_t R = real_size(cfg.form);
+    Replica &rp = rep[r];
+    size_t RS = real_size(cfg.form);
     int rc;
     double u[3];
-    shift_draws(cfg.seed, (uint64_t)cycles, u);
-    if ((rc = rmpcd_random_shift(&box, u))) return rc;
-    rmpcd_particles_t cv = colloid_view();
-    if ((rc = rmpcd_apply_pbc(&cv, box.edges))) return rc;
-    if ((rc = sort_solvent(true))) return rc;
-    if ((rc = build_list())) return rc;
-    rmpcd_particles_t sv = solvent_view();
-    RMPCD_HIP(hipMemcpyAsync(sol_pos[1 - cur], sv.pos, 3 * n * R, hipMemcpyDeviceToDevice, st));
-    RMPCD_HIP(hipMemcpyAsync(col_pos_old, col_pos, 6 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    if ((rc = rmpcd_simple_mpcd_step(&box, &cells, &sv, nullptr, cfg.seed, (uint64_t)cycles, 0, 0.0, 0, cfg.T, 1)))
+    shift_draws(rp.seed, (uint64_t)rp.cycles, u);
+    if ((rc = rmpcd_random_shift(&rp.box, u))) return rc;
+    rmpcd_particles_t cv = colloid_view(r);
+    if ((rc = rmpcd_apply_pbc(&cv, rp.box.edges))) return rc;
+    if ((rc = sort_solvent(r, true))) return rc;
+    if ((rc = build_list(r))) return rc;
+    rmpcd_particles_t sv = solvent_view(r);
+    rmpcd_cells_t c = cells_view(r);
+    RMPCD_HIP(hipMemcpyAsync(solvent_alt(r).pos, sv.pos, 3 * rp.n * RS, hipMemcpyDeviceToDevice, st));
+    RMPCD_HIP(hipMemcpyAsync(col_pos_old + 6 * r, cv.pos, 6 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // SRD axes: Marsaglia on threefry((cycle, cell + 2^40 k), (seed, SRD)) pairs
+    if ((rc = rmpcd_simple_mpcd_step(&rp.box, &c, &sv, nullptr, rp.seed, (uint64_t)rp.cycles, 0, 0.0, 0, cfg.T, 1)))
         return rc;
     if (cfg.bulk_rmpcd) {
-        if ((rc = rmpcd_bulk_reaction(&box, &cells, &sv, 2, 1, cfg.bulk_rate, cfg.tau, nullptr, cfg.seed,
-                                      (uint64_t)cycles)))
+        // one draw per eligible cell: threefry((cycle, cell), (seed, REACTION))
+        if ((rc = rmpcd_bulk_reaction(&rp.box, &c, &sv, 2, 1, cfg.bulk_rate, cfg.tau, nullptr, rp.seed,
+                                      (uint64_t)rp.cycles)))
             return rc;
     } else {
-        if ((rc = rmpcd_refuel(&sv, &cv, box.edges, cfg.L[0]))) return rc;
+        if ((rc = rmpcd_refuel(&sv, &cv, rp.box.edges, rp.box.L[0]))) return rc;
     }
-    ++cycles;
+    ++rp.cycles;
     return RMPCD_OK;
 }
 
 int DimerSim::run(int n_cycles)
 {
+    if (cfg.mode == RMPCD_SIM_FUSED) return run_fused_cycles(*this, n_cycles);
     for (int c = 0; c < n_cycles; ++c) {
-        int rc;
-        if (cfg.mode == RMPCD_SIM_FUSED) {
-            rc = run_fused_cycle(*this);
-        } else {
+        for (int r = 0; r < R; ++r) {
+            int rc;
             for (int j = 0; j < cfg.n_md; ++j)
-                if ((rc = md_step_reference())) return rc;
-            rc = collide_reference();
+                if ((rc = md_step_reference(r))) return rc;
+            if ((rc = collide_reference(r))) return rc;
         }
-        if (rc) return rc;
     }
     RMPCD_HIP(hipStreamSynchronize(stream()));
     return RMPCD_OK;
@@ -457,7 +514,7 @@ int rmpcd_dimer_create(const rmpcd_dimer_config_t *cfg, rmpcd_dimer_t **out)
     *out = nullptr;
     DimerSim *s = new (std::nothrow) DimerSim();
     if (!s) return set_error(RMPCD_ERR_ARG, "rmpcd_dimer_create: out of host memory");
-    int rc = s->setup(*cfg);
+    int rc = s->setup(*cfg, cfg->replica_L);
     if (rc) {
         delete s;
         return rc;
@@ -472,33 +529,41 @@ int rmpcd_dimer_run(rmpcd_dimer_t *sim, int32_t n_cycles)
     return ((DimerSim *)sim)->run(n_cycles);
 }
 
-int rmpcd_dimer_state(rmpcd_dimer_t *sim, rmpcd_particles_t *solvent, rmpcd_particles_t *colloids,
-                      rmpcd_box_t *box, rmpcd_cells_t *cells, rmpcd_dimer_stats_t *stats)
+int rmpcd_dimer_state(rmpcd_dimer_t *sim, int32_t replica, rmpcd_particles_t *solvent,
+                      rmpcd_particles_t *colloids, rmpcd_box_t *box, rmpcd_cells_t *cells,
+                      rmpcd_dimer_stats_t *stats)
 {
     RMPCD_CHECK(sim, "rmpcd_dimer_state: null simulation");
     DimerSim *s = (DimerSim *)sim;
+    RMPCD_CHECK(replica >= 0 && replica < s->R, "rmpcd_dimer_state: replica out of range");
     int rc = s->sync_state();
     if (rc) return rc;
-    if (solvent) *solvent = s->solvent_view();
-    if (colloids) *colloids = s->colloid_view();
-    if (box) *box = s->box;
-    if (cells) *cells = s->cells;
+    int r = replica;
+    const Replica &rp = s->rep[r];
+    if (solvent) *solvent = s->solvent_view(r);
+    if (colloids) *colloids = s->colloid_view(r);
+    if (box) *box = rp.box;
+    if (cells) *cells = s->cells_view(r);
     if (stats) {
         memset(stats, 0, sizeof(*stats));
-        stats->n_solvent = s->n;
-        stats->md_steps = s->md_steps;
-        stats->cycles = s->cycles;
-        stats->n_extra_sorting = s->n_extra_sorting;
-        stats->e1 = s->e1;
-        stats->e2 = s->e2;
+        stats->n_solvent = rp.n;
+        stats->md_steps = rp.md_steps;
+        stats->cycles = rp.cycles;
+        stats->n_extra_sorting = rp.n_extra;
+        stats->e1 = rp.e1;
+        stats->e2 = rp.e2;
         stats->nmax = s->nmax;
-        stats->list = s->list;
-        stats->nlist = s->nlist;
-        stats->pos_old = s->sol_pos[1 - s->cur];
-        stats->colloid_pos_old = s->col_pos_old;
+        stats->list = s->list + 2 * (size_t)r * s->nmax;
+        stats->nlist = s->nlist + 2 * r;
+        stats->pos_old = s->cfg.mode == RMPCD_SIM_FUSED
+                             ? (void *)((char *)s->sol_pos_old + 3 * rp.p_off * real_size(s->cfg.form))
+                             : s->solvent_alt(r).pos;
+        stats->colloid_pos_old = s->col_pos_old + 6 * r;
         stats->mass[0] = s->mass[0];
         stats->mass[1] = s->mass[1];
         stats->dt = s->dt;
+        stats->n_replicas = s->R;
+        stats->n_solvent_total = s->ntot;
     }
     return RMPCD_OK;
 }

@@ -1,0 +1,81 @@
+"""The single-dimer program on the GPU (programs/single_dimer_pbc.f90): the fused,
+graph-captured step must reproduce the reference call sequence bit for bit (same draws,
+same sort decisions), for one and for several replicas of uneven size; plus the
+invariants of dimer_test.py (momentum conservation) and RATTLE."""
+import numpy as np
+import pytest
+
+from rmpcdmd_amd import capi
+from rmpcdmd_amd.sim import SingleDimer
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("pos", "vel", "force", "force_old", "image", "id", "species", "flags", "colloid_pos",
+        "colloid_vel", "colloid_force", "colloid_force_old", "colloid_image", "cell_count", "cell_start",
+        "nlist", "origin")
+
+
+def compare(a, b, replica_msg=""):
+    for k in KEYS:
+        assert a[k].shape == b[k].shape, k
+        assert a[k].tobytes() == b[k].tobytes(), "%s differs %s" % (k, replica_msg)
+    for i in range(2):
+        n = a["nlist"][i]
+        assert (a["list"][i, :n] == b["list"][i, :n]).all()
+    for k in ("md_steps", "cycles", "n_extra_sorting", "e1", "e2"):
+        assert a[k] == b[k], k
+
+
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+def test_fused_equals_reference_sequence(gpu, form):
+    kw = dict(L=(16, 16, 16), n_md=10, tau=0.1, form=form, seed=77)
+    ref = SingleDimer(mode=capi.SIM_REFERENCE, **kw)
+    fus = SingleDimer(mode=capi.SIM_FUSED, **kw)
+    for cycle in range(3):
+        ref.run(1)
+        fus.run(1)
+        a, b = ref.download(), fus.download()
+        compare(a, b, "after cycle %d" % cycle)
+    assert a["n_extra_sorting"] >= 1  # the first step re-sorts (pos_old is the set-up order)
+    ref.close()
+    fus.close()
+
+
+def test_fused_replicas_of_uneven_size(gpu):
+    Ls = [(12, 12, 12), (16, 12, 14), (12, 16, 12)]
+    kw = dict(n_md=10, tau=0.1, form=capi.FP64, seed=5, replica_L=Ls)
+    ref = SingleDimer(mode=capi.SIM_REFERENCE, **kw)
+    fus = SingleDimer(mode=capi.SIM_FUSED, **kw)
+    ref.run(2)
+    fus.run(2)
+    for r in range(len(Ls)):
+        compare(ref.download(r), fus.download(r), "replica %d" % r)
+    # replicas are independent systems with their own seeds
+    a0, a2 = fus.download(0), fus.download(2)
+    assert not np.array_equal(a0["vel"][:100], a2["vel"][:100])
+    ref.close()
+    fus.close()
+
+
+def test_dimer_invariants(gpu):
+    sim = SingleDimer(L=(20, 20, 20), form=capi.FP64, mode=capi.SIM_FUSED, n_md=25, tau=0.25, seed=3)
+    st = sim.state()[4]
+    m = np.array(st.mass[:])
+
+    def momentum(s):
+        return s["vel"].sum(axis=0) + m[0] * s["colloid_vel"][0] + m[1] * s["colloid_vel"][1]
+
+    s0 = sim.download()
+    p0 = momentum(s0)
+    sim.run(4)
+    s1 = sim.download()
+    p1 = momentum(s1)
+    # dimer_test.py: centre-of-mass velocity drift below 1e-15 per particle (FP64)
+    assert np.abs(p1 - p0).max() / (s1["n_solvent"] + m.sum()) < 1e-15
+    r = s1["colloid_pos"][0] - s1["colloid_pos"][1]
+    r -= np.round(r / 20.0) * 20.0
+    assert abs(np.sqrt(r @ r) - 4.5) < 1e-9
+    assert s1["cycles"] == 4 and s1["md_steps"] == 100
+    # the catalytic sphere converted some A into B
+    assert (s1["species"] == 2).sum() > 0
+    sim.close()
