@@ -4,10 +4,15 @@ one MI355X.
 A step is one collision cycle: N_MD velocity-Verlet steps of the solvent and the dimer
 (forces, RATTLE, catalytic conversion, skin-triggered re-sorts) followed by the grid
 shift, the cell sort, the neighbour-list rebuild and the SRD collision.  The metric
-counts solvent particle-steps: N_solvent x N_MD per cycle.
+counts solvent particle-steps: N_solvent x N_MD per cycle, summed over replicas.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fp32|fp64]
-                    [--mode fused|reference] [--L 64]
+                    [--mode fused|reference] [--L 64] [--replicas R] [--no-cpu]
+
+The JSON line carries the roofline of the dominant kernel (the streaming pass over all
+solvent, 9 words per particle: read position and velocity, write position), timed with
+HIP event nodes inside the captured graph during the timed cycles, and the CPU baseline:
+the oracle (FP64 C with OpenMP) running one cycle of the same workload on this host.
 """
 from __future__ import annotations
 
@@ -15,6 +20,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -28,18 +34,19 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4, help="collision cycles timed")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed collision cycles")
+    ap.add_argument("--steps", type=int, default=6, help="collision cycles timed")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed collision cycles")
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--mode", choices=["fused", "reference"], default="fused")
     ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--replicas", type=int, default=1, help="independent L^3 systems in one launch")
     ap.add_argument("--n-md", type=int, default=50)
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     return ap.parse_args()
 
 
-def run_gpu(form, mode, L, n_md, steps, warmup):
+def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile):
     import torch
 
     from rmpcdmd_amd import capi
@@ -47,8 +54,11 @@ def run_gpu(form, mode, L, n_md, steps, warmup):
 
     f = capi.FP32 if form == "fp32" else capi.FP64
     m = capi.SIM_FUSED if mode == "fused" else capi.SIM_REFERENCE
-    sim = SingleDimer(L=(L, L, L), n_md=n_md, form=f, mode=m)
-    n = sim.n_solvent
+    sim = SingleDimer(L=(L, L, L), n_md=n_md, form=f, mode=m, n_replicas=replicas)
+    st0 = sim.state()[4]
+    n_total = int(st0.n_solvent_total)
+    if profile and mode == "fused":
+        capi.call("rmpcd_dimer_profile", sim._h, 1)
     if warmup:
         sim.run(warmup)
     torch.cuda.synchronize()
@@ -56,12 +66,55 @@ def run_gpu(form, mode, L, n_md, steps, warmup):
     sim.run(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    st = sim.state()[4]
     sec = t1 - t0
-    out = dict(n=n, seconds=sec, ps_per_s=n * n_md * steps / sec, ms_per_cycle=1e3 * sec / steps,
-               extra_sorts=int(st.n_extra_sorting), cycles=int(st.cycles))
+    out = dict(n=n_total, seconds=sec, ps_per_s=n_total * n_md * steps / sec, ms_per_cycle=1e3 * sec / steps)
+    extra = 0
+    for r in range(replicas):
+        extra += int(sim.state(r)[4].n_extra_sorting)
+    out["extra_sorts"] = extra
+    out["cycles"] = warmup + steps
+    if profile and mode == "fused":
+        ms, k = C.c_double(), C.c_int32()
+        capi.call("rmpcd_dimer_stream_time", sim._h, C.byref(ms), C.byref(k))
+        out["stream_ms"] = ms.value
+        out["stream_samples"] = k.value
     sim.close()
     return out
+
+
+def cpu_baseline(L, n_md):
+    """the oracle on this host's cores: one collision cycle of the same workload"""
+    from oracle import oracle as O
+    from oracle.dimer import NumpyDraws, synthetic_dimer
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    O.lib().orc_set_threads(cores)
+    drv = synthetic_dimer((L, L, L), seed=1, tau=0.5, n_md=n_md)
+    draws = NumpyDraws(2)
+    t0 = time.perf_counter()
+    drv.cycle(draws)
+    sec = time.perf_counter() - t0
+    O.lib().orc_set_threads(1)
+    return {"value": drv.s.n * n_md / sec, "unit": "solvent particle-steps/s", "cores": cores, "kind": "port",
+            "sample": "one collision cycle (%d MD steps + grid shift, sort, list rebuild, SRD) of the L=%d dimer "
+                      "workload, %d solvent, on the FP64 C/OpenMP oracle in %.1f s" % (n_md, L, drv.s.n, sec)}
+
+
+def measured_traffic(form, L, replicas):
+    """HBM bytes per launch of the streaming kernel, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), when one matches"""
+    path = os.path.join(REPO, "profiles", "stream_kernel_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    key = "%s L=%d replicas=%d" % (form, L, replicas)
+    for line in open(path):
+        m = re.match(r"\s*traffic_bytes_per_launch\[(.*)\]\s*=\s*([0-9.eE+]+)", line)
+        if m and m.group(1) == key:
+            return float(m.group(2))
+    return None
 
 
 def main():
@@ -77,10 +130,19 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.zeros(1, device="cuda")
     capi.call("rmpcd_init", 0)
-    main_run = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup)
+    main_run = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, a.replicas, profile=True)
     compare = None
     if not a.no_compare and a.form == "fp32":
-        compare = run_gpu("fp64", a.mode, a.L, a.n_md, a.steps, a.warmup)
+        compare = run_gpu("fp64", a.mode, a.L, a.n_md, a.steps, a.warmup, a.replicas, profile=False)
+    w = 4 if a.form == "fp32" else 8
+    roofline = None
+    if "stream_ms" in main_run and main_run["stream_ms"] > 0:
+        alg = 9 * w * main_run["n"]
+        achieved = alg / (main_run["stream_ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.form, a.L, a.replicas),
+                    "kernel": "k_fz_stream", "algorithmic_bytes_per_launch": alg,
+                    "avg_launch_ms": main_run["stream_ms"], "launches_timed": main_run["stream_samples"]}
     line = {
         "metric": METRIC,
         "value": main_run["ps_per_s"],
@@ -96,19 +158,27 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "single_dimer_pbc: dimer nanomotor (sigma_C=sigma_N=2, d=4.5) in a periodic "
-                        "L=%d^3 box, rho=10, tau=0.5, N_MD=%d, T=1; a step is one collision cycle"
-                        % (a.L, a.n_md),
+                        "L=%d^3 box, rho=10, tau=0.5, N_MD=%d, T=1, x%d replica(s); a step is one collision "
+                        "cycle" % (a.L, a.n_md, a.replicas),
             "n_solvent": main_run["n"],
+            "replicas": a.replicas,
             "form": "fp32-mixed (FP32 solvent, FP64 colloids/forces/RATTLE)" if a.form == "fp32" else "fp64",
             "mode": a.mode,
             "extra_sorts": main_run["extra_sorts"],
         },
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": None,
     }
     if compare:
         line["fp64_form"] = {"value": compare["ps_per_s"], "ms_per_step": compare["ms_per_cycle"],
                              "mixed_over_fp64": main_run["ps_per_s"] / compare["ps_per_s"]}
+    # the whole step's algorithmic bytes per particle-step: 9w per MD step plus the sorts
+    n_sorts = (main_run["extra_sorts"] / max(1, main_run["cycles"]) + 1) / a.n_md
+    S = 38 if a.form == "fp32" else 68
+    line["step_bytes_per_particle_step"] = 9 * w + n_sorts * (2 * S + 16 / 10)
+    line["step_effective_GBs"] = line["step_bytes_per_particle_step"] * main_run["ps_per_s"] / 1e9
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(a.L, a.n_md)
     print(json.dumps(line))
 
 

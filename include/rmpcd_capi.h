@@ -282,6 +282,11 @@ int rmpcd_dimer_state(rmpcd_dimer_t *sim, int32_t replica, rmpcd_particles_t *so
                       rmpcd_particles_t *colloids, rmpcd_box_t *box, rmpcd_cells_t *cells,
                       rmpcd_dimer_stats_t *stats);
 int rmpcd_dimer_destroy(rmpcd_dimer_t *sim);
+/* fused mode: record events around the streaming kernel of every stride-th MD step of
+ * the captured cycle (0 switches it off); rmpcd_dimer_stream_time returns the mean
+ * duration of those launches in the last cycle run */
+int rmpcd_dimer_profile(rmpcd_dimer_t *sim, int32_t stride);
+int rmpcd_dimer_stream_time(rmpcd_dimer_t *sim, double *avg_ms, int32_t *samples);
 
 /* ---- counter-based generator used when draws are not injected ------------------------ */
 /* uniform doubles in [0,1): out[i] = U(seed, stream, step, i), device output */

@@ -122,7 +122,7 @@ def cartesian_indices(c, x):
     return tuple(out)
 
 
-def count_particles(c, pos, nthreads=1):
+def count_particles(c, pos, nthreads=0):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     cnt = np.zeros(c.N, np.int32)
     st = np.zeros(c.N, np.int32)
@@ -155,7 +155,7 @@ class PState:
                          P(self.image), P(self.id), P(self.species), P(self.flags))
 
 
-def sort(c, ps, nthreads=1):
+def sort(c, ps, nthreads=0):
     """particle_system%sort: returns (sorted PState, id_to_idx, cell_count, cell_start);
     the unsorted input plays the reference's pos_old after the pointer switch."""
     out = PState(ps.n)

@@ -18,6 +18,17 @@
 #include <omp.h>
 #endif
 
+/* thread count of the parallel loops (1 = the reference's single-thread order, used for
+ * parity; more threads only for the CPU baseline, where the loops that the reference
+ * parallelises with OpenMP run in parallel too) */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_get_threads(void) { return g_threads; }
+#define ORC_PRAGMA(x) _Pragma(#x)
+#define ORC_PARALLEL_FOR ORC_PRAGMA(omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1))
+#define ORC_PARALLEL_FOR_REDUCE(...) \
+    ORC_PRAGMA(omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1) reduction(__VA_ARGS__))
+
 /* ======================= hilbert.f90 =========================================== */
 /* src/hilbert.f90:14-15 */
 #define HDIM 3
@@ -226,7 +237,7 @@ void orc_count_particles(const orc_cells_t *c, int64_t n, const double *pos,
                          int32_t *cell_count, int32_t *cell_start, int nthreads)
 {
     /* cell_system.f90:86-129: per-thread histograms, reduction, serial scan */
-    int nt = nthreads < 1 ? 1 : nthreads;
+    int nt = nthreads > 0 ? nthreads : g_threads;
     int32_t *tmp = (int32_t *)calloc((size_t)c->N * nt, sizeof(int32_t));
 #pragma omp parallel num_threads(nt)
     {
@@ -258,7 +269,7 @@ void orc_sort(const orc_cells_t *c, const orc_particles_t *src, orc_particles_t 
 {
     /* particle_system.f90:304-355 (the species==0 test at :325 is a no-op: all
      * particles are sorted) */
-    int nt = nthreads < 1 ? 1 : nthreads;
+    int nt = nthreads > 0 ? nthreads : g_threads;
     orc_count_particles(c, src->n, src->pos, cell_count, cell_start, nt);
 #pragma omp parallel for num_threads(nt) schedule(static)
     for (int64_t i = 0; i < src->n; ++i) {
@@ -285,6 +296,7 @@ double orc_maximum_displacement(int64_t n, const double *pos, const double *pos_
                                 const int32_t *species) /* particle_system.f90:357-378 */
 {
     double rmax_sq = 0;
+    ORC_PARALLEL_FOR_REDUCE(max : rmax_sq)
     for (int64_t i = 0; i < n; ++i) {
         if (species[i] >= 0) {
             double d0 = pos[3 * i] - pos_old[3 * i];
@@ -380,6 +392,7 @@ void orc_simple_mpcd_step(const orc_cells_t *c, const int32_t *cell_count,
         sin_a = 1;
         cos_a = 0;
     }
+    ORC_PARALLEL_FOR
     for (int ci = 0; ci < c->N; ++ci) {
         if (cell_count[ci] <= 1) continue;
         int start = cell_start[ci], n = cell_count[ci];
@@ -428,6 +441,7 @@ double orc_compute_temperature(const orc_cells_t *c, const int32_t *cell_count,
 {
     int count = 0;
     double te = 0;
+    ORC_PARALLEL_FOR_REDUCE(+ : count, te)
     for (int ci = 0; ci < c->N; ++ci) {
         if (cell_count[ci] <= 1) continue;
         int start = cell_start[ci], n = cell_count[ci];
@@ -462,6 +476,7 @@ void orc_mpcd_stream_periodic(const orc_cells_t *c, int64_t n, double *pos,
                               const double *vel, int32_t *image, double dt)
 {
     /* mpcd.f90:440-457 */
+    ORC_PARALLEL_FOR
     for (int64_t i = 0; i < n; ++i) {
         for (int k = 0; k < 3; ++k) {
             double x = pos[3 * i + k] + vel[3 * i + k] * dt;
@@ -478,6 +493,7 @@ void orc_bulk_reaction(const orc_cells_t *c, const int32_t *cell_count,
                        const double *draws)
 {
     /* mpcd.f90:605-643 */
+    ORC_PARALLEL_FOR
     for (int ci = 0; ci < c->N; ++ci) {
         if (cell_count[ci] <= 1 || !is_reac[ci]) continue;
         int start = cell_start[ci], n = cell_count[ci];
@@ -537,6 +553,7 @@ void orc_md_pos(int64_t n, double *pos, const double *vel, const double *force, 
 {
     /* md.f90:61-77 */
     double dt_sq = (dt * dt) / 2;
+    ORC_PARALLEL_FOR
     for (int64_t i = 0; i < 3 * n; ++i) pos[i] = (pos[i] + dt * vel[i]) + dt_sq * force[i];
 }
 
@@ -544,6 +561,7 @@ void orc_md_vel(int64_t n, double *vel, const double *force, const double *force
                 int32_t *flags, double dt)
 {
     /* md.f90:219-237 */
+    ORC_PARALLEL_FOR
     for (int64_t i = 0; i < n; ++i) {
         if (!(flags[i] & (1 << ORC_WALL_BIT))) {
             for (int k = 0; k < 3; ++k)
@@ -557,6 +575,7 @@ void orc_md_vel(int64_t n, double *vel, const double *force, const double *force
 void orc_apply_pbc(int64_t n, double *pos, int32_t *image, const double edges[3])
 {
     /* md.f90:202-217 */
+    ORC_PARALLEL_FOR
     for (int64_t i = 0; i < n; ++i) {
         for (int k = 0; k < 3; ++k) {
             int jump = (int)floor(pos[3 * i + k] / edges[k]);
@@ -840,6 +859,7 @@ double orc_compute_force(int64_t n1, const double *pos1, const int32_t *species1
 {
     /* neighbor_list.f90:159-225 */
     const int MD = 1 << ORC_MD_BIT, PAST = 1 << ORC_PAST_MD_BIT, CAT = 1 << ORC_CATALYZED_BIT;
+    ORC_PARALLEL_FOR
     for (int64_t i = 0; i < n2; ++i) {
         int f = flags2[i];
         f = (f & MD) ? (f | PAST) : (f & ~PAST);
@@ -926,6 +946,7 @@ void orc_change_species(int64_t n, const double *cpos, const double *pos, int32_
                         double cut_sq_n, double prob, const double *draws)
 {
     /* programs/single_dimer_pbc.f90:511-542 */
+    ORC_PARALLEL_FOR
     for (int64_t m = 0; m < n; ++m) {
         if (!(flags[m] & (1 << ORC_REAC_BIT))) continue;
         double x[3];
@@ -947,6 +968,7 @@ void orc_refuel(int64_t n, const double *cpos, const double *pos, int32_t *speci
     float farf = (float)L1 * 0.45f;
     farf = farf * farf;
     double far = (double)farf;
+    ORC_PARALLEL_FOR
     for (int64_t m = 0; m < n; ++m) {
         if (species[m] != 2) continue;
         double x[3];

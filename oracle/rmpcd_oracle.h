@@ -26,6 +26,10 @@
 extern "C" {
 #endif
 
+/* threads of the OpenMP loops; 1 (the default) gives the single-thread order */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 /* flag bits, src/common.f90:49-61 */
 enum { ORC_REAC_BIT = 1, ORC_MD_BIT = 2, ORC_WALL_BIT = 3, ORC_PAST_MD_BIT = 4,
        ORC_CATALYZED_BIT = 6 };

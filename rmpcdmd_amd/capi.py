@@ -133,6 +133,8 @@ SIGNATURES = {
     "rmpcd_dimer_run": [_P, _I],
     "rmpcd_dimer_state": [_P, _I, _PARTP, _PARTP, _BOXP, _CELLP, C.POINTER(DimerStats)],
     "rmpcd_dimer_destroy": [_P],
+    "rmpcd_dimer_profile": [_P, _I],
+    "rmpcd_dimer_stream_time": [_P, _D3, C.POINTER(_I)],
     "rmpcd_threefry_uniform": [_U64, _U64, _U64, _I64, _P],
     "rmpcd_threefry2x64_20": [C.POINTER(_U64), C.POINTER(_U64), C.POINTER(_U64)],
 }

@@ -1030,7 +1030,8 @@ static FP make_fp(const DimerSim &s)
 }
 
 template <class F>
-static void enqueue_md_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st)
+static int enqueue_md_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, hipEvent_t ev0,
+                           hipEvent_t ev1)
 {
     const unsigned Rr = (unsigned)s.R;
     const unsigned gs = (unsigned)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, STREAM_BLOCKS);
@@ -1040,7 +1041,10 @@ static void enqueue_md_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_
     const unsigned gt = (unsigned)((s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE);
     const int64_t lim = std::max(s.nmax_rep, s.ncmax_rep);
     const unsigned gg = (unsigned)std::min<int64_t>((lim + FT - 1) / FT, RED_BLOCKS);
+    // optional event nodes around the streaming kernel (its duration inside the graph)
+    if (ev0) RMPCD_HIP(hipEventRecordWithFlags(ev0, st, hipEventRecordExternal));
     k_fz_stream<F><<<dim3(gs, Rr), FT, 0, st>>>(b, p);
+    if (ev1) RMPCD_HIP(hipEventRecordWithFlags(ev1, st, hipEventRecordExternal));
     k_fz_list_a<F><<<dim3(gl, Rr), FT, 0, st>>>(b, p);
     k_fz_exact<F><<<dim3(gr, Rr), FT, 0, st>>>(b, p);
     k_fz_key<F><<<dim3(gr, Rr), FT, 0, st>>>(b, p, 0);
@@ -1052,6 +1056,7 @@ static void enqueue_md_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_
     k_fz_list_build<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, 0);
     k_fz_force<F><<<dim3(gf, Rr), FT, 0, st>>>(b, p);
     k_fz_list_b<F><<<dim3(gl, Rr), FT, 0, st>>>(b, p);
+    return RMPCD_OK;
 }
 
 template <class F>
@@ -1079,14 +1084,66 @@ static int build_graph(DimerSim &s)
     FB<F> b = make_fb<F>(s);
     FP p = make_fp(s);
     hipStream_t st = s.fstream;
+    if (s.graph_exec) {
+        RMPCD_HIP(hipGraphExecDestroy(s.graph_exec));
+        s.graph_exec = nullptr;
+    }
+    if (s.graph) {
+        RMPCD_HIP(hipGraphDestroy(s.graph));
+        s.graph = nullptr;
+    }
     RMPCD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < s.cfg.n_md; ++j) enqueue_md_step<F>(s, b, p, st);
+    int rc = RMPCD_OK;
+    size_t pair = 0;
+    for (int j = 0; j < s.cfg.n_md && rc == RMPCD_OK; ++j) {
+        bool timed = s.prof_stride > 0 && j % s.prof_stride == 0 && 2 * pair + 1 < s.prof_ev.size();
+        rc = enqueue_md_step<F>(s, b, p, st, timed ? s.prof_ev[2 * pair] : nullptr,
+                                timed ? s.prof_ev[2 * pair + 1] : nullptr);
+        if (timed) ++pair;
+    }
     enqueue_collision<F>(s, b, p, st);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
     if (e != hipSuccess) return set_error(RMPCD_ERR_HIP, "dimer: graph capture failed: %s", hipGetErrorString(e));
     s.graph = g;
+    s.prof_pairs = (int)pair;
     RMPCD_HIP(hipGraphInstantiate(&s.graph_exec, g, nullptr, nullptr, 0));
+    return RMPCD_OK;
+}
+
+int fused_set_profile(DimerSim &s, int stride)
+{
+    if (!s.fused_ready) return set_error(RMPCD_ERR_ARG, "dimer: profiling needs the fused mode");
+    RMPCD_HIP(hipStreamSynchronize(s.fstream));
+    for (hipEvent_t e : s.prof_ev) (void)hipEventDestroy(e);
+    s.prof_ev.clear();
+    s.prof_stride = stride > 0 ? stride : 0;
+    if (s.prof_stride) {
+        int pairs = (s.cfg.n_md + s.prof_stride - 1) / s.prof_stride;
+        s.prof_ev.resize(2 * (size_t)pairs);
+        for (hipEvent_t &e : s.prof_ev) RMPCD_HIP(hipEventCreate(&e));
+    }
+    return s.cfg.form == RMPCD_FP64 ? build_graph<F64>(s) : build_graph<F32>(s);
+}
+
+int fused_stream_time(DimerSim &s, double *avg_ms, int32_t *samples)
+{
+    *avg_ms = 0;
+    *samples = 0;
+    if (s.prof_pairs == 0) return set_error(RMPCD_ERR_ARG, "dimer: no profiled run yet");
+    RMPCD_HIP(hipStreamSynchronize(s.fstream));
+    double tot = 0;
+    for (int k = 0; k < s.prof_pairs; ++k) {
+        float ms = 0;
+        RMPCD_HIP(hipEventElapsedTime(&ms, s.prof_ev[2 * k], s.prof_ev[2 * k + 1]));
+        tot += ms;
+    }
+    *avg_ms = tot / s.prof_pairs;
+    *samples = s.prof_pairs;
     return RMPCD_OK;
 }
 

@@ -170,6 +170,7 @@ DimerSim::~DimerSim()
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (fstream) (void)hipStreamDestroy(fstream);
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     for (void *p : allocs) (void)hipFree(p);
 }
 
@@ -566,6 +567,18 @@ int rmpcd_dimer_state(rmpcd_dimer_t *sim, int32_t replica, rmpcd_particles_t *so
         stats->n_solvent_total = s->ntot;
     }
     return RMPCD_OK;
+}
+
+int rmpcd_dimer_profile(rmpcd_dimer_t *sim, int32_t stride)
+{
+    RMPCD_CHECK(sim && stride >= 0, "rmpcd_dimer_profile: bad argument");
+    return fused_set_profile(*(DimerSim *)sim, stride);
+}
+
+int rmpcd_dimer_stream_time(rmpcd_dimer_t *sim, double *avg_ms, int32_t *samples)
+{
+    RMPCD_CHECK(sim && avg_ms && samples, "rmpcd_dimer_stream_time: null argument");
+    return fused_stream_time(*(DimerSim *)sim, avg_ms, samples);
 }
 
 int rmpcd_dimer_destroy(rmpcd_dimer_t *sim)

@@ -71,6 +71,8 @@ struct DimerSim {
     hipGraph_t graph = nullptr;
     hipStream_t fstream = nullptr;
     bool fused_ready = false;
+    int prof_stride = 0, prof_pairs = 0;  // event nodes around the streaming kernel
+    std::vector<hipEvent_t> prof_ev;
     std::vector<void *> allocs;
 
     ~DimerSim();
@@ -91,5 +93,7 @@ struct DimerSim {
 
 int fused_prepare(DimerSim &s);
 int run_fused_cycles(DimerSim &s, int n_cycles);
+int fused_set_profile(DimerSim &s, int stride);
+int fused_stream_time(DimerSim &s, double *avg_ms, int32_t *samples);
 
 }  // namespace rmpcd
