@@ -31,6 +31,39 @@ def _compile(src):
     return out
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FBIN = os.path.join(FORTRAN, "bin")
+FLIB = os.path.join(FORTRAN, "librmpcd_fortran.so")
+AMDFLANG = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
+FPROGRAMS = ["fortran_cells", "fortran_fluid", "fortran_dimer"]
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("%s failed:\n%s%s" % (what, r.stdout, r.stderr))
+
+
+def build_fortran(verbose=False):
+    """the Fortran module API (amdflang) over librmpcd_amd.so, and its driver programs"""
+    mod = os.path.join(OBJ, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    os.makedirs(FBIN, exist_ok=True)
+    objs = []
+    for src in ("rmpcd_capi.f90", "rmpcdmd.f90"):
+        out = os.path.join(OBJ, src.replace(".f90", ".o"))
+        _run([AMDFLANG, "-c", "-O2", "-fPIC", "-J", mod, os.path.join(FORTRAN, src), "-o", out], src)
+        objs.append(out)
+    _run([AMDFLANG, "-shared", "-fPIC", "-o", FLIB, *objs, "-L" + HERE, "-lrmpcd_amd",
+          "-Wl,-rpath,$ORIGIN/.."], "librmpcd_fortran.so")
+    for prog in FPROGRAMS:
+        _run([AMDFLANG, "-O2", "-J", mod, "-I", mod, os.path.join(FORTRAN, "programs", prog + ".f90"),
+              "-o", os.path.join(FBIN, prog), "-L" + FORTRAN, "-lrmpcd_fortran", "-L" + HERE, "-lrmpcd_amd",
+              "-Wl,-rpath,$ORIGIN/..:$ORIGIN/../.."], prog)
+    if verbose:
+        print("built", FLIB, "and", ", ".join(FPROGRAMS))
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -44,6 +77,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -1,0 +1,266 @@
+! rmpcd_capi.f90 -- iso_c_binding view of include/rmpcd_capi.h (the C ABI of the HIP
+! library) for the Fortran modules that keep the RMPCDMD module API.
+module rmpcd_capi
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: RMPCD_OK = 0, RMPCD_ERR_OVERFLOW = -3
+  integer(c_int32_t), parameter :: RMPCD_FP64 = 0, RMPCD_FP32 = 1
+  integer(c_int32_t), parameter :: H2D = 0, D2H = 1, D2D = 2
+
+  type, bind(c) :: rmpcd_box_t
+     integer(c_int32_t) :: L(3), M(3), ncells, has_walls
+     real(c_double) :: a, edges(3), origin(3)
+     integer(c_int32_t) :: bc(3)
+     real(c_double) :: max_v
+  end type rmpcd_box_t
+
+  type, bind(c) :: rmpcd_particles_t
+     integer(c_int64_t) :: n
+     integer(c_int32_t) :: form, pad_
+     type(c_ptr) :: pos = c_null_ptr, vel = c_null_ptr, force = c_null_ptr, force_old = c_null_ptr
+     type(c_ptr) :: image = c_null_ptr, id = c_null_ptr, species = c_null_ptr, flags = c_null_ptr
+  end type rmpcd_particles_t
+
+  type, bind(c) :: rmpcd_cells_t
+     type(c_ptr) :: cell_count = c_null_ptr, cell_start = c_null_ptr
+     type(c_ptr) :: is_md = c_null_ptr, is_reac = c_null_ptr
+  end type rmpcd_cells_t
+
+  type, bind(c) :: rmpcd_lj_t
+     integer(c_int32_t) :: n1, n2
+     real(c_double) :: eps(16), sigma(16), cut(16), cut_sq(16), cut_energy(16)
+  end type rmpcd_lj_t
+
+  interface
+     function rmpcd_last_error() bind(c) result(p)
+       import :: c_ptr
+       type(c_ptr) :: p
+     end function rmpcd_last_error
+     integer(c_int) function rmpcd_init(device) bind(c)
+       import :: c_int
+       integer(c_int), value :: device
+     end function rmpcd_init
+     integer(c_int) function rmpcd_malloc(ptr, bytes) bind(c)
+       import :: c_int, c_ptr, c_size_t
+       type(c_ptr) :: ptr
+       integer(c_size_t), value :: bytes
+     end function rmpcd_malloc
+     integer(c_int) function rmpcd_free(ptr) bind(c)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ptr
+     end function rmpcd_free
+     integer(c_int) function rmpcd_memcpy(dst, src, bytes, kind) bind(c)
+       import :: c_int, c_ptr, c_size_t, c_int32_t
+       type(c_ptr), value :: dst, src
+       integer(c_size_t), value :: bytes
+       integer(c_int32_t), value :: kind
+     end function rmpcd_memcpy
+     integer(c_int32_t) function rmpcd_compact_p_to_h(p, M) bind(c)
+       import :: c_int32_t
+       integer(c_int32_t), intent(in) :: p(3), M(3)
+     end function rmpcd_compact_p_to_h
+     subroutine rmpcd_compact_h_to_p(h, M, p) bind(c)
+       import :: c_int32_t
+       integer(c_int32_t), value :: h
+       integer(c_int32_t), intent(in) :: M(3)
+       integer(c_int32_t), intent(out) :: p(3)
+     end subroutine rmpcd_compact_h_to_p
+     integer(c_int) function rmpcd_box_init(box, L, a, has_walls) bind(c)
+       import :: c_int, c_int32_t, c_double, rmpcd_box_t
+       type(rmpcd_box_t) :: box
+       integer(c_int32_t), intent(in) :: L(3)
+       real(c_double), value :: a
+       integer(c_int32_t), value :: has_walls
+     end function rmpcd_box_init
+     integer(c_int) function rmpcd_count_particles(box, p, cells) bind(c)
+       import :: c_int, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_particles_t), intent(in) :: p
+       type(rmpcd_cells_t) :: cells
+     end function rmpcd_count_particles
+     integer(c_int) function rmpcd_sort(box, src, dst, id_to_idx, cells, apply_pbc) bind(c)
+       import :: c_int, c_ptr, c_int32_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_particles_t), intent(in) :: src
+       type(rmpcd_particles_t) :: dst
+       type(c_ptr), value :: id_to_idx
+       type(rmpcd_cells_t) :: cells
+       integer(c_int32_t), value :: apply_pbc
+     end function rmpcd_sort
+     integer(c_int) function rmpcd_maximum_displacement(p, pos_old, r) bind(c)
+       import :: c_int, c_ptr, c_double, rmpcd_particles_t
+       type(rmpcd_particles_t), intent(in) :: p
+       type(c_ptr), value :: pos_old
+       real(c_double) :: r
+     end function rmpcd_maximum_displacement
+     integer(c_int) function rmpcd_cell_maximum_displacement(box, cells, p, pos_old, delta_t, r) bind(c)
+       import :: c_int, c_ptr, c_double, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t), intent(in) :: p
+       type(c_ptr), value :: pos_old
+       real(c_double), value :: delta_t
+       real(c_double) :: r
+     end function rmpcd_cell_maximum_displacement
+     integer(c_int) function rmpcd_simple_mpcd_step(box, cells, p, axes, seed, step, alpha_given, &
+          alpha, thermostat, T, hydro) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t) :: p
+       type(c_ptr), value :: axes
+       integer(c_int64_t), value :: seed, step
+       integer(c_int32_t), value :: alpha_given, thermostat, hydro
+       real(c_double), value :: alpha, T
+     end function rmpcd_simple_mpcd_step
+     integer(c_int) function rmpcd_compute_temperature(box, cells, p, te) bind(c)
+       import :: c_int, c_double, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t), intent(in) :: p
+       real(c_double) :: te
+     end function rmpcd_compute_temperature
+     integer(c_int) function rmpcd_mpcd_stream_periodic(box, p, dt) bind(c)
+       import :: c_int, c_double, rmpcd_box_t, rmpcd_particles_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt
+     end function rmpcd_mpcd_stream_periodic
+     integer(c_int) function rmpcd_bulk_reaction(box, cells, p, from, to, rate, tau, draws, seed, step) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t) :: p
+       integer(c_int32_t), value :: from, to
+       real(c_double), value :: rate, tau
+       type(c_ptr), value :: draws
+       integer(c_int64_t), value :: seed, step
+     end function rmpcd_bulk_reaction
+     integer(c_int) function rmpcd_mpcd_stream_nogravity_zwall(box, p, dt) bind(c)
+       import :: c_int, c_double, rmpcd_box_t, rmpcd_particles_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt
+     end function rmpcd_mpcd_stream_nogravity_zwall
+     integer(c_int) function rmpcd_md_pos(p, dt) bind(c)
+       import :: c_int, c_double, rmpcd_particles_t
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt
+     end function rmpcd_md_pos
+     integer(c_int) function rmpcd_md_vel(p, dt) bind(c)
+       import :: c_int, c_double, rmpcd_particles_t
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt
+     end function rmpcd_md_vel
+     integer(c_int) function rmpcd_apply_pbc(p, edges) bind(c)
+       import :: c_int, c_double, rmpcd_particles_t
+       type(rmpcd_particles_t) :: p
+       real(c_double), intent(in) :: edges(3)
+     end function rmpcd_apply_pbc
+     integer(c_int) function rmpcd_cell_md_pos(box, cells, p, dt, md_flag) bind(c)
+       import :: c_int, c_double, c_int32_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt
+       integer(c_int32_t), value :: md_flag
+     end function rmpcd_cell_md_pos
+     integer(c_int) function rmpcd_cell_md_vel(box, cells, p, dt, md_flag) bind(c)
+       import :: c_int, c_double, c_int32_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt
+       integer(c_int32_t), value :: md_flag
+     end function rmpcd_cell_md_vel
+     integer(c_int) function rmpcd_cell_md_pos_wall(box, cells, p, dt, md_flag, axis) bind(c)
+       import :: c_int, c_double, c_int32_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt
+       integer(c_int32_t), value :: md_flag, axis
+     end function rmpcd_cell_md_pos_wall
+     integer(c_int) function rmpcd_rattle_dimer_pos(c, pos_rattle, mass, n_mass, d, dt, edges) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, rmpcd_particles_t
+       type(rmpcd_particles_t) :: c
+       type(c_ptr), value :: pos_rattle
+       real(c_double), intent(in) :: mass(*)
+       integer(c_int32_t), value :: n_mass
+       real(c_double), value :: d, dt
+       real(c_double), intent(in) :: edges(3)
+     end function rmpcd_rattle_dimer_pos
+     integer(c_int) function rmpcd_rattle_dimer_vel(c, mass, n_mass, d, dt, edges) bind(c)
+       import :: c_int, c_double, c_int32_t, rmpcd_particles_t
+       type(rmpcd_particles_t) :: c
+       real(c_double), intent(in) :: mass(*)
+       integer(c_int32_t), value :: n_mass
+       real(c_double), value :: d, dt
+       real(c_double), intent(in) :: edges(3)
+     end function rmpcd_rattle_dimer_vel
+     integer(c_int) function rmpcd_lj_params_init(lj, n1, n2, eps, sigma, cut) bind(c)
+       import :: c_int, c_double, c_int32_t, rmpcd_lj_t
+       type(rmpcd_lj_t) :: lj
+       integer(c_int32_t), value :: n1, n2
+       real(c_double), intent(in) :: eps(*), sigma(*), cut(*)
+     end function rmpcd_lj_params_init
+     integer(c_int32_t) function rmpcd_make_stencil(a, cut, stencil, cap) bind(c)
+       import :: c_int32_t, c_double
+       real(c_double), value :: a, cut
+       integer(c_int32_t) :: stencil(*)
+       integer(c_int32_t), value :: cap
+     end function rmpcd_make_stencil
+     integer(c_int) function rmpcd_update_list(box, cells, points, solvent, radius, stencil, n_stencil, &
+          lj, nmax, list, n) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t) :: cells
+       type(rmpcd_particles_t), intent(in) :: points, solvent
+       real(c_double), value :: radius
+       type(c_ptr), value :: stencil, lj, list, n
+       integer(c_int32_t), value :: n_stencil, nmax
+     end function rmpcd_update_list
+     integer(c_int) function rmpcd_compute_force(ps1, ps2, list, n, nmax, L, lj, e) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, rmpcd_particles_t, rmpcd_lj_t
+       type(rmpcd_particles_t) :: ps1, ps2
+       type(c_ptr), value :: list, n
+       integer(c_int32_t), value :: nmax
+       real(c_double), intent(in) :: L(3)
+       type(rmpcd_lj_t), intent(in) :: lj
+       real(c_double) :: e
+     end function rmpcd_compute_force
+     integer(c_int) function rmpcd_compute_force_n2(ps, L, lj, e) bind(c)
+       import :: c_int, c_double, rmpcd_particles_t, rmpcd_lj_t
+       type(rmpcd_particles_t) :: ps
+       real(c_double), intent(in) :: L(3)
+       type(rmpcd_lj_t), intent(in) :: lj
+       real(c_double) :: e
+     end function rmpcd_compute_force_n2
+     subroutine rmpcd_threefry2x64_20(ctr, key, out) bind(c)
+       import :: c_int64_t
+       integer(c_int64_t), intent(in) :: ctr(2), key(2)
+       integer(c_int64_t), intent(out) :: out(2)
+     end subroutine rmpcd_threefry2x64_20
+  end interface
+
+contains
+
+  !> stop with the library's message when a call fails (the reference's error stop)
+  subroutine check(rc)
+    integer(c_int), intent(in) :: rc
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: i
+    if (rc == RMPCD_OK) return
+    call c_f_pointer(rmpcd_last_error(), msg, [512])
+    text = ''
+    do i = 1, 512
+       if (msg(i) == c_null_char) exit
+       text(i:i) = msg(i)
+    end do
+    write(*, '(a)') trim(text)
+    error stop 'rmpcdmd_amd: library call failed'
+  end subroutine check
+
+end module rmpcd_capi
