@@ -146,6 +146,34 @@ class DimerOracle:
         self.collide(draws)
 
 
+def from_download(st, L, d, tau, n_md, sigma_C=2.0, sigma_N=2.0, **kw):
+    """a DimerOracle holding a state downloaded from the device program (FP64 form):
+    st is rmpcdmd_amd.sim.SingleDimer.download()'s dict. The neighbour lists and the
+    cell counts are recomputed by the oracle (and can be compared with st's)."""
+    n = int(st["n_solvent"])
+    s = PState_from(st, "", n)
+    col = PState_from(st, "colloid_", 2)
+    col.species[:] = [1, 2]
+    drv = DimerOracle(L, s, col, np.array(st["pos_old"], np.float64), np.array(st["colloid_pos_old"]),
+                      st["mass"], d, tau, n_md, sigma_C=sigma_C, sigma_N=sigma_N, nmax=int(st["nmax"]),
+                      origin=st["origin"], **kw)
+    drv.update_list()
+    drv.md_steps, drv.cycles, drv.n_extra = int(st["md_steps"]), int(st["cycles"]), int(st["n_extra_sorting"])
+    drv.e1, drv.e2 = float(st["e1"]), float(st["e2"])
+    return drv
+
+
+def PState_from(st, prefix, n):
+    p = O.PState(n)
+    for k in ("pos", "vel", "force", "force_old", "image"):
+        getattr(p, k)[:] = st[prefix + k]
+    if not prefix:
+        p.id[:] = st["id"]
+        p.species[:] = st["species"]
+        p.flags[:] = st["flags"]
+    return p
+
+
 def synthetic_dimer(L, rho=10, seed=0, sigma=2.0, d=4.5, T=1.0, **kw):
     """a set-up like single_dimer_pbc.f90:240-296 with numpy draws: Maxwell-Boltzmann
     velocities without net momentum, solvent placed outside the colloid cutoffs, sorted,

@@ -85,11 +85,14 @@ class SingleDimer:
         out["colloid_image"] = get(col.image, 6, np.int32).reshape(3, 2).T.copy()
         out["cell_count"] = get(cells.cell_count, box.ncells, np.int32)
         out["cell_start"] = get(cells.cell_start, box.ncells, np.int32)
+        out["pos_old"] = get(st.pos_old, 3 * n, real).reshape(3, n).T.copy()
+        out["colloid_pos_old"] = get(st.colloid_pos_old, 6, np.float64).reshape(3, 2).T.copy()
         out["list"] = get(st.list, 2 * st.nmax, np.int32).reshape(2, st.nmax)
         out["nlist"] = get(st.nlist, 2, np.int32)
         out["origin"] = np.array(box.origin[:])
-        for k in ("md_steps", "cycles", "n_extra_sorting", "e1", "e2", "n_solvent"):
+        for k in ("md_steps", "cycles", "n_extra_sorting", "e1", "e2", "n_solvent", "nmax", "dt"):
             out[k] = getattr(st, k)
+        out["mass"] = np.array(st.mass[:])
         return out
 
     def close(self):

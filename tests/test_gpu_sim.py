@@ -79,3 +79,54 @@ def test_dimer_invariants(gpu):
     # the catalytic sphere converted some A into B
     assert (s1["species"] == 2).sum() > 0
     sim.close()
+
+
+def _close(a, b, rtol=1e-12):
+    """|a - b| <= rtol * max(1, |b|) elementwise (the FP64 bar of the whole-step check)"""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return a.shape == b.shape and bool((np.abs(a - b) <= rtol * np.maximum(1.0, np.abs(b))).all())
+
+
+@pytest.mark.parametrize("bulk", [0, 1])
+def test_reference_sequence_matches_oracle(gpu, bulk):
+    """the whole single_dimer_pbc cycle (MD steps with sort decisions, forces, RATTLE,
+    change_species; grid shift, sort, SRD, refuel or bulk reaction) on the GPU against
+    the oracle's loop consuming the same Threefry draws. Integer state (sort order, ids,
+    species, flags, images, cell counts, lists) bit-exact; FP64 within 1e-12 relative:
+    the colloid force sums are tree-reduced on the GPU, everything else keeps the
+    reference's evaluation order."""
+    from oracle.dimer import from_download
+    from oracle.threefry import ThreefryDraws
+    L = (12, 12, 12)
+    kw = dict(L=L, n_md=10, tau=0.1, d=4.5, sigma_C=2.0, sigma_N=2.0, bulk_rmpcd=bulk, bulk_rate=0.5)
+    sim = SingleDimer(mode=capi.SIM_REFERENCE, form=capi.FP64, seed=41, **kw)
+    st = sim.download()
+    orc = from_download(st, L, 4.5, 0.1, 10, bulk_rmpcd=bool(bulk), bulk_rate=0.5)
+    assert (orc.nl == st["nlist"]).all()
+    for j in range(2):
+        assert (orc.lst[j, :orc.nl[j]] == st["list"][j, :st["nlist"][j]]).all()
+    draws = ThreefryDraws(41)
+    for cycle in range(3):
+        sim.run(1)
+        orc.cycle(draws)
+        g = sim.download()
+        msg = "after cycle %d" % cycle
+        assert g["n_extra_sorting"] == orc.n_extra and g["md_steps"] == orc.md_steps, msg
+        assert (g["id"] == orc.s.id).all(), msg
+        assert (g["species"] == orc.s.species).all(), msg
+        assert (g["flags"] == orc.s.flags).all(), msg
+        assert (g["image"] == orc.s.image).all(), msg
+        assert (g["colloid_image"] == orc.col.image).all(), msg
+        assert (g["cell_count"] == orc.cnt).all() and (g["cell_start"] == orc.st).all(), msg
+        assert (g["nlist"] == orc.nl).all(), msg
+        for j in range(2):
+            assert (g["list"][j, :g["nlist"][j]] == orc.lst[j, :orc.nl[j]]).all(), msg
+        assert _close(g["origin"], orc.c.origin[:]), msg
+        for k in ("pos", "vel", "force", "force_old"):
+            assert _close(g[k], getattr(orc.s, k)), "%s %s" % (k, msg)
+            assert _close(g["colloid_" + k], getattr(orc.col, k)), "colloid %s %s" % (k, msg)
+        assert _close(g["e1"], orc.e1) and _close(g["e2"], orc.e2), msg
+    if bulk:
+        assert (g["species"] == 2).sum() > 0
+    sim.close()

@@ -479,3 +479,33 @@ def test_catalytic_helpers():
     assert sp[0] == 2 and fl[0] == 0
     O.refuel(cpos, pos, sp, L, 20)  # far = (20*0.45)^2 = 81 in single precision
     assert list(sp) == [2, 1, 2, 1, 1]
+
+
+def test_numpy_threefry_known_answers():
+    # Random123 kat_vectors, threefry2x64 with 20 rounds: zero input, and the digits of pi
+    from oracle import threefry as T
+    a, b = T.threefry2x64(0, 0, 0, 0)
+    assert (int(a), int(b)) == (0xC2B6E3A8C2C69865, 0x6F81ED42F350084D)
+    a, b = T.threefry2x64(0x243F6A8885A308D3, 0x13198A2E03707344, 0xA4093822299F31D0, 0x082EFA98EC4E6C89)
+    assert (int(a), int(b)) == (0x263C7D30BB0F0AF1, 0x56BE8361D3311526)
+
+
+def test_numpy_threefry_matches_library_and_sphere():
+    import ctypes as C
+    from oracle import threefry as T
+    from rmpcdmd_amd import capi
+    rng = np.random.default_rng(3)
+    v = rng.integers(0, 2 ** 63, size=(50, 4), dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    a, b = T.threefry2x64(v[:, 0], v[:, 1], v[:, 2], v[:, 3])
+    for i in range(50):
+        o = (C.c_uint64 * 2)()
+        capi.lib().rmpcd_threefry2x64_20((C.c_uint64 * 2)(int(v[i, 0]), int(v[i, 1])),
+                                         (C.c_uint64 * 2)(int(v[i, 2]), int(v[i, 3])), o)
+        assert (o[0], o[1]) == (int(a[i]), int(b[i]))
+    ax = T.srd_axes(7, 3, 4096)
+    assert np.allclose((ax * ax).sum(axis=1), 1.0, atol=1e-14)
+    assert np.abs(ax.mean(axis=0)).max() < 0.05  # uniform on the sphere
+    assert abs((ax[:, 2] ** 2).mean() - 1 / 3) < 0.03
+    # the shift draws are uniforms in [0, 1)
+    u = np.array([T.shift_uniforms(5, s) for s in range(200)])
+    assert (u >= 0).all() and (u < 1).all() and abs(u.mean() - 0.5) < 0.05
