@@ -1,0 +1,98 @@
+"""The FP32-mixed form: FP32 solvent state and Kahan-compensated FP32 cell sums, FP64
+colloids, colloid forces and RATTLE. It is not bit-comparable with the FP64 oracle, so
+it is held to the statistical bar instead:
+* per-cell momentum (and relative kinetic energy) conserved by the SRD collision to
+  1e-6 relative;
+* temperature within 0.1% of the FP64 form on the same initial state and draws;
+* Maxwell-Boltzmann moments of the velocities;
+* total momentum conserved by the whole dimer cycle (dimer_test.py's check, relaxed
+  from 1e-15 to the FP32 rounding level)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from rmpcdmd_amd import capi
+from rmpcdmd_amd.sim import SingleDimer
+
+from _helpers import cells_to_device, sorted_state, to_device
+
+pytestmark = pytest.mark.gpu
+
+
+def cell_sums(v, cnt, st):
+    """per-cell sums of v (n, k) over the sorted layout; cells with count > 1"""
+    keep = np.nonzero(cnt > 1)[0]
+    idx = np.repeat(np.arange(keep.size), cnt[keep])
+    rows = np.concatenate([np.arange(st[c], st[c] + cnt[c]) for c in keep])
+    out = np.zeros((keep.size,) + v.shape[1:])
+    np.add.at(out, idx, v[rows])
+    return out, keep
+
+
+def test_fp32_srd_conserves_cell_momentum(gpu):
+    c, srt, cnt, st = sorted_state((16, 16, 16), 10, 31)
+    d = to_device(srt, form=capi.FP32)
+    cs = cells_to_device(c, cnt, st)
+    v0 = d.host("vel").astype(np.float64)
+    capi.call("rmpcd_simple_mpcd_step", cs.boxp(), cs.cellsp(), d.struct(), None, 99, 3, 0, 0.0, 0, 1.0, 1)
+    v1 = d.host("vel").astype(np.float64)
+    assert not np.allclose(v0, v1)
+    p0, keep = cell_sums(v0, cnt, st)
+    p1, _ = cell_sums(v1, cnt, st)
+    scale, _ = cell_sums(np.abs(v0), cnt, st)
+    assert (np.abs(p1 - p0) <= 1e-6 * scale).all()
+    # the rotation keeps each cell's kinetic energy
+    e0, _ = cell_sums((v0 * v0).sum(axis=1), cnt, st)
+    e1, _ = cell_sums((v1 * v1).sum(axis=1), cnt, st)
+    assert (np.abs(e1 - e0) <= 1e-6 * e0).all()
+
+
+def temperature(sim):
+    sol, col, box, cells, st = sim.state()
+    te = C.c_double()
+    capi.call("rmpcd_compute_temperature", C.byref(box), C.byref(cells), C.byref(sol), C.byref(te))
+    return te.value
+
+
+def test_fp32_form_against_fp64_form(gpu):
+    """same initial state and draws in both forms; the per-observable FP32 error"""
+    kw = dict(L=(24, 24, 24), n_md=10, tau=0.1, seed=9, mode=capi.SIM_FUSED)
+    s64 = SingleDimer(form=capi.FP64, **kw)
+    s32 = SingleDimer(form=capi.FP32, **kw)
+    m = np.array(s64.state()[4].mass[:])
+
+    def momentum(s):
+        return (s["vel"].astype(np.float64).sum(axis=0) + m[0] * s["colloid_vel"][0]
+                + m[1] * s["colloid_vel"][1])
+
+    a0 = s32.download()
+    p0 = momentum(a0)
+    errors = {}
+    for cycle in range(4):
+        s64.run(1)
+        s32.run(1)
+        t64, t32 = temperature(s64), temperature(s32)
+        errors.setdefault("temperature", []).append(abs(t32 - t64) / t64)
+    a, b = s32.download(), s64.download()
+    n = a["n_solvent"]
+    v = a["vel"].astype(np.float64)
+    ke32, ke64 = (v * v).sum(), (b["vel"] * b["vel"]).sum()
+    errors["kinetic_energy"] = abs(ke32 - ke64) / ke64
+    errors["dimer_position"] = float(np.abs(a["colloid_pos"] - b["colloid_pos"]).max())
+    errors["n_product"] = int(abs((a["species"] == 2).sum() - (b["species"] == 2).sum()))
+    print("FP32-mixed vs FP64 form after 4 cycles:", errors)
+    assert max(errors["temperature"]) < 1e-3
+    assert errors["kinetic_energy"] < 1e-3
+    assert errors["dimer_position"] < 1e-3
+    # momentum: conserved to the FP32 rounding level of the velocities
+    p1 = momentum(a)
+    assert np.abs(p1 - p0).max() / (n + m.sum()) < 1e-6
+    # Maxwell-Boltzmann: zero mean, variance T, kurtosis 3 per component
+    T = 1.0
+    assert np.abs(v.mean(axis=0)).max() < 0.02
+    assert np.abs(v.var(axis=0) / T - 1).max() < 0.03
+    kurt = (v ** 4).mean(axis=0) / v.var(axis=0) ** 2
+    assert np.abs(kurt - 3).max() < 0.08
+    s64.close()
+    s32.close()
