@@ -233,6 +233,18 @@ int rmpcd_change_species(rmpcd_particles_t *solvent, const rmpcd_particles_t *co
 /* refuel (:544-565) */
 int rmpcd_refuel(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
                  const double edges[3], int32_t L1);
+/* flag_particles_nl of the fixed-sphere driver
+ * (test/setup_single_catalytic_fixed_sphere.f90:311-329): a species-1 particle of colloid
+ * 0's list inside cut_sq_c draws u and gets the REAC bit when u <= prob; draws: NULL
+ * (Threefry keyed by (seed, step, id)) or device array, one per list slot */
+int rmpcd_flag_particles_nl_prob(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
+                                 const int32_t *list, const int32_t *n, int32_t nmax,
+                                 const double edges[3], double cut_sq_c, double prob,
+                                 const double *draws, uint64_t seed, uint64_t step);
+/* change_species of the same driver (:331-349): a flagged particle farther than cut_sq_c
+ * from colloid 0 becomes species 2 and loses the flag (no draw) */
+int rmpcd_change_species_sphere(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
+                                const double edges[3], double cut_sq_c);
 
 /* ---- the single-dimer nanomotor program (programs/single_dimer_pbc.f90) -------------- */
 /* The time loop of the driver on device-resident state: set-up (:183-296), the MD step
@@ -287,6 +299,55 @@ int rmpcd_dimer_destroy(rmpcd_dimer_t *sim);
  * duration of those launches in the last cycle run */
 int rmpcd_dimer_profile(rmpcd_dimer_t *sim, int32_t stride);
 int rmpcd_dimer_stream_time(rmpcd_dimer_t *sim, double *avg_ms, int32_t *samples);
+
+/* ---- the other driver loops on device-resident state --------------------------------
+ * RMPCD_PROG_FLUID     test/setup_simple_fluid.f90:89-127: SRD, streaming, grid shift,
+ *                      sort; n_md streaming sub-steps of tau/n_md per collision (1 is
+ *                      the driver's single mpcd_stream_periodic of tau)
+ * RMPCD_PROG_SPHERE    test/setup_single_catalytic_fixed_sphere.f90:89-253: a fixed
+ *                      catalytic sphere, solvent Verlet with neighbour-list forces,
+ *                      flag/change at the sphere, SRD and the bulk B->A reaction
+ * RMPCD_PROG_COLLOIDS  programs/n_colloids_pbc.f90:123-346 (production loop, no
+ *                      thermostat phase): many LJ colloids, lazy cell-wise streaming
+ *                      (cell_md_pos/vel), colloid-colloid n^2 forces, SRD
+ * Each runs the driver's call sequence through the entry points above. */
+#define RMPCD_PROG_FLUID 0
+#define RMPCD_PROG_SPHERE 1
+#define RMPCD_PROG_COLLOIDS 2
+typedef struct rmpcd_program_config {
+    int32_t kind;
+    int32_t L[3];
+    int32_t rho;
+    int32_t n_md;             /* MD (or streaming) steps per collision, dt = tau / n_md */
+    int32_t form;             /* RMPCD_FP64 or RMPCD_FP32 for the solvent */
+    int32_t n_colloids;       /* COLLOIDS */
+    int32_t nmax_list;        /* 0: the driver's capacity */
+    int32_t pad_;
+    double T, tau;
+    double sigma, epsilon;    /* solvent-colloid LJ (SPHERE, COLLOIDS) */
+    double epsilon_colloids;  /* COLLOIDS: colloid-colloid LJ, sigma 2 sigma, cut 6 sigma */
+    double prob;              /* SPHERE: flag probability */
+    double bulk_rate;         /* SPHERE: bulk B->A rate */
+    double skin;              /* 0: the driver's value (SPHERE 1.5, COLLOIDS 0.8) */
+    uint64_t seed;
+} rmpcd_program_config_t;
+typedef struct rmpcd_program_stats {
+    int64_t n_solvent, n_colloids, md_steps, cycles, n_extra_sorting;
+    double e1, e2;            /* last compute_force / compute_force_n2 energies */
+    double temperature;       /* last compute_temperature of the collision cycle */
+    double dt, mass;
+    int32_t nmax, last_sort;  /* list capacity; COLLOIDS: md step of the last sort */
+    int32_t *list, *nlist;    /* device: the colloids' neighbour lists (nmax each) */
+    void *pos_old;            /* device: solvent pos_old (the sort's other buffer) */
+    double *colloid_pos_old;  /* device */
+} rmpcd_program_stats_t;
+typedef struct rmpcd_program rmpcd_program_t;
+int rmpcd_program_create(const rmpcd_program_config_t *cfg, rmpcd_program_t **prog);
+int rmpcd_program_run(rmpcd_program_t *prog, int32_t n_cycles);
+int rmpcd_program_state(rmpcd_program_t *prog, rmpcd_particles_t *solvent,
+                        rmpcd_particles_t *colloids, rmpcd_box_t *box, rmpcd_cells_t *cells,
+                        rmpcd_program_stats_t *stats);
+int rmpcd_program_destroy(rmpcd_program_t *prog);
 
 /* ---- counter-based generator used when draws are not injected ------------------------ */
 /* uniform doubles in [0,1): out[i] = U(seed, stream, step, i), device output */

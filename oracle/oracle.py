@@ -351,3 +351,15 @@ def change_species(cpos, pos, species, flags, edges, cut_sq_c, cut_sq_n, prob, d
 def refuel(cpos, pos, species, edges, L1):
     ea = (C.c_double * 3)(*[float(v) for v in edges])
     lib().orc_refuel(I64(pos.shape[0]), P(cpos), P(pos), P(species), ea, int(L1))
+
+
+def flag_particles_nl_prob(list1, n1, cpos, pos, species, flags, edges, cut_sq_c, prob, draws):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_flag_particles_nl_prob(P(list1), int(n1), P(cpos), P(pos), P(species), P(flags), ea,
+                                     D(cut_sq_c), D(prob), P(draws))
+
+
+def change_species_sphere(cpos, pos, species, flags, edges, cut_sq_c):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_change_species_sphere(I64(pos.shape[0]), P(cpos), P(pos), P(species), P(flags), ea,
+                                    D(cut_sq_c))

@@ -979,3 +979,37 @@ void orc_refuel(int64_t n, const double *cpos, const double *pos, int32_t *speci
         if (dc > far && dn > far) species[m] = 1;
     }
 }
+
+void orc_flag_particles_nl_prob(const int32_t *list1, int32_t n1, const double *cpos,
+                                const double *pos, const int32_t *species, int32_t *flags,
+                                const double edges[3], double cut_sq_c, double prob,
+                                const double *draws)
+{
+    /* test/setup_single_catalytic_fixed_sphere.f90:311-329: a candidate inside the cutoff
+     * consumes one draw (draws[s] for list slot s) and is flagged when it is <= prob */
+    for (int s = 0; s < n1; ++s) {
+        int r = list1[s];
+        if (species[r] == 1) {
+            double x[3];
+            orc_rel_pos(cpos, pos + 3 * r, edges, x);
+            if (dot3(x, x) < cut_sq_c && draws[s] <= prob) flags[r] |= 1 << ORC_REAC_BIT;
+        }
+    }
+}
+
+void orc_change_species_sphere(int64_t n, const double *cpos, const double *pos,
+                               int32_t *species, int32_t *flags, const double edges[3],
+                               double cut_sq_c)
+{
+    /* test/setup_single_catalytic_fixed_sphere.f90:331-349 */
+    ORC_PARALLEL_FOR
+    for (int64_t m = 0; m < n; ++m) {
+        if (!(flags[m] & (1 << ORC_REAC_BIT))) continue;
+        double x[3];
+        orc_rel_pos(cpos, pos + 3 * m, edges, x);
+        if (dot3(x, x) > cut_sq_c) {
+            species[m] = 2;
+            flags[m] &= ~(1 << ORC_REAC_BIT);
+        }
+    }
+}

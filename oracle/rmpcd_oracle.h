@@ -180,6 +180,14 @@ void orc_change_species(int64_t n, const double *cpos, const double *pos,
                         const double *draws);
 void orc_refuel(int64_t n, const double *cpos, const double *pos, int32_t *species,
                 const double edges[3], int L1);
+/* the fixed-sphere driver's helpers (test/setup_single_catalytic_fixed_sphere.f90:311-349) */
+void orc_flag_particles_nl_prob(const int32_t *list1, int32_t n1, const double *cpos,
+                                const double *pos, const int32_t *species, int32_t *flags,
+                                const double edges[3], double cut_sq_c, double prob,
+                                const double *draws);
+void orc_change_species_sphere(int64_t n, const double *cpos, const double *pos,
+                               int32_t *species, int32_t *flags, const double edges[3],
+                               double cut_sq_c);
 
 #ifdef __cplusplus
 }

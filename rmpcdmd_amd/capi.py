@@ -83,6 +83,25 @@ _BOXP = C.POINTER(Box)
 _PARTP = C.POINTER(Particles)
 _CELLP = C.POINTER(Cells)
 _LJP = C.POINTER(LJ)
+PROG_FLUID, PROG_SPHERE, PROG_COLLOIDS = 0, 1, 2
+
+
+class ProgramConfig(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("L", C.c_int32 * 3), ("rho", C.c_int32), ("n_md", C.c_int32),
+                ("form", C.c_int32), ("n_colloids", C.c_int32), ("nmax_list", C.c_int32), ("pad_", C.c_int32),
+                ("T", C.c_double), ("tau", C.c_double), ("sigma", C.c_double), ("epsilon", C.c_double),
+                ("epsilon_colloids", C.c_double), ("prob", C.c_double), ("bulk_rate", C.c_double),
+                ("skin", C.c_double), ("seed", C.c_uint64)]
+
+
+class ProgramStats(C.Structure):
+    _fields_ = [("n_solvent", C.c_int64), ("n_colloids", C.c_int64), ("md_steps", C.c_int64),
+                ("cycles", C.c_int64), ("n_extra_sorting", C.c_int64), ("e1", C.c_double), ("e2", C.c_double),
+                ("temperature", C.c_double), ("dt", C.c_double), ("mass", C.c_double), ("nmax", C.c_int32),
+                ("last_sort", C.c_int32), ("list", C.c_void_p), ("nlist", C.c_void_p), ("pos_old", C.c_void_p),
+                ("colloid_pos_old", C.c_void_p)]
+
+
 _D3 = C.POINTER(C.c_double)
 
 # name -> argtypes; every function returns int status unless listed in _RESTYPE
@@ -129,6 +148,12 @@ SIGNATURES = {
     "rmpcd_flag_particles_nl": [_PARTP, _PARTP, _P, _P, _I, _D3, _D],
     "rmpcd_change_species": [_PARTP, _PARTP, _D3, _D, _D, _D, _P, _U64, _U64],
     "rmpcd_refuel": [_PARTP, _PARTP, _D3, _I],
+    "rmpcd_flag_particles_nl_prob": [_PARTP, _PARTP, _P, _P, _I, _D3, _D, _D, _P, _U64, _U64],
+    "rmpcd_change_species_sphere": [_PARTP, _PARTP, _D3, _D],
+    "rmpcd_program_create": [C.POINTER(ProgramConfig), C.POINTER(C.c_void_p)],
+    "rmpcd_program_run": [_P, _I],
+    "rmpcd_program_state": [_P, _PARTP, _PARTP, _BOXP, _CELLP, C.POINTER(ProgramStats)],
+    "rmpcd_program_destroy": [_P],
     "rmpcd_dimer_create": [C.POINTER(DimerConfig), C.POINTER(C.c_void_p)],
     "rmpcd_dimer_run": [_P, _I],
     "rmpcd_dimer_state": [_P, _I, _PARTP, _PARTP, _BOXP, _CELLP, C.POINTER(DimerStats)],

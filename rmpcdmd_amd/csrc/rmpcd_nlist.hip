@@ -303,6 +303,54 @@ __global__ void k_change_species(PView<F> sol, PView<F64> c, double L0, double L
     }
 }
 
+// flag_particles_nl of the fixed-sphere driver
+// (setup_single_catalytic_fixed_sphere.f90:311-329): a species-1 list particle inside
+// the cutoff draws u (list slot k: draws[k], or threefry((step, id), (seed, CHANGE)))
+// and is flagged when u <= prob
+template <class F>
+__global__ void k_flag_nl_prob(PView<F> sol, PView<F64> c, const int32_t *__restrict__ list,
+                               const int32_t *__restrict__ nl, int32_t nmax, double L0, double L1, double L2,
+                               double cut_sq_c, double prob, const double *__restrict__ draws, uint64_t seed,
+                               uint64_t step)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= min(nl[0], nmax)) return;
+    int r = list[k];
+    if ((int)sol.species[r] != 1) return;
+    double d0 = min_image(c.pos[0] - (double)sol.pos[r], L0);
+    double d1 = min_image(c.pos[c.n] - (double)sol.pos[sol.n + r], L1);
+    double d2 = min_image(c.pos[2 * c.n] - (double)sol.pos[2 * sol.n + r], L2);
+    if ((d0 * d0 + d1 * d1) + d2 * d2 < cut_sq_c) {
+        double u;
+        if (draws) {
+            u = draws[k];
+        } else {
+            uint64_t a, b;
+            threefry2x64(step, (uint64_t)sol.id[r], seed, STREAM_CHANGE, a, b);
+            u = u01(a);
+        }
+        if (u <= prob) sol.flags[r] = (typename F::small)((int)sol.flags[r] | REAC_MASK);
+    }
+}
+
+// change_species of the fixed-sphere driver (setup_single_catalytic_fixed_sphere.f90:331-349)
+template <class F>
+__global__ void k_change_species_sphere(PView<F> sol, PView<F64> c, double L0, double L1, double L2,
+                                        double cut_sq_c)
+{
+    int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= sol.n) return;
+    int f = (int)sol.flags[m];
+    if (!(f & REAC_MASK)) return;
+    double a0 = min_image(c.pos[0] - (double)sol.pos[m], L0);
+    double a1 = min_image(c.pos[c.n] - (double)sol.pos[sol.n + m], L1);
+    double a2 = min_image(c.pos[2 * c.n] - (double)sol.pos[2 * sol.n + m], L2);
+    if ((a0 * a0 + a1 * a1) + a2 * a2 > cut_sq_c) {
+        sol.species[m] = (typename F::small)2;
+        sol.flags[m] = (typename F::small)(f & ~REAC_MASK);
+    }
+}
+
 // refuel (single_dimer_pbc.f90:544-565)
 template <class F>
 __global__ void k_refuel(PView<F> sol, PView<F64> c, double L0, double L1, double L2, double far)
@@ -481,6 +529,48 @@ int rmpcd_change_species(rmpcd_particles_t *solvent, const rmpcd_particles_t *co
         k_change_species<F32><<<blocks_for(solvent->n, 256), 256, 0, s>>>(
             view<F32>(solvent), view<F64>(colloids), edges[0], edges[1], edges[2], cut_sq_c, cut_sq_n, prob,
             draws, seed, step);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+int rmpcd_flag_particles_nl_prob(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
+                                 const int32_t *list, const int32_t *n, int32_t nmax, const double edges[3],
+                                 double cut_sq_c, double prob, const double *draws, uint64_t seed, uint64_t step)
+{
+    int rc = check_particles(solvent, "flag_particles_nl", false);
+    if (!rc) rc = check_particles(colloids, "flag_particles_nl", false);
+    if (rc) return rc;
+    RMPCD_CHECK(colloids->form == RMPCD_FP64, "flag_particles_nl: colloids must use the FP64 form");
+    RMPCD_CHECK(list && n && edges && nmax > 0, "flag_particles_nl: null argument");
+    RMPCD_CHECK(draws || solvent->id, "flag_particles_nl: null argument");
+    hipStream_t s = stream();
+    if (solvent->form == RMPCD_FP64)
+        k_flag_nl_prob<F64><<<blocks_for(nmax, 256), 256, 0, s>>>(view<F64>(solvent), view<F64>(colloids), list,
+                                                                  n, nmax, edges[0], edges[1], edges[2],
+                                                                  cut_sq_c, prob, draws, seed, step);
+    else
+        k_flag_nl_prob<F32><<<blocks_for(nmax, 256), 256, 0, s>>>(view<F32>(solvent), view<F64>(colloids), list,
+                                                                  n, nmax, edges[0], edges[1], edges[2],
+                                                                  cut_sq_c, prob, draws, seed, step);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+int rmpcd_change_species_sphere(rmpcd_particles_t *solvent, const rmpcd_particles_t *colloids,
+                                const double edges[3], double cut_sq_c)
+{
+    int rc = check_particles(solvent, "change_species", false);
+    if (!rc) rc = check_particles(colloids, "change_species", false);
+    if (rc) return rc;
+    RMPCD_CHECK(colloids->form == RMPCD_FP64 && colloids->n >= 1, "change_species: needs an FP64 colloid");
+    RMPCD_CHECK(edges != nullptr, "change_species: null edges");
+    hipStream_t s = stream();
+    if (solvent->form == RMPCD_FP64)
+        k_change_species_sphere<F64><<<blocks_for(solvent->n, 256), 256, 0, s>>>(
+            view<F64>(solvent), view<F64>(colloids), edges[0], edges[1], edges[2], cut_sq_c);
+    else
+        k_change_species_sphere<F32><<<blocks_for(solvent->n, 256), 256, 0, s>>>(
+            view<F32>(solvent), view<F64>(colloids), edges[0], edges[1], edges[2], cut_sq_c);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
