@@ -29,6 +29,7 @@ sys.path.insert(0, REPO)
 
 METRIC = "solvent particle-steps/s, FP32-mixed vs FP64 at stated error; % of HBM peak"
 HBM_PEAK_GBS = 8000.0
+PROFILE_STRIDE = 5  # event nodes around every 5th streaming launch of the captured cycle
 
 
 def parse():
@@ -57,8 +58,13 @@ def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile):
     sim = SingleDimer(L=(L, L, L), n_md=n_md, form=f, mode=m, n_replicas=replicas)
     st0 = sim.state()[4]
     n_total = int(st0.n_solvent_total)
+    out = {}
     if profile and mode == "fused":
-        capi.call("rmpcd_dimer_profile", sim._h, 1)
+        try:
+            capi.call("rmpcd_dimer_profile", sim._h, PROFILE_STRIDE)
+        except capi.RmpcdError as e:
+            out["profile_error"] = str(e)
+            profile = False
     if warmup:
         sim.run(warmup)
     torch.cuda.synchronize()
@@ -67,7 +73,7 @@ def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     sec = t1 - t0
-    out = dict(n=n_total, seconds=sec, ps_per_s=n_total * n_md * steps / sec, ms_per_cycle=1e3 * sec / steps)
+    out.update(n=n_total, seconds=sec, ps_per_s=n_total * n_md * steps / sec, ms_per_cycle=1e3 * sec / steps)
     extra = 0
     for r in range(replicas):
         extra += int(sim.state(r)[4].n_extra_sorting)
@@ -169,6 +175,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    if "profile_error" in main_run:
+        line["roofline_error"] = main_run["profile_error"]
     if compare:
         line["fp64_form"] = {"value": compare["ps_per_s"], "ms_per_step": compare["ms_per_cycle"],
                              "mixed_over_fp64": main_run["ps_per_s"] / compare["ps_per_s"]}

@@ -1030,8 +1030,7 @@ static FP make_fp(const DimerSim &s)
 }
 
 template <class F>
-static int enqueue_md_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, hipEvent_t ev0,
-                           hipEvent_t ev1)
+static int enqueue_md_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st)
 {
     const unsigned Rr = (unsigned)s.R;
     const unsigned gs = (unsigned)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, STREAM_BLOCKS);
@@ -1041,10 +1040,7 @@ static int enqueue_md_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t
     const unsigned gt = (unsigned)((s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE);
     const int64_t lim = std::max(s.nmax_rep, s.ncmax_rep);
     const unsigned gg = (unsigned)std::min<int64_t>((lim + FT - 1) / FT, RED_BLOCKS);
-    // optional event nodes around the streaming kernel (its duration inside the graph)
-    if (ev0) RMPCD_HIP(hipEventRecordWithFlags(ev0, st, hipEventRecordExternal));
     k_fz_stream<F><<<dim3(gs, Rr), FT, 0, st>>>(b, p);
-    if (ev1) RMPCD_HIP(hipEventRecordWithFlags(ev1, st, hipEventRecordExternal));
     k_fz_list_a<F><<<dim3(gl, Rr), FT, 0, st>>>(b, p);
     k_fz_exact<F><<<dim3(gr, Rr), FT, 0, st>>>(b, p);
     k_fz_key<F><<<dim3(gr, Rr), FT, 0, st>>>(b, p, 0);
@@ -1078,6 +1074,79 @@ static void enqueue_collision(DimerSim &s, const FB<F> &b, const FP &p, hipStrea
     k_fz_collide<F><<<dim3(gc, Rr), CT, 0, st>>>(b, p);
 }
 
+// Event-record nodes around every prof_stride-th launch of the streaming kernel in the
+// captured cycle, so that its duration is measured inside the graph replays. The nodes
+// are visited in dependency order (Kahn); the streaming kernel is recognised by its
+// function handle.
+template <class F>
+static int add_stream_timers(DimerSim &s, hipGraph_t g)
+{
+    size_t nn = 0;
+    RMPCD_HIP(hipGraphGetNodes(g, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn), order;
+    RMPCD_HIP(hipGraphGetNodes(g, nodes.data(), &nn));
+    std::vector<size_t> indeg(nn);
+    std::vector<std::vector<size_t>> succ(nn);
+    auto index_of = [&](hipGraphNode_t x) {
+        for (size_t i = 0; i < nn; ++i)
+            if (nodes[i] == x) return i;
+        return nn;
+    };
+    for (size_t i = 0; i < nn; ++i) {
+        size_t nd = 0;
+        RMPCD_HIP(hipGraphNodeGetDependentNodes(nodes[i], nullptr, &nd));
+        std::vector<hipGraphNode_t> d(nd);
+        if (nd) RMPCD_HIP(hipGraphNodeGetDependentNodes(nodes[i], d.data(), &nd));
+        for (hipGraphNode_t x : d) {
+            size_t k = index_of(x);
+            if (k == nn) return set_error(RMPCD_ERR_HIP, "dimer: inconsistent captured graph");
+            succ[i].push_back(k);
+            ++indeg[k];
+        }
+    }
+    std::vector<size_t> ready;
+    for (size_t i = 0; i < nn; ++i)
+        if (!indeg[i]) ready.push_back(i);
+    while (!ready.empty()) {
+        size_t i = ready.front();
+        ready.erase(ready.begin());
+        order.push_back(nodes[i]);
+        for (size_t k : succ[i])
+            if (--indeg[k] == 0) ready.push_back(k);
+    }
+    const void *stream_fn = reinterpret_cast<const void *>(&k_fz_stream<F>);
+    size_t seen = 0, pair = 0;
+    for (hipGraphNode_t x : order) {
+        hipGraphNodeType t;
+        RMPCD_HIP(hipGraphNodeGetType(x, &t));
+        if (t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp;
+        RMPCD_HIP(hipGraphKernelNodeGetParams(x, &kp));
+        if (kp.func != stream_fn) continue;
+        if (seen++ % (size_t)s.prof_stride || 2 * pair + 1 >= s.prof_ev.size()) continue;
+        size_t nd = 0, nn2 = 0;
+        RMPCD_HIP(hipGraphNodeGetDependencies(x, nullptr, &nd));
+        std::vector<hipGraphNode_t> dep(nd);
+        if (nd) RMPCD_HIP(hipGraphNodeGetDependencies(x, dep.data(), &nd));
+        RMPCD_HIP(hipGraphNodeGetDependentNodes(x, nullptr, &nn2));
+        std::vector<hipGraphNode_t> out(nn2);
+        if (nn2) RMPCD_HIP(hipGraphNodeGetDependentNodes(x, out.data(), &nn2));
+        hipGraphNode_t e0, e1;
+        RMPCD_HIP(hipGraphAddEventRecordNode(&e0, g, dep.data(), nd, s.prof_ev[2 * pair]));
+        for (hipGraphNode_t d : dep) RMPCD_HIP(hipGraphRemoveDependencies(g, &d, &x, 1));
+        RMPCD_HIP(hipGraphAddDependencies(g, &e0, &x, 1));
+        RMPCD_HIP(hipGraphAddEventRecordNode(&e1, g, &x, 1, s.prof_ev[2 * pair + 1]));
+        for (hipGraphNode_t d : out) {
+            RMPCD_HIP(hipGraphRemoveDependencies(g, &x, &d, 1));
+            RMPCD_HIP(hipGraphAddDependencies(g, &e1, &d, 1));
+        }
+        ++pair;
+    }
+    if (!pair) return set_error(RMPCD_ERR_HIP, "dimer: streaming kernel not found in the captured graph");
+    s.prof_pairs = (int)pair;
+    return RMPCD_OK;
+}
+
 template <class F>
 static int build_graph(DimerSim &s)
 {
@@ -1094,13 +1163,7 @@ static int build_graph(DimerSim &s)
     }
     RMPCD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = RMPCD_OK;
-    size_t pair = 0;
-    for (int j = 0; j < s.cfg.n_md && rc == RMPCD_OK; ++j) {
-        bool timed = s.prof_stride > 0 && j % s.prof_stride == 0 && 2 * pair + 1 < s.prof_ev.size();
-        rc = enqueue_md_step<F>(s, b, p, st, timed ? s.prof_ev[2 * pair] : nullptr,
-                                timed ? s.prof_ev[2 * pair + 1] : nullptr);
-        if (timed) ++pair;
-    }
+    for (int j = 0; j < s.cfg.n_md && rc == RMPCD_OK; ++j) rc = enqueue_md_step<F>(s, b, p, st);
     enqueue_collision<F>(s, b, p, st);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
@@ -1110,7 +1173,8 @@ static int build_graph(DimerSim &s)
     }
     if (e != hipSuccess) return set_error(RMPCD_ERR_HIP, "dimer: graph capture failed: %s", hipGetErrorString(e));
     s.graph = g;
-    s.prof_pairs = (int)pair;
+    s.prof_pairs = 0;
+    if (s.prof_stride > 0 && (rc = add_stream_timers<F>(s, g))) return rc;
     RMPCD_HIP(hipGraphInstantiate(&s.graph_exec, g, nullptr, nullptr, 0));
     return RMPCD_OK;
 }

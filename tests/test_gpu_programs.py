@@ -61,14 +61,14 @@ def test_fluid_matches_oracle(gpu, n_md):
 def test_sphere_matches_oracle(gpu):
     from oracle.programs import SphereOracle
     L = (16, 16, 16)
-    kw = dict(L=L, n_md=5, tau=0.05, sigma=2.0, T=1.0 / 3.0, bulk_rate=0.5, prob=0.7)
+    kw = dict(L=L, n_md=10, tau=0.5, sigma=2.0, T=1.0, bulk_rate=0.05, prob=0.7)
     prog = Program(PROG_SPHERE, form=capi.FP64, seed=12, **kw)
     st = prog.download()
     # set-up: no solvent inside the sphere's cutoff, one draw-free first list
     cut = 2.0 * 2 ** (1 / 6)
     d = st["pos"] - st["colloid_pos"][0]
     assert ((d * d).sum(axis=1) >= cut * cut).all()
-    orc = SphereOracle(st, L, 0.05, 5, 12, sigma=2.0, prob=0.7, bulk_rate=0.5)
+    orc = SphereOracle(st, L, 0.5, 10, 12, sigma=2.0, prob=0.7, bulk_rate=0.05)
     assert (orc.nl == st["nlist"]).all()
     for cycle in range(3):
         prog.run(1)
