@@ -9,10 +9,13 @@ counts solvent particle-steps: N_solvent x N_MD per cycle, summed over replicas.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fp32|fp64]
                     [--mode fused|reference] [--L 64] [--replicas R] [--no-cpu]
 
-The JSON line carries the roofline of the dominant kernel (the streaming pass over all
-solvent, 9 words per particle: read position and velocity, write position), timed with
-HIP event nodes inside the captured graph during the timed cycles, and the CPU baseline:
-the oracle (FP64 C with OpenMP) running one cycle of the same workload on this host.
+The headline runs R independent L=64 systems (default R=16) in every launch, FP32-mixed,
+with the FP64 form and a single system alongside.  The JSON line carries the roofline
+of the dominant kernel, k_fz_step (the streaming pass over all solvent, 9 words per
+particle: read position and velocity, write position; the list members' force work
+rides along), timed with HIP event-record nodes inside the captured graph during the
+timed cycles, and the CPU baseline: the oracle (FP64 C with OpenMP) running one cycle of
+the single-system workload on this host.
 """
 from __future__ import annotations
 
@@ -40,10 +43,11 @@ def parse():
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--mode", choices=["fused", "reference"], default="fused")
     ap.add_argument("--L", type=int, default=64)
-    ap.add_argument("--replicas", type=int, default=1, help="independent L^3 systems in one launch")
+    ap.add_argument("--replicas", type=int, default=16, help="independent L^3 systems in every launch")
     ap.add_argument("--n-md", type=int, default=50)
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-replica run")
     return ap.parse_args()
 
 
@@ -147,7 +151,7 @@ def main():
         achieved = alg / (main_run["stream_ms"] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.form, a.L, a.replicas),
-                    "kernel": "k_fz_stream", "algorithmic_bytes_per_launch": alg,
+                    "kernel": "k_fz_step", "algorithmic_bytes_per_launch": alg,
                     "avg_launch_ms": main_run["stream_ms"], "launches_timed": main_run["stream_samples"]}
     line = {
         "metric": METRIC,
@@ -180,11 +184,16 @@ def main():
     if compare:
         line["fp64_form"] = {"value": compare["ps_per_s"], "ms_per_step": compare["ms_per_cycle"],
                              "mixed_over_fp64": main_run["ps_per_s"] / compare["ps_per_s"]}
+    if not a.no_single and a.replicas > 1:
+        single = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, 1, profile=False)
+        line["single_replica"] = {"value": single["ps_per_s"], "ms_per_step": single["ms_per_cycle"],
+                                  "n_solvent": single["n"]}
     # the whole step's algorithmic bytes per particle-step: 9w per MD step plus the sorts
-    n_sorts = (main_run["extra_sorts"] / max(1, main_run["cycles"]) + 1) / a.n_md
+    n_sorts = (main_run["extra_sorts"] / max(1, main_run["cycles"] * a.replicas) + 1) / a.n_md
     S = 38 if a.form == "fp32" else 68
     line["step_bytes_per_particle_step"] = 9 * w + n_sorts * (2 * S + 16 / 10)
     line["step_effective_GBs"] = line["step_bytes_per_particle_step"] * main_run["ps_per_s"] / 1e9
+    line["step_frac_of_hbm_peak"] = line["step_effective_GBs"] / HBM_PEAK_GBS
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.L, a.n_md)
     print(json.dumps(line))

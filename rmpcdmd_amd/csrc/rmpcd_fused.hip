@@ -2,30 +2,23 @@
 // kernels over every replica at once (blockIdx.y = replica), captured as one hipGraph
 // per collision cycle.
 //
-// Per MD step (programs/single_dimer_pbc.f90:306-383):
-//   stream    all solvent: x += dt v (+ dt^2/2 f where the MD bit says a force exists)
-//   list_a    neighbour-list members: zero the next force buffer and roll the MD bits
-//             (each particle once, by an epoch claim), exact displacement of the members;
-//             the last block moves the colloids (Verlet + RATTLE) and takes the skin
-//             decision on the device
-//   exact     only when the bound below is inconclusive: the exact maximum displacement
-//   sort      only when the skin test fires: key+count, scan, place, stable gather, list
-//   force     colloid-solvent LJ over the list; the last block adds the colloid forces,
-//             colloid-colloid LJ, colloid Verlet and RATTLE
-//   list_b    list members: velocity Verlet, catalytic flagging and conversion
-// Per collision: the same sort with a fresh grid shift, then SRD + refuel (or the bulk
-// reaction) per cell.
+// Per MD step (programs/single_dimer_pbc.f90:306-383), one kernel k_fz_step: the
+// neighbour-list members finish the previous step (forces, md_vel, catalytic flagging
+// and conversion) and stream; every other particle streams; the last block finishes the
+// colloids and takes the skin decision.  Then, only when the skin test fires, the sort:
+// key+count, scan, place, stable gather, list.  Per collision: the same sort with a
+// fresh grid shift, then SRD + refuel (or the bulk reaction) per cell.
 //
-// Why list-only work is exact: forces only reach particles in a colloid's list, the
+// Why member-only work is exact: forces only reach particles in a colloid's list, the
 // list keeps every particle within max_cut + skin, and the skin test rebuilds it before
 // any particle that carried a force, an MD/PAST_MD bit or the REAC bit can leave it, so
 // the reference's full passes (md_vel, the MD-bit roll, force zeroing, change_species)
 // leave every other particle unchanged.  Skin decision without a full pass: between two
-// sorts a particle outside the list moves ballistically, so its displacement after k
+// sorts a particle outside the lists moves ballistically, so its displacement after k
 // steps is at most k (dt |v|max (1 + 1e-6) + rounding slack), with |v|max taken over all
 // particles at the last sort or collision; list members are measured exactly.  When
 // that bound stays below the threshold the reference's test is false too; otherwise the
-// exact pass decides.
+// stream role measures every particle exactly in the same pass.
 #include <math.h>
 #include <string.h>
 
@@ -37,20 +30,23 @@ namespace rmpcd {
 
 constexpr int FT = 256;            // threads per block
 constexpr int PS = 32768;          // partial-sum slots per replica
-constexpr int RED_BLOCKS = 1024;   // blocks per replica for reductions over particles
-constexpr int STREAM_BLOCKS = 8192;
+constexpr int STREAM_BLOCKS = 16384;  // stream-role blocks per replica (grid-stride beyond)
+constexpr int SORT_BLOCKS = 4096;     // blocks per replica of the particle passes of the sort
 constexpr int CT = 128;            // cells per block in the collision kernel
 constexpr int CTILE = 2048;        // staged particles per collision block
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
+constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
+constexpr int KPT = 4, KCHUNK = FT * KPT;  // particles per thread and per chunk of the key pass
+constexpr int KHASH_BITS = 11, KHASH = 1 << KHASH_BITS;  // LDS hash entries, 2 x KCHUNK
 
-enum { D_LISTA = 0, D_EXACT, D_KEY, D_FORCE, D_LISTB, D_GATHER, D_COLLIDE };
 
 template <class F>
 struct FB {
     typename F::real *pos[2], *vel[2], *force[4], *pos_old;
     int32_t *image[2], *id[2];
     typename F::small *species[2], *flags[2];
-    int32_t *id_to_idx, *claim, *key, *slot, *tmp;
+    int32_t *id_to_idx, *key, *slot, *tmp;
+    uint8_t *memb;
     int32_t *count[2], *start;
     uint8_t *is_md, *is_reac;
     int32_t *list, *nlist;
@@ -58,8 +54,10 @@ struct FB {
     double *cpos, *cpos_old, *cpos_rattle, *cvel, *cforce[2], *cmass;
     int32_t *cimage, *cspecies;
     double *part;
-    int32_t *scan_sums;
+    int32_t *scan_sums, *slocal;
     int64_t scan_stride;
+    int32_t *mslots, *mcount;  // per replica and sort buffer: the member slots of its last lists
+    int32_t nmax;
     const RepDev *rep;
     CtlDev *ctl;
 };
@@ -71,18 +69,6 @@ struct FP {
     double radius_sq;
     LJ sc, cc;
 };
-
-// last-block bookkeeping: every block stores its partial, the last one to arrive sees them
-__device__ inline bool arrive_last(uint32_t *counter, unsigned nblocks, bool *s_last)
-{
-    if (threadIdx.x == 0) {
-        __threadfence();
-        *s_last = atomicAdd(counter, 1u) == nblocks - 1;
-    }
-    __syncthreads();
-    if (*s_last) __threadfence();
-    return *s_last;
-}
 
 __device__ inline int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
 __device__ inline int64_t gindex() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
@@ -132,168 +118,325 @@ __device__ inline void rattle_vel_2(const double *cp, double *cv, const int32_t 
     }
 }
 
-// ---- stream ------------------------------------------------------------------------
-template <class F>
-__global__ void __launch_bounds__(FT) k_fz_stream(FB<F> B, FP P)
-{
-    using R = typename F::real;
-    const int r = blockIdx.y;
-    const RepDev &rp = B.rep[r];
-    const int cur = B.ctl[r].cur, fp = B.ctl[r].fpar;
-    const int64_t n = rp.n;
-    R *__restrict__ pos = B.pos[cur] + 3 * rp.p_off;
-    const R *__restrict__ vel = B.vel[cur] + 3 * rp.p_off;
-    const R *__restrict__ f = B.force[2 * cur + fp] + 3 * rp.p_off;
-    const typename F::small *__restrict__ fl = B.flags[cur] + rp.p_off;
-    const R rdt = (R)P.dt, dsq = (R)P.dt_sq;
-    for (int64_t i = gindex(); i < n; i += gstride()) {
-        const bool md = ((int)fl[i] & MD_MASK) != 0;
-        R x0 = pos[i] + rdt * vel[i];
-        R x1 = pos[n + i] + rdt * vel[n + i];
-        R x2 = pos[2 * n + i] + rdt * vel[2 * n + i];
-        if (md) {
-            x0 = x0 + dsq * f[i];
-            x1 = x1 + dsq * f[n + i];
-            x2 = x2 + dsq * f[2 * n + i];
-        }
-        pos[i] = x0;
-        pos[n + i] = x1;
-        pos[2 * n + i] = x2;
-    }
-}
+// ---- the MD step kernel ---------------------------------------------------------------
+// Blocks [0, nbm) take the entries of the two neighbour lists (member role: a particle in
+// both lists is taken by its colloid-0 entry, so each member is handled by one thread);
+// the other blocks stream every solvent particle that is not a member (stream role).
+//   mode 0  first step of a cycle: the members stream with the force of the last step
+//   mode 1  the members finish the previous step -- the MD-bit roll and LJ forces of
+//           compute_force (neighbor_list.f90:180-222), md_vel (md.f90:219-237),
+//           flag_particles_nl and change_species (single_dimer_pbc.f90:493-542) -- and
+//           stream for this step (md_pos, md.f90:61-77); the non-members stream
+//   mode 2  the members finish the previous step only (the last step before a collision)
+// The last block to finish completes the colloids' share of the previous step (force
+// sums, compute_force_n2, Verlet velocity, RATTLE), moves them for this step (Verlet
+// position, RATTLE) and takes the skin decision (single_dimer_pbc.f90:307-361).
+// Member partials: MPV values per member block (colloid-0 force, energy, colloid-1
+// force, energy, maximum squared displacement); stream-role maxima follow at MOFF.
+constexpr int MPV = 9;
+constexpr int MAX_MEMBER_BLOCKS = 256;  // member blocks per replica (lists of up to 32768 entries each)
 
-// ---- list_a: claims, force zeroing, MD roll, member displacement; colloid step ------
 template <class F>
-__global__ void __launch_bounds__(FT) k_fz_list_a(FB<F> B, FP P)
+__global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm)
 {
     using R = typename F::real;
+    using S = typename F::small;
     __shared__ double red[16];
-    __shared__ bool s_last;
     const int r = blockIdx.y;
     const RepDev &rp = B.rep[r];
     CtlDev &C = B.ctl[r];
     const int cur = C.cur, fp = C.fpar;
-    const int64_t n = rp.n;
-    const int e = blockIdx.x * FT + threadIdx.x;
-    double r2 = 0;
-    if (e < 2 * P.nmax) {
+    const bool exact = C.need_exact != 0;
+    const int64_t n = rp.n, po3 = 3 * rp.p_off;
+    R *__restrict__ pos = B.pos[cur] + po3;
+    R *__restrict__ vel = B.vel[cur] + po3;
+    const R *__restrict__ po = B.pos_old + po3;
+    const uint8_t *__restrict__ memb = B.memb + rp.p_off;
+    const R rdt = (R)P.dt, dsq = (R)P.dt_sq;
+    double *part = B.part + (int64_t)r * PS;
+    const int moff = nbm * MPV;
+    if ((int)blockIdx.x < nbm) {
+        double c[MPV] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int e = blockIdx.x * FT + threadIdx.x;
         const int col = e / P.nmax, k = e - col * P.nmax;
-        const int nl = min(B.nlist[2 * r + col], P.nmax);
-        if (k < nl) {
+        if (col < 2 && k < min(B.nlist[2 * r + col], P.nmax)) {
             const int idx = B.list[(int64_t)(2 * r + col) * P.nmax + k];
-            const unsigned epoch = (unsigned)(C.md_step * 4 + 1);
-            if (atomicExch((unsigned *)&B.claim[rp.p_off + idx], epoch) != epoch) {
-                R *fn = B.force[2 * cur + (1 - fp)] + 3 * rp.p_off;
-                fn[idx] = 0;
-                fn[n + idx] = 0;
-                fn[2 * n + idx] = 0;
-                typename F::small *fl = B.flags[cur] + rp.p_off;
-                int f = (int)fl[idx];
-                f = (f & MD_MASK) ? (f | PAST_MD_MASK) : (f & ~PAST_MD_MASK);
-                fl[idx] = (typename F::small)(f & ~MD_MASK);
+            const int mb = memb[idx];
+            if (col == 0 || !(mb & 1)) {
+                R *fnew = B.force[2 * cur + (1 - fp)] + po3;
+                const R *fcur = B.force[2 * cur + fp] + po3;
+                R x[3] = {pos[idx], pos[n + idx], pos[2 * n + idx]};
+                R v[3] = {vel[idx], vel[n + idx], vel[2 * n + idx]};
+                R fa[3] = {fcur[idx], fcur[n + idx], fcur[2 * n + idx]};
+                if (mode >= 1) {
+                    S *spc = B.species[cur] + rp.p_off;
+                    S *flg = B.flags[cur] + rp.p_off;
+                    const double *cp = B.cpos + 6 * r;
+                    const double *L = rp.box.edges;
+                    const int s2 = (int)spc[idx];
+                    int fl = (int)flg[idx];
+                    fl = (fl & MD_MASK) ? (fl | PAST_MD_MASK) : (fl & ~PAST_MD_MASK);
+                    fl &= ~MD_MASK;
+                    const double y[3] = {(double)x[0], (double)x[1], (double)x[2]};
+                    double dsq_c[2] = {0, 0};
+                    R fn[3] = {0, 0, 0};
+                    for (int cc = 0; cc < 2; ++cc) {
+                        const double d0 = min_image(cp[cc] - y[0], L[0]);
+                        const double d1 = min_image(cp[2 + cc] - y[1], L[1]);
+                        const double d2 = min_image(cp[4 + cc] - y[2], L[2]);
+                        const double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
+                        dsq_c[cc] = r_sq;
+                        if (!(mb & (1 << cc)) || s2 <= 0) continue;
+                        const int s1 = B.cspecies[2 * r + cc];
+                        const int pidx = (s2 - 1) + P.sc.n1 * (s1 - 1);
+                        if (r_sq <= P.sc.cut_sq[pidx]) {
+                            const double fr = lj_force_over_r(r_sq, P.sc.eps[pidx], P.sc.sigma[pidx]);
+                            const double g0 = fr * d0, g1 = fr * d1, g2 = fr * d2;
+                            fn[0] = fn[0] + (R)(-g0);
+                            fn[1] = fn[1] + (R)(-g1);
+                            fn[2] = fn[2] + (R)(-g2);
+                            fl |= MD_MASK | (s1 == 1 ? CATALYZED_MASK : 0);
+                            c[4 * cc] = g0;
+                            c[4 * cc + 1] = g1;
+                            c[4 * cc + 2] = g2;
+                            c[4 * cc + 3] = lj_energy(r_sq, P.sc.eps[pidx], P.sc.sigma[pidx], P.sc.cut_energy[pidx]);
+                        }
+                    }
+                    fnew[idx] = fn[0];
+                    fnew[n + idx] = fn[1];
+                    fnew[2 * n + idx] = fn[2];
+                    if (!(fl & WALL_MASK)) {
+                        for (int q = 0; q < 3; ++q) v[q] = v[q] + (rdt * (fn[q] + fa[q])) / (R)2;
+                        vel[idx] = v[0];
+                        vel[n + idx] = v[1];
+                        vel[2 * n + idx] = v[2];
+                    } else {
+                        fl &= ~WALL_MASK;
+                    }
+                    const double cut_c = P.sc.cut_sq[0], cut_n = P.sc.cut_sq[2];
+                    int snew = s2;
+                    if ((mb & 1) && s2 == 1 && dsq_c[0] < cut_c) fl |= REAC_MASK;
+                    if ((fl & REAC_MASK) && dsq_c[0] > cut_c && dsq_c[1] > cut_n) {
+                        // one draw per conversion candidate: threefry((md_step, id), (seed, CHANGE))
+                        uint64_t u0, u1;
+                        threefry2x64((uint64_t)C.md_step, (uint64_t)B.id[cur][rp.p_off + idx], rp.seed,
+                                     STREAM_CHANGE, u0, u1);
+                        if (u01(u0) <= P.prob) snew = 2;
+                        fl &= ~REAC_MASK;
+                    }
+                    if (snew != s2) spc[idx] = (S)snew;
+                    flg[idx] = (S)fl;
+                    fa[0] = fn[0];
+                    fa[1] = fn[1];
+                    fa[2] = fn[2];
+                }
+                if (mode <= 1) {
+                    for (int q = 0; q < 3; ++q) x[q] = (x[q] + rdt * v[q]) + dsq * fa[q];
+                    pos[idx] = x[0];
+                    pos[n + idx] = x[1];
+                    pos[2 * n + idx] = x[2];
+                    const double d0 = (double)(x[0] - po[idx]);
+                    const double d1 = (double)(x[1] - po[n + idx]);
+                    const double d2 = (double)(x[2] - po[2 * n + idx]);
+                    c[8] = (d0 * d0 + d1 * d1) + d2 * d2;
+                }
             }
-            const R *pos = B.pos[cur] + 3 * rp.p_off;
-            const R *po = B.pos_old + 3 * rp.p_off;
-            double d0 = (double)(pos[idx] - po[idx]);
-            double d1 = (double)(pos[n + idx] - po[n + idx]);
-            double d2 = (double)(pos[2 * n + idx] - po[2 * n + idx]);
-            r2 = (d0 * d0 + d1 * d1) + d2 * d2;
         }
-    }
-    double m = block_max(r2, red);
-    if (threadIdx.x == 0) B.part[(int64_t)r * PS + blockIdx.x] = m;
-    if (!arrive_last(&C.done[D_LISTA], gridDim.x, &s_last)) return;
-    if (threadIdx.x != 0) return;
-    double lm = 0;
-    for (unsigned q = 0; q < gridDim.x; ++q) lm = fmax(lm, B.part[(int64_t)r * PS + q]);
-    C.done[D_LISTA] = 0;
-    C.list_max2 = lm;
-    // colloid Verlet position step and RATTLE (single_dimer_pbc.f90:311-317)
-    double *cp = B.cpos + 6 * r, *cv = B.cvel + 6 * r, *cr = B.cpos_rattle + 6 * r, *co = B.cpos_old + 6 * r;
-    const double *cf = B.cforce[C.cfpar] + 6 * r;
-    const double *cm = B.cmass + 2 * r;
-    const int32_t *cs = B.cspecies + 2 * r;
-    const double dt = P.dt;
-    for (int q = 0; q < 6; ++q) cr[q] = cp[q];
-    for (int i = 0; i < 2; ++i)
-        for (int k = 0; k < 3; ++k) {
-            int q = 2 * k + i;
-            cp[q] = (cp[q] + dt * cv[q]) + ((dt * dt) * cf[q]) / (2 * cm[i]);
+        // nine reductions at once: wave shuffles, then one exchange through LDS
+        __shared__ double wred[FT / 64][MPV];
+        for (int off = 32; off > 0; off >>= 1) {
+            for (int q = 0; q < 8; ++q) c[q] += __shfl_xor(c[q], off, 64);
+            const double o = __shfl_xor(c[8], off, 64);
+            c[8] = o > c[8] ? o : c[8];
         }
-    rattle_pos_2(cp, cv, cr, cs, P.mass, P.d, dt, rp.box.edges);
-    // maximum displacements and the skin test (single_dimer_pbc.f90:320-323)
-    double cmax = 0;
-    for (int i = 0; i < 2; ++i) {
-        double d0 = cp[i] - co[i], d1 = cp[2 + i] - co[2 + i], d2 = cp[4 + i] - co[4 + i];
-        double q2 = (d0 * d0 + d1 * d1) + d2 * d2;
-        cmax = q2 > cmax ? q2 : cmax;
-    }
-    double co_max = sqrt(cmax);
-    C.co_max = co_max;
-    int k = C.k_since + 1;
-    double bound = (double)k * (P.dt * sqrt(C.vmax2) * (1.0 + 1e-6) + P.slack);
-    double so_est = fmax(sqrt(lm), bound);
-    if (co_max >= P.skin * 0.1) {
-        C.need_sort = 1;
-        C.need_exact = 0;
-    } else if (so_est < P.skin * 0.89) {
-        C.need_sort = 0;
-        C.need_exact = 0;
+        if ((threadIdx.x & 63) == 0)
+            for (int q = 0; q < MPV; ++q) wred[threadIdx.x >> 6][q] = c[q];
+        __syncthreads();
+        if (threadIdx.x < MPV) {
+            double t = wred[0][threadIdx.x];
+            for (int w = 1; w < FT / 64; ++w) {
+                const double u = wred[w][threadIdx.x];
+                t = threadIdx.x == 8 ? (u > t ? u : t) : t + u;
+            }
+            part[blockIdx.x * MPV + threadIdx.x] = t;
+        }
     } else {
-        C.need_sort = 0;
-        C.need_exact = 1;
+        const int64_t gs = (int64_t)(gridDim.x - nbm) * FT;
+        const S *spc = B.species[cur] + rp.p_off;
+        double mx = 0;
+        for (int64_t i = (int64_t)(blockIdx.x - nbm) * FT + threadIdx.x; i < n; i += gs) {
+            if (memb[i]) continue;
+            const R x0 = pos[i] + rdt * vel[i];
+            const R x1 = pos[n + i] + rdt * vel[n + i];
+            const R x2 = pos[2 * n + i] + rdt * vel[2 * n + i];
+            pos[i] = x0;
+            pos[n + i] = x1;
+            pos[2 * n + i] = x2;
+            if (exact && (int)spc[i] >= 0) {
+                const double d0 = (double)(x0 - po[i]);
+                const double d1 = (double)(x1 - po[n + i]);
+                const double d2 = (double)(x2 - po[2 * n + i]);
+                const double q2 = (d0 * d0 + d1 * d1) + d2 * d2;
+                mx = q2 > mx ? q2 : mx;
+            }
+        }
+        if (exact) {
+            const double m = block_max(mx, red);
+            if (threadIdx.x == 0) part[moff + (blockIdx.x - nbm)] = m;
+        }
     }
-    // switch(force, force_old) and zero the new colloid force (single_dimer_pbc.f90:341-348)
-    C.fpar ^= 1;
-    C.cfpar ^= 1;
-    double *cfn = B.cforce[C.cfpar] + 6 * r;
-    for (int q = 0; q < 6; ++q) cfn[q] = 0;
 }
 
-// ---- exact maximum displacement (particle_system.f90:357-378) -----------------------
+// The colloids' share of the step, one block per replica after k_fz_step: the member
+// partial sums in block order (deterministic), the stream-role maxima, then the
+// colloid force, compute_force_n2, Verlet velocity and RATTLE of the previous step and
+// the Verlet position step, RATTLE and skin decision of this one.
 template <class F>
-__global__ void __launch_bounds__(FT) k_fz_exact(FB<F> B, FP P)
+__global__ void __launch_bounds__(1024) k_fz_colloid(FB<F> B, FP P, int mode, int nbm, int nstream)
 {
-    using R = typename F::real;
     __shared__ double red[16];
-    __shared__ bool s_last;
+    __shared__ double acc[MPV];
+    __shared__ double lp[MAX_MEMBER_BLOCKS * MPV];
     const int r = blockIdx.y;
-    CtlDev &C = B.ctl[r];
-    if (!C.need_exact) return;
     const RepDev &rp = B.rep[r];
-    const int64_t n = rp.n;
-    const int cur = C.cur;
-    const R *pos = B.pos[cur] + 3 * rp.p_off, *po = B.pos_old + 3 * rp.p_off;
-    const typename F::small *sp = B.species[cur] + rp.p_off;
-    double mx = 0;
-    for (int64_t i = gindex(); i < n; i += gstride()) {
-        if ((int)sp[i] >= 0) {
-            double d0 = (double)(pos[i] - po[i]);
-            double d1 = (double)(pos[n + i] - po[n + i]);
-            double d2 = (double)(pos[2 * n + i] - po[2 * n + i]);
-            double q2 = (d0 * d0 + d1 * d1) + d2 * d2;
-            mx = q2 > mx ? q2 : mx;
+    CtlDev &C = B.ctl[r];
+    const bool exact = C.need_exact != 0;
+    const double *part = B.part + (int64_t)r * PS;
+    const int moff = nbm * MPV;
+    for (int q = threadIdx.x; q < nbm * MPV; q += blockDim.x) lp[q] = part[q];
+    __syncthreads();
+    if (threadIdx.x < MPV) {
+        double a = 0;
+        for (int b = 0; b < nbm; ++b) {
+            const double t = lp[b * MPV + threadIdx.x];
+            if (threadIdx.x == 8)
+                a = t > a ? t : a;
+            else
+                a = a + t;
+        }
+        acc[threadIdx.x] = a;
+    }
+    double smax = 0;
+    if (exact && mode <= 1) {
+        for (int b = threadIdx.x; b < nstream; b += blockDim.x) {
+            const double t = part[moff + b];
+            smax = t > smax ? t : smax;
         }
     }
-    double m = block_max(mx, red);
-    if (threadIdx.x == 0) B.part[(int64_t)r * PS + blockIdx.x] = m;
-    if (!arrive_last(&C.done[D_EXACT], gridDim.x, &s_last)) return;
+    smax = block_max(smax, red);
     if (threadIdx.x != 0) return;
-    double all = 0;
-    for (unsigned q = 0; q < gridDim.x; ++q) all = fmax(all, B.part[(int64_t)r * PS + q]);
-    C.done[D_EXACT] = 0;
-    C.need_sort = sqrt(all) >= P.skin * 0.89 ? 1 : 0;
-    C.need_exact = 0;
+    // the colloid state in registers: one batch of loads, one batch of stores
+    const double *L = rp.box.edges;
+    const double dt = P.dt;
+    const int cfpar = C.cfpar, fpar = C.fpar, k_since = C.k_since;
+    const int64_t md_step = C.md_step;
+    const double vmax2 = C.vmax2;
+    double cp[6], cv[6], cf[6], cfo[6], co[6], cm[2];
+    int32_t cs[2];
+    for (int q = 0; q < 6; ++q) {
+        cp[q] = B.cpos[6 * r + q];
+        cv[q] = B.cvel[6 * r + q];
+        cf[q] = B.cforce[cfpar][6 * r + q];
+        cfo[q] = B.cforce[cfpar ^ 1][6 * r + q];
+        co[q] = B.cpos_old[6 * r + q];
+    }
+    cm[0] = B.cmass[2 * r];
+    cm[1] = B.cmass[2 * r + 1];
+    cs[0] = B.cspecies[2 * r];
+    cs[1] = B.cspecies[2 * r + 1];
+    int ncf = cfpar;
+    if (mode >= 1) {
+        // switch(force, force_old) of the colloids; the new force is the list sum
+        // (single_dimer_pbc.f90:341-352), then compute_force_n2 (neighbor_list.f90:243-262)
+        ncf = cfpar ^ 1;
+        for (int q = 0; q < 6; ++q) cfo[q] = cf[q];
+        for (int cc = 0; cc < 2; ++cc) {
+            cf[cc] = 0.0 + acc[4 * cc];
+            cf[2 + cc] = 0.0 + acc[4 * cc + 1];
+            cf[4 + cc] = 0.0 + acc[4 * cc + 2];
+        }
+        double e1 = 0;
+        e1 = e1 + acc[3];
+        e1 = e1 + acc[7];
+        C.e1 = e1;
+        {
+            const double d0 = min_image(cp[0] - cp[1], L[0]);
+            const double d1 = min_image(cp[2] - cp[3], L[1]);
+            const double d2 = min_image(cp[4] - cp[5], L[2]);
+            const double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
+            const int sa = cs[0], sb = cs[1];
+            const int pc = (sa - 1) + P.cc.n1 * (sb - 1), pe = (sb - 1) + P.cc.n1 * (sa - 1);
+            double e2 = 0, e0 = 0;
+            if (r_sq <= P.cc.cut_sq[pc]) {
+                const double fr = lj_force_over_r(r_sq, P.cc.eps[pe], P.cc.sigma[pe]);
+                cf[0] = cf[0] + fr * d0;
+                cf[2] = cf[2] + fr * d1;
+                cf[4] = cf[4] + fr * d2;
+                cf[1] = cf[1] - fr * d0;
+                cf[3] = cf[3] - fr * d1;
+                cf[5] = cf[5] - fr * d2;
+                e0 = e0 + lj_energy(r_sq, P.cc.eps[pe], P.cc.sigma[pe], P.cc.cut_energy[pe]);
+            }
+            e2 = e2 + e0;
+            e2 = e2 + 0.0;
+            C.e2 = e2;
+        }
+        // colloid velocity step and RATTLE (single_dimer_pbc.f90:356-361)
+        for (int i = 0; i < 2; ++i)
+            for (int q3 = 0; q3 < 3; ++q3) {
+                const int q = 2 * q3 + i;
+                cv[q] = cv[q] + (dt * (cf[q] + cfo[q])) / (2 * cm[i]);
+            }
+        rattle_vel_2(cp, cv, cs, P.mass, P.d, L);
+        for (int q = 0; q < 6; ++q) B.cforce[ncf][6 * r + q] = cf[q];
+        C.cfpar = ncf;
+        C.fpar = fpar ^ 1;
+        C.md_step = md_step + 1;
+    }
+    if (mode <= 1) {
+        // colloid Verlet position step and RATTLE (single_dimer_pbc.f90:311-317), with the
+        // force of the last completed step
+        double cr[6];
+        for (int q = 0; q < 6; ++q) cr[q] = cp[q];
+        for (int i = 0; i < 2; ++i)
+            for (int q3 = 0; q3 < 3; ++q3) {
+                const int q = 2 * q3 + i;
+                cp[q] = (cp[q] + dt * cv[q]) + ((dt * dt) * cf[q]) / (2 * cm[i]);
+            }
+        rattle_pos_2(cp, cv, cr, cs, P.mass, P.d, dt, L);
+        for (int q = 0; q < 6; ++q) B.cpos_rattle[6 * r + q] = cr[q];
+        // maximum displacements and the skin test (single_dimer_pbc.f90:320-323)
+        double cmax = 0;
+        for (int i = 0; i < 2; ++i) {
+            const double d0 = cp[i] - co[i], d1 = cp[2 + i] - co[2 + i], d2 = cp[4 + i] - co[4 + i];
+            const double q2 = (d0 * d0 + d1 * d1) + d2 * d2;
+            cmax = q2 > cmax ? q2 : cmax;
+        }
+        const double co_max = sqrt(cmax);
+        C.co_max = co_max;
+        const int k = k_since + 1;
+        const double per = P.dt * sqrt(vmax2) * (1.0 + 1e-6) + P.slack;
+        const double mem = acc[8];
+        C.list_max2 = mem;
+        const double so_max = exact ? sqrt(mem > smax ? mem : smax) : fmax(sqrt(mem), (double)k * per);
+        C.need_sort = (co_max >= P.skin * 0.1 || so_max >= P.skin * 0.89) ? 1 : 0;
+        C.k_since = k;
+        // is the ballistic bound conclusive for the next step?
+        C.need_exact = (double)(k + 1) * per >= P.skin * 0.89 ? 1 : 0;
+    }
+    for (int q = 0; q < 6; ++q) {
+        B.cpos[6 * r + q] = cp[q];
+        B.cvel[6 * r + q] = cv[q];
+    }
 }
 
 // ---- sort ---------------------------------------------------------------------------
-// S1: grid shift (collision sort), periodic wrap, key and arrival slot; the last block
-// wraps the colloids and records their positions as pos_old
+// S1: grid shift (collision sort), periodic wrap, key and arrival slot; block 0 wraps
+// the colloids and records their positions as pos_old
 template <class F>
 __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced)
 {
     using R = typename F::real;
-    __shared__ bool s_last;
     const int r = blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (!forced && !C.need_sort) return;
@@ -316,29 +459,65 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced)
     const R *pos = B.pos[cur] + 3 * rp.p_off;
     int32_t *cnt = B.count[C.cnt ^ 1] + rp.c_off;
     const int32_t *lut = B.lut + rp.lut_off;
-    for (int64_t i = gindex(); i < n; i += gstride()) {
-        R x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
-        for (int k = 0; k < 3; ++k) {
-            R e = (R)b.edges[k];
-            int jump = (int)floor((double)(x[k] / e));
-            x[k] = x[k] - (R)jump * e;
+    // Chunks of KCHUNK consecutive particles: the keys of a chunk are counted in an LDS
+    // hash table, then one global atomic per distinct key reserves the chunk's slots in
+    // that cell (arrival order within a cell is made stable by the gather).
+    __shared__ int hkey[KHASH], hcnt[KHASH];
+    for (int64_t c0 = (int64_t)blockIdx.x * KCHUNK; c0 < n; c0 += (int64_t)gridDim.x * KCHUNK) {
+        for (int q = threadIdx.x; q < KHASH; q += FT) {
+            hkey[q] = -1;
+            hcnt[q] = 0;
         }
-        int key;
-        {
-            int p0, p1, p2;
-            cell_of<R>(b, x[0], x[1], x[2], p0, p1, p2);
-            if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
-                (unsigned)p2 < (unsigned)b.L[2])
-                key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
-            else
-                key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+        __syncthreads();
+        int keys[KPT], ent[KPT], loc[KPT];
+        for (int u = 0; u < KPT; ++u) {
+            const int64_t i = c0 + u * FT + threadIdx.x;
+            keys[u] = -1;
+            ent[u] = 0;
+            loc[u] = 0;
+            if (i >= n) continue;
+            R x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+            for (int k = 0; k < 3; ++k) {
+                R e = (R)b.edges[k];
+                int jump = (int)floor((double)(x[k] / e));
+                x[k] = x[k] - (R)jump * e;
+            }
+            int key;
+            {
+                int p0, p1, p2;
+                cell_of<R>(b, x[0], x[1], x[2], p0, p1, p2);
+                if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
+                    (unsigned)p2 < (unsigned)b.L[2])
+                    key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+                else
+                    key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+            }
+            keys[u] = key;
+            // open addressing; the table has twice as many entries as a chunk has keys
+            unsigned h = ((unsigned)key * 2654435761u) >> (32 - KHASH_BITS);
+            for (int probe = 0; probe < KHASH; ++probe) {
+                const int prev = atomicCAS(&hkey[h], -1, key);
+                if (prev == -1 || prev == key) break;
+                h = (h + 1) & (KHASH - 1);
+            }
+            ent[u] = (int)h;
+            loc[u] = atomicAdd(&hcnt[h], 1);
         }
-        B.key[rp.p_off + i] = key;
-        B.slot[rp.p_off + i] = atomicAdd(&cnt[key], 1);
+        __syncthreads();
+        for (int q = threadIdx.x; q < KHASH; q += FT)
+            if (hkey[q] >= 0) hcnt[q] = atomicAdd(&cnt[hkey[q]], hcnt[q]);
+        __syncthreads();
+        for (int u = 0; u < KPT; ++u) {
+            const int64_t i = c0 + u * FT + threadIdx.x;
+            if (keys[u] < 0) continue;
+            B.key[rp.p_off + i] = keys[u];
+            B.slot[rp.p_off + i] = hcnt[ent[u]] + loc[u];
+        }
+        __syncthreads();
     }
-    if (!arrive_last(&C.done[D_KEY], gridDim.x, &s_last)) return;
-    if (threadIdx.x != 0) return;
-    C.done[D_KEY] = 0;
+    // block 0 records the grid shift and wraps the colloids; the other blocks computed the
+    // same origin and read no colloid data
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int k = 0; k < 3; ++k) C.origin[k] = b.origin[k];
     // apply_pbc(colloids) (md.f90:202-217) and colloids%pos_old = colloids%pos
     double *cp = B.cpos + 6 * r, *co = B.cpos_old + 6 * r;
@@ -373,6 +552,8 @@ __device__ inline int scan_block_exclusive(int v, int *lds, int *total)
     return off + inc - v;
 }
 
+// S2: exclusive scan of the new cell counts inside tiles of SCAN_TILE cells (tile-local
+// offsets into slocal, tile totals into scan_sums)
 template <class F>
 __global__ void __launch_bounds__(SCAN_T) k_fz_scan_tiles(FB<F> B, int forced)
 {
@@ -385,7 +566,7 @@ __global__ void __launch_bounds__(SCAN_T) k_fz_scan_tiles(FB<F> B, int forced)
     const int64_t tile0 = (int64_t)blockIdx.x * SCAN_TILE;
     if (tile0 >= nc) return;  // uniform per block
     const int32_t *in = B.count[C.cnt ^ 1] + rp.c_off;
-    int32_t *out = B.start + rp.c_off;
+    int32_t *out = B.slocal + rp.c_off;
     int64_t base = tile0 + (int64_t)threadIdx.x * SCAN_I;
     int v[SCAN_I], s = 0;
     for (int k = 0; k < SCAN_I; ++k) {
@@ -401,65 +582,79 @@ __global__ void __launch_bounds__(SCAN_T) k_fz_scan_tiles(FB<F> B, int forced)
     if (threadIdx.x == 0) B.scan_sums[r * B.scan_stride + blockIdx.x] = tot;
 }
 
-template <class F>
-__global__ void __launch_bounds__(1024) k_fz_scan_top(FB<F> B, int forced)
-{
-    __shared__ int lds[16];
-    const int r = blockIdx.y;
-    const CtlDev &C = B.ctl[r];
-    if (!sort_active(C, forced)) return;
-    const int64_t ntiles = (B.rep[r].box.ncells + SCAN_TILE - 1) / SCAN_TILE;
-    int32_t *sums = B.scan_sums + r * B.scan_stride;
-    int carry = 0;
-    for (int64_t c0 = 0; c0 < ntiles; c0 += blockDim.x) {
-        int64_t i = c0 + threadIdx.x;
-        int v = i < ntiles ? sums[i] : 0;
-        int tot;
-        int ex = scan_block_exclusive(v, lds, &tot);
-        if (i < ntiles) sums[i] = ex + carry;
-        carry += tot;
-    }
-}
-
-template <class F>
-__global__ void __launch_bounds__(SCAN_T) k_fz_scan_add(FB<F> B, int forced)
-{
-    const int r = blockIdx.y;
-    const CtlDev &C = B.ctl[r];
-    if (!sort_active(C, forced)) return;
-    const RepDev &rp = B.rep[r];
-    const int64_t nc = rp.box.ncells;
-    const int64_t tile0 = (int64_t)blockIdx.x * SCAN_TILE;
-    if (tile0 >= nc) return;
-    int32_t *out = B.start + rp.c_off;
-    const int add = B.scan_sums[r * B.scan_stride + blockIdx.x];
-    for (int k = threadIdx.x; k < SCAN_TILE; k += SCAN_T)
-        if (tile0 + k < nc) out[tile0 + k] += add;
-}
-
+// S3: every block scans the tile totals in LDS; particles go to their arrival slot in
+// the cell (tmp[start + slot] = i) and the final cell_start is written (cell_start =
+// tile-local offset + tile prefix)
 template <class F>
 __global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int forced)
 {
+    __shared__ int pre[MAX_TILES];
+    __shared__ int lds[FT / 64];
     const int r = blockIdx.y;
-    const CtlDev &C = B.ctl[r];
+    CtlDev &C = B.ctl[r];
     if (!sort_active(C, forced)) return;
     const RepDev &rp = B.rep[r];
-    const int64_t n = rp.n;
-    const int32_t *start = B.start + rp.c_off;
+    const int64_t n = rp.n, nc = rp.box.ncells;
+    const int nt = (int)((nc + SCAN_TILE - 1) / SCAN_TILE);
+    const int32_t *sums = B.scan_sums + r * B.scan_stride;
+    const int per = (nt + FT - 1) / FT;
+    const int t0 = threadIdx.x * per;
+    int loc = 0;
+    for (int t = t0; t < t0 + per && t < nt; ++t) loc += sums[t];
+    int tot;
+    int off = scan_block_exclusive(loc, lds, &tot);
+    for (int t = t0; t < t0 + per && t < nt; ++t) {
+        pre[t] = off;
+        off += sums[t];
+    }
+    __syncthreads();
+    const int32_t *sl = B.slocal + rp.c_off;
     const int32_t *key = B.key + rp.p_off, *slot = B.slot + rp.p_off;
     int32_t *tmp = B.tmp + rp.p_off;
-    for (int64_t i = gindex(); i < n; i += gstride()) tmp[start[key[i]] + slot[i]] = (int32_t)i;
+    for (int64_t i = gindex(); i < n; i += gstride()) {
+        const int c = key[i];
+        tmp[sl[c] + pre[c / SCAN_TILE] + slot[i]] = (int32_t)i;
+    }
+    int32_t *start = B.start + rp.c_off;
+    for (int64_t c = gindex(); c < nc; c += gstride()) start[c] = sl[c] + pre[c / SCAN_TILE];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        C.cur_next = 1 - C.cur;
+        C.cnt_next = C.cnt ^ 1;
+    }
+    // the gather writes forces only where they are non-zero: zero the destination
+    // buffers' force slots of the members of the lists saved when they were last in use
+    // (only those can hold a force)
+    using R = typename F::real;
+    const int dst = 1 - C.cur;
+    const int64_t po3 = 3 * rp.p_off;
+    R *f0 = B.force[2 * dst] + po3, *f1 = B.force[2 * dst + 1] + po3;
+    const int32_t *ms = B.mslots + ((int64_t)r * 2 + dst) * 2 * B.nmax;
+    const int mc0 = B.mcount[(r * 2 + dst) * 2], mc1 = B.mcount[(r * 2 + dst) * 2 + 1];
+    for (int64_t e = gindex(); e < 2 * (int64_t)B.nmax; e += gstride()) {
+        const int col = (int)(e / B.nmax), k = (int)(e - (int64_t)col * B.nmax);
+        if (k >= (col ? mc1 : mc0)) continue;
+        const int q = ms[e];
+        for (int kk = 0; kk < 3; ++kk) {
+            f0[kk * n + q] = 0;
+            f1[kk * n + q] = 0;
+        }
+    }
 }
 
 // S4: stable gather into the other buffers (wrap fused), pos_old, |v|max; clears the
-// previous cell counts and the is_md/is_reac marks for the list build
+// previous cell counts and the is_md/is_reac marks for the list build.  A block takes
+// FT consecutive destination slots at a time and keeps their source indices in LDS: the
+// rank of a particle among its cell's arrivals reads them there (the reference's
+// single-thread order is the source order).  Forces are copied only for particles that
+// carry one (MD/PAST_MD bits); every other destination force slot is already zero (see
+// k_fz_place).
 template <class F>
 __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
 {
     using R = typename F::real;
     using S = typename F::small;
     __shared__ double red[16];
-    __shared__ bool s_last;
+    __shared__ int32_t ltmp[FT];
     const int r = blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (!sort_active(C, forced)) return;
@@ -480,13 +675,21 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
     int32_t *old_cnt = B.count[C.cnt] + rp.c_off;
     const int64_t lim = n > nc ? n : nc;
     double vm = 0;
-    for (int64_t j = gindex(); j < lim; j += gstride()) {
+    for (int64_t t0 = (int64_t)blockIdx.x * FT; t0 < lim; t0 += (int64_t)gridDim.x * FT) {
+        const int64_t j = t0 + threadIdx.x;
+        __syncthreads();
+        const int32_t v = j < n ? tmp[j] : -1;
+        ltmp[threadIdx.x] = v;
+        __syncthreads();
         if (j < n) {
-            const int32_t v = tmp[j];
             const int c = key[v];
             const int s = start[c], m = cnt[c];
             int rank = 0;
-            for (int q = s; q < s + m; ++q) rank += tmp[q] < v ? 1 : 0;
+            for (int q = s; q < s + m; ++q) {
+                const int64_t lq = q - t0;
+                const int32_t w = (lq >= 0 && lq < FT) ? ltmp[lq] : tmp[q];
+                rank += w < v ? 1 : 0;
+            }
             const int64_t d = s + rank;
             const int fl = (int)sfl[v];
             const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
@@ -502,14 +705,16 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
                 R vv = svel[k * n + v];
                 dvel[k * n + d] = vv;
                 v2 += (double)vv * (double)vv;
-                df[k * n + d] = has_f ? sf[k * n + v] : (R)0;
-                dfo[k * n + d] = has_f ? sfo[k * n + v] : (R)0;
+                if (has_f) {
+                    df[k * n + d] = sf[k * n + v];
+                    dfo[k * n + d] = sfo[k * n + v];
+                }
             }
             const int32_t id = sid[v];
             did[d] = id;
             dsp[d] = ssp[v];
             dfl[d] = (S)fl;
-            if ((uint32_t)id < (uint32_t)n) B.id_to_idx[rp.p_off + id] = (int32_t)d;
+            B.memb[rp.p_off + d] = 0;
             vm = v2 > vm ? v2 : vm;
         }
         if (j < nc) {
@@ -520,273 +725,109 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
     }
     double m = block_max(vm, red);
     if (threadIdx.x == 0) B.part[(int64_t)r * PS + blockIdx.x] = m;
-    if (!arrive_last(&C.done[D_GATHER], gridDim.x, &s_last)) return;
-    if (threadIdx.x != 0) return;
-    double all = 0;
-    for (unsigned q = 0; q < gridDim.x; ++q) all = fmax(all, B.part[(int64_t)r * PS + q]);
-    C.done[D_GATHER] = 0;
-    C.vmax2 = all;
-    C.cur = dst;
-    C.cnt ^= 1;
-    C.k_since = 0;
-    if (!forced) C.n_extra += 1;
-    // need_sort stays set for the list build; the next list_a pass recomputes it
 }
 
-// S5: update_list (neighbor_list.f90:76-157) for colloid blockIdx.x of replica r, one
-// workgroup: count per stencil cell, block scan, fill in stencil order
+// S5: update_list (neighbor_list.f90:76-157) for colloid blockIdx.x, one workgroup:
+// count per stencil cell, block scan, fill in stencil order; block 0 also commits the
+// sort (|v|max from the gather partials, buffer flip, counters).  The blocks take the
+// new buffer indices from cur_next/cnt_next (set by k_fz_place), so the flip by block 0
+// cannot reach the other block.
 template <class F>
-__global__ void __launch_bounds__(1024) k_fz_list_build(FB<F> B, FP P, int forced)
+__global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forced, int ngather)
 {
     __shared__ int lds[16];
-    const int r = blockIdx.y, col = blockIdx.x;
+    __shared__ double red[16];
+    const int r = blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (!sort_active(C, forced)) return;
     const RepDev &rp = B.rep[r];
+    const int cur = C.cur_next, cnt_live = C.cnt_next;  // the buffers the gather wrote
     Box b = rp.box;
     for (int k = 0; k < 3; ++k) b.origin[k] = C.origin[k];
     const int64_t n = rp.n;
-    const int cur = C.cur;
     const typename F::real *pos = B.pos[cur] + 3 * rp.p_off;
     const typename F::small *sp = B.species[cur] + rp.p_off;
-    const int32_t *cnt = B.count[C.cnt] + rp.c_off, *start = B.start + rp.c_off;
+    const int32_t *cnt = B.count[cnt_live] + rp.c_off, *start = B.start + rp.c_off;
     uint8_t *is_md = B.is_md + rp.c_off, *is_reac = B.is_reac + rp.c_off;
-    const double *cp = B.cpos + 6 * r;
-    const double x[3] = {cp[col], cp[2 + col], cp[4 + col]};
-    const int si = B.cspecies[2 * r + col];
     const double L0 = b.L[0] * b.a, L1 = b.L[1] * b.a, L2 = b.L[2] * b.a;  // neighbor_list.f90:92
-    int32_t *lst = B.list + (int64_t)(2 * r + col) * P.nmax;
-    int carry = 0;
-    for (int base = 0; base < P.n_stencil; base += blockDim.x) {
-        const int j = base + threadIdx.x;
-        int key = 0, s = 0, m = 0, k = 0;
-        bool md = false, noreac = false;
-        if (j < P.n_stencil) {
-            key = stencil_key_dev(b, x, B.stencil, j);
-            s = start[key];
-            m = cnt[key];
-            for (int q = s; q < s + m; ++q) {
-                const int spc = (int)sp[q];
-                if (spc <= 0) continue;
-                double d0 = min_image(x[0] - (double)pos[q], L0);
-                double d1 = min_image(x[1] - (double)pos[n + q], L1);
-                double d2 = min_image(x[2] - (double)pos[2 * n + q], L2);
-                double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
-                if (rsq < P.radius_sq) {
-                    md = true;
-                    if (rsq <= P.sc.cut_sq[(spc - 1) + P.sc.n1 * (si - 1)]) noreac = true;
-                    ++k;
-                }
-            }
-            if (md) {
-                is_md[key] = 1;
-                if (noreac) is_reac[key] = 0;
-            }
-        }
-        int tot;
-        int off = carry + scan_block_exclusive(k, lds, &tot);
-        if (j < P.n_stencil && k > 0) {
-            int w = 0;
-            for (int q = s; q < s + m; ++q) {
-                const int spc = (int)sp[q];
-                if (spc <= 0) continue;
-                double d0 = min_image(x[0] - (double)pos[q], L0);
-                double d1 = min_image(x[1] - (double)pos[n + q], L1);
-                double d2 = min_image(x[2] - (double)pos[2 * n + q], L2);
-                double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
-                if (rsq < P.radius_sq) {
-                    if (off + w < P.nmax) lst[off + w] = q;
-                    ++w;
-                }
-            }
-        }
-        carry += tot;
-    }
-    if (threadIdx.x == 0) {
-        B.nlist[2 * r + col] = carry > P.nmax ? P.nmax : carry;
-        if (carry > P.nmax) C.error = RMPCD_ERR_OVERFLOW;
-    }
-}
-
-// ---- force: colloid-solvent LJ over the lists (neighbor_list.f90:191-222) -------------
-template <class F>
-__global__ void __launch_bounds__(FT) k_fz_force(FB<F> B, FP P)
-{
-    using R = typename F::real;
-    __shared__ double red[16];
-    __shared__ bool s_last;
-    const int r = blockIdx.y;
-    const RepDev &rp = B.rep[r];
-    CtlDev &C = B.ctl[r];
-    const int nb = (P.nmax + FT - 1) / FT;
-    const int col = blockIdx.x / nb, bb = blockIdx.x - col * nb;
-    const int k = bb * FT + threadIdx.x;
-    const int64_t n = rp.n;
-    const int cur = C.cur, fp = C.fpar;
-    const double *cp = B.cpos + 6 * r;
-    const double x0 = cp[col], x1 = cp[2 + col], x2 = cp[4 + col];
-    const int s1 = B.cspecies[2 * r + col];
-    const double *L = rp.box.edges;
-    const int nl = min(B.nlist[2 * r + col], P.nmax);
-    double f0 = 0, f1 = 0, f2 = 0, e = 0;
-    if (k < nl) {
-        const int idx = B.list[(int64_t)(2 * r + col) * P.nmax + k];
-        const int s2 = (int)B.species[cur][rp.p_off + idx];
-        if (s2 > 0) {
-            const R *pos = B.pos[cur] + 3 * rp.p_off;
-            double d0 = min_image(x0 - (double)pos[idx], L[0]);
-            double d1 = min_image(x1 - (double)pos[n + idx], L[1]);
-            double d2 = min_image(x2 - (double)pos[2 * n + idx], L[2]);
-            double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
-            int pidx = (s2 - 1) + P.sc.n1 * (s1 - 1);
-            if (r_sq <= P.sc.cut_sq[pidx]) {
-                double fr = lj_force_over_r(r_sq, P.sc.eps[pidx], P.sc.sigma[pidx]);
-                f0 = fr * d0;
-                f1 = fr * d1;
-                f2 = fr * d2;
-                e = lj_energy(r_sq, P.sc.eps[pidx], P.sc.sigma[pidx], P.sc.cut_energy[pidx]);
-                R *force = B.force[2 * cur + fp] + 3 * rp.p_off;
-                atomicAdd(&force[idx], (R)(-f0));
-                atomicAdd(&force[n + idx], (R)(-f1));
-                atomicAdd(&force[2 * n + idx], (R)(-f2));
-                atomic_or_flag(&B.flags[cur][rp.p_off + idx], MD_MASK | (s1 == 1 ? CATALYZED_MASK : 0));
-            }
-        }
-    }
-    double b0 = block_sum(f0, red), b1 = block_sum(f1, red), b2 = block_sum(f2, red), be = block_sum(e, red);
-    if (threadIdx.x == 0) {
-        double *pp = B.part + (int64_t)r * PS + (int64_t)(col * nb + bb) * 4;
-        pp[0] = b0;
-        pp[1] = b1;
-        pp[2] = b2;
-        pp[3] = be;
-    }
-    if (!arrive_last(&C.done[D_FORCE], gridDim.x, &s_last)) return;
-    if (threadIdx.x != 0) return;
-    C.done[D_FORCE] = 0;
-    double *cf = B.cforce[C.cfpar] + 6 * r;
-    const double *cfo = B.cforce[1 - C.cfpar] + 6 * r;
-    double ecol[2];
-    for (int c = 0; c < 2; ++c) {
-        double a0 = 0, a1 = 0, a2 = 0, ae = 0;
-        for (int q = 0; q < nb; ++q) {
-            const double *pp = B.part + (int64_t)r * PS + (int64_t)(c * nb + q) * 4;
-            a0 += pp[0];
-            a1 += pp[1];
-            a2 += pp[2];
-            ae += pp[3];
-        }
-        cf[c] = cf[c] + a0;
-        cf[2 + c] = cf[2 + c] + a1;
-        cf[4 + c] = cf[4 + c] + a2;
-        ecol[c] = ae;
-    }
-    double e1 = 0;
-    e1 = e1 + ecol[0];
-    e1 = e1 + ecol[1];
-    C.e1 = e1;
-    // compute_force_n2 for the pair (neighbor_list.f90:243-262)
+    int total = 0;
     {
-        double d0 = min_image(cp[0] - cp[1], L[0]);
-        double d1 = min_image(cp[2] - cp[3], L[1]);
-        double d2 = min_image(cp[4] - cp[5], L[2]);
-        double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
-        const int sa = B.cspecies[2 * r], sb = B.cspecies[2 * r + 1];
-        int pc = (sa - 1) + P.cc.n1 * (sb - 1), pe = (sb - 1) + P.cc.n1 * (sa - 1);
-        double e2 = 0, e0 = 0;
-        if (r_sq <= P.cc.cut_sq[pc]) {
-            double fr = lj_force_over_r(r_sq, P.cc.eps[pe], P.cc.sigma[pe]);
-            cf[0] = cf[0] + fr * d0;
-            cf[2] = cf[2] + fr * d1;
-            cf[4] = cf[4] + fr * d2;
-            cf[1] = cf[1] - fr * d0;
-            cf[3] = cf[3] - fr * d1;
-            cf[5] = cf[5] - fr * d2;
-            e0 = e0 + lj_energy(r_sq, P.cc.eps[pe], P.cc.sigma[pe], P.cc.cut_energy[pe]);
-        }
-        e2 = e2 + e0;
-        e2 = e2 + 0.0;
-        C.e2 = e2;
-    }
-    // colloid velocity step and RATTLE (single_dimer_pbc.f90:356-361)
-    double *cv = B.cvel + 6 * r;
-    const double *cm = B.cmass + 2 * r;
-    for (int i = 0; i < 2; ++i)
-        for (int q3 = 0; q3 < 3; ++q3) {
-            int q = 2 * q3 + i;
-            cv[q] = cv[q] + (P.dt * (cf[q] + cfo[q])) / (2 * cm[i]);
-        }
-    rattle_vel_2(cp, cv, B.cspecies + 2 * r, P.mass, P.d, L);
-}
-
-// ---- list_b: velocity Verlet of the members, flagging and conversion ------------------
-template <class F>
-__global__ void __launch_bounds__(FT) k_fz_list_b(FB<F> B, FP P)
-{
-    using R = typename F::real;
-    using S = typename F::small;
-    __shared__ bool s_last;
-    const int r = blockIdx.y;
-    const RepDev &rp = B.rep[r];
-    CtlDev &C = B.ctl[r];
-    const int cur = C.cur, fp = C.fpar;
-    const int64_t n = rp.n;
-    const int e = blockIdx.x * FT + threadIdx.x;
-    if (e < 2 * P.nmax) {
-        const int col = e / P.nmax, k = e - col * P.nmax;
-        const int nl = min(B.nlist[2 * r + col], P.nmax);
-        if (k < nl) {
-            const int idx = B.list[(int64_t)(2 * r + col) * P.nmax + k];
-            const int64_t g = rp.p_off + idx;
-            const R *pos = B.pos[cur] + 3 * rp.p_off;
-            const double *cp = B.cpos + 6 * r;
-            const double *L = rp.box.edges;
-            const double y0 = (double)pos[idx], y1 = (double)pos[n + idx], y2 = (double)pos[2 * n + idx];
-            S *spc = B.species[cur] + rp.p_off;
-            S *fl = B.flags[cur] + rp.p_off;
-            const double cut_c = P.sc.cut_sq[0], cut_n = P.sc.cut_sq[2];
-            const unsigned epoch = (unsigned)(C.md_step * 4 + 2);
-            if (atomicExch((unsigned *)&B.claim[g], epoch) != epoch) {
-                // md_vel (md.f90:219-237)
-                const int f = (int)fl[idx];
-                if (!(f & WALL_MASK)) {
-                    R *vel = B.vel[cur] + 3 * rp.p_off;
-                    const R *fo = B.force[2 * cur + fp] + 3 * rp.p_off;
-                    const R *fold = B.force[2 * cur + 1 - fp] + 3 * rp.p_off;
-                    const R rdt = (R)P.dt;
-                    for (int q = 0; q < 3; ++q)
-                        vel[q * n + idx] = vel[q * n + idx] + (rdt * (fo[q * n + idx] + fold[q * n + idx])) / (R)2;
-                } else {
-                    atomic_and_flag(&fl[idx], ~WALL_MASK);
+        const int col = blockIdx.x;
+        const double *cp = B.cpos + 6 * r;
+        const double x[3] = {cp[col], cp[2 + col], cp[4 + col]};
+        const int si = B.cspecies[2 * r + col];
+        int32_t *lst = B.list + (int64_t)(2 * r + col) * P.nmax;
+        int32_t *msl = B.mslots + ((int64_t)r * 2 + cur) * 2 * P.nmax + (int64_t)col * P.nmax;
+        int carry = 0;
+        for (int base = 0; base < P.n_stencil; base += blockDim.x) {
+            const int j = base + threadIdx.x;
+            int key = 0, s = 0, m = 0, k = 0;
+            bool md = false, noreac = false;
+            if (j < P.n_stencil) {
+                key = stencil_key_dev(b, x, B.stencil, j);
+                s = start[key];
+                m = cnt[key];
+                for (int q = s; q < s + m; ++q) {
+                    const int spc = (int)sp[q];
+                    if (spc <= 0) continue;
+                    double d0 = min_image(x[0] - (double)pos[q], L0);
+                    double d1 = min_image(x[1] - (double)pos[n + q], L1);
+                    double d2 = min_image(x[2] - (double)pos[2 * n + q], L2);
+                    double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
+                    if (rsq < P.radius_sq) {
+                        md = true;
+                        if (rsq <= P.sc.cut_sq[(spc - 1) + P.sc.n1 * (si - 1)]) noreac = true;
+                        ++k;
+                    }
                 }
-                // change_species (single_dimer_pbc.f90:511-542), draw threefry((md_step, id), (seed, CHANGE))
-                if (f & REAC_MASK) {
-                    double a0 = min_image(cp[0] - y0, L[0]), a1 = min_image(cp[2] - y1, L[1]),
-                           a2 = min_image(cp[4] - y2, L[2]);
-                    double b0 = min_image(cp[1] - y0, L[0]), b1 = min_image(cp[3] - y1, L[1]),
-                           b2 = min_image(cp[5] - y2, L[2]);
-                    double dc = (a0 * a0 + a1 * a1) + a2 * a2, dn = (b0 * b0 + b1 * b1) + b2 * b2;
-                    if (dc > cut_c && dn > cut_n) {
-                        uint64_t u0, u1;
-                        threefry2x64((uint64_t)C.md_step, (uint64_t)B.id[cur][g], rp.seed, STREAM_CHANGE, u0, u1);
-                        if (u01(u0) <= P.prob) spc[idx] = (S)2;
-                        atomic_and_flag(&fl[idx], ~REAC_MASK);
+                if (md) {
+                    is_md[key] = 1;
+                    if (noreac) is_reac[key] = 0;
+                }
+            }
+            int tot;
+            int off = carry + scan_block_exclusive(k, lds, &tot);
+            if (j < P.n_stencil && k > 0) {
+                int w = 0;
+                for (int q = s; q < s + m; ++q) {
+                    const int spc = (int)sp[q];
+                    if (spc <= 0) continue;
+                    double d0 = min_image(x[0] - (double)pos[q], L0);
+                    double d1 = min_image(x[1] - (double)pos[n + q], L1);
+                    double d2 = min_image(x[2] - (double)pos[2 * n + q], L2);
+                    double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
+                    if (rsq < P.radius_sq) {
+                        if (off + w < P.nmax) {
+                            lst[off + w] = q;
+                            msl[off + w] = q;
+                            atomic_or_flag(&B.memb[rp.p_off + q], 1 << col);
+                        }
+                        ++w;
                     }
                 }
             }
-            // flag_particles_nl (single_dimer_pbc.f90:493-509), colloid 1's list
-            if (col == 0 && (int)spc[idx] == 1) {
-                double a0 = min_image(cp[0] - y0, L[0]), a1 = min_image(cp[2] - y1, L[1]),
-                       a2 = min_image(cp[4] - y2, L[2]);
-                if ((a0 * a0 + a1 * a1) + a2 * a2 < cut_c) atomic_or_flag(&fl[idx], REAC_MASK);
-            }
+            carry += tot;
         }
+        total = carry;
     }
-    if (!arrive_last(&C.done[D_LISTB], gridDim.x, &s_last)) return;
+    const int col = blockIdx.x;
+    if (threadIdx.x == 0) {
+        B.nlist[2 * r + col] = total > P.nmax ? P.nmax : total;
+        B.mcount[(r * 2 + cur) * 2 + col] = total > P.nmax ? P.nmax : total;
+        if (total > P.nmax) C.error = RMPCD_ERR_OVERFLOW;
+    }
+    if (col != 0) return;
+    double all = 0;
+    for (int q = threadIdx.x; q < ngather; q += blockDim.x) all = fmax(all, B.part[(int64_t)r * PS + q]);
+    all = block_max(all, red);
     if (threadIdx.x != 0) return;
-    C.done[D_LISTB] = 0;
-    C.md_step += 1;
-    C.k_since += 1;
+    C.vmax2 = all;
+    C.cur = cur;
+    C.cnt = cnt_live;
+    C.k_since = 0;
+    C.need_exact = P.dt * sqrt(all) * (1.0 + 1e-6) + P.slack >= P.skin * 0.89 ? 1 : 0;
+    if (!forced) C.n_extra += 1;
+    // need_sort stays set; the colloid kernel of the next step recomputes it
 }
 
 // ---- collision: SRD (mpcd.f90:120-180) + refuel or bulk reaction, |v|max ---------------
@@ -799,7 +840,6 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
     __shared__ S lsp[CTILE];
     __shared__ double red[16];
     __shared__ int64_t s_range[2];
-    __shared__ bool s_last;
     const int r = blockIdx.y;
     const RepDev &rp = B.rep[r];
     CtlDev &C = B.ctl[r];
@@ -944,13 +984,39 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
     }
     double mm = block_max(vm, red);
     if (threadIdx.x == 0) B.part[(int64_t)r * PS + blockIdx.x] = mm;
-    if (!arrive_last(&C.done[D_COLLIDE], gridDim.x, &s_last)) return;
-    if (threadIdx.x != 0) return;
+}
+
+// |v|max after the collision and the cycle counter, one block per replica
+template <class F>
+__global__ void __launch_bounds__(1024) k_fz_collide_commit(FB<F> B, FP P, int ncollide)
+{
+    __shared__ double red[16];
+    const int r = blockIdx.y;
+    CtlDev &C = B.ctl[r];
     double all = 0;
-    for (unsigned q = 0; q < gridDim.x; ++q) all = fmax(all, B.part[(int64_t)r * PS + q]);
-    C.done[D_COLLIDE] = 0;
+    for (int q = threadIdx.x; q < ncollide; q += blockDim.x) all = fmax(all, B.part[(int64_t)r * PS + q]);
+    all = block_max(all, red);
+    if (threadIdx.x != 0) return;
     C.vmax2 = all;
+    C.need_exact = P.dt * sqrt(all) * (1.0 + 1e-6) + P.slack >= P.skin * 0.89 ? 1 : 0;
     C.cycle += 1;
+}
+
+// list membership bits from the lists built at set-up (memb starts zeroed)
+template <class F>
+__global__ void k_fz_memb_init(FB<F> B, FP P)
+{
+    const int r = blockIdx.y;
+    const int e = blockIdx.x * FT + threadIdx.x;
+    const int col = e / P.nmax, k = e - col * P.nmax;
+    if (col >= 2) return;
+    const int cur = B.ctl[r].cur, nl = min(B.nlist[2 * r + col], P.nmax);
+    if (k == 0) B.mcount[(r * 2 + cur) * 2 + col] = nl;
+    if (k < nl) {
+        const int q = B.list[(int64_t)(2 * r + col) * P.nmax + k];
+        atomic_or_flag(&B.memb[B.rep[r].p_off + q], 1 << col);
+        B.mslots[((int64_t)r * 2 + cur) * 2 * P.nmax + (int64_t)col * P.nmax + k] = q;
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -974,7 +1040,7 @@ static FB<F> make_fb(DimerSim &s)
     for (int k = 0; k < 4; ++k) b.force[k] = (R *)s.sol_force[k];
     b.pos_old = (R *)s.sol_pos_old;
     b.id_to_idx = s.id_to_idx;
-    b.claim = s.claim;
+    b.memb = s.memb;
     b.key = s.f_key;
     b.slot = s.f_slot;
     b.tmp = s.f_tmp;
@@ -994,6 +1060,10 @@ static FB<F> make_fb(DimerSim &s)
     b.cspecies = s.col_species;
     b.part = s.part;
     b.scan_sums = s.scan_sums;
+    b.mslots = s.f_mslots;
+    b.mcount = s.f_mcount;
+    b.nmax = s.nmax;
+    b.slocal = s.f_slocal;
     b.scan_stride = (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE + 1;
     b.rep = s.rep_dev;
     b.ctl = s.ctl_dev;
@@ -1029,49 +1099,47 @@ static FP make_fp(const DimerSim &s)
     return p;
 }
 
+// grid of the step kernel: the member blocks, then the stream blocks
+static int member_blocks(const DimerSim &s) { return (int)((2 * (int64_t)s.nmax + FT - 1) / FT); }
+static int stream_blocks(const DimerSim &s)
+{
+    return (int)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, STREAM_BLOCKS);
+}
+
 template <class F>
-static int enqueue_md_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st)
+static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int forced)
 {
     const unsigned Rr = (unsigned)s.R;
-    const unsigned gs = (unsigned)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, STREAM_BLOCKS);
-    const unsigned gr = (unsigned)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, RED_BLOCKS);
-    const unsigned gl = (unsigned)((2 * (int64_t)s.nmax + FT - 1) / FT);
-    const unsigned gf = (unsigned)(2 * ((s.nmax + FT - 1) / FT));
+    const unsigned gp = (unsigned)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, SORT_BLOCKS);
     const unsigned gt = (unsigned)((s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE);
     const int64_t lim = std::max(s.nmax_rep, s.ncmax_rep);
-    const unsigned gg = (unsigned)std::min<int64_t>((lim + FT - 1) / FT, RED_BLOCKS);
-    k_fz_stream<F><<<dim3(gs, Rr), FT, 0, st>>>(b, p);
-    k_fz_list_a<F><<<dim3(gl, Rr), FT, 0, st>>>(b, p);
-    k_fz_exact<F><<<dim3(gr, Rr), FT, 0, st>>>(b, p);
-    k_fz_key<F><<<dim3(gr, Rr), FT, 0, st>>>(b, p, 0);
-    k_fz_scan_tiles<F><<<dim3(gt, Rr), SCAN_T, 0, st>>>(b, 0);
-    k_fz_scan_top<F><<<dim3(1, Rr), 1024, 0, st>>>(b, 0);
-    k_fz_scan_add<F><<<dim3(gt, Rr), SCAN_T, 0, st>>>(b, 0);
-    k_fz_place<F><<<dim3(gr, Rr), FT, 0, st>>>(b, 0);
-    k_fz_gather<F><<<dim3(gg, Rr), FT, 0, st>>>(b, p, 0);
-    k_fz_list_build<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, 0);
-    k_fz_force<F><<<dim3(gf, Rr), FT, 0, st>>>(b, p);
-    k_fz_list_b<F><<<dim3(gl, Rr), FT, 0, st>>>(b, p);
-    return RMPCD_OK;
+    const unsigned gg = (unsigned)std::min<int64_t>((lim + FT - 1) / FT, SORT_BLOCKS);
+    k_fz_key<F><<<dim3(gp, Rr), FT, 0, st>>>(b, p, forced);
+    k_fz_scan_tiles<F><<<dim3(gt, Rr), SCAN_T, 0, st>>>(b, forced);
+    k_fz_place<F><<<dim3(gp, Rr), FT, 0, st>>>(b, forced);
+    k_fz_gather<F><<<dim3(gg, Rr), FT, 0, st>>>(b, p, forced);
+    k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, (int)gg);
+}
+
+template <class F>
+static void enqueue_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int mode)
+{
+    const unsigned Rr = (unsigned)s.R;
+    const int nbm = member_blocks(s);
+    const unsigned grid = (unsigned)(nbm + (mode <= 1 ? stream_blocks(s) : 0));
+    k_fz_step<F><<<dim3(grid, Rr), FT, 0, st>>>(b, p, mode, nbm);
+    k_fz_colloid<F><<<dim3(1, Rr), 1024, 0, st>>>(b, p, mode, nbm, mode <= 1 ? stream_blocks(s) : 0);
+    if (mode <= 1) enqueue_sort<F>(s, b, p, st, 0);
 }
 
 template <class F>
 static void enqueue_collision(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st)
 {
     const unsigned Rr = (unsigned)s.R;
-    const unsigned gr = (unsigned)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, RED_BLOCKS);
-    const unsigned gt = (unsigned)((s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE);
-    const int64_t lim = std::max(s.nmax_rep, s.ncmax_rep);
-    const unsigned gg = (unsigned)std::min<int64_t>((lim + FT - 1) / FT, RED_BLOCKS);
     const unsigned gc = (unsigned)((s.ncmax_rep + CT - 1) / CT);
-    k_fz_key<F><<<dim3(gr, Rr), FT, 0, st>>>(b, p, 1);
-    k_fz_scan_tiles<F><<<dim3(gt, Rr), SCAN_T, 0, st>>>(b, 1);
-    k_fz_scan_top<F><<<dim3(1, Rr), 1024, 0, st>>>(b, 1);
-    k_fz_scan_add<F><<<dim3(gt, Rr), SCAN_T, 0, st>>>(b, 1);
-    k_fz_place<F><<<dim3(gr, Rr), FT, 0, st>>>(b, 1);
-    k_fz_gather<F><<<dim3(gg, Rr), FT, 0, st>>>(b, p, 1);
-    k_fz_list_build<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, 1);
+    enqueue_sort<F>(s, b, p, st, 1);
     k_fz_collide<F><<<dim3(gc, Rr), CT, 0, st>>>(b, p);
+    k_fz_collide_commit<F><<<dim3(1, Rr), 1024, 0, st>>>(b, p, (int)gc);
 }
 
 // Event-record nodes around every prof_stride-th launch of the streaming kernel in the
@@ -1114,7 +1182,8 @@ static int add_stream_timers(DimerSim &s, hipGraph_t g)
         for (size_t k : succ[i])
             if (--indeg[k] == 0) ready.push_back(k);
     }
-    const void *stream_fn = reinterpret_cast<const void *>(&k_fz_stream<F>);
+    const void *stream_fn = reinterpret_cast<const void *>(&k_fz_step<F>);
+    const unsigned full = (unsigned)(member_blocks(s) + stream_blocks(s));
     size_t seen = 0, pair = 0;
     for (hipGraphNode_t x : order) {
         hipGraphNodeType t;
@@ -1122,7 +1191,7 @@ static int add_stream_timers(DimerSim &s, hipGraph_t g)
         if (t != hipGraphNodeTypeKernel) continue;
         hipKernelNodeParams kp;
         RMPCD_HIP(hipGraphKernelNodeGetParams(x, &kp));
-        if (kp.func != stream_fn) continue;
+        if (kp.func != stream_fn || kp.gridDim.x != full) continue;
         if (seen++ % (size_t)s.prof_stride || 2 * pair + 1 >= s.prof_ev.size()) continue;
         size_t nd = 0, nn2 = 0;
         RMPCD_HIP(hipGraphNodeGetDependencies(x, nullptr, &nd));
@@ -1163,7 +1232,10 @@ static int build_graph(DimerSim &s)
     }
     RMPCD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = RMPCD_OK;
-    for (int j = 0; j < s.cfg.n_md && rc == RMPCD_OK; ++j) rc = enqueue_md_step<F>(s, b, p, st);
+    // the first step streams with the last force; each later step first finishes its
+    // predecessor; the last step is finished right before the collision
+    for (int j = 0; j < s.cfg.n_md; ++j) enqueue_step<F>(s, b, p, st, j == 0 ? 0 : 1);
+    enqueue_step<F>(s, b, p, st, 2);
     enqueue_collision<F>(s, b, p, st);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
@@ -1216,17 +1288,21 @@ int fused_prepare(DimerSim &s)
     size_t RS = s.cfg.form == RMPCD_FP64 ? 8 : 4;
     int rc;
     // every per-block partial of a replica must fit its PS slots
-    RMPCD_CHECK((s.ncmax_rep + CT - 1) / CT <= PS && 8 * ((int64_t)s.nmax + FT - 1) / FT <= PS &&
-                    (2 * (int64_t)s.nmax + FT - 1) / FT <= PS,
-                "dimer: box or list too large for the fused partial sums");
+    RMPCD_CHECK((s.ncmax_rep + CT - 1) / CT <= PS && (int64_t)member_blocks(s) * MPV + STREAM_BLOCKS <= PS &&
+                    SORT_BLOCKS <= PS && member_blocks(s) <= MAX_MEMBER_BLOCKS &&
+                    (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE <= MAX_TILES,
+                "dimer: box or list too large for the fused kernels");
     if ((rc = s.alloc(&s.sol_pos_old, 3 * s.ntot * RS))) return rc;
-    if ((rc = s.alloc((void **)&s.claim, s.ntot * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.memb, s.ntot))) return rc;
     if ((rc = s.alloc((void **)&s.f_key, s.ntot * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_slot, s.ntot * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_tmp, s.ntot * 4))) return rc;
     if ((rc = s.alloc((void **)&s.lut, s.luttot * 4))) return rc;
     int64_t sstride = (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE + 1;
     if ((rc = s.alloc((void **)&s.scan_sums, (size_t)s.R * sstride * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.f_slocal, s.nctot * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.f_mslots, (size_t)s.R * 4 * s.nmax * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.f_mcount, (size_t)s.R * 4 * 4))) return rc;
     if ((rc = s.alloc((void **)&s.part, (size_t)s.R * PS * 8))) return rc;
     if ((rc = s.alloc((void **)&s.rep_dev, sizeof(RepDev) * s.R))) return rc;
     if ((rc = s.alloc((void **)&s.ctl_dev, sizeof(CtlDev) * s.R))) return rc;
@@ -1252,6 +1328,7 @@ int fused_prepare(DimerSim &s)
         c.n_extra = rp.n_extra;
         for (int k = 0; k < 3; ++k) c.origin[k] = rp.box.origin[k];
         c.vmax2 = INFINITY;  // the first step compares against the unsorted set-up positions
+        c.need_exact = 1;
         c.e1 = rp.e1;
         c.e2 = rp.e2;
         // pos_old starts as the positions before the set-up sort (the reference never
@@ -1264,6 +1341,11 @@ int fused_prepare(DimerSim &s)
     }
     RMPCD_HIP(hipMemcpy(s.rep_dev, rd.data(), sizeof(RepDev) * s.R, hipMemcpyHostToDevice));
     RMPCD_HIP(hipMemcpy(s.ctl_dev, cd.data(), sizeof(CtlDev) * s.R, hipMemcpyHostToDevice));
+    if (s.cfg.form == RMPCD_FP64)
+        k_fz_memb_init<F64><<<dim3(member_blocks(s), s.R), FT, 0, s.fstream>>>(make_fb<F64>(s), make_fp(s));
+    else
+        k_fz_memb_init<F32><<<dim3(member_blocks(s), s.R), FT, 0, s.fstream>>>(make_fb<F32>(s), make_fp(s));
+    RMPCD_LAUNCHED();
     RMPCD_HIP(hipDeviceSynchronize());
     rc = s.cfg.form == RMPCD_FP64 ? build_graph<F64>(s) : build_graph<F32>(s);
     if (rc) return rc;

@@ -30,6 +30,7 @@ struct RepDev {
 struct CtlDev {
     int32_t cur, fpar, cfpar, need_sort;
     int32_t need_exact, k_since, error, cnt;  // cnt: which cell_count array is live
+    int32_t cur_next, cnt_next;               // a sort in progress: the buffers it writes
     int64_t md_step, cycle, n_extra;
     double origin[3];
     double vmax2, list_max2, co_max, e1, e2;
@@ -51,7 +52,8 @@ struct DimerSim {
     int32_t *sol_image[2] = {}, *sol_id[2] = {};
     void *sol_force[4] = {};
     void *sol_pos_old = nullptr;  // fused mode: positions at the last sort
-    int32_t *id_to_idx = nullptr, *claim = nullptr;
+    int32_t *id_to_idx = nullptr;
+    uint8_t *memb = nullptr;  // fused mode: bit c set for a member of colloid c's list
     int32_t *cell_count[2] = {}, *cell_start = nullptr;
     uint8_t *is_md = nullptr, *is_reac = nullptr;
     double *col_pos = nullptr, *col_pos_old = nullptr, *col_pos_rattle = nullptr, *col_vel = nullptr;
@@ -60,7 +62,8 @@ struct DimerSim {
     double *col_mass = nullptr;
     int32_t *list = nullptr, *nlist = nullptr, *stencil = nullptr;
     int32_t *lut = nullptr;
-    int32_t *f_key = nullptr, *f_slot = nullptr, *f_tmp = nullptr, *scan_sums = nullptr;
+    int32_t *f_key = nullptr, *f_slot = nullptr, *f_tmp = nullptr, *scan_sums = nullptr, *f_slocal = nullptr;
+    int32_t *f_mslots = nullptr, *f_mcount = nullptr;
     double *part = nullptr;
     void *scratch = nullptr;
 

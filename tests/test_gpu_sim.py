@@ -1,7 +1,9 @@
 """The single-dimer program on the GPU (programs/single_dimer_pbc.f90): the fused,
-graph-captured step must reproduce the reference call sequence bit for bit (same draws,
-same sort decisions), for one and for several replicas of uneven size; plus the
-invariants of dimer_test.py (momentum conservation) and RATTLE."""
+graph-captured step must reproduce the reference call sequence (same draws, same sort
+decisions; integer state bit for bit, floating point to the FP64 bar of 1e-12 relative,
+as the colloid force sums run in another order), for one and for several replicas of
+uneven size; both must match the oracle's loop; plus the invariants of dimer_test.py
+(momentum conservation) and RATTLE."""
 import numpy as np
 import pytest
 
@@ -15,15 +17,26 @@ KEYS = ("pos", "vel", "force", "force_old", "image", "id", "species", "flags", "
         "nlist", "origin")
 
 
-def compare(a, b, replica_msg=""):
+def _close(a, b, rtol=1e-12):
+    """|a - b| <= rtol * max(1, |b|) elementwise (the FP64 bar of the whole-step check)"""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return a.shape == b.shape and bool((np.abs(a - b) <= rtol * np.maximum(1.0, np.abs(b))).all())
+
+
+def compare(a, b, replica_msg="", rtol=1e-12):
     for k in KEYS:
         assert a[k].shape == b[k].shape, k
-        assert a[k].tobytes() == b[k].tobytes(), "%s differs %s" % (k, replica_msg)
+        if a[k].dtype.kind == "f":
+            assert _close(a[k], b[k], rtol), "%s differs %s" % (k, replica_msg)
+        else:
+            assert a[k].tobytes() == b[k].tobytes(), "%s differs %s" % (k, replica_msg)
     for i in range(2):
         n = a["nlist"][i]
         assert (a["list"][i, :n] == b["list"][i, :n]).all()
-    for k in ("md_steps", "cycles", "n_extra_sorting", "e1", "e2"):
+    for k in ("md_steps", "cycles", "n_extra_sorting"):
         assert a[k] == b[k], k
+    assert _close(a["e1"], b["e1"], rtol) and _close(a["e2"], b["e2"], rtol)
 
 
 @pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
@@ -35,7 +48,8 @@ def test_fused_equals_reference_sequence(gpu, form):
         ref.run(1)
         fus.run(1)
         a, b = ref.download(), fus.download()
-        compare(a, b, "after cycle %d" % cycle)
+        # FP32: the solvent state is single precision, the colloids FP64
+        compare(a, b, "after cycle %d" % cycle, 1e-12 if form == capi.FP64 else 1e-5)
     assert a["n_extra_sorting"] >= 1  # the first step re-sorts (pos_old is the set-up order)
     ref.close()
     fus.close()
@@ -81,15 +95,9 @@ def test_dimer_invariants(gpu):
     sim.close()
 
 
-def _close(a, b, rtol=1e-12):
-    """|a - b| <= rtol * max(1, |b|) elementwise (the FP64 bar of the whole-step check)"""
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
-    return a.shape == b.shape and bool((np.abs(a - b) <= rtol * np.maximum(1.0, np.abs(b))).all())
-
-
+@pytest.mark.parametrize("mode", [capi.SIM_REFERENCE, capi.SIM_FUSED])
 @pytest.mark.parametrize("bulk", [0, 1])
-def test_reference_sequence_matches_oracle(gpu, bulk):
+def test_sequence_matches_oracle(gpu, bulk, mode):
     """the whole single_dimer_pbc cycle (MD steps with sort decisions, forces, RATTLE,
     change_species; grid shift, sort, SRD, refuel or bulk reaction) on the GPU against
     the oracle's loop consuming the same Threefry draws. Integer state (sort order, ids,
@@ -100,7 +108,7 @@ def test_reference_sequence_matches_oracle(gpu, bulk):
     from oracle.threefry import ThreefryDraws
     L = (12, 12, 12)
     kw = dict(L=L, n_md=10, tau=0.1, d=4.5, sigma_C=2.0, sigma_N=2.0, bulk_rmpcd=bulk, bulk_rate=0.5)
-    sim = SingleDimer(mode=capi.SIM_REFERENCE, form=capi.FP64, seed=41, **kw)
+    sim = SingleDimer(mode=mode, form=capi.FP64, seed=41, **kw)
     st = sim.download()
     orc = from_download(st, L, 4.5, 0.1, 10, bulk_rmpcd=bool(bulk), bulk_rate=0.5)
     assert (orc.nl == st["nlist"]).all()
