@@ -48,10 +48,21 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-single", action="store_true", help="skip the single-replica run")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the 128-replica run with L drawn from {24, 32, 40, 48}")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="first copy MB megabytes device to device once (the reference dispatch for "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
-def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile):
+def c4_sizes():
+    """128 replicas with L drawn uniformly from {24, 32, 40, 48} (seeded)"""
+    import numpy as np
+    return [int(v) for v in np.array([24, 32, 40, 48])[np.random.default_rng(20261202).integers(0, 4, 128)]]
+
+
+def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile, sizes=None):
     import torch
 
     from rmpcdmd_amd import capi
@@ -59,7 +70,11 @@ def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile):
 
     f = capi.FP32 if form == "fp32" else capi.FP64
     m = capi.SIM_FUSED if mode == "fused" else capi.SIM_REFERENCE
-    sim = SingleDimer(L=(L, L, L), n_md=n_md, form=f, mode=m, n_replicas=replicas)
+    if sizes is not None:
+        replicas = len(sizes)
+        sim = SingleDimer(n_md=n_md, form=f, mode=m, replica_L=[(q, q, q) for q in sizes])
+    else:
+        sim = SingleDimer(L=(L, L, L), n_md=n_md, form=f, mode=m, n_replicas=replicas)
     st0 = sim.state()[4]
     n_total = int(st0.n_solvent_total)
     out = {}
@@ -140,6 +155,14 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.zeros(1, device="cuda")
     capi.call("rmpcd_init", 0)
+    if a.copy_calibration:
+        nbytes = a.copy_calibration << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        capi.call("rmpcd_malloc", C.byref(src), nbytes)
+        capi.call("rmpcd_malloc", C.byref(dst), nbytes)
+        capi.call("rmpcd_memcpy", dst, src, nbytes, 2)
+        capi.call("rmpcd_free", src)
+        capi.call("rmpcd_free", dst)
     main_run = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, a.replicas, profile=True)
     compare = None
     if not a.no_compare and a.form == "fp32":
@@ -184,6 +207,11 @@ def main():
     if compare:
         line["fp64_form"] = {"value": compare["ps_per_s"], "ms_per_step": compare["ms_per_cycle"],
                              "mixed_over_fp64": main_run["ps_per_s"] / compare["ps_per_s"]}
+    if not a.no_c4:
+        sizes = c4_sizes()
+        c4 = run_gpu(a.form, a.mode, 0, a.n_md, max(2, a.steps // 2), 1, 0, profile=False, sizes=sizes)
+        line["batched_c4"] = {"value": c4["ps_per_s"], "ms_per_step": c4["ms_per_cycle"], "n_solvent": c4["n"],
+                              "replicas": len(sizes), "L": "128 replicas, L uniform in {24,32,40,48}, seeded"}
     if not a.no_single and a.replicas > 1:
         single = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, 1, profile=False)
         line["single_replica"] = {"value": single["ps_per_s"], "ms_per_step": single["ms_per_cycle"],

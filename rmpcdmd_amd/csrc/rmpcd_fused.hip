@@ -22,6 +22,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "rmpcd_sim.hpp"
@@ -36,6 +37,7 @@ constexpr int CT = 128;            // cells per block in the collision kernel
 constexpr int CTILE = 2048;        // staged particles per collision block
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
+constexpr int MAX_STENCIL = 2048;  // stencil cells of the fused list build
 constexpr int KPT = 4, KCHUNK = FT * KPT;  // particles per thread and per chunk of the key pass
 constexpr int KHASH_BITS = 11, KHASH = 1 << KHASH_BITS;  // LDS hash entries, 2 x KCHUNK
 
@@ -72,6 +74,7 @@ struct FP {
 
 __device__ inline int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
 __device__ inline int64_t gindex() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
 
 // ---- colloid helpers (one thread, replica r; the arithmetic of k_colloid_pos,
 // k_rattle_pos, k_max_disp, k_force_nl's last block, k_force_n2, k_colloid_vel and
@@ -264,6 +267,39 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
         const int64_t gs = (int64_t)(gridDim.x - nbm) * FT;
         const S *spc = B.species[cur] + rp.p_off;
         double mx = 0;
+        if (!exact && rp.vec) {
+            // 16-byte loads and stores: VW particles per thread and plane (the replica's
+            // planes and membership bytes are 16-byte aligned)
+            constexpr int VW = 16 / sizeof(R);
+            using V = typename std::conditional<sizeof(R) == 4, float4, double2>::type;
+            const int64_t nv = n / VW;
+            for (int64_t q = (int64_t)(blockIdx.x - nbm) * FT + threadIdx.x; q < nv; q += gs) {
+                uint32_t mb = 0;
+                for (int u = 0; u < VW; ++u) mb |= (uint32_t)memb[q * VW + u] << (8 * u);
+                V pv[3], vv[3];
+                for (int k = 0; k < 3; ++k) {
+                    pv[k] = reinterpret_cast<const V *>(pos + k * n)[q];
+                    vv[k] = reinterpret_cast<const V *>(vel + k * n)[q];
+                }
+                for (int k = 0; k < 3; ++k) {
+                    R *pe = reinterpret_cast<R *>(&pv[k]);
+                    const R *ve = reinterpret_cast<const R *>(&vv[k]);
+                    for (int u = 0; u < VW; ++u)
+                        if (!((mb >> (8 * u)) & 0xffu)) pe[u] = pe[u] + rdt * ve[u];
+                }
+                if (!mb) {
+                    for (int k = 0; k < 3; ++k) reinterpret_cast<V *>(pos + k * n)[q] = pv[k];
+                } else {
+                    // a member is streamed by its member thread: write the others only
+                    for (int k = 0; k < 3; ++k) {
+                        const R *pe = reinterpret_cast<const R *>(&pv[k]);
+                        for (int u = 0; u < VW; ++u)
+                            if (!((mb >> (8 * u)) & 0xffu)) pos[k * n + q * VW + u] = pe[u];
+                    }
+                }
+            }
+            return;
+        }
         for (int64_t i = (int64_t)(blockIdx.x - nbm) * FT + threadIdx.x; i < n; i += gs) {
             if (memb[i]) continue;
             const R x0 = pos[i] + rdt * vel[i];
@@ -643,18 +679,20 @@ __global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int forced)
 
 // S4: stable gather into the other buffers (wrap fused), pos_old, |v|max; clears the
 // previous cell counts and the is_md/is_reac marks for the list build.  A block takes
-// FT consecutive destination slots at a time and keeps their source indices in LDS: the
+// FT consecutive destination slots at a time.  Their source indices are kept in LDS: the
 // rank of a particle among its cell's arrivals reads them there (the reference's
-// single-thread order is the source order).  Forces are copied only for particles that
-// carry one (MD/PAST_MD bits); every other destination force slot is already zero (see
-// k_fz_place).
+// single-thread order is the source order).  The tile's (source, destination) pairs are
+// then re-dealt to the threads in increasing source order, so that a wave reads a few
+// contiguous source runs and writes within the tile: both sides stay close to coalesced.
+// Forces are copied only for particles that carry one (MD/PAST_MD bits); every other
+// destination force slot is already zero (see k_fz_place).
 template <class F>
 __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
 {
     using R = typename F::real;
     using S = typename F::small;
     __shared__ double red[16];
-    __shared__ int32_t ltmp[FT];
+    __shared__ int32_t ltmp[FT], lsrc[FT], ldst[FT];
     const int r = blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (!sort_active(C, forced)) return;
@@ -678,9 +716,11 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
     for (int64_t t0 = (int64_t)blockIdx.x * FT; t0 < lim; t0 += (int64_t)gridDim.x * FT) {
         const int64_t j = t0 + threadIdx.x;
         __syncthreads();
-        const int32_t v = j < n ? tmp[j] : -1;
+        const int32_t v = j < n ? tmp[j] : INT32_MAX;
         ltmp[threadIdx.x] = v;
         __syncthreads();
+        int64_t d = -1;
+        int place = 0;
         if (j < n) {
             const int c = key[v];
             const int s = start[c], m = cnt[c];
@@ -690,31 +730,41 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
                 const int32_t w = (lq >= 0 && lq < FT) ? ltmp[lq] : tmp[q];
                 rank += w < v ? 1 : 0;
             }
-            const int64_t d = s + rank;
-            const int fl = (int)sfl[v];
+            d = s + rank;
+        }
+        // position of this pair in source order within the tile (sources are distinct;
+        // the unused slots of a partial tile sort last)
+        for (int k = 0; k < FT; ++k) place += ltmp[k] < v ? 1 : 0;
+        if (v == INT32_MAX) place = threadIdx.x;
+        lsrc[place] = v;
+        ldst[place] = (int32_t)d;
+        __syncthreads();
+        const int32_t u = lsrc[threadIdx.x];
+        const int64_t e = ldst[threadIdx.x];
+        if (u != INT32_MAX) {
+            const int fl = (int)sfl[u];
             const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
             double v2 = 0;
             for (int k = 0; k < 3; ++k) {
-                R x = spos[k * n + v];
-                R e = (R)rp.box.edges[k];
-                int jump = (int)floor((double)(x / e));
-                x = x - (R)jump * e;
-                dpos[k * n + d] = x;
-                dold[k * n + d] = x;
-                dim[k * n + d] = sim[k * n + v] + jump;
-                R vv = svel[k * n + v];
-                dvel[k * n + d] = vv;
+                R x = spos[k * n + u];
+                R ed = (R)rp.box.edges[k];
+                int jump = (int)floor((double)(x / ed));
+                x = x - (R)jump * ed;
+                dpos[k * n + e] = x;
+                dold[k * n + e] = x;
+                dim[k * n + e] = sim[k * n + u] + jump;
+                R vv = svel[k * n + u];
+                dvel[k * n + e] = vv;
                 v2 += (double)vv * (double)vv;
                 if (has_f) {
-                    df[k * n + d] = sf[k * n + v];
-                    dfo[k * n + d] = sfo[k * n + v];
+                    df[k * n + e] = sf[k * n + u];
+                    dfo[k * n + e] = sfo[k * n + u];
                 }
             }
-            const int32_t id = sid[v];
-            did[d] = id;
-            dsp[d] = ssp[v];
-            dfl[d] = (S)fl;
-            B.memb[rp.p_off + d] = 0;
+            did[e] = sid[u];
+            dsp[e] = ssp[u];
+            dfl[e] = (S)fl;
+            B.memb[rp.p_off + e] = 0;
             vm = v2 > vm ? v2 : vm;
         }
         if (j < nc) {
@@ -750,67 +800,79 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
     const int32_t *cnt = B.count[cnt_live] + rp.c_off, *start = B.start + rp.c_off;
     uint8_t *is_md = B.is_md + rp.c_off, *is_reac = B.is_reac + rp.c_off;
     const double L0 = b.L[0] * b.a, L1 = b.L[1] * b.a, L2 = b.L[2] * b.a;  // neighbor_list.f90:92
-    int total = 0;
-    {
-        const int col = blockIdx.x;
-        const double *cp = B.cpos + 6 * r;
-        const double x[3] = {cp[col], cp[2 + col], cp[4 + col]};
-        const int si = B.cspecies[2 * r + col];
-        int32_t *lst = B.list + (int64_t)(2 * r + col) * P.nmax;
-        int32_t *msl = B.mslots + ((int64_t)r * 2 + cur) * 2 * P.nmax + (int64_t)col * P.nmax;
-        int carry = 0;
-        for (int base = 0; base < P.n_stencil; base += blockDim.x) {
-            const int j = base + threadIdx.x;
-            int key = 0, s = 0, m = 0, k = 0;
-            bool md = false, noreac = false;
-            if (j < P.n_stencil) {
-                key = stencil_key_dev(b, x, B.stencil, j);
-                s = start[key];
-                m = cnt[key];
-                for (int q = s; q < s + m; ++q) {
-                    const int spc = (int)sp[q];
-                    if (spc <= 0) continue;
-                    double d0 = min_image(x[0] - (double)pos[q], L0);
-                    double d1 = min_image(x[1] - (double)pos[n + q], L1);
-                    double d2 = min_image(x[2] - (double)pos[2 * n + q], L2);
-                    double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
-                    if (rsq < P.radius_sq) {
-                        md = true;
-                        if (rsq <= P.sc.cut_sq[(spc - 1) + P.sc.n1 * (si - 1)]) noreac = true;
-                        ++k;
-                    }
-                }
-                if (md) {
-                    is_md[key] = 1;
-                    if (noreac) is_reac[key] = 0;
-                }
-            }
-            int tot;
-            int off = carry + scan_block_exclusive(k, lds, &tot);
-            if (j < P.n_stencil && k > 0) {
-                int w = 0;
-                for (int q = s; q < s + m; ++q) {
-                    const int spc = (int)sp[q];
-                    if (spc <= 0) continue;
-                    double d0 = min_image(x[0] - (double)pos[q], L0);
-                    double d1 = min_image(x[1] - (double)pos[n + q], L1);
-                    double d2 = min_image(x[2] - (double)pos[2 * n + q], L2);
-                    double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
-                    if (rsq < P.radius_sq) {
-                        if (off + w < P.nmax) {
-                            lst[off + w] = q;
-                            msl[off + w] = q;
-                            atomic_or_flag(&B.memb[rp.p_off + q], 1 << col);
-                        }
-                        ++w;
-                    }
-                }
-            }
-            carry += tot;
-        }
-        total = carry;
-    }
+    // The stencil's cells and their particles are flattened into one index range (cell
+    // j owns [soff[j], soff[j+1])), so every thread tests one candidate at a time and the
+    // loads of different candidates overlap; a block scan per chunk keeps the reference's
+    // order (stencil order, then cell order).
+    __shared__ int skey[MAX_STENCIL], sstart[MAX_STENCIL], soff[MAX_STENCIL + 1], sflag[MAX_STENCIL];
     const int col = blockIdx.x;
+    const double *cp = B.cpos + 6 * r;
+    const double x[3] = {cp[col], cp[2 + col], cp[4 + col]};
+    const int si = B.cspecies[2 * r + col];
+    int32_t *lst = B.list + (int64_t)(2 * r + col) * P.nmax;
+    int32_t *msl = B.mslots + ((int64_t)r * 2 + cur) * 2 * P.nmax + (int64_t)col * P.nmax;
+    const int ns = P.n_stencil;
+    int carry = 0;
+    for (int base = 0; base < ns; base += blockDim.x) {
+        const int j = base + threadIdx.x;
+        int m = 0;
+        if (j < ns) {
+            const int key = stencil_key_dev(b, x, B.stencil, j);
+            skey[j] = key;
+            sstart[j] = start[key];
+            sflag[j] = 0;
+            m = cnt[key];
+        }
+        int tot;
+        const int off = carry + scan_block_exclusive(m, lds, &tot);
+        if (j < ns) soff[j] = off;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) soff[ns] = carry;
+    __syncthreads();
+    const int T = carry;
+    int total = 0;
+    for (int f0 = 0; f0 < T; f0 += blockDim.x) {
+        const int f = f0 + threadIdx.x;
+        int q = -1, in = 0;
+        if (f < T) {
+            int lo = 0, hi = ns - 1;  // the last j with soff[j] <= f
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (soff[mid] <= f)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            q = sstart[lo] + (f - soff[lo]);
+            const int spc = (int)sp[q];
+            if (spc > 0) {
+                const double d0 = min_image(x[0] - (double)pos[q], L0);
+                const double d1 = min_image(x[1] - (double)pos[n + q], L1);
+                const double d2 = min_image(x[2] - (double)pos[2 * n + q], L2);
+                const double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
+                if (rsq < P.radius_sq) {
+                    in = 1;
+                    const bool noreac = rsq <= P.sc.cut_sq[(spc - 1) + P.sc.n1 * (si - 1)];
+                    atomicOr(&sflag[lo], noreac ? 3 : 1);
+                }
+            }
+        }
+        int tot;
+        const int off = total + scan_block_exclusive(in, lds, &tot);
+        if (in && off < P.nmax) {
+            lst[off] = q;
+            msl[off] = q;
+            atomic_or_flag(&B.memb[rp.p_off + q], 1 << col);
+        }
+        total += tot;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        const int fl = sflag[j];
+        if (fl & 1) is_md[skey[j]] = 1;
+        if (fl & 2) is_reac[skey[j]] = 0;
+    }
     if (threadIdx.x == 0) {
         B.nlist[2 * r + col] = total > P.nmax ? P.nmax : total;
         B.mcount[(r * 2 + cur) * 2 + col] = total > P.nmax ? P.nmax : total;
@@ -1289,7 +1351,7 @@ int fused_prepare(DimerSim &s)
     int rc;
     // every per-block partial of a replica must fit its PS slots
     RMPCD_CHECK((s.ncmax_rep + CT - 1) / CT <= PS && (int64_t)member_blocks(s) * MPV + STREAM_BLOCKS <= PS &&
-                    SORT_BLOCKS <= PS && member_blocks(s) <= MAX_MEMBER_BLOCKS &&
+                    SORT_BLOCKS <= PS && member_blocks(s) <= MAX_MEMBER_BLOCKS && s.n_stencil <= MAX_STENCIL &&
                     (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE <= MAX_TILES,
                 "dimer: box or list too large for the fused kernels");
     if ((rc = s.alloc(&s.sol_pos_old, 3 * s.ntot * RS))) return rc;
@@ -1317,6 +1379,7 @@ int fused_prepare(DimerSim &s)
         rd[r].c_off = rp.c_off;
         rd[r].lut_off = rp.lut_off;
         rd[r].seed = rp.seed;
+        rd[r].vec = (rp.n % 4 == 0 && rp.p_off % 4 == 0) ? 1 : 0;
         CtlDev &c = cd[r];
         memset(&c, 0, sizeof(c));
         c.cur = rp.cur;

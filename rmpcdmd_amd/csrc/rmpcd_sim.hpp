@@ -26,6 +26,7 @@ struct RepDev {
     Box box;
     int64_t n, p_off, c_off, lut_off;
     uint64_t seed;
+    int32_t vec, pad_;  // vec: planes and membership bytes allow 16-byte accesses
 };
 struct CtlDev {
     int32_t cur, fpar, cfpar, need_sort;

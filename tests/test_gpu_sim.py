@@ -138,3 +138,24 @@ def test_sequence_matches_oracle(gpu, bulk, mode):
     if bulk:
         assert (g["species"] == 2).sum() > 0
     sim.close()
+
+
+def test_batched_replicas_of_the_c4_shape(gpu):
+    """replicas of unequal L (the batched configuration draws L from {24, 32, 40, 48}),
+    FP64, tau=0.5 and N_MD=50 as in the dimer runs: the fused batch against the reference
+    call sequence replica by replica, and replica 0 of the batch against the same system
+    run on its own"""
+    Ls = [(24, 24, 24), (32, 32, 32), (24, 24, 24), (32, 32, 32)]
+    kw = dict(n_md=50, tau=0.5, form=capi.FP64, seed=17)
+    ref = SingleDimer(mode=capi.SIM_REFERENCE, replica_L=Ls, **kw)
+    fus = SingleDimer(mode=capi.SIM_FUSED, replica_L=Ls, **kw)
+    one = SingleDimer(mode=capi.SIM_FUSED, L=Ls[0], **kw)
+    ref.run(1)
+    fus.run(1)
+    one.run(1)
+    for r in range(len(Ls)):
+        compare(ref.download(r), fus.download(r), "replica %d" % r)
+    compare(fus.download(0), one.download(0), "replica 0 alone")
+    ref.close()
+    fus.close()
+    one.close()
