@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librmpcd_amd.so")
+# RMPCD_AMD_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("RMPCD_AMD_LIB") or os.path.join(HERE, "librmpcd_amd.so")
 
 RMPCD_OK = 0
 RMPCD_ERR_ARG = -1

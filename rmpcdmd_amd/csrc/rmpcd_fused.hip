@@ -32,7 +32,8 @@ namespace rmpcd {
 constexpr int FT = 256;            // threads per block
 constexpr int PS = 32768;          // partial-sum slots per replica
 constexpr int STREAM_BLOCKS = 16384;  // stream-role blocks per replica (grid-stride beyond)
-constexpr int SORT_BLOCKS = 4096;     // blocks per replica of the particle passes of the sort
+constexpr int SORT_BLOCKS = 4096;     // blocks of the particle passes of a sort (all replicas)
+constexpr int SCAN_BLOCKS = 1024;     // blocks of the cell-count tile scan (all replicas)
 constexpr int CT = 128;            // cells per block in the collision kernel
 constexpr int CTILE = 2048;        // staged particles per collision block
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
@@ -59,6 +60,8 @@ struct FB {
     int32_t *scan_sums, *slocal;
     int64_t scan_stride;
     int32_t *mslots, *mcount;  // per replica and sort buffer: the member slots of its last lists
+    int64_t *plan;             // the replicas a sort chain works on (k_fz_plan)
+    double *gpart;             // |v|^2 maximum per gather unit
     int32_t nmax;
     const RepDev *rep;
     CtlDev *ctl;
@@ -74,6 +77,7 @@ struct FP {
 
 __device__ inline int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
 __device__ inline int64_t gindex() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
 
 
 // ---- colloid helpers (one thread, replica r; the arithmetic of k_colloid_pos,
@@ -466,108 +470,13 @@ __global__ void __launch_bounds__(1024) k_fz_colloid(FB<F> B, FP P, int mode, in
     }
 }
 
-// ---- sort ---------------------------------------------------------------------------
-// S1: grid shift (collision sort), periodic wrap, key and arrival slot; block 0 wraps
-// the colloids and records their positions as pos_old
-template <class F>
-__global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced)
-{
-    using R = typename F::real;
-    const int r = blockIdx.y;
-    CtlDev &C = B.ctl[r];
-    if (!forced && !C.need_sort) return;
-    const RepDev &rp = B.rep[r];
-    Box b = rp.box;
-    if (forced) {
-        double u[3];
-        uint64_t a0, a1, a2, a3;
-        threefry2x64((uint64_t)C.cycle, 0, rp.seed, STREAM_SHIFT, a0, a1);
-        threefry2x64((uint64_t)C.cycle, 1, rp.seed, STREAM_SHIFT, a2, a3);
-        u[0] = u01(a0);
-        u[1] = u01(a1);
-        u[2] = u01(a2);
-        for (int k = 0; k < 3; ++k) b.origin[k] = u[k] - 1;  // random_shift
-    } else {
-        for (int k = 0; k < 3; ++k) b.origin[k] = C.origin[k];
-    }
-    const int64_t n = rp.n;
-    const int cur = C.cur;
-    const R *pos = B.pos[cur] + 3 * rp.p_off;
-    int32_t *cnt = B.count[C.cnt ^ 1] + rp.c_off;
-    const int32_t *lut = B.lut + rp.lut_off;
-    // Chunks of KCHUNK consecutive particles: the keys of a chunk are counted in an LDS
-    // hash table, then one global atomic per distinct key reserves the chunk's slots in
-    // that cell (arrival order within a cell is made stable by the gather).
-    __shared__ int hkey[KHASH], hcnt[KHASH];
-    for (int64_t c0 = (int64_t)blockIdx.x * KCHUNK; c0 < n; c0 += (int64_t)gridDim.x * KCHUNK) {
-        for (int q = threadIdx.x; q < KHASH; q += FT) {
-            hkey[q] = -1;
-            hcnt[q] = 0;
-        }
-        __syncthreads();
-        int keys[KPT], ent[KPT], loc[KPT];
-        for (int u = 0; u < KPT; ++u) {
-            const int64_t i = c0 + u * FT + threadIdx.x;
-            keys[u] = -1;
-            ent[u] = 0;
-            loc[u] = 0;
-            if (i >= n) continue;
-            R x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
-            for (int k = 0; k < 3; ++k) {
-                R e = (R)b.edges[k];
-                int jump = (int)floor((double)(x[k] / e));
-                x[k] = x[k] - (R)jump * e;
-            }
-            int key;
-            {
-                int p0, p1, p2;
-                cell_of<R>(b, x[0], x[1], x[2], p0, p1, p2);
-                if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
-                    (unsigned)p2 < (unsigned)b.L[2])
-                    key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
-                else
-                    key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
-            }
-            keys[u] = key;
-            // open addressing; the table has twice as many entries as a chunk has keys
-            unsigned h = ((unsigned)key * 2654435761u) >> (32 - KHASH_BITS);
-            for (int probe = 0; probe < KHASH; ++probe) {
-                const int prev = atomicCAS(&hkey[h], -1, key);
-                if (prev == -1 || prev == key) break;
-                h = (h + 1) & (KHASH - 1);
-            }
-            ent[u] = (int)h;
-            loc[u] = atomicAdd(&hcnt[h], 1);
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < KHASH; q += FT)
-            if (hkey[q] >= 0) hcnt[q] = atomicAdd(&cnt[hkey[q]], hcnt[q]);
-        __syncthreads();
-        for (int u = 0; u < KPT; ++u) {
-            const int64_t i = c0 + u * FT + threadIdx.x;
-            if (keys[u] < 0) continue;
-            B.key[rp.p_off + i] = keys[u];
-            B.slot[rp.p_off + i] = hcnt[ent[u]] + loc[u];
-        }
-        __syncthreads();
-    }
-    // block 0 records the grid shift and wraps the colloids; the other blocks computed the
-    // same origin and read no colloid data
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int k = 0; k < 3; ++k) C.origin[k] = b.origin[k];
-    // apply_pbc(colloids) (md.f90:202-217) and colloids%pos_old = colloids%pos
-    double *cp = B.cpos + 6 * r, *co = B.cpos_old + 6 * r;
-    int32_t *ci = B.cimage + 6 * r;
-    for (int k = 0; k < 3; ++k)
-        for (int i = 0; i < 2; ++i) {
-            int q = 2 * k + i;
-            double x = cp[q];
-            int jump = (int)floor(x / b.edges[k]);
-            ci[q] += jump;
-            cp[q] = x - jump * b.edges[k];
-        }
-    for (int q = 0; q < 6; ++q) co[q] = cp[q];
-}
+// ---- sort -------------------------------------------------------------------------
+// The particle passes of a sort run on a fixed grid over the replicas that sort (a
+// graph cannot branch, and a grid of blocks per replica that mostly exit at once costs
+// microseconds per launch in a large batch).  k_fz_plan lists those replicas and the
+// prefix sums of their work units per pass; a pass's blocks take contiguous unit ranges.
+enum { U_KEY = 0, U_SCAN, U_PLACE, U_GATHER, U_KINDS };
+constexpr int PT = 4 * FT;  // particles or cells per place unit
 
 __device__ inline bool sort_active(const CtlDev &C, int forced) { return forced || C.need_sort; }
 
@@ -590,191 +499,397 @@ __device__ inline int scan_block_exclusive(int v, int *lds, int *total)
 
 // S2: exclusive scan of the new cell counts inside tiles of SCAN_TILE cells (tile-local
 // offsets into slocal, tile totals into scan_sums)
-template <class F>
-__global__ void __launch_bounds__(SCAN_T) k_fz_scan_tiles(FB<F> B, int forced)
+
+__device__ inline int64_t units_of(const RepDev &rp, int kind)
 {
-    __shared__ int lds[SCAN_T / 64];
-    const int r = blockIdx.y;
-    const CtlDev &C = B.ctl[r];
-    if (!sort_active(C, forced)) return;
-    const RepDev &rp = B.rep[r];
-    const int64_t nc = rp.box.ncells;
-    const int64_t tile0 = (int64_t)blockIdx.x * SCAN_TILE;
-    if (tile0 >= nc) return;  // uniform per block
-    const int32_t *in = B.count[C.cnt ^ 1] + rp.c_off;
-    int32_t *out = B.slocal + rp.c_off;
-    int64_t base = tile0 + (int64_t)threadIdx.x * SCAN_I;
-    int v[SCAN_I], s = 0;
-    for (int k = 0; k < SCAN_I; ++k) {
-        v[k] = base + k < nc ? in[base + k] : 0;
-        s += v[k];
+    const int64_t n = rp.n, nc = rp.box.ncells;
+    switch (kind) {
+    case U_KEY:
+        return (n + KCHUNK - 1) / KCHUNK;
+    case U_SCAN:
+        return (nc + SCAN_TILE - 1) / SCAN_TILE;
+    case U_PLACE:
+        return (n + PT - 1) / PT + (nc + PT - 1) / PT;
+    default:
+        return ((n > nc ? n : nc) + FT - 1) / FT;
     }
-    int tot;
-    int off = scan_block_exclusive(s, lds, &tot);
-    for (int k = 0; k < SCAN_I; ++k) {
-        if (base + k < nc) out[base + k] = off;
-        off += v[k];
-    }
-    if (threadIdx.x == 0) B.scan_sums[r * B.scan_stride + blockIdx.x] = tot;
 }
 
-// S3: every block scans the tile totals in LDS; particles go to their arrival slot in
-// the cell (tmp[start + slot] = i) and the final cell_start is written (cell_start =
-// tile-local offset + tile prefix)
-template <class F>
-__global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int forced)
+// plan layout (int64): [0] active count, [1 + kind] total units, then act[R] (replica of
+// active index a), amap[R] (active index of replica r, or -1), pre[kind][R + 1]
+__device__ inline const int64_t *plan_act(const int64_t *plan) { return plan + 1 + U_KINDS; }
+__device__ inline const int64_t *plan_amap(const int64_t *plan, int R) { return plan + 1 + U_KINDS + R; }
+__device__ inline const int64_t *plan_pre(const int64_t *plan, int R, int kind)
 {
-    __shared__ int pre[MAX_TILES];
+    return plan + 1 + U_KINDS + 2 * (int64_t)R + (int64_t)kind * (R + 1);
+}
+
+template <class F>
+__global__ void __launch_bounds__(FT) k_fz_plan(FB<F> B, int forced, int R)
+{
     __shared__ int lds[FT / 64];
-    const int r = blockIdx.y;
-    CtlDev &C = B.ctl[r];
-    if (!sort_active(C, forced)) return;
-    const RepDev &rp = B.rep[r];
-    const int64_t n = rp.n, nc = rp.box.ncells;
-    const int nt = (int)((nc + SCAN_TILE - 1) / SCAN_TILE);
-    const int32_t *sums = B.scan_sums + r * B.scan_stride;
-    const int per = (nt + FT - 1) / FT;
-    const int t0 = threadIdx.x * per;
-    int loc = 0;
-    for (int t = t0; t < t0 + per && t < nt; ++t) loc += sums[t];
-    int tot;
-    int off = scan_block_exclusive(loc, lds, &tot);
-    for (int t = t0; t < t0 + per && t < nt; ++t) {
-        pre[t] = off;
-        off += sums[t];
+    int64_t *plan = B.plan;
+    int64_t *act = plan + 1 + U_KINDS, *amap = act + R;
+    // the common case first: no replica sorts
+    bool any = forced != 0;
+    for (int r = threadIdx.x; r < R && !any; r += blockDim.x) any = B.ctl[r].need_sort != 0;
+    if (!__syncthreads_or(any)) {
+        if (threadIdx.x <= U_KINDS) plan[threadIdx.x] = 0;
+        return;
     }
-    __syncthreads();
-    const int32_t *sl = B.slocal + rp.c_off;
-    const int32_t *key = B.key + rp.p_off, *slot = B.slot + rp.p_off;
-    int32_t *tmp = B.tmp + rp.p_off;
-    for (int64_t i = gindex(); i < n; i += gstride()) {
-        const int c = key[i];
-        tmp[sl[c] + pre[c / SCAN_TILE] + slot[i]] = (int32_t)i;
+    int carry = 0;
+    int64_t csum[U_KINDS] = {0, 0, 0, 0};
+    for (int base = 0; base < R; base += blockDim.x) {
+        const int r = base + threadIdx.x;
+        const bool on = r < R && sort_active(B.ctl[r], forced);
+        int tot;
+        const int off = carry + scan_block_exclusive(on ? 1 : 0, lds, &tot);
+        if (r < R) amap[r] = on ? off : -1;
+        if (on) act[off] = r;
+        for (int kind = 0; kind < U_KINDS; ++kind) {
+            const int u = on ? (int)units_of(B.rep[r], kind) : 0;
+            int ktot;
+            const int kx = scan_block_exclusive(u, lds, &ktot);
+            if (on) plan[1 + U_KINDS + 2 * (int64_t)R + (int64_t)kind * (R + 1) + off] = csum[kind] + kx;
+            csum[kind] += ktot;
+        }
+        carry += tot;
     }
-    int32_t *start = B.start + rp.c_off;
-    for (int64_t c = gindex(); c < nc; c += gstride()) start[c] = sl[c] + pre[c / SCAN_TILE];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        C.cur_next = 1 - C.cur;
-        C.cnt_next = C.cnt ^ 1;
-    }
-    // the gather writes forces only where they are non-zero: zero the destination
-    // buffers' force slots of the members of the lists saved when they were last in use
-    // (only those can hold a force)
-    using R = typename F::real;
-    const int dst = 1 - C.cur;
-    const int64_t po3 = 3 * rp.p_off;
-    R *f0 = B.force[2 * dst] + po3, *f1 = B.force[2 * dst + 1] + po3;
-    const int32_t *ms = B.mslots + ((int64_t)r * 2 + dst) * 2 * B.nmax;
-    const int mc0 = B.mcount[(r * 2 + dst) * 2], mc1 = B.mcount[(r * 2 + dst) * 2 + 1];
-    for (int64_t e = gindex(); e < 2 * (int64_t)B.nmax; e += gstride()) {
-        const int col = (int)(e / B.nmax), k = (int)(e - (int64_t)col * B.nmax);
-        if (k >= (col ? mc1 : mc0)) continue;
-        const int q = ms[e];
-        for (int kk = 0; kk < 3; ++kk) {
-            f0[kk * n + q] = 0;
-            f1[kk * n + q] = 0;
+    if (threadIdx.x == 0) {
+        plan[0] = carry;
+        for (int kind = 0; kind < U_KINDS; ++kind) {
+            plan[1 + kind] = csum[kind];
+            plan[1 + U_KINDS + 2 * (int64_t)R + (int64_t)kind * (R + 1) + carry] = csum[kind];
         }
     }
 }
 
-// S4: stable gather into the other buffers (wrap fused), pos_old, |v|max; clears the
-// previous cell counts and the is_md/is_reac marks for the list build.  A block takes
-// FT consecutive destination slots at a time.  Their source indices are kept in LDS: the
-// rank of a particle among its cell's arrivals reads them there (the reference's
-// single-thread order is the source order).  The tile's (source, destination) pairs are
-// then re-dealt to the threads in increasing source order, so that a wave reads a few
-// contiguous source runs and writes within the tile: both sides stay close to coalesced.
-// Forces are copied only for particles that carry one (MD/PAST_MD bits); every other
-// destination force slot is already zero (see k_fz_place).
-template <class F>
-__global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
+// the active index whose unit range holds unit u (the last a with pre[a] <= u)
+__device__ inline int find_unit(const int64_t *pre, int nact, int64_t u)
 {
-    using R = typename F::real;
+    int lo = 0, hi = nact - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// the unit range [u0, u1) of this block for pass `kind`
+struct UnitRange {
+    int64_t u0, u1;
+    int nact;
+    const int64_t *pre, *act;
+};
+__device__ inline UnitRange unit_range(const int64_t *plan, int R, int kind)
+{
+    UnitRange w;
+    w.nact = (int)plan[0];
+    const int64_t T = plan[1 + kind];
+    const int64_t per = (T + gridDim.x - 1) / gridDim.x;
+    w.u0 = (int64_t)blockIdx.x * per;
+    w.u1 = w.u0 + per < T ? w.u0 + per : T;
+    w.pre = plan_pre(plan, R, kind);
+    w.act = plan_act(plan);
+    return w;
+}
+
+// S1: grid shift (collision sort), periodic wrap, key and slot in the cell.  A unit is
+// a chunk of KCHUNK consecutive particles: its keys are counted in an LDS hash table,
+// then one global atomic per distinct key reserves the chunk's slots in that cell
+// (arrival order within a cell is made stable by the gather).  The block with a
+// replica's first chunk records the grid shift and wraps the colloids.
+template <class F>
+__global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
+{
+    using Rl = typename F::real;
+    __shared__ int hkey[KHASH], hcnt[KHASH];
+    const UnitRange w = unit_range(B.plan, R, U_KEY);
+    int a = -1;
+    for (int64_t u = w.u0; u < w.u1; ++u) {
+        if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
+        const int r = (int)w.act[a];
+        const int64_t c0 = (u - w.pre[a]) * KCHUNK;
+        CtlDev &C = B.ctl[r];
+        const RepDev &rp = B.rep[r];
+        Box b = rp.box;
+        if (forced) {
+            uint64_t a0, a1, a2, a3;
+            threefry2x64((uint64_t)C.cycle, 0, rp.seed, STREAM_SHIFT, a0, a1);
+            threefry2x64((uint64_t)C.cycle, 1, rp.seed, STREAM_SHIFT, a2, a3);
+            b.origin[0] = u01(a0) - 1;  // random_shift
+            b.origin[1] = u01(a1) - 1;
+            b.origin[2] = u01(a2) - 1;
+        } else {
+            for (int k = 0; k < 3; ++k) b.origin[k] = C.origin[k];
+        }
+        const int64_t n = rp.n;
+        const Rl *pos = B.pos[C.cur] + 3 * rp.p_off;
+        int32_t *cnt = B.count[C.cnt ^ 1] + rp.c_off;
+        const int32_t *lut = B.lut + rp.lut_off;
+        for (int q = threadIdx.x; q < KHASH; q += FT) {
+            hkey[q] = -1;
+            hcnt[q] = 0;
+        }
+        __syncthreads();
+        int keys[KPT], ent[KPT], loc[KPT];
+        for (int v = 0; v < KPT; ++v) {
+            const int64_t i = c0 + v * FT + threadIdx.x;
+            keys[v] = -1;
+            ent[v] = 0;
+            loc[v] = 0;
+            if (i >= n) continue;
+            Rl x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+            for (int k = 0; k < 3; ++k) {
+                Rl e = (Rl)b.edges[k];
+                int jump = (int)floor((double)(x[k] / e));
+                x[k] = x[k] - (Rl)jump * e;
+            }
+            int key;
+            {
+                int p0, p1, p2;
+                cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
+                if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
+                    (unsigned)p2 < (unsigned)b.L[2])
+                    key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+                else
+                    key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+            }
+            keys[v] = key;
+            // open addressing; the table has twice as many entries as a chunk has keys
+            unsigned h = ((unsigned)key * 2654435761u) >> (32 - KHASH_BITS);
+            for (int probe = 0; probe < KHASH; ++probe) {
+                const int prev = atomicCAS(&hkey[h], -1, key);
+                if (prev == -1 || prev == key) break;
+                h = (h + 1) & (KHASH - 1);
+            }
+            ent[v] = (int)h;
+            loc[v] = atomicAdd(&hcnt[h], 1);
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < KHASH; q += FT)
+            if (hkey[q] >= 0) hcnt[q] = atomicAdd(&cnt[hkey[q]], hcnt[q]);
+        __syncthreads();
+        for (int v = 0; v < KPT; ++v) {
+            const int64_t i = c0 + v * FT + threadIdx.x;
+            if (keys[v] < 0) continue;
+            B.key[rp.p_off + i] = keys[v];
+            B.slot[rp.p_off + i] = hcnt[ent[v]] + loc[v];
+        }
+        __syncthreads();
+        if (c0 == 0 && threadIdx.x == 0) {
+            for (int k = 0; k < 3; ++k) C.origin[k] = b.origin[k];
+            // apply_pbc(colloids) (md.f90:202-217) and colloids%pos_old = colloids%pos
+            double *cp = B.cpos + 6 * r, *co = B.cpos_old + 6 * r;
+            int32_t *ci = B.cimage + 6 * r;
+            for (int k = 0; k < 3; ++k)
+                for (int i = 0; i < 2; ++i) {
+                    int q = 2 * k + i;
+                    double x = cp[q];
+                    int jump = (int)floor(x / b.edges[k]);
+                    ci[q] += jump;
+                    cp[q] = x - jump * b.edges[k];
+                }
+            for (int q = 0; q < 6; ++q) co[q] = cp[q];
+        }
+    }
+}
+
+// S2: exclusive scan of the new cell counts inside tiles of SCAN_TILE cells (tile-local
+// offsets into slocal, tile totals into scan_sums); a unit is a tile
+template <class F>
+__global__ void __launch_bounds__(SCAN_T) k_fz_scan_tiles(FB<F> B, int R)
+{
+    __shared__ int lds[SCAN_T / 64];
+    const UnitRange w = unit_range(B.plan, R, U_SCAN);
+    int a = -1;
+    for (int64_t u = w.u0; u < w.u1; ++u) {
+        if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
+        const int r = (int)w.act[a];
+        const int64_t tile = u - w.pre[a];
+        const CtlDev &C = B.ctl[r];
+        const RepDev &rp = B.rep[r];
+        const int64_t nc = rp.box.ncells;
+        const int64_t tile0 = tile * SCAN_TILE;
+        const int32_t *in = B.count[C.cnt ^ 1] + rp.c_off;
+        int32_t *out = B.slocal + rp.c_off;
+        int64_t base = tile0 + (int64_t)threadIdx.x * SCAN_I;
+        int v[SCAN_I], s = 0;
+        for (int k = 0; k < SCAN_I; ++k) {
+            v[k] = base + k < nc ? in[base + k] : 0;
+            s += v[k];
+        }
+        int tot;
+        int off = scan_block_exclusive(s, lds, &tot);
+        for (int k = 0; k < SCAN_I; ++k) {
+            if (base + k < nc) out[base + k] = off;
+            off += v[k];
+        }
+        if (threadIdx.x == 0) B.scan_sums[r * B.scan_stride + tile] = tot;
+    }
+}
+
+// S3: a unit is PT particles (tmp[start + slot] = i) or PT cells (the final cell_start =
+// tile-local offset + tile prefix); the tile prefix of a replica is scanned in LDS by every
+// block that works on it.  The block with a replica's first unit marks the buffers the
+// sort writes and zeroes the destination buffers' force slots of the members of the
+// lists saved when those buffers were last in use (only those can hold a force; the
+// gather copies forces only where they are non-zero).
+template <class F>
+__global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int R)
+{
+    using Rl = typename F::real;
+    __shared__ int pre[MAX_TILES];
+    __shared__ int lds[FT / 64];
+    const UnitRange w = unit_range(B.plan, R, U_PLACE);
+    int a = -1, r_pre = -1;
+    for (int64_t u = w.u0; u < w.u1; ++u) {
+        if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
+        const int r = (int)w.act[a];
+        const int64_t local = u - w.pre[a];
+        CtlDev &C = B.ctl[r];
+        const RepDev &rp = B.rep[r];
+        const int64_t n = rp.n, nc = rp.box.ncells;
+        if (r != r_pre) {
+            const int nt = (int)((nc + SCAN_TILE - 1) / SCAN_TILE);
+            const int32_t *sums = B.scan_sums + r * B.scan_stride;
+            const int per = (nt + FT - 1) / FT;
+            const int t0 = threadIdx.x * per;
+            int loc = 0;
+            for (int t = t0; t < t0 + per && t < nt; ++t) loc += sums[t];
+            int tot;
+            int off = scan_block_exclusive(loc, lds, &tot);
+            for (int t = t0; t < t0 + per && t < nt; ++t) {
+                pre[t] = off;
+                off += sums[t];
+            }
+            __syncthreads();
+            r_pre = r;
+        }
+        const int32_t *sl = B.slocal + rp.c_off;
+        const int64_t np_units = (n + PT - 1) / PT;
+        if (local < np_units) {
+            const int32_t *key = B.key + rp.p_off, *slot = B.slot + rp.p_off;
+            int32_t *tmp = B.tmp + rp.p_off;
+            for (int64_t i = local * PT + threadIdx.x; i < n && i < (local + 1) * PT; i += FT) {
+                const int c = key[i];
+                tmp[sl[c] + pre[c / SCAN_TILE] + slot[i]] = (int32_t)i;
+            }
+        } else {
+            int32_t *start = B.start + rp.c_off;
+            const int64_t c0 = (local - np_units) * PT;
+            for (int64_t c = c0 + threadIdx.x; c < nc && c < c0 + PT; c += FT) start[c] = sl[c] + pre[c / SCAN_TILE];
+        }
+        if (local == 0) {
+            const int dst = 1 - C.cur;
+            const int64_t po3 = 3 * rp.p_off;
+            Rl *f0 = B.force[2 * dst] + po3, *f1 = B.force[2 * dst + 1] + po3;
+            const int32_t *ms = B.mslots + ((int64_t)r * 2 + dst) * 2 * B.nmax;
+            const int mc0 = B.mcount[(r * 2 + dst) * 2], mc1 = B.mcount[(r * 2 + dst) * 2 + 1];
+            for (int64_t e = threadIdx.x; e < 2 * (int64_t)B.nmax; e += FT) {
+                const int col = (int)(e / B.nmax), k = (int)(e - (int64_t)col * B.nmax);
+                if (k >= (col ? mc1 : mc0)) continue;
+                const int q = ms[e];
+                for (int kk = 0; kk < 3; ++kk) {
+                    f0[kk * n + q] = 0;
+                    f1[kk * n + q] = 0;
+                }
+            }
+            if (threadIdx.x == 0) {
+                C.cur_next = 1 - C.cur;
+                C.cnt_next = C.cnt ^ 1;
+            }
+        }
+    }
+}
+
+// S4: stable gather into the other buffers (wrap fused), pos_old, |v|max per unit;
+// clears the previous cell counts and the is_md/is_reac marks for the list build.  A unit
+// is a tile of FT destination slots; its source indices are kept in LDS, where the rank of
+// a particle among its cell's arrivals reads them (the reference's single-thread order is
+// the source order).  Forces are copied only for particles that carry one (MD/PAST_MD
+// bits); every other destination force slot is already zero (see k_fz_place).
+template <class F>
+__global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
+{
+    using Rl = typename F::real;
     using S = typename F::small;
     __shared__ double red[16];
-    __shared__ int32_t ltmp[FT], lsrc[FT], ldst[FT];
-    const int r = blockIdx.y;
-    CtlDev &C = B.ctl[r];
-    if (!sort_active(C, forced)) return;
-    const RepDev &rp = B.rep[r];
-    const int64_t n = rp.n, nc = rp.box.ncells;
-    const int cur = C.cur, dst = 1 - cur, fp = C.fpar;
-    const int64_t po3 = 3 * rp.p_off;
-    const R *spos = B.pos[cur] + po3, *svel = B.vel[cur] + po3;
-    const R *sf = B.force[2 * cur + fp] + po3, *sfo = B.force[2 * cur + 1 - fp] + po3;
-    const int32_t *sim = B.image[cur] + po3, *sid = B.id[cur] + rp.p_off;
-    const S *ssp = B.species[cur] + rp.p_off, *sfl = B.flags[cur] + rp.p_off;
-    R *dpos = B.pos[dst] + po3, *dvel = B.vel[dst] + po3, *dold = B.pos_old + po3;
-    R *df = B.force[2 * dst + fp] + po3, *dfo = B.force[2 * dst + 1 - fp] + po3;
-    int32_t *dim = B.image[dst] + po3, *did = B.id[dst] + rp.p_off;
-    S *dsp = B.species[dst] + rp.p_off, *dfl = B.flags[dst] + rp.p_off;
-    const int32_t *tmp = B.tmp + rp.p_off, *key = B.key + rp.p_off;
-    const int32_t *start = B.start + rp.c_off, *cnt = B.count[C.cnt ^ 1] + rp.c_off;
-    int32_t *old_cnt = B.count[C.cnt] + rp.c_off;
-    const int64_t lim = n > nc ? n : nc;
-    double vm = 0;
-    for (int64_t t0 = (int64_t)blockIdx.x * FT; t0 < lim; t0 += (int64_t)gridDim.x * FT) {
+    __shared__ int32_t ltmp[FT];
+    const UnitRange w = unit_range(B.plan, R, U_GATHER);
+    int a = -1;
+    for (int64_t u = w.u0; u < w.u1; ++u) {
+        if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
+        const int r = (int)w.act[a];
+        const int64_t t0 = (u - w.pre[a]) * FT;
+        CtlDev &C = B.ctl[r];
+        const RepDev &rp = B.rep[r];
+        const int64_t n = rp.n, nc = rp.box.ncells;
+        const int cur = C.cur, dst = 1 - cur, fp = C.fpar;
+        const int64_t po3 = 3 * rp.p_off;
+        const Rl *spos = B.pos[cur] + po3, *svel = B.vel[cur] + po3;
+        const Rl *sf = B.force[2 * cur + fp] + po3, *sfo = B.force[2 * cur + 1 - fp] + po3;
+        const int32_t *sim = B.image[cur] + po3, *sid = B.id[cur] + rp.p_off;
+        const S *ssp = B.species[cur] + rp.p_off, *sfl = B.flags[cur] + rp.p_off;
+        Rl *dpos = B.pos[dst] + po3, *dvel = B.vel[dst] + po3, *dold = B.pos_old + po3;
+        Rl *df = B.force[2 * dst + fp] + po3, *dfo = B.force[2 * dst + 1 - fp] + po3;
+        int32_t *dim = B.image[dst] + po3, *did = B.id[dst] + rp.p_off;
+        S *dsp = B.species[dst] + rp.p_off, *dfl = B.flags[dst] + rp.p_off;
+        const int32_t *tmp = B.tmp + rp.p_off, *key = B.key + rp.p_off;
+        const int32_t *start = B.start + rp.c_off, *cnt = B.count[C.cnt ^ 1] + rp.c_off;
+        int32_t *old_cnt = B.count[C.cnt] + rp.c_off;
         const int64_t j = t0 + threadIdx.x;
         __syncthreads();
         const int32_t v = j < n ? tmp[j] : INT32_MAX;
         ltmp[threadIdx.x] = v;
         __syncthreads();
-        int64_t d = -1;
-        int place = 0;
+        double vm = 0;
         if (j < n) {
             const int c = key[v];
             const int s = start[c], m = cnt[c];
             int rank = 0;
             for (int q = s; q < s + m; ++q) {
                 const int64_t lq = q - t0;
-                const int32_t w = (lq >= 0 && lq < FT) ? ltmp[lq] : tmp[q];
-                rank += w < v ? 1 : 0;
+                const int32_t wv = (lq >= 0 && lq < FT) ? ltmp[lq] : tmp[q];
+                rank += wv < v ? 1 : 0;
             }
-            d = s + rank;
-        }
-        // position of this pair in source order within the tile (sources are distinct;
-        // the unused slots of a partial tile sort last)
-        for (int k = 0; k < FT; ++k) place += ltmp[k] < v ? 1 : 0;
-        if (v == INT32_MAX) place = threadIdx.x;
-        lsrc[place] = v;
-        ldst[place] = (int32_t)d;
-        __syncthreads();
-        const int32_t u = lsrc[threadIdx.x];
-        const int64_t e = ldst[threadIdx.x];
-        if (u != INT32_MAX) {
-            const int fl = (int)sfl[u];
+            const int64_t d = s + rank;
+            const int fl = (int)sfl[v];
             const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
             double v2 = 0;
             for (int k = 0; k < 3; ++k) {
-                R x = spos[k * n + u];
-                R ed = (R)rp.box.edges[k];
+                Rl x = spos[k * n + v];
+                Rl ed = (Rl)rp.box.edges[k];
                 int jump = (int)floor((double)(x / ed));
-                x = x - (R)jump * ed;
-                dpos[k * n + e] = x;
-                dold[k * n + e] = x;
-                dim[k * n + e] = sim[k * n + u] + jump;
-                R vv = svel[k * n + u];
-                dvel[k * n + e] = vv;
+                x = x - (Rl)jump * ed;
+                dpos[k * n + d] = x;
+                dold[k * n + d] = x;
+                dim[k * n + d] = sim[k * n + v] + jump;
+                Rl vv = svel[k * n + v];
+                dvel[k * n + d] = vv;
                 v2 += (double)vv * (double)vv;
                 if (has_f) {
-                    df[k * n + e] = sf[k * n + u];
-                    dfo[k * n + e] = sfo[k * n + u];
+                    df[k * n + d] = sf[k * n + v];
+                    dfo[k * n + d] = sfo[k * n + v];
                 }
             }
-            did[e] = sid[u];
-            dsp[e] = ssp[u];
-            dfl[e] = (S)fl;
-            B.memb[rp.p_off + e] = 0;
-            vm = v2 > vm ? v2 : vm;
+            did[d] = sid[v];
+            dsp[d] = ssp[v];
+            dfl[d] = (S)fl;
+            B.memb[rp.p_off + d] = 0;
+            vm = v2;
         }
         if (j < nc) {
             old_cnt[j] = 0;
             B.is_md[rp.c_off + j] = 0;
             B.is_reac[rp.c_off + j] = 1;
         }
+        const double m = block_max(vm, red);
+        if (threadIdx.x == 0) B.gpart[u] = m;
     }
-    double m = block_max(vm, red);
-    if (threadIdx.x == 0) B.part[(int64_t)r * PS + blockIdx.x] = m;
 }
 
 // S5: update_list (neighbor_list.f90:76-157) for colloid blockIdx.x, one workgroup:
@@ -783,7 +898,7 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int forced)
 // new buffer indices from cur_next/cnt_next (set by k_fz_place), so the flip by block 0
 // cannot reach the other block.
 template <class F>
-__global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forced, int ngather)
+__global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forced, int R)
 {
     __shared__ int lds[16];
     __shared__ double red[16];
@@ -879,8 +994,11 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
         if (total > P.nmax) C.error = RMPCD_ERR_OVERFLOW;
     }
     if (col != 0) return;
+    // |v|max over this replica's gather units
+    const int a = (int)plan_amap(B.plan, R)[r];
+    const int64_t *gpre = plan_pre(B.plan, R, U_GATHER);
     double all = 0;
-    for (int q = threadIdx.x; q < ngather; q += blockDim.x) all = fmax(all, B.part[(int64_t)r * PS + q]);
+    for (int64_t q = gpre[a] + threadIdx.x; q < gpre[a + 1]; q += blockDim.x) all = fmax(all, B.gpart[q]);
     all = block_max(all, red);
     if (threadIdx.x != 0) return;
     C.vmax2 = all;
@@ -1123,6 +1241,8 @@ static FB<F> make_fb(DimerSim &s)
     b.part = s.part;
     b.scan_sums = s.scan_sums;
     b.mslots = s.f_mslots;
+    b.plan = s.f_plan;
+    b.gpart = s.f_gpart;
     b.mcount = s.f_mcount;
     b.nmax = s.nmax;
     b.slocal = s.f_slocal;
@@ -1172,15 +1292,12 @@ template <class F>
 static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int forced)
 {
     const unsigned Rr = (unsigned)s.R;
-    const unsigned gp = (unsigned)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, SORT_BLOCKS);
-    const unsigned gt = (unsigned)((s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE);
-    const int64_t lim = std::max(s.nmax_rep, s.ncmax_rep);
-    const unsigned gg = (unsigned)std::min<int64_t>((lim + FT - 1) / FT, SORT_BLOCKS);
-    k_fz_key<F><<<dim3(gp, Rr), FT, 0, st>>>(b, p, forced);
-    k_fz_scan_tiles<F><<<dim3(gt, Rr), SCAN_T, 0, st>>>(b, forced);
-    k_fz_place<F><<<dim3(gp, Rr), FT, 0, st>>>(b, forced);
-    k_fz_gather<F><<<dim3(gg, Rr), FT, 0, st>>>(b, p, forced);
-    k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, (int)gg);
+    k_fz_plan<F><<<1, FT, 0, st>>>(b, forced, s.R);
+    k_fz_key<F><<<SORT_BLOCKS, FT, 0, st>>>(b, p, forced, s.R);
+    k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, s.R);
+    k_fz_place<F><<<SORT_BLOCKS, FT, 0, st>>>(b, s.R);
+    k_fz_gather<F><<<SORT_BLOCKS, FT, 0, st>>>(b, p, s.R);
+    k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, s.R);
 }
 
 template <class F>
@@ -1364,6 +1481,10 @@ int fused_prepare(DimerSim &s)
     if ((rc = s.alloc((void **)&s.scan_sums, (size_t)s.R * sstride * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_slocal, s.nctot * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_mslots, (size_t)s.R * 4 * s.nmax * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.f_plan, (1 + 4 + 2 * (size_t)s.R + 4 * ((size_t)s.R + 1)) * 8))) return rc;
+    int64_t gunits = 0;
+    for (const Replica &rp : s.rep) gunits += (std::max<int64_t>(rp.n, rp.box.ncells) + FT - 1) / FT;
+    if ((rc = s.alloc((void **)&s.f_gpart, (size_t)gunits * 8))) return rc;
     if ((rc = s.alloc((void **)&s.f_mcount, (size_t)s.R * 4 * 4))) return rc;
     if ((rc = s.alloc((void **)&s.part, (size_t)s.R * PS * 8))) return rc;
     if ((rc = s.alloc((void **)&s.rep_dev, sizeof(RepDev) * s.R))) return rc;

@@ -65,6 +65,8 @@ struct DimerSim {
     int32_t *lut = nullptr;
     int32_t *f_key = nullptr, *f_slot = nullptr, *f_tmp = nullptr, *scan_sums = nullptr, *f_slocal = nullptr;
     int32_t *f_mslots = nullptr, *f_mcount = nullptr;
+    int64_t *f_plan = nullptr;
+    double *f_gpart = nullptr;
     double *part = nullptr;
     void *scratch = nullptr;
 
