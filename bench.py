@@ -70,11 +70,14 @@ def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile, sizes=None):
 
     f = capi.FP32 if form == "fp32" else capi.FP64
     m = capi.SIM_FUSED if mode == "fused" else capi.SIM_REFERENCE
+    from rmpcdmd_amd.autotune import lookup
     if sizes is not None:
         replicas = len(sizes)
-        sim = SingleDimer(n_md=n_md, form=f, mode=m, replica_L=[(q, q, q) for q in sizes])
+        shape = lookup(form, 0, replicas, mixed=True) if mode == "fused" else {}
+        sim = SingleDimer(n_md=n_md, form=f, mode=m, replica_L=[(q, q, q) for q in sizes], **shape)
     else:
-        sim = SingleDimer(L=(L, L, L), n_md=n_md, form=f, mode=m, n_replicas=replicas)
+        shape = lookup(form, L, replicas) if mode == "fused" else {}
+        sim = SingleDimer(L=(L, L, L), n_md=n_md, form=f, mode=m, n_replicas=replicas, **shape)
     st0 = sim.state()[4]
     n_total = int(st0.n_solvent_total)
     out = {}
@@ -92,7 +95,8 @@ def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile, sizes=None):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     sec = t1 - t0
-    out.update(n=n_total, seconds=sec, ps_per_s=n_total * n_md * steps / sec, ms_per_cycle=1e3 * sec / steps)
+    out.update(n=n_total, seconds=sec, ps_per_s=n_total * n_md * steps / sec, ms_per_cycle=1e3 * sec / steps,
+               launch_shape=shape or "defaults")
     extra = 0
     for r in range(replicas):
         extra += int(sim.state(r)[4].n_extra_sorting)
@@ -198,6 +202,7 @@ def main():
             "form": "fp32-mixed (FP32 solvent, FP64 colloids/forces/RATTLE)" if a.form == "fp32" else "fp64",
             "mode": a.mode,
             "extra_sorts": main_run["extra_sorts"],
+            "launch_shape": main_run["launch_shape"],
         },
         "roofline": roofline,
         "cpu_baseline": None,

@@ -273,6 +273,10 @@ typedef struct rmpcd_dimer_config {
     int32_t n_replicas;   /* independent systems advanced together (0 or 1: one) */
     uint64_t seed;        /* replica r uses seed + r */
     const int32_t *replica_L;  /* host, 3 per replica; NULL: every replica uses L */
+    int32_t stream_blocks;     /* fused mode launch shape, 0: default (16384): stream-role
+                                  workgroups per replica of the step kernel */
+    int32_t sort_blocks;       /* fused mode launch shape, 0: default (4096): workgroups of
+                                  the key, place and gather passes */
 } rmpcd_dimer_config_t;
 typedef struct rmpcd_dimer_stats {
     int64_t n_solvent, md_steps, cycles, n_extra_sorting;  /* of the replica asked for */

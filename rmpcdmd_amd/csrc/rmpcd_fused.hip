@@ -31,8 +31,8 @@ namespace rmpcd {
 
 constexpr int FT = 256;            // threads per block
 constexpr int PS = 32768;          // partial-sum slots per replica
-constexpr int STREAM_BLOCKS = 16384;  // stream-role blocks per replica (grid-stride beyond)
-constexpr int SORT_BLOCKS = 4096;     // blocks of the particle passes of a sort (all replicas)
+constexpr int STREAM_BLOCKS = 16384;  // stream-role blocks per replica, default and maximum
+constexpr int SORT_BLOCKS = 4096;     // blocks of the particle passes of a sort (all replicas), default
 constexpr int SCAN_BLOCKS = 1024;     // blocks of the cell-count tile scan (all replicas)
 constexpr int CT = 128;            // cells per block in the collision kernel
 constexpr int CTILE = 2048;        // staged particles per collision block
@@ -1285,7 +1285,7 @@ static FP make_fp(const DimerSim &s)
 static int member_blocks(const DimerSim &s) { return (int)((2 * (int64_t)s.nmax + FT - 1) / FT); }
 static int stream_blocks(const DimerSim &s)
 {
-    return (int)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, STREAM_BLOCKS);
+    return (int)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, s.stream_blocks);
 }
 
 template <class F>
@@ -1293,10 +1293,10 @@ static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t s
 {
     const unsigned Rr = (unsigned)s.R;
     k_fz_plan<F><<<1, FT, 0, st>>>(b, forced, s.R);
-    k_fz_key<F><<<SORT_BLOCKS, FT, 0, st>>>(b, p, forced, s.R);
+    k_fz_key<F><<<s.sort_blocks, FT, 0, st>>>(b, p, forced, s.R);
     k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, s.R);
-    k_fz_place<F><<<SORT_BLOCKS, FT, 0, st>>>(b, s.R);
-    k_fz_gather<F><<<SORT_BLOCKS, FT, 0, st>>>(b, p, s.R);
+    k_fz_place<F><<<s.sort_blocks, FT, 0, st>>>(b, s.R);
+    k_fz_gather<F><<<s.sort_blocks, FT, 0, st>>>(b, p, s.R);
     k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, s.R);
 }
 
@@ -1467,8 +1467,12 @@ int fused_prepare(DimerSim &s)
     size_t RS = s.cfg.form == RMPCD_FP64 ? 8 : 4;
     int rc;
     // every per-block partial of a replica must fit its PS slots
+    s.stream_blocks = s.cfg.stream_blocks > 0 ? s.cfg.stream_blocks : STREAM_BLOCKS;
+    s.sort_blocks = s.cfg.sort_blocks > 0 ? s.cfg.sort_blocks : SORT_BLOCKS;
+    RMPCD_CHECK(s.stream_blocks <= STREAM_BLOCKS && s.sort_blocks <= 65536,
+                "dimer: stream_blocks must be at most 16384 and sort_blocks at most 65536");
     RMPCD_CHECK((s.ncmax_rep + CT - 1) / CT <= PS && (int64_t)member_blocks(s) * MPV + STREAM_BLOCKS <= PS &&
-                    SORT_BLOCKS <= PS && member_blocks(s) <= MAX_MEMBER_BLOCKS && s.n_stencil <= MAX_STENCIL &&
+                    member_blocks(s) <= MAX_MEMBER_BLOCKS && s.n_stencil <= MAX_STENCIL &&
                     (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE <= MAX_TILES,
                 "dimer: box or list too large for the fused kernels");
     if ((rc = s.alloc(&s.sol_pos_old, 3 * s.ntot * RS))) return rc;
