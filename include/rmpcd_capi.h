@@ -277,6 +277,11 @@ typedef struct rmpcd_dimer_config {
                                   workgroups per replica of the step kernel */
     int32_t sort_blocks;       /* fused mode launch shape, 0: default (4096): workgroups of
                                   the key, place and gather passes */
+    int32_t lazy;              /* fused mode: 1 = particles outside the neighbour lists are
+                                  moved only when read (at a sort or an exact skin test), by
+                                  replaying their per-step additions; the same trajectory
+                                  bit for bit, without the per-step pass over all solvent */
+    int32_t pad_;
 } rmpcd_dimer_config_t;
 typedef struct rmpcd_dimer_stats {
     int64_t n_solvent, md_steps, cycles, n_extra_sorting;  /* of the replica asked for */

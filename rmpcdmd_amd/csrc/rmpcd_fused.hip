@@ -32,6 +32,7 @@ namespace rmpcd {
 constexpr int FT = 256;            // threads per block
 constexpr int PS = 32768;          // partial-sum slots per replica
 constexpr int STREAM_BLOCKS = 16384;  // stream-role blocks per replica, default and maximum
+constexpr int LAZY_STREAM_BLOCKS = 1024;  // stream-role blocks per replica in lazy mode
 constexpr int SORT_BLOCKS = 4096;     // blocks of the particle passes of a sort (all replicas), default
 constexpr int SCAN_BLOCKS = 1024;     // blocks of the cell-count tile scan (all replicas)
 constexpr int CT = 128;            // cells per block in the collision kernel
@@ -70,7 +71,7 @@ struct FB {
 struct FP {
     double dt, dt_sq, skin, d, prob, bulk_rate, tau, slack;
     double mass[2];
-    int nmax, n_stencil, bulk, pad;
+    int nmax, n_stencil, bulk, lazy;
     double radius_sq;
     LJ sc, cc;
 };
@@ -271,6 +272,27 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
         const int64_t gs = (int64_t)(gridDim.x - nbm) * FT;
         const S *spc = B.species[cur] + rp.p_off;
         double mx = 0;
+        if (P.lazy) {
+            // non-members keep the position of the last sort (pos == pos_old) until a sort
+            // reads them; an exact skin test replays their k additions x = x + dt v
+            if (!exact) return;
+            const int k = C.k_since + 1;
+            for (int64_t i = (int64_t)(blockIdx.x - nbm) * FT + threadIdx.x; i < n; i += gs) {
+                if (memb[i] || (int)spc[i] < 0) continue;
+                double q2 = 0;
+                for (int c = 0; c < 3; ++c) {
+                    const R t = rdt * vel[c * n + i];
+                    R x = pos[c * n + i];
+                    for (int s2 = 0; s2 < k; ++s2) x = x + t;
+                    const double dd = (double)(x - po[c * n + i]);
+                    q2 = c == 0 ? dd * dd : (c == 1 ? q2 + dd * dd : q2 + dd * dd);
+                }
+                mx = q2 > mx ? q2 : mx;
+            }
+            const double m = block_max(mx, red);
+            if (threadIdx.x == 0) part[moff + (blockIdx.x - nbm)] = m;
+            return;
+        }
         if (!exact && rp.vec) {
             // 16-byte loads and stores: VW particles per thread and plane (the replica's
             // planes and membership bytes are 16-byte aligned)
@@ -627,8 +649,9 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
             for (int k = 0; k < 3; ++k) b.origin[k] = C.origin[k];
         }
         const int64_t n = rp.n;
-        const Rl *pos = B.pos[C.cur] + 3 * rp.p_off;
+        Rl *pos = B.pos[C.cur] + 3 * rp.p_off;
         int32_t *cnt = B.count[C.cnt ^ 1] + rp.c_off;
+        const int lazy_k = P.lazy ? C.k_since : 0;
         const int32_t *lut = B.lut + rp.lut_off;
         for (int q = threadIdx.x; q < KHASH; q += FT) {
             hkey[q] = -1;
@@ -643,6 +666,16 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
             loc[v] = 0;
             if (i >= n) continue;
             Rl x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+            if (lazy_k > 0 && !B.memb[rp.p_off + i]) {
+                // replay the k_since additions x = x + dt v of the lazily moved particle
+                const Rl rdt = (Rl)P.dt;
+                const Rl *vel = B.vel[C.cur] + 3 * rp.p_off;
+                for (int k = 0; k < 3; ++k) {
+                    const Rl t = rdt * vel[k * n + i];
+                    for (int s2 = 0; s2 < lazy_k; ++s2) x[k] = x[k] + t;
+                    pos[k * n + i] = x[k];
+                }
+            }
             for (int k = 0; k < 3; ++k) {
                 Rl e = (Rl)b.edges[k];
                 int jump = (int)floor((double)(x[k] / e));
@@ -1263,6 +1296,7 @@ static FP make_fp(const DimerSim &s)
     p.prob = s.cfg.prob;
     p.bulk_rate = s.cfg.bulk_rate;
     p.tau = s.cfg.tau;
+    p.lazy = s.cfg.lazy ? 1 : 0;
     p.mass[0] = s.mass[0];
     p.mass[1] = s.mass[1];
     p.nmax = s.nmax;
@@ -1285,7 +1319,9 @@ static FP make_fp(const DimerSim &s)
 static int member_blocks(const DimerSim &s) { return (int)((2 * (int64_t)s.nmax + FT - 1) / FT); }
 static int stream_blocks(const DimerSim &s)
 {
-    return (int)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, s.stream_blocks);
+    // lazy mode: the stream role only measures displacements, and only for an exact test
+    const int cap = s.cfg.lazy ? std::min(s.stream_blocks, LAZY_STREAM_BLOCKS) : s.stream_blocks;
+    return (int)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, cap);
 }
 
 template <class F>

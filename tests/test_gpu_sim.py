@@ -159,3 +159,23 @@ def test_batched_replicas_of_the_c4_shape(gpu):
     ref.close()
     fus.close()
     one.close()
+
+
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+def test_lazy_streaming_equals_eager(gpu, form):
+    """lazy mode replays the per-step additions of the particles outside the lists when a
+    sort reads them: the trajectory must be the eager one bit for bit, for every replica"""
+    Ls = [(16, 16, 16), (16, 12, 14)]
+    kw = dict(n_md=20, tau=0.4, form=form, seed=23, replica_L=Ls, mode=capi.SIM_FUSED)
+    eager = SingleDimer(**kw)
+    lazy = SingleDimer(lazy=1, **kw)
+    for cycle in range(3):
+        eager.run(1)
+        lazy.run(1)
+        for r in range(len(Ls)):
+            a, b = eager.download(r), lazy.download(r)
+            compare(a, b, "replica %d cycle %d" % (r, cycle), rtol=0.0)
+            assert a["pos_old"].tobytes() == b["pos_old"].tobytes()
+    assert a["n_extra_sorting"] >= 1
+    eager.close()
+    lazy.close()
