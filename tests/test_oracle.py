@@ -509,3 +509,72 @@ def test_numpy_threefry_matches_library_and_sphere():
     # the shift draws are uniforms in [0, 1)
     u = np.array([T.shift_uniforms(5, s) for s in range(200)])
     assert (u >= 0).all() and (u < 1).all() and abs(u.mean() - 0.5) < 0.05
+
+
+def _program_state(L, rho, nc, seed, sigma):
+    """a synthetic download-style state (what the device programs hand to the oracle
+    loops): sorted solvent outside the colloid cutoffs, colloids apart"""
+    rng = np.random.default_rng(seed)
+    c = O.cells_init(L)
+    n = rho * int(np.prod(L))
+    ps = O.PState(n)
+    cut = sigma * 2 ** (1 / 6)
+    cp = np.array(L, float) / 2 + np.zeros((nc, 3))
+    if nc > 1:
+        cp = rng.random((nc, 3)) * np.array(L)
+    pos = rng.random((n, 3)) * np.array(L)
+    for _ in range(50):
+        bad = np.zeros(n, bool)
+        for j in range(nc):
+            d = pos - cp[j]
+            d -= np.round(d / np.array(L)) * np.array(L)
+            bad |= (d * d).sum(axis=1) < cut * cut
+        if not bad.any():
+            break
+        pos[bad] = rng.random((bad.sum(), 3)) * np.array(L)
+    ps.pos[:] = pos
+    ps.vel[:] = rng.normal(size=(n, 3))
+    ps.vel -= ps.vel.mean(axis=0)
+    srt, _, cnt, st = O.sort(c, ps)
+    d = dict(n_solvent=n, pos=srt.pos, vel=srt.vel, force=np.zeros((n, 3)), force_old=np.zeros((n, 3)),
+             image=srt.image, id=srt.id, species=srt.species, flags=srt.flags, pos_old=srt.pos.copy(),
+             cell_count=cnt, cell_start=st, origin=np.zeros(3), md_steps=0, cycles=0, n_extra_sorting=0,
+             e1=0.0, e2=0.0, temperature=0.0, n_colloids=nc, nmax=6000, mass=50.0, last_sort=0)
+    if nc:
+        for k in ("vel", "force", "force_old"):
+            d["colloid_" + k] = np.zeros((nc, 3))
+        d["colloid_pos"] = cp
+        d["colloid_image"] = np.zeros((nc, 3), np.int32)
+        d["colloid_pos_old"] = cp.copy()
+    return d
+
+
+def test_oracle_program_loops_conserve():
+    """the oracle's driver loops (the checker of the device programs): the fluid loop keeps
+    the total momentum and the energy of SRD + free streaming; the many-colloid loop keeps
+    the total momentum (pairwise LJ, SRD); the fixed-sphere loop converts A to B at the
+    sphere and keeps every particle in the box bookkeeping (id set unchanged)"""
+    from oracle.programs import ColloidsOracle, FluidOracle, SphereOracle
+    st = _program_state((8, 6, 10), 10, 0, 1, 1.0)
+    f = FluidOracle(st, (8, 6, 10), 0.1, 2, 3)
+    p0, e0 = f.s.vel.sum(axis=0), (f.s.vel ** 2).sum()
+    for _ in range(3):
+        f.cycle()
+    assert np.allclose(f.s.vel.sum(axis=0), p0, atol=1e-9)
+    assert abs((f.s.vel ** 2).sum() - e0) < 1e-9 * e0
+    assert sorted(f.s.id.tolist()) == list(range(f.s.n))
+    st = _program_state((12, 12, 12), 10, 1, 2, 2.0)
+    s = SphereOracle(st, (12, 12, 12), 0.5, 10, 4, sigma=2.0, bulk_rate=0.01)
+    for _ in range(2):
+        s.cycle()
+    assert sorted(s.s.id.tolist()) == list(range(s.s.n))
+    assert (s.s.species == 2).sum() > 0
+    st = _program_state((16, 16, 16), 10, 4, 3, 1.5)
+    c = ColloidsOracle(st, (16, 16, 16), 0.2, 10, 5, sigma=1.5)
+    mass = float(st["mass"])
+    p0 = c.s.vel.sum(axis=0) + mass * c.col.vel.sum(axis=0)
+    for _ in range(2):
+        c.cycle()
+    p1 = c.s.vel.sum(axis=0) + mass * c.col.vel.sum(axis=0)
+    assert np.abs(p1 - p0).max() / c.s.n < 1e-12
+    assert c.last_sort == c.md_steps
