@@ -76,6 +76,9 @@ struct FP {
     LJ sc, cc;
 };
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
 __device__ inline int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
 __device__ inline int64_t gindex() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -294,18 +297,19 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
             return;
         }
         if (!exact && rp.vec) {
-            // 16-byte loads and stores: VW particles per thread and plane (the replica's
-            // planes and membership bytes are 16-byte aligned)
+            // 16-byte non-temporal loads and stores: VW particles per thread and plane (the
+            // replica's planes and membership bytes are 16-byte aligned).  Nothing read
+            // here is read again in this launch, so the lines need not stay in L2.
             constexpr int VW = 16 / sizeof(R);
-            using V = typename std::conditional<sizeof(R) == 4, float4, double2>::type;
+            using V = typename std::conditional<sizeof(R) == 4, v4f, v2d>::type;
+            using M = typename std::conditional<sizeof(R) == 4, uint32_t, uint16_t>::type;
             const int64_t nv = n / VW;
             for (int64_t q = (int64_t)(blockIdx.x - nbm) * FT + threadIdx.x; q < nv; q += gs) {
-                uint32_t mb = 0;
-                for (int u = 0; u < VW; ++u) mb |= (uint32_t)memb[q * VW + u] << (8 * u);
+                const uint32_t mb = (uint32_t)reinterpret_cast<const M *>(memb)[q];
                 V pv[3], vv[3];
                 for (int k = 0; k < 3; ++k) {
-                    pv[k] = reinterpret_cast<const V *>(pos + k * n)[q];
-                    vv[k] = reinterpret_cast<const V *>(vel + k * n)[q];
+                    pv[k] = __builtin_nontemporal_load(reinterpret_cast<const V *>(pos + k * n) + q);
+                    vv[k] = __builtin_nontemporal_load(reinterpret_cast<const V *>(vel + k * n) + q);
                 }
                 for (int k = 0; k < 3; ++k) {
                     R *pe = reinterpret_cast<R *>(&pv[k]);
@@ -314,7 +318,8 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
                         if (!((mb >> (8 * u)) & 0xffu)) pe[u] = pe[u] + rdt * ve[u];
                 }
                 if (!mb) {
-                    for (int k = 0; k < 3; ++k) reinterpret_cast<V *>(pos + k * n)[q] = pv[k];
+                    for (int k = 0; k < 3; ++k)
+                        __builtin_nontemporal_store(pv[k], reinterpret_cast<V *>(pos + k * n) + q);
                 } else {
                     // a member is streamed by its member thread: write the others only
                     for (int k = 0; k < 3; ++k) {
