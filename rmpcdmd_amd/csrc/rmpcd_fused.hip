@@ -40,16 +40,49 @@ constexpr int CTILE = 2048;        // staged particles per collision block
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
 constexpr int MAX_STENCIL = 2048;  // stencil cells of the fused list build
+constexpr int GPT = 2, GT = FT * GPT;  // destination slots per thread and per unit of the gather
 constexpr int KPT = 4, KCHUNK = FT * KPT;  // particles per thread and per chunk of the key pass
 constexpr int KHASH_BITS = 11, KHASH = 1 << KHASH_BITS;  // LDS hash entries, 2 x KCHUNK
 
 
+// The staging record of a sort: a particle's wrapped state, written by the key pass in
+// source order and read whole by the gather (one or two cache lines per particle where
+// the planes would cost one line per component).
+template <class Rl, class S>
+struct SortRec;
+template <>
+struct alignas(16) SortRec<float, uint8_t> {
+    float p[3], v[3];
+    int32_t im[3], id, key;
+    uint32_t sf;  // species | flags << 8
+    __device__ int species() const { return (int)(sf & 0xffu); }
+    __device__ int flags() const { return (int)(sf >> 8); }
+    __device__ void set_sf(int sp, int fl) { sf = (uint32_t)(sp & 0xff) | ((uint32_t)(fl & 0xff) << 8); }
+};
+template <>
+struct alignas(16) SortRec<double, int32_t> {
+    double p[3], v[3];
+    int32_t im[3], id, key, sp, fl, pad;
+    __device__ int species() const { return sp; }
+    __device__ int flags() const { return fl; }
+    __device__ void set_sf(int s_, int f_) { sp = s_; fl = f_; pad = 0; }
+};
+static_assert(sizeof(SortRec<float, uint8_t>) == 48, "FP32 sort record");
+static_assert(sizeof(SortRec<double, int32_t>) == 80, "FP64 sort record");
+
 template <class F>
 struct FB {
+    using Rec = SortRec<typename F::real, typename F::small>;
     typename F::real *pos[2], *vel[2], *force[4], *pos_old;
+    Rec *rec;
     int32_t *image[2], *id[2];
     typename F::small *species[2], *flags[2];
-    int32_t *id_to_idx, *key, *slot, *tmp;
+    int32_t *id_to_idx, *key, *slot;
+    // After an extra sort the particles stay where they are: perm[d] is the storage index
+    // of the particle at reference position d, inv its inverse, pair the (reference
+    // position, storage index) of each slot that the extra sort's place pass fills.
+    int32_t *perm, *inv;
+    int2 *pair;
     uint8_t *memb;
     int32_t *count[2], *start;
     uint8_t *is_md, *is_reac;
@@ -538,7 +571,7 @@ __device__ inline int64_t units_of(const RepDev &rp, int kind)
     case U_PLACE:
         return (n + PT - 1) / PT + (nc + PT - 1) / PT;
     default:
-        return ((n > nc ? n : nc) + FT - 1) / FT;
+        return ((n > nc ? n : nc) + GT - 1) / GT;
     }
 }
 
@@ -624,8 +657,9 @@ __device__ inline UnitRange unit_range(const int64_t *plan, int R, int kind)
     return w;
 }
 
-// S1: grid shift (collision sort), periodic wrap, key and slot in the cell.  A unit is
-// a chunk of KCHUNK consecutive particles: its keys are counted in an LDS hash table,
+// S1: grid shift (collision sort), periodic wrap, key and slot in the cell, and the
+// particle's staging record for the gather.  A unit is a chunk of KCHUNK consecutive
+// particles: its keys are counted in an LDS hash table,
 // then one global atomic per distinct key reserves the chunk's slots in that cell
 // (arrival order within a cell is made stable by the gather).  The block with a
 // replica's first chunk records the grid shift and wraps the colloids.
@@ -634,6 +668,7 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
 {
     using Rl = typename F::real;
     __shared__ int hkey[KHASH], hcnt[KHASH];
+    __shared__ double red[16];
     const UnitRange w = unit_range(B.plan, R, U_KEY);
     int a = -1;
     for (int64_t u = w.u0; u < w.u1; ++u) {
@@ -654,16 +689,21 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
             for (int k = 0; k < 3; ++k) b.origin[k] = C.origin[k];
         }
         const int64_t n = rp.n;
-        Rl *pos = B.pos[C.cur] + 3 * rp.p_off;
+        Rl *pos = B.pos[C.cur] + 3 * rp.p_off, *pold = B.pos_old + 3 * rp.p_off;
+        const Rl *vel = B.vel[C.cur] + 3 * rp.p_off;
+        int32_t *img = B.image[C.cur] + 3 * rp.p_off;
+        const int32_t *ids = B.id[C.cur] + rp.p_off;
+        const typename F::small *spc = B.species[C.cur] + rp.p_off, *flg = B.flags[C.cur] + rp.p_off;
+        uint8_t *memb = B.memb + rp.p_off;
+        typename FB<F>::Rec *rec = B.rec + rp.p_off;
         int32_t *cnt = B.count[C.cnt ^ 1] + rp.c_off;
         const int lazy_k = P.lazy ? C.k_since : 0;
+        // before the first sort the bound is not set: take every particle's speed
+        const bool all_v = !(C.vmax2 < INFINITY);
         const int32_t *lut = B.lut + rp.lut_off;
-        for (int q = threadIdx.x; q < KHASH; q += FT) {
-            hkey[q] = -1;
-            hcnt[q] = 0;
-        }
-        __syncthreads();
+        const Rl rdt = (Rl)P.dt;
         int keys[KPT], ent[KPT], loc[KPT];
+        double vm = 0;  // extra sort: |v|^2 maximum over the chunk's list members (all_v: all)
         for (int v = 0; v < KPT; ++v) {
             const int64_t i = c0 + v * FT + threadIdx.x;
             keys[v] = -1;
@@ -671,32 +711,106 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
             loc[v] = 0;
             if (i >= n) continue;
             Rl x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
-            if (lazy_k > 0 && !B.memb[rp.p_off + i]) {
-                // replay the k_since additions x = x + dt v of the lazily moved particle
-                const Rl rdt = (Rl)P.dt;
-                const Rl *vel = B.vel[C.cur] + 3 * rp.p_off;
-                for (int k = 0; k < 3; ++k) {
-                    const Rl t = rdt * vel[k * n + i];
-                    for (int s2 = 0; s2 < lazy_k; ++s2) x[k] = x[k] + t;
-                    pos[k * n + i] = x[k];
+            int jmp[3];
+            if (forced) {
+                const Rl vl[3] = {vel[i], vel[n + i], vel[2 * n + i]};
+                if (lazy_k > 0 && !memb[i]) {
+                    // replay the k_since additions x = x + dt v of the lazily moved particle
+                    for (int k = 0; k < 3; ++k) {
+                        const Rl t = rdt * vl[k];
+                        for (int s2 = 0; s2 < lazy_k; ++s2) x[k] = x[k] + t;
+                    }
                 }
+                for (int k = 0; k < 3; ++k) {
+                    Rl e = (Rl)b.edges[k];
+                    int jump = (int)floor((double)(x[k] / e));
+                    x[k] = x[k] - (Rl)jump * e;
+                    jmp[k] = jump;
+                }
+                // the staging record: wrapped position (apply_pbc, md.f90:202-217), image,
+                // and the rest of the particle's state
+                typename FB<F>::Rec q;
+                for (int k = 0; k < 3; ++k) {
+                    q.p[k] = x[k];
+                    q.v[k] = vl[k];
+                    q.im[k] = img[k * n + i] + jmp[k];
+                }
+                q.id = ids[i];
+                q.set_sf((int)spc[i], (int)flg[i]);
+                int key;
+                {
+                    int p0, p1, p2;
+                    cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
+                    if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
+                        (unsigned)p2 < (unsigned)b.L[2])
+                        key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+                    else
+                        key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+                }
+                q.key = key;
+                keys[v] = key;
+                rec[i] = q;
+            } else {
+                // extra sort: the particle stays in place; apply_pbc and pos_old = pos
+                // (single_dimer_pbc.f90:325-335) are done here
+                const int mb = memb[i];
+                bool write = false;
+                if (lazy_k > 0 && !mb) {
+                    for (int k = 0; k < 3; ++k) {
+                        const Rl t = rdt * vel[k * n + i];
+                        for (int s2 = 0; s2 < lazy_k; ++s2) x[k] = x[k] + t;
+                    }
+                    write = true;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    Rl e = (Rl)b.edges[k];
+                    int jump = (int)floor((double)(x[k] / e));
+                    x[k] = x[k] - (Rl)jump * e;
+                    if (jump != 0) {
+                        img[k * n + i] += jump;
+                        write = true;
+                    }
+                }
+                for (int k = 0; k < 3; ++k) {
+                    if (write) pos[k * n + i] = x[k];
+                    pold[k * n + i] = x[k];
+                }
+                if (mb || all_v) {
+                    double v2 = 0;
+                    for (int k = 0; k < 3; ++k) {
+                        const double vv = (double)vel[k * n + i];
+                        v2 += vv * vv;
+                    }
+                    vm = v2 > vm ? v2 : vm;
+                    if (mb) memb[i] = 0;
+                }
+                int key;
+                {
+                    int p0, p1, p2;
+                    cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
+                    if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
+                        (unsigned)p2 < (unsigned)b.L[2])
+                        key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+                    else
+                        key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+                }
+                keys[v] = key;
             }
-            for (int k = 0; k < 3; ++k) {
-                Rl e = (Rl)b.edges[k];
-                int jump = (int)floor((double)(x[k] / e));
-                x[k] = x[k] - (Rl)jump * e;
-            }
-            int key;
-            {
-                int p0, p1, p2;
-                cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
-                if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
-                    (unsigned)p2 < (unsigned)b.L[2])
-                    key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
-                else
-                    key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
-            }
-            keys[v] = key;
+            B.key[rp.p_off + i] = keys[v];
+        }
+        if (!forced) {
+            const double m = block_max(vm, red);
+            if (threadIdx.x == 0) B.gpart[u] = m;
+        }
+        // the LDS hash after all of the thread's loads are issued
+        for (int q = threadIdx.x; q < KHASH; q += FT) {
+            hkey[q] = -1;
+            hcnt[q] = 0;
+        }
+        __syncthreads();
+        for (int v = 0; v < KPT; ++v) {
+            const int key = keys[v];
+            if (key < 0) continue;
             // open addressing; the table has twice as many entries as a chunk has keys
             unsigned h = ((unsigned)key * 2654435761u) >> (32 - KHASH_BITS);
             for (int probe = 0; probe < KHASH; ++probe) {
@@ -714,7 +828,6 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
         for (int v = 0; v < KPT; ++v) {
             const int64_t i = c0 + v * FT + threadIdx.x;
             if (keys[v] < 0) continue;
-            B.key[rp.p_off + i] = keys[v];
             B.slot[rp.p_off + i] = hcnt[ent[v]] + loc[v];
         }
         __syncthreads();
@@ -770,14 +883,16 @@ __global__ void __launch_bounds__(SCAN_T) k_fz_scan_tiles(FB<F> B, int R)
     }
 }
 
-// S3: a unit is PT particles (tmp[start + slot] = i) or PT cells (the final cell_start =
-// tile-local offset + tile prefix); the tile prefix of a replica is scanned in LDS by every
-// block that works on it.  The block with a replica's first unit marks the buffers the
-// sort writes and zeroes the destination buffers' force slots of the members of the
-// lists saved when those buffers were last in use (only those can hold a force; the
-// gather copies forces only where they are non-zero).
+// S3: a unit is PT particles (pair[start + slot] = (reference position, storage index))
+// or PT cells (the final
+// cell_start = tile-local offset + tile prefix); the tile prefix of a replica is scanned
+// in LDS by every block that works on it.  The block with a replica's first unit marks
+// the buffers the sort writes; in a collision sort it also zeroes the destination
+// buffers' force slots of the members of the lists saved when those buffers were last in
+// use (only those can hold a force; the gather copies forces only where they are
+// non-zero).
 template <class F>
-__global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int R)
+__global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int forced, int R)
 {
     using Rl = typename F::real;
     __shared__ int pre[MAX_TILES];
@@ -810,18 +925,26 @@ __global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int R)
         const int32_t *sl = B.slocal + rp.c_off;
         const int64_t np_units = (n + PT - 1) / PT;
         if (local < np_units) {
-            const int32_t *key = B.key + rp.p_off, *slot = B.slot + rp.p_off;
-            int32_t *tmp = B.tmp + rp.p_off;
+            // a slot's entry: the particle's reference position (its storage index unless
+            // an extra sort left the particles in place) and its storage index
+            const int32_t *key = B.key + rp.p_off, *slot = B.slot + rp.p_off, *inv = B.inv + rp.p_off;
+            int2 *pair = B.pair + rp.p_off;
+            const bool virt_in = C.virt != 0;
             for (int64_t i = local * PT + threadIdx.x; i < n && i < (local + 1) * PT; i += FT) {
                 const int c = key[i];
-                tmp[sl[c] + pre[c / SCAN_TILE] + slot[i]] = (int32_t)i;
+                pair[sl[c] + pre[c / SCAN_TILE] + slot[i]] = make_int2(virt_in ? inv[i] : (int32_t)i, (int32_t)i);
             }
         } else {
             int32_t *start = B.start + rp.c_off;
             const int64_t c0 = (local - np_units) * PT;
             for (int64_t c = c0 + threadIdx.x; c < nc && c < c0 + PT; c += FT) start[c] = sl[c] + pre[c / SCAN_TILE];
         }
-        if (local == 0) {
+        if (local == 0 && !forced) {
+            if (threadIdx.x == 0) {
+                C.cur_next = C.cur;  // an extra sort leaves the particles in place
+                C.cnt_next = C.cnt ^ 1;
+            }
+        } else if (local == 0) {
             const int dst = 1 - C.cur;
             const int64_t po3 = 3 * rp.p_off;
             Rl *f0 = B.force[2 * dst] + po3, *f1 = B.force[2 * dst + 1] + po3;
@@ -844,89 +967,168 @@ __global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int R)
     }
 }
 
-// S4: stable gather into the other buffers (wrap fused), pos_old, |v|max per unit;
-// clears the previous cell counts and the is_md/is_reac marks for the list build.  A unit
-// is a tile of FT destination slots; its source indices are kept in LDS, where the rank of
-// a particle among its cell's arrivals reads them (the reference's single-thread order is
-// the source order).  Forces are copied only for particles that carry one (MD/PAST_MD
-// bits); every other destination force slot is already zero (see k_fz_place).
+// S4: stable gather of the staging records into the other buffers, pos_old, |v|max per
+// unit; clears the previous cell counts and the is_md/is_reac marks for the list build.  A
+// unit is a tile of GT destination slots (GPT per thread, their loads issued together);
+// the slots' reference positions are kept in LDS, where the rank of a particle among its
+// cell's arrivals reads them (the reference's single-thread sort keeps the order of the
+// reference positions within a cell).  Forces are copied only for particles that carry
+// one (MD/PAST_MD bits); every other destination force slot is already zero (see
+// k_fz_place).
 template <class F>
 __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
 {
     using Rl = typename F::real;
     using S = typename F::small;
     __shared__ double red[16];
-    __shared__ int32_t ltmp[FT];
+    __shared__ int32_t ltmp[GT];
     const UnitRange w = unit_range(B.plan, R, U_GATHER);
     int a = -1;
     for (int64_t u = w.u0; u < w.u1; ++u) {
         if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
         const int r = (int)w.act[a];
-        const int64_t t0 = (u - w.pre[a]) * FT;
+        const int64_t t0 = (u - w.pre[a]) * GT;
         CtlDev &C = B.ctl[r];
         const RepDev &rp = B.rep[r];
         const int64_t n = rp.n, nc = rp.box.ncells;
         const int cur = C.cur, dst = 1 - cur, fp = C.fpar;
         const int64_t po3 = 3 * rp.p_off;
-        const Rl *spos = B.pos[cur] + po3, *svel = B.vel[cur] + po3;
+        const typename FB<F>::Rec *rec = B.rec + rp.p_off;
         const Rl *sf = B.force[2 * cur + fp] + po3, *sfo = B.force[2 * cur + 1 - fp] + po3;
-        const int32_t *sim = B.image[cur] + po3, *sid = B.id[cur] + rp.p_off;
-        const S *ssp = B.species[cur] + rp.p_off, *sfl = B.flags[cur] + rp.p_off;
         Rl *dpos = B.pos[dst] + po3, *dvel = B.vel[dst] + po3, *dold = B.pos_old + po3;
         Rl *df = B.force[2 * dst + fp] + po3, *dfo = B.force[2 * dst + 1 - fp] + po3;
         int32_t *dim = B.image[dst] + po3, *did = B.id[dst] + rp.p_off;
         S *dsp = B.species[dst] + rp.p_off, *dfl = B.flags[dst] + rp.p_off;
-        const int32_t *tmp = B.tmp + rp.p_off, *key = B.key + rp.p_off;
+        const int2 *pair = B.pair + rp.p_off;
         const int32_t *start = B.start + rp.c_off, *cnt = B.count[C.cnt ^ 1] + rp.c_off;
         int32_t *old_cnt = B.count[C.cnt] + rp.c_off;
-        const int64_t j = t0 + threadIdx.x;
+        // GPT destination slots per thread, their loads issued together; a slot holds the
+        // particle's reference position (the rank's order) and its storage index
+        int32_t v[GPT], vs[GPT];
+        typename FB<F>::Rec q[GPT];
+        int s0[GPT], m0[GPT];
         __syncthreads();
-        const int32_t v = j < n ? tmp[j] : INT32_MAX;
-        ltmp[threadIdx.x] = v;
+        for (int h = 0; h < GPT; ++h) {
+            const int64_t j = t0 + h * FT + threadIdx.x;
+            const int2 pr = j < n ? pair[j] : make_int2(INT32_MAX, 0);
+            v[h] = pr.x;
+            vs[h] = pr.y;
+            ltmp[h * FT + threadIdx.x] = v[h];
+        }
+        for (int h = 0; h < GPT; ++h)
+            if (v[h] != INT32_MAX) q[h] = rec[vs[h]];
+        for (int h = 0; h < GPT; ++h) {
+            if (v[h] == INT32_MAX) continue;
+            s0[h] = start[q[h].key];
+            m0[h] = cnt[q[h].key];
+        }
         __syncthreads();
         double vm = 0;
-        if (j < n) {
-            const int c = key[v];
-            const int s = start[c], m = cnt[c];
+        for (int h = 0; h < GPT; ++h) {
+            if (v[h] == INT32_MAX) continue;
+            const int s = s0[h], m = m0[h];
             int rank = 0;
-            for (int q = s; q < s + m; ++q) {
-                const int64_t lq = q - t0;
-                const int32_t wv = (lq >= 0 && lq < FT) ? ltmp[lq] : tmp[q];
-                rank += wv < v ? 1 : 0;
+            for (int e = s; e < s + m; ++e) {
+                const int64_t le = e - t0;
+                const int32_t wv = (le >= 0 && le < GT) ? ltmp[le] : pair[e].x;
+                rank += wv < v[h] ? 1 : 0;
             }
             const int64_t d = s + rank;
-            const int fl = (int)sfl[v];
+            const int fl = q[h].flags();
             const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
             double v2 = 0;
             for (int k = 0; k < 3; ++k) {
-                Rl x = spos[k * n + v];
-                Rl ed = (Rl)rp.box.edges[k];
-                int jump = (int)floor((double)(x / ed));
-                x = x - (Rl)jump * ed;
-                dpos[k * n + d] = x;
-                dold[k * n + d] = x;
-                dim[k * n + d] = sim[k * n + v] + jump;
-                Rl vv = svel[k * n + v];
+                dpos[k * n + d] = q[h].p[k];
+                dold[k * n + d] = q[h].p[k];
+                dim[k * n + d] = q[h].im[k];
+                const Rl vv = q[h].v[k];
                 dvel[k * n + d] = vv;
                 v2 += (double)vv * (double)vv;
                 if (has_f) {
-                    df[k * n + d] = sf[k * n + v];
-                    dfo[k * n + d] = sfo[k * n + v];
+                    df[k * n + d] = sf[k * n + vs[h]];
+                    dfo[k * n + d] = sfo[k * n + vs[h]];
                 }
             }
-            did[d] = sid[v];
-            dsp[d] = ssp[v];
+            did[d] = q[h].id;
+            dsp[d] = (S)q[h].species();
             dfl[d] = (S)fl;
             B.memb[rp.p_off + d] = 0;
-            vm = v2;
+            vm = v2 > vm ? v2 : vm;
         }
-        if (j < nc) {
-            old_cnt[j] = 0;
-            B.is_md[rp.c_off + j] = 0;
-            B.is_reac[rp.c_off + j] = 1;
+        for (int h = 0; h < GPT; ++h) {
+            const int64_t jc = t0 + h * FT + threadIdx.x;
+            if (jc < nc) {
+                old_cnt[jc] = 0;
+                B.is_md[rp.c_off + jc] = 0;
+                B.is_reac[rp.c_off + jc] = 1;
+            }
         }
         const double m = block_max(vm, red);
         if (threadIdx.x == 0) B.gpart[u] = m;
+    }
+}
+
+// S4 of an extra sort: the particles stay where they are and only the order is computed.
+// The stable rank of a slot's particle among its cell's arrivals (by reference position,
+// read from LDS within the unit) gives its new reference position d: perm[d] = storage
+// index, inv[storage index] = d.  Clears the previous cell counts and the is_md/is_reac
+// marks like the gather.
+template <class F>
+__global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
+{
+    __shared__ int32_t lref[GT];
+    const UnitRange w = unit_range(B.plan, R, U_GATHER);
+    int a = -1;
+    for (int64_t u = w.u0; u < w.u1; ++u) {
+        if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
+        const int r = (int)w.act[a];
+        const int64_t t0 = (u - w.pre[a]) * GT;
+        const CtlDev &C = B.ctl[r];
+        const RepDev &rp = B.rep[r];
+        const int64_t n = rp.n, nc = rp.box.ncells;
+        const int2 *pair = B.pair + rp.p_off;
+        const int32_t *key = B.key + rp.p_off;
+        int32_t *perm = B.perm + rp.p_off, *inv = B.inv + rp.p_off;
+        const int32_t *start = B.start + rp.c_off, *cnt = B.count[C.cnt ^ 1] + rp.c_off;
+        int32_t *old_cnt = B.count[C.cnt] + rp.c_off;
+        int32_t ref[GPT], sto[GPT];
+        int s0[GPT], m0[GPT];
+        __syncthreads();
+        for (int h = 0; h < GPT; ++h) {
+            const int64_t j = t0 + h * FT + threadIdx.x;
+            const int2 pr = j < n ? pair[j] : make_int2(INT32_MAX, 0);
+            ref[h] = pr.x;
+            sto[h] = pr.y;
+            lref[h * FT + threadIdx.x] = pr.x;
+        }
+        for (int h = 0; h < GPT; ++h) {
+            if (ref[h] == INT32_MAX) continue;
+            const int c = key[sto[h]];
+            s0[h] = start[c];
+            m0[h] = cnt[c];
+        }
+        __syncthreads();
+        for (int h = 0; h < GPT; ++h) {
+            if (ref[h] == INT32_MAX) continue;
+            const int s = s0[h], m = m0[h];
+            int rank = 0;
+            for (int e = s; e < s + m; ++e) {
+                const int64_t le = e - t0;
+                const int32_t wv = (le >= 0 && le < GT) ? lref[le] : pair[e].x;
+                rank += wv < ref[h] ? 1 : 0;
+            }
+            const int32_t d = s + rank;
+            perm[d] = sto[h];
+            inv[sto[h]] = d;
+        }
+        for (int h = 0; h < GPT; ++h) {
+            const int64_t jc = t0 + h * FT + threadIdx.x;
+            if (jc < nc) {
+                old_cnt[jc] = 0;
+                B.is_md[rp.c_off + jc] = 0;
+                B.is_reac[rp.c_off + jc] = 1;
+            }
+        }
     }
 }
 
@@ -998,6 +1200,7 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
                     hi = mid - 1;
             }
             q = sstart[lo] + (f - soff[lo]);
+            if (!forced) q = B.perm[rp.p_off + q];  // an extra sort: the storage index
             const int spc = (int)sp[q];
             if (spc > 0) {
                 const double d0 = min_image(x[0] - (double)pos[q], L0);
@@ -1032,14 +1235,17 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
         if (total > P.nmax) C.error = RMPCD_ERR_OVERFLOW;
     }
     if (col != 0) return;
-    // |v|max over this replica's gather units
+    // |v|max over this replica's gather units; after an extra sort, the previous bound
+    // raised to the list members' speeds (the key pass's units): only list members have
+    // changed their velocity since the bound was taken
     const int a = (int)plan_amap(B.plan, R)[r];
-    const int64_t *gpre = plan_pre(B.plan, R, U_GATHER);
-    double all = 0;
+    const int64_t *gpre = plan_pre(B.plan, R, forced ? U_GATHER : U_KEY);
+    double all = forced || !(C.vmax2 < INFINITY) ? 0.0 : C.vmax2;
     for (int64_t q = gpre[a] + threadIdx.x; q < gpre[a + 1]; q += blockDim.x) all = fmax(all, B.gpart[q]);
     all = block_max(all, red);
     if (threadIdx.x != 0) return;
     C.vmax2 = all;
+    C.virt = forced ? 0 : 1;
     C.cur = cur;
     C.cnt = cnt_live;
     C.k_since = 0;
@@ -1260,8 +1466,11 @@ static FB<F> make_fb(DimerSim &s)
     b.id_to_idx = s.id_to_idx;
     b.memb = s.memb;
     b.key = s.f_key;
+    b.perm = s.f_perm;
+    b.inv = s.f_inv;
+    b.pair = reinterpret_cast<int2 *>(s.f_pair);
+    b.rec = reinterpret_cast<typename FB<F>::Rec *>(s.f_rec);
     b.slot = s.f_slot;
-    b.tmp = s.f_tmp;
     b.start = s.cell_start;
     b.is_md = s.is_md;
     b.is_reac = s.is_reac;
@@ -1336,8 +1545,11 @@ static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t s
     k_fz_plan<F><<<1, FT, 0, st>>>(b, forced, s.R);
     k_fz_key<F><<<s.sort_blocks, FT, 0, st>>>(b, p, forced, s.R);
     k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, s.R);
-    k_fz_place<F><<<s.sort_blocks, FT, 0, st>>>(b, s.R);
-    k_fz_gather<F><<<s.sort_blocks, FT, 0, st>>>(b, p, s.R);
+    k_fz_place<F><<<s.sort_blocks, FT, 0, st>>>(b, forced, s.R);
+    if (forced)
+        k_fz_gather<F><<<s.sort_blocks, FT, 0, st>>>(b, p, s.R);
+    else
+        k_fz_rank<F><<<s.sort_blocks, FT, 0, st>>>(b, s.R);
     k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, s.R);
 }
 
@@ -1519,8 +1731,13 @@ int fused_prepare(DimerSim &s)
     if ((rc = s.alloc(&s.sol_pos_old, 3 * s.ntot * RS))) return rc;
     if ((rc = s.alloc((void **)&s.memb, s.ntot))) return rc;
     if ((rc = s.alloc((void **)&s.f_key, s.ntot * 4))) return rc;
+    if ((rc = s.alloc(&s.f_rec, s.ntot * (s.cfg.form == RMPCD_FP32 ? sizeof(SortRec<float, uint8_t>)
+                                                                     : sizeof(SortRec<double, int32_t>)))))
+        return rc;
     if ((rc = s.alloc((void **)&s.f_slot, s.ntot * 4))) return rc;
-    if ((rc = s.alloc((void **)&s.f_tmp, s.ntot * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.f_perm, s.ntot * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.f_inv, s.ntot * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.f_pair, s.ntot * 8))) return rc;
     if ((rc = s.alloc((void **)&s.lut, s.luttot * 4))) return rc;
     int64_t sstride = (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE + 1;
     if ((rc = s.alloc((void **)&s.scan_sums, (size_t)s.R * sstride * 4))) return rc;
@@ -1528,7 +1745,7 @@ int fused_prepare(DimerSim &s)
     if ((rc = s.alloc((void **)&s.f_mslots, (size_t)s.R * 4 * s.nmax * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_plan, (1 + 4 + 2 * (size_t)s.R + 4 * ((size_t)s.R + 1)) * 8))) return rc;
     int64_t gunits = 0;
-    for (const Replica &rp : s.rep) gunits += (std::max<int64_t>(rp.n, rp.box.ncells) + FT - 1) / FT;
+    for (const Replica &rp : s.rep) gunits += (std::max<int64_t>(rp.n, rp.box.ncells) + GT - 1) / GT;
     if ((rc = s.alloc((void **)&s.f_gpart, (size_t)gunits * 8))) return rc;
     if ((rc = s.alloc((void **)&s.f_mcount, (size_t)s.R * 4 * 4))) return rc;
     if ((rc = s.alloc((void **)&s.part, (size_t)s.R * PS * 8))) return rc;

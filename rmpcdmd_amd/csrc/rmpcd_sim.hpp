@@ -32,6 +32,8 @@ struct CtlDev {
     int32_t cur, fpar, cfpar, need_sort;
     int32_t need_exact, k_since, error, cnt;  // cnt: which cell_count array is live
     int32_t cur_next, cnt_next;               // a sort in progress: the buffers it writes
+    int32_t virt, pad_;  // virt: an extra sort since the last collision sort left the
+                         // particles in place (the reference order is in perm/inv)
     int64_t md_step, cycle, n_extra;
     double origin[3];
     double vmax2, list_max2, co_max, e1, e2;
@@ -64,7 +66,9 @@ struct DimerSim {
     double *col_mass = nullptr;
     int32_t *list = nullptr, *nlist = nullptr, *stencil = nullptr;
     int32_t *lut = nullptr;
-    int32_t *f_key = nullptr, *f_slot = nullptr, *f_tmp = nullptr, *scan_sums = nullptr, *f_slocal = nullptr;
+    void *f_rec = nullptr;  // fused mode: staging records of a sort
+    int32_t *f_perm = nullptr, *f_inv = nullptr, *f_pair = nullptr;  // fused mode: extra sorts
+    int32_t *f_key = nullptr, *f_slot = nullptr, *scan_sums = nullptr, *f_slocal = nullptr;
     int32_t *f_mslots = nullptr, *f_mcount = nullptr;
     int64_t *f_plan = nullptr;
     double *f_gpart = nullptr;
