@@ -667,7 +667,8 @@ template <class F>
 __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
 {
     using Rl = typename F::real;
-    __shared__ int hkey[KHASH], hcnt[KHASH];
+    __shared__ int hkey[KHASH], hcnt[KHASH], occ[KHASH];
+    __shared__ int lds[FT / 64];
     __shared__ double red[16];
     const UnitRange w = unit_range(B.plan, R, U_KEY);
     int a = -1;
@@ -822,8 +823,22 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
             loc[v] = atomicAdd(&hcnt[h], 1);
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < KHASH; q += FT)
-            if (hkey[q] >= 0) hcnt[q] = atomicAdd(&cnt[hkey[q]], hcnt[q]);
+        {
+            // the occupied entries compacted first, so that a thread waits for at most
+            // one global atomic per round (a chunk has a few hundred distinct keys)
+            constexpr int PER = KHASH / FT;
+            int c = 0;
+            for (int j = 0; j < PER; ++j) c += hkey[threadIdx.x * PER + j] >= 0 ? 1 : 0;
+            int tot;
+            int o = scan_block_exclusive(c, lds, &tot);
+            for (int j = 0; j < PER; ++j)
+                if (hkey[threadIdx.x * PER + j] >= 0) occ[o++] = threadIdx.x * PER + j;
+            __syncthreads();
+            for (int e = threadIdx.x; e < tot; e += FT) {
+                const int q = occ[e];
+                hcnt[q] = atomicAdd(&cnt[hkey[q]], hcnt[q]);
+            }
+        }
         __syncthreads();
         for (int v = 0; v < KPT; ++v) {
             const int64_t i = c0 + v * FT + threadIdx.x;
