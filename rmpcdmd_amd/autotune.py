@@ -20,7 +20,7 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-CANDIDATES = {"stream_blocks": [2048, 4096, 8192, 16384], "sort_blocks": [1024, 2048, 4096, 8192]}
+CANDIDATES = {"stream_blocks": [2048, 4096, 8192, 16384], "sort_blocks": [2048, 4096, 8192, 16384, 32768]}
 
 
 def workload_key(form, L, replicas, mixed=False):

@@ -40,7 +40,7 @@ constexpr int CTILE = 2048;        // staged particles per collision block
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
 constexpr int MAX_STENCIL = 2048;  // stencil cells of the fused list build
-constexpr int GPT = 2, GT = FT * GPT;  // destination slots per thread and per unit of the gather
+constexpr int GPT = 4, GT = FT * GPT;  // destination slots per thread and per unit of the gather
 constexpr int KPT = 4, KCHUNK = FT * KPT;  // particles per thread and per chunk of the key pass
 constexpr int KHASH_BITS = 11, KHASH = 1 << KHASH_BITS;  // LDS hash entries, 2 x KCHUNK
 
