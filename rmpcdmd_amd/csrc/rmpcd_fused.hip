@@ -41,6 +41,7 @@ constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
 constexpr int MAX_STENCIL = 2048;  // stencil cells of the fused list build
 constexpr int GPT = 4, GT = FT * GPT;  // destination slots per thread and per unit of the gather
+constexpr int GHALO = 64;              // slots staged on either side of a gather or rank unit
 constexpr int KPT = 4, KCHUNK = FT * KPT;  // particles per thread and per chunk of the key pass
 constexpr int KHASH_BITS = 11, KHASH = 1 << KHASH_BITS;  // LDS hash entries, 2 x KCHUNK
 
@@ -982,6 +983,40 @@ __global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int forced, int R)
     }
 }
 
+// The reference positions of the GHALO slots on either side of a unit's tile, so that
+// the rank of a particle in a cell that straddles the tile edge reads LDS only (a cell
+// holds far fewer than GHALO particles at the densities of the programs; a larger one
+// reads the rest from memory).
+__device__ inline void load_halo(int32_t *lv, const int2 *pair, int64_t t0, int64_t n)
+{
+    for (int q = threadIdx.x; q < 2 * GHALO; q += FT) {
+        const int64_t j = q < GHALO ? t0 - GHALO + q : t0 + GT + (q - GHALO);
+        lv[q < GHALO ? q : GT + q] = (j >= 0 && j < n) ? pair[j].x : INT32_MAX;
+    }
+}
+
+// The number of a cell's arrivals [s, s + m) whose reference position is below ref: the
+// window [t0 - GHALO, t0 + GT + GHALO) staged in LDS, anything outside it read from
+// memory (two separate loops, so that the common one is LDS only).
+__device__ inline int rank_in_cell(const int32_t *lv, const int2 *pair, int64_t t0, int s, int m, int32_t ref)
+{
+    const int64_t w0 = t0 - GHALO, w1 = t0 + GT + GHALO;
+    const int lo = (int)(s > w0 ? s : w0), hi = (int)(s + m < w1 ? s + m : w1);
+    int rank = 0;
+    const int32_t *l = lv - w0;
+    int e = lo;
+    for (; e + 4 <= hi; e += 4) {  // four LDS reads in flight per wait
+        const int32_t a0 = l[e], a1 = l[e + 1], a2 = l[e + 2], a3 = l[e + 3];
+        rank += (a0 < ref ? 1 : 0) + (a1 < ref ? 1 : 0) + (a2 < ref ? 1 : 0) + (a3 < ref ? 1 : 0);
+    }
+    for (; e < hi; ++e) rank += l[e] < ref ? 1 : 0;
+    // the parts before and after the window (empty unless the cell is larger than GHALO)
+    const int a = lo < s + m ? lo : s + m, b = hi > a ? hi : a;
+    for (int e = s; e < a; ++e) rank += pair[e].x < ref ? 1 : 0;
+    for (int e = b; e < s + m; ++e) rank += pair[e].x < ref ? 1 : 0;
+    return rank;
+}
+
 // S4: stable gather of the staging records into the other buffers, pos_old, |v|max per
 // unit; clears the previous cell counts and the is_md/is_reac marks for the list build.  A
 // unit is a tile of GT destination slots (GPT per thread, their loads issued together);
@@ -996,7 +1031,7 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
     using Rl = typename F::real;
     using S = typename F::small;
     __shared__ double red[16];
-    __shared__ int32_t ltmp[GT];
+    __shared__ int32_t ltmp[GT + 2 * GHALO];
     const UnitRange w = unit_range(B.plan, R, U_GATHER);
     int a = -1;
     for (int64_t u = w.u0; u < w.u1; ++u) {
@@ -1017,38 +1052,30 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
         const int2 *pair = B.pair + rp.p_off;
         const int32_t *start = B.start + rp.c_off, *cnt = B.count[C.cnt ^ 1] + rp.c_off;
         int32_t *old_cnt = B.count[C.cnt] + rp.c_off;
-        // GPT destination slots per thread, their loads issued together; a slot holds the
-        // particle's reference position (the rank's order) and its storage index
-        int32_t v[GPT], vs[GPT];
+        // GPT destination slots per thread, their loads issued together (unconditionally: a
+        // slot past the end reads the last one).  A slot holds the particle's reference
+        // position (the rank's order) and its storage index.
+        int32_t v[GPT], vs[GPT], s0[GPT], m0[GPT];
         typename FB<F>::Rec q[GPT];
-        int s0[GPT], m0[GPT];
         __syncthreads();
         for (int h = 0; h < GPT; ++h) {
             const int64_t j = t0 + h * FT + threadIdx.x;
-            const int2 pr = j < n ? pair[j] : make_int2(INT32_MAX, 0);
-            v[h] = pr.x;
+            const int2 pr = pair[j < n ? j : n - 1];
+            v[h] = j < n ? pr.x : INT32_MAX;
             vs[h] = pr.y;
-            ltmp[h * FT + threadIdx.x] = v[h];
+            ltmp[GHALO + h * FT + threadIdx.x] = v[h];
         }
-        for (int h = 0; h < GPT; ++h)
-            if (v[h] != INT32_MAX) q[h] = rec[vs[h]];
+        for (int h = 0; h < GPT; ++h) q[h] = rec[vs[h]];
         for (int h = 0; h < GPT; ++h) {
-            if (v[h] == INT32_MAX) continue;
             s0[h] = start[q[h].key];
             m0[h] = cnt[q[h].key];
         }
+        load_halo(ltmp, pair, t0, n);
         __syncthreads();
         double vm = 0;
         for (int h = 0; h < GPT; ++h) {
             if (v[h] == INT32_MAX) continue;
-            const int s = s0[h], m = m0[h];
-            int rank = 0;
-            for (int e = s; e < s + m; ++e) {
-                const int64_t le = e - t0;
-                const int32_t wv = (le >= 0 && le < GT) ? ltmp[le] : pair[e].x;
-                rank += wv < v[h] ? 1 : 0;
-            }
-            const int64_t d = s + rank;
+            const int64_t d = s0[h] + rank_in_cell(ltmp, pair, t0, s0[h], m0[h], v[h]);
             const int fl = q[h].flags();
             const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
             double v2 = 0;
@@ -1091,7 +1118,7 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
 template <class F>
 __global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
 {
-    __shared__ int32_t lref[GT];
+    __shared__ int32_t lref[GT + 2 * GHALO];
     const UnitRange w = unit_range(B.plan, R, U_GATHER);
     int a = -1;
     for (int64_t u = w.u0; u < w.u1; ++u) {
@@ -1106,18 +1133,17 @@ __global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
         int32_t *perm = B.perm + rp.p_off, *inv = B.inv + rp.p_off;
         const int32_t *start = B.start + rp.c_off, *cnt = B.count[C.cnt ^ 1] + rp.c_off;
         int32_t *old_cnt = B.count[C.cnt] + rp.c_off;
-        int32_t ref[GPT], sto[GPT];
-        int s0[GPT], m0[GPT];
+        int32_t ref[GPT], sto[GPT], s0[GPT], m0[GPT];
         __syncthreads();
         for (int h = 0; h < GPT; ++h) {
             const int64_t j = t0 + h * FT + threadIdx.x;
-            const int2 pr = j < n ? pair[j] : make_int2(INT32_MAX, 0);
-            ref[h] = pr.x;
+            const int2 pr = pair[j < n ? j : n - 1];
+            ref[h] = j < n ? pr.x : INT32_MAX;
             sto[h] = pr.y;
-            lref[h * FT + threadIdx.x] = pr.x;
+            lref[GHALO + h * FT + threadIdx.x] = ref[h];
         }
+        load_halo(lref, pair, t0, n);
         for (int h = 0; h < GPT; ++h) {
-            if (ref[h] == INT32_MAX) continue;
             const int c = key[sto[h]];
             s0[h] = start[c];
             m0[h] = cnt[c];
@@ -1125,14 +1151,7 @@ __global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
         __syncthreads();
         for (int h = 0; h < GPT; ++h) {
             if (ref[h] == INT32_MAX) continue;
-            const int s = s0[h], m = m0[h];
-            int rank = 0;
-            for (int e = s; e < s + m; ++e) {
-                const int64_t le = e - t0;
-                const int32_t wv = (le >= 0 && le < GT) ? lref[le] : pair[e].x;
-                rank += wv < ref[h] ? 1 : 0;
-            }
-            const int32_t d = s + rank;
+            const int32_t d = s0[h] + rank_in_cell(lref, pair, t0, s0[h], m0[h], ref[h]);
             perm[d] = sto[h];
             inv[sto[h]] = d;
         }
