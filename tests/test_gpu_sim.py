@@ -179,3 +179,29 @@ def test_lazy_streaming_equals_eager(gpu, form):
     assert a["n_extra_sorting"] >= 1
     eager.close()
     lazy.close()
+
+
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+def test_many_extra_sorts_per_cycle(gpu, form):
+    """a small skin (0.6) makes the skin test fire several times per collision cycle, so
+    extra sorts follow each other before the collision sort (the fused mode keeps the
+    particles in place at an extra sort and carries the order in perm/inv): the fused batch
+    must still equal the reference call sequence, and lazy streaming the eager one"""
+    Ls = [(16, 16, 16), (20, 16, 12)]
+    kw = dict(n_md=50, tau=0.5, form=form, seed=31, replica_L=Ls, skin=0.6)
+    ref = SingleDimer(mode=capi.SIM_REFERENCE, **kw)
+    fus = SingleDimer(mode=capi.SIM_FUSED, **kw)
+    lazy = SingleDimer(mode=capi.SIM_FUSED, lazy=1, **kw)
+    for cycle in range(2):
+        ref.run(1)
+        fus.run(1)
+        lazy.run(1)
+        for r in range(len(Ls)):
+            a, b, c = ref.download(r), fus.download(r), lazy.download(r)
+            compare(a, b, "replica %d cycle %d" % (r, cycle), 1e-12 if form == capi.FP64 else 1e-5)
+            compare(b, c, "lazy replica %d cycle %d" % (r, cycle), rtol=0.0)
+    for r in range(len(Ls)):
+        assert fus.download(r)["n_extra_sorting"] >= 6, "replica %d" % r
+    ref.close()
+    fus.close()
+    lazy.close()
