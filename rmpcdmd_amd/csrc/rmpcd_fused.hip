@@ -330,10 +330,11 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
             if (threadIdx.x == 0) part[moff + (blockIdx.x - nbm)] = m;
             return;
         }
-        if (!exact && rp.vec) {
+        if (rp.vec) {
             // 16-byte non-temporal loads and stores: VW particles per thread and plane (the
             // replica's planes and membership bytes are 16-byte aligned).  Nothing read
-            // here is read again in this launch, so the lines need not stay in L2.
+            // here is read again in this launch, so the lines need not stay in L2.  An
+            // exact skin test also reads pos_old and takes the largest displacement.
             constexpr int VW = 16 / sizeof(R);
             using V = typename std::conditional<sizeof(R) == 4, v4f, v2d>::type;
             using M = typename std::conditional<sizeof(R) == 4, uint32_t, uint16_t>::type;
@@ -351,6 +352,19 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
                     for (int u = 0; u < VW; ++u)
                         if (!((mb >> (8 * u)) & 0xffu)) pe[u] = pe[u] + rdt * ve[u];
                 }
+                if (exact) {
+                    V ov[3];
+                    for (int k = 0; k < 3; ++k)
+                        ov[k] = __builtin_nontemporal_load(reinterpret_cast<const V *>(po + k * n) + q);
+                    for (int u = 0; u < VW; ++u) {
+                        if (((mb >> (8 * u)) & 0xffu) || (int)spc[q * VW + u] < 0) continue;
+                        const double d0 = (double)(pv[0][u] - ov[0][u]);
+                        const double d1 = (double)(pv[1][u] - ov[1][u]);
+                        const double d2 = (double)(pv[2][u] - ov[2][u]);
+                        const double q2 = (d0 * d0 + d1 * d1) + d2 * d2;
+                        mx = q2 > mx ? q2 : mx;
+                    }
+                }
                 if (!mb) {
                     for (int k = 0; k < 3; ++k)
                         __builtin_nontemporal_store(pv[k], reinterpret_cast<V *>(pos + k * n) + q);
@@ -362,6 +376,10 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
                             if (!((mb >> (8 * u)) & 0xffu)) pos[k * n + q * VW + u] = pe[u];
                     }
                 }
+            }
+            if (exact) {
+                const double m = block_max(mx, red);
+                if (threadIdx.x == 0) part[moff + (blockIdx.x - nbm)] = m;
             }
             return;
         }
