@@ -22,6 +22,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -84,6 +86,13 @@ struct FB {
     // position, storage index) of each slot that the extra sort's place pass fills.
     int32_t *perm, *inv;
     int2 *pair;
+    // A deferred extra sort keeps the order of the previous sort in perm/inv and the new
+    // cells in dkey (the order it stands for: cell, then previous order); cand lists the
+    // particles near the colloids for its list build (cand_cap per replica), stage holds
+    // the list build's working entries (nmax per colloid).
+    int32_t *dkey, *cand;
+    int4 *stage;
+    int64_t cand_cap;
     uint8_t *memb;
     int32_t *count[2], *start;
     uint8_t *is_md, *is_reac;
@@ -106,6 +115,7 @@ struct FP {
     double dt, dt_sq, skin, d, prob, bulk_rate, tau, slack;
     double mass[2];
     int nmax, n_stencil, bulk, lazy;
+    int stencil_m, pad_;  // stencil offsets lie in [-stencil_m, stencil_m]^3
     double radius_sq;
     LJ sc, cc;
 };
@@ -621,12 +631,19 @@ __global__ void __launch_bounds__(FT) k_fz_plan(FB<F> B, int forced, int R)
     for (int base = 0; base < R; base += blockDim.x) {
         const int r = base + threadIdx.x;
         const bool on = r < R && sort_active(B.ctl[r], forced);
+        // an extra sort is deferred when the previous one was not and the box allows it:
+        // only the key pass and the list build run for it
+        const bool dfr = on && !forced && B.ctl[r].def == 0 && B.rep[r].defer != 0;
+        if (on) {
+            B.ctl[r].kind = dfr ? 1 : 0;
+            B.ctl[r].ccount = 0;
+        }
         int tot;
         const int off = carry + scan_block_exclusive(on ? 1 : 0, lds, &tot);
         if (r < R) amap[r] = on ? off : -1;
         if (on) act[off] = r;
         for (int kind = 0; kind < U_KINDS; ++kind) {
-            const int u = on ? (int)units_of(B.rep[r], kind) : 0;
+            const int u = on && !(dfr && kind != U_KEY) ? (int)units_of(B.rep[r], kind) : 0;
             int ktot;
             const int kx = scan_block_exclusive(u, lds, &ktot);
             if (on) plan[1 + U_KINDS + 2 * (int64_t)R + (int64_t)kind * (R + 1) + off] = csum[kind] + kx;
@@ -724,6 +741,13 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
         const Rl rdt = (Rl)P.dt;
         int keys[KPT], ent[KPT], loc[KPT];
         double vm = 0;  // extra sort: |v|^2 maximum over the chunk's list members (all_v: all)
+        const bool dfr = !forced && C.kind == 1;
+        bool candf[KPT];
+        double ccol[2][3];
+        for (int v = 0; v < KPT; ++v) candf[v] = false;
+        for (int cc = 0; cc < 2; ++cc)
+            for (int k = 0; k < 3; ++k) ccol[cc][k] = dfr ? B.cpos[6 * r + 2 * k + cc] : 0.0;
+        const double cand_r2 = (sqrt(P.radius_sq) + 1e-3) * (sqrt(P.radius_sq) + 1e-3);
         for (int v = 0; v < KPT; ++v) {
             const int64_t i = c0 + v * FT + threadIdx.x;
             keys[v] = -1;
@@ -804,24 +828,54 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
                     vm = v2 > vm ? v2 : vm;
                     if (mb) memb[i] = 0;
                 }
-                int key;
-                {
-                    int p0, p1, p2;
-                    cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
-                    if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
-                        (unsigned)p2 < (unsigned)b.L[2])
-                        key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
-                    else
-                        key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
-                }
+                int key, p0, p1, p2;
+                cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
+                if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
+                    (unsigned)p2 < (unsigned)b.L[2])
+                    key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+                else
+                    key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
                 keys[v] = key;
+                if (dfr) {
+                    // a deferred sort keeps the cell; the particle is a candidate of the
+                    // list build when it lies within the list radius of either colloid
+                    // (with a margin: the colloids may be read before or after their wrap)
+                    for (int cc = 0; cc < 2; ++cc) {
+                        const double d0 = min_image(ccol[cc][0] - (double)x[0], b.edges[0]);
+                        const double d1 = min_image(ccol[cc][1] - (double)x[1], b.edges[1]);
+                        const double d2 = min_image(ccol[cc][2] - (double)x[2], b.edges[2]);
+                        candf[v] = candf[v] || (d0 * d0 + d1 * d1) + d2 * d2 < cand_r2;
+                    }
+                }
             }
-            B.key[rp.p_off + i] = keys[v];
+            if (dfr)
+                B.dkey[rp.p_off + i] = keys[v];
+            else
+                B.key[rp.p_off + i] = keys[v];
         }
         if (!forced) {
             const double m = block_max(vm, red);
             if (threadIdx.x == 0) B.gpart[u] = m;
         }
+        if (dfr) {
+            // append the chunk's candidates: a block scan, then one atomic per chunk
+            int c = 0;
+            for (int v = 0; v < KPT; ++v) c += candf[v] ? 1 : 0;
+            int tot;
+            int o = scan_block_exclusive(c, lds, &tot);
+            if (tot) {
+                if (threadIdx.x == 0) hcnt[0] = atomicAdd(&C.ccount, tot);
+                __syncthreads();
+                const int64_t base = hcnt[0];
+                int32_t *cand = B.cand + (int64_t)r * B.cand_cap;
+                for (int v = 0; v < KPT; ++v)
+                    if (candf[v]) {
+                        if (base + o < B.cand_cap) cand[base + o] = (int32_t)(c0 + v * FT + threadIdx.x);
+                        ++o;
+                    }
+            }
+        }
+        if (!dfr) {
         // the LDS hash after all of the thread's loads are issued
         for (int q = threadIdx.x; q < KHASH; q += FT) {
             hkey[q] = -1;
@@ -863,6 +917,7 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
             const int64_t i = c0 + v * FT + threadIdx.x;
             if (keys[v] < 0) continue;
             B.slot[rp.p_off + i] = hcnt[ent[v]] + loc[v];
+        }
         }
         __syncthreads();
         if (c0 == 0 && threadIdx.x == 0) {
@@ -1005,12 +1060,38 @@ __global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int forced, int R)
 // the rank of a particle in a cell that straddles the tile edge reads LDS only (a cell
 // holds far fewer than GHALO particles at the densities of the programs; a larger one
 // reads the rest from memory).
-__device__ inline void load_halo(int32_t *lv, const int2 *pair, int64_t t0, int64_t n)
+__device__ inline void load_halo(int32_t *lv, int32_t *lk, const int2 *pair, const int32_t *dkey, int64_t t0,
+                                 int64_t n)
 {
     for (int q = threadIdx.x; q < 2 * GHALO; q += FT) {
         const int64_t j = q < GHALO ? t0 - GHALO + q : t0 + GT + (q - GHALO);
-        lv[q < GHALO ? q : GT + q] = (j >= 0 && j < n) ? pair[j].x : INT32_MAX;
+        const bool in = j >= 0 && j < n;
+        const int2 pr = in ? pair[j] : make_int2(INT32_MAX, 0);
+        lv[q < GHALO ? q : GT + q] = pr.x;
+        if (dkey) lk[q < GHALO ? q : GT + q] = in ? dkey[pr.y] : INT32_MAX;
     }
+}
+
+// rank_in_cell when the previous extra sort was deferred: the reference order is (the
+// deferred sort's cell, then the order before it), lexicographic
+__device__ inline int rank_in_cell_deferred(const int32_t *lv, const int32_t *lk, const int2 *pair,
+                                            const int32_t *dkey, int64_t t0, int s, int m, int32_t ref, int32_t dk)
+{
+    const int64_t w0 = t0 - GHALO, w1 = t0 + GT + GHALO;
+    const int lo = (int)(s > w0 ? s : w0), hi = (int)(s + m < w1 ? s + m : w1);
+    int rank = 0;
+    for (int e = lo; e < hi; ++e) {
+        const int32_t k = lk[e - w0];
+        rank += (k < dk || (k == dk && lv[e - w0] < ref)) ? 1 : 0;
+    }
+    const int a = lo < s + m ? lo : s + m, b = hi > a ? hi : a;
+    for (int e = s; e < s + m; ++e) {
+        if (e >= a && e < b) continue;
+        const int2 pr = pair[e];
+        const int32_t k = dkey[pr.y];
+        rank += (k < dk || (k == dk && pr.x < ref)) ? 1 : 0;
+    }
+    return rank;
 }
 
 // The number of a cell's arrivals [s, s + m) whose reference position is below ref: the
@@ -1049,7 +1130,7 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
     using Rl = typename F::real;
     using S = typename F::small;
     __shared__ double red[16];
-    __shared__ int32_t ltmp[GT + 2 * GHALO];
+    __shared__ int32_t ltmp[GT + 2 * GHALO], lkey[GT + 2 * GHALO];
     const UnitRange w = unit_range(B.plan, R, U_GATHER);
     int a = -1;
     for (int64_t u = w.u0; u < w.u1; ++u) {
@@ -1088,12 +1169,21 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
             s0[h] = start[q[h].key];
             m0[h] = cnt[q[h].key];
         }
-        load_halo(ltmp, pair, t0, n);
+        // after a deferred extra sort the order also compares the cells it kept
+        const int32_t *dkey = C.def ? B.dkey + rp.p_off : nullptr;
+        int32_t dk[GPT];
+        for (int h = 0; h < GPT; ++h) {
+            dk[h] = dkey ? dkey[vs[h]] : 0;
+            if (dkey) lkey[GHALO + h * FT + threadIdx.x] = v[h] == INT32_MAX ? INT32_MAX : dk[h];
+        }
+        load_halo(ltmp, lkey, pair, dkey, t0, n);
         __syncthreads();
         double vm = 0;
         for (int h = 0; h < GPT; ++h) {
             if (v[h] == INT32_MAX) continue;
-            const int64_t d = s0[h] + rank_in_cell(ltmp, pair, t0, s0[h], m0[h], v[h]);
+            const int64_t d =
+                s0[h] + (dkey ? rank_in_cell_deferred(ltmp, lkey, pair, dkey, t0, s0[h], m0[h], v[h], dk[h])
+                              : rank_in_cell(ltmp, pair, t0, s0[h], m0[h], v[h]));
             const int fl = q[h].flags();
             const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
             double v2 = 0;
@@ -1136,7 +1226,7 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
 template <class F>
 __global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
 {
-    __shared__ int32_t lref[GT + 2 * GHALO];
+    __shared__ int32_t lref[GT + 2 * GHALO], lkey[GT + 2 * GHALO];
     const UnitRange w = unit_range(B.plan, R, U_GATHER);
     int a = -1;
     for (int64_t u = w.u0; u < w.u1; ++u) {
@@ -1160,7 +1250,13 @@ __global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
             sto[h] = pr.y;
             lref[GHALO + h * FT + threadIdx.x] = ref[h];
         }
-        load_halo(lref, pair, t0, n);
+        const int32_t *dkey = C.def ? B.dkey + rp.p_off : nullptr;
+        int32_t dk[GPT];
+        for (int h = 0; h < GPT; ++h) {
+            dk[h] = dkey ? dkey[sto[h]] : 0;
+            if (dkey) lkey[GHALO + h * FT + threadIdx.x] = ref[h] == INT32_MAX ? INT32_MAX : dk[h];
+        }
+        load_halo(lref, lkey, pair, dkey, t0, n);
         for (int h = 0; h < GPT; ++h) {
             const int c = key[sto[h]];
             s0[h] = start[c];
@@ -1169,7 +1265,9 @@ __global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
         __syncthreads();
         for (int h = 0; h < GPT; ++h) {
             if (ref[h] == INT32_MAX) continue;
-            const int32_t d = s0[h] + rank_in_cell(lref, pair, t0, s0[h], m0[h], ref[h]);
+            const int32_t d =
+                s0[h] + (dkey ? rank_in_cell_deferred(lref, lkey, pair, dkey, t0, s0[h], m0[h], ref[h], dk[h])
+                              : rank_in_cell(lref, pair, t0, s0[h], m0[h], ref[h]));
             perm[d] = sto[h];
             inv[sto[h]] = d;
         }
@@ -1182,6 +1280,143 @@ __global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
             }
         }
     }
+}
+
+// S5 of a deferred extra sort: update_list (neighbor_list.f90:76-157) for colloid col from
+// the key pass's candidates, with no cell lists.  A candidate belongs to stencil cell j
+// when its cell is the colloid's cell plus stencil offset j (the box holds the stencil
+// box, so j is unique); the list keeps the reference's order: stencil order, then, within
+// a cell, the order before the sort (a stable sort keeps it).  Counting per stencil cell,
+// a scan and a rank within each cell by that order place every entry.  Block 0 commits
+// the sort; the particles and the buffers stay as they are.
+constexpr int DEFER_BOX = 4096;  // stencil boxes up to (2m + 1)^3 cells
+template <class F>
+__device__ void deferred_list(FB<F> &B, const FP &P, int R, int r, int col, int *lds, double *red, short *dlut,
+                              int *cntj, int *offj, int *cursor, int &nsel)
+{
+    CtlDev &C = B.ctl[r];
+    const RepDev &rp = B.rep[r];
+    Box b = rp.box;
+    for (int k = 0; k < 3; ++k) b.origin[k] = C.origin[k];
+    const int64_t n = rp.n;
+    const int cur = C.cur;
+    const typename F::real *pos = B.pos[cur] + 3 * rp.p_off;
+    const typename F::small *sp = B.species[cur] + rp.p_off;
+    const int32_t *inv = B.inv + rp.p_off;
+    const bool virt_in = C.virt != 0;
+    const double L0 = b.L[0] * b.a, L1 = b.L[1] * b.a, L2 = b.L[2] * b.a;  // neighbor_list.f90:92
+    const int m = P.stencil_m, wd = 2 * m + 1, ns = P.n_stencil;
+    for (int q = threadIdx.x; q < wd * wd * wd; q += blockDim.x) dlut[q] = -1;
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        cntj[j] = 0;
+        cursor[j] = 0;
+    }
+    if (threadIdx.x == 0) nsel = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        const int32_t *o = B.stencil + 3 * j;
+        dlut[((o[0] + m) * wd + (o[1] + m)) * wd + (o[2] + m)] = (short)j;
+    }
+    const double *cp = B.cpos + 6 * r;
+    const double x[3] = {cp[col], cp[2 + col], cp[4 + col]};
+    int pc[3];
+    cell_of<double>(b, x[0], x[1], x[2], pc[0], pc[1], pc[2]);
+    int32_t *lst = B.list + (int64_t)(2 * r + col) * P.nmax;
+    int32_t *msl = B.mslots + ((int64_t)r * 2 + cur) * 2 * P.nmax + (int64_t)col * P.nmax;
+    int4 *stg = B.stage + ((int64_t)r * 2 + col) * 2 * P.nmax, *stg2 = stg + P.nmax;
+    const int64_t ncand = C.ccount < B.cand_cap ? C.ccount : B.cand_cap;
+    const int32_t *cand = B.cand + (int64_t)r * B.cand_cap;
+    __syncthreads();
+    constexpr int CU = 4;  // candidates per thread and round, their loads issued together
+    for (int64_t e0 = threadIdx.x; e0 < ncand; e0 += CU * blockDim.x) {
+        int iv[CU], sv[CU];
+        typename F::real xv[CU][3];
+        for (int u = 0; u < CU; ++u) {
+            const int64_t e = e0 + u * blockDim.x;
+            iv[u] = cand[e < ncand ? e : ncand - 1];
+        }
+        for (int u = 0; u < CU; ++u) {
+            sv[u] = (int)sp[iv[u]];
+            for (int k = 0; k < 3; ++k) xv[u][k] = pos[k * n + iv[u]];
+        }
+        for (int u = 0; u < CU; ++u) {
+        if (e0 + u * blockDim.x >= ncand) break;
+        const int i = iv[u];
+        const int spc = sv[u];
+        if (spc <= 0) continue;
+        const typename F::real xi[3] = {xv[u][0], xv[u][1], xv[u][2]};
+        int q[3];
+        cell_of<typename F::real>(b, xi[0], xi[1], xi[2], q[0], q[1], q[2]);
+        int box = 0;
+        bool in = true;
+        for (int k = 0; k < 3; ++k) {
+            int o = (q[k] - pc[k]) % b.L[k];
+            o += o < 0 ? b.L[k] : 0;
+            o = o > b.L[k] / 2 ? o - b.L[k] : o;
+            in = in && o >= -m && o <= m;
+            box = box * wd + (o + m);
+        }
+        if (!in) continue;
+        const int j = (int)dlut[box];
+        if (j < 0) continue;
+        const double d0 = min_image(x[0] - (double)xi[0], L0);
+        const double d1 = min_image(x[1] - (double)xi[1], L1);
+        const double d2 = min_image(x[2] - (double)xi[2], L2);
+        const double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
+        if (!(rsq < P.radius_sq)) continue;
+        const int at = atomicAdd(&nsel, 1);
+        if (at < P.nmax) {
+            stg[at] = make_int4(i, j, virt_in ? inv[i] : i, 0);
+            atomicAdd(&cntj[j], 1);
+        }
+        }
+    }
+    __syncthreads();
+    const int total = nsel;
+    const bool over = total > P.nmax || C.ccount > B.cand_cap;
+    int carry = 0;
+    for (int base = 0; base < ns; base += blockDim.x) {
+        const int j = base + threadIdx.x;
+        int tot;
+        const int off = carry + scan_block_exclusive(j < ns ? cntj[j] : 0, lds, &tot);
+        if (j < ns) offj[j] = off;
+        carry += tot;
+    }
+    __syncthreads();
+    const int tl = over ? 0 : total;
+    for (int e = threadIdx.x; e < tl; e += blockDim.x) {
+        const int4 t = stg[e];
+        stg2[offj[t.y] + atomicAdd(&cursor[t.y], 1)] = t;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int e = threadIdx.x; e < tl; e += blockDim.x) {
+        const int4 t = stg2[e];
+        const int s0 = offj[t.y], c = cntj[t.y];
+        int rank = 0;
+        for (int q = s0; q < s0 + c; ++q) rank += stg2[q].z < t.z ? 1 : 0;
+        lst[s0 + rank] = t.x;
+        msl[s0 + rank] = t.x;
+        atomic_or_flag(&B.memb[rp.p_off + t.x], 1 << col);
+    }
+    if (threadIdx.x == 0) {
+        B.nlist[2 * r + col] = tl;
+        B.mcount[(r * 2 + cur) * 2 + col] = tl;
+        if (over) C.error = RMPCD_ERR_OVERFLOW;
+    }
+    if (col != 0) return;
+    // |v|max: the previous bound raised to the list members' speeds (the key pass's units)
+    const int a = (int)plan_amap(B.plan, R)[r];
+    const int64_t *kpre = plan_pre(B.plan, R, U_KEY);
+    double all = !(C.vmax2 < INFINITY) ? 0.0 : C.vmax2;
+    for (int64_t q = kpre[a] + threadIdx.x; q < kpre[a + 1]; q += blockDim.x) all = fmax(all, B.gpart[q]);
+    all = block_max(all, red);
+    if (threadIdx.x != 0) return;
+    C.vmax2 = all;
+    C.def = 1;
+    C.k_since = 0;
+    C.need_exact = P.dt * sqrt(all) * (1.0 + 1e-6) + P.slack >= P.skin * 0.89 ? 1 : 0;
+    C.n_extra += 1;
 }
 
 // S5: update_list (neighbor_list.f90:76-157) for colloid blockIdx.x, one workgroup:
@@ -1197,6 +1432,13 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
     const int r = blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (!sort_active(C, forced)) return;
+    __shared__ int skey[MAX_STENCIL], sstart[MAX_STENCIL], soff[MAX_STENCIL + 1], sflag[MAX_STENCIL];
+    if (!forced && C.kind == 1) {
+        __shared__ short dlut[DEFER_BOX];
+        __shared__ int nsel;
+        deferred_list<F>(B, P, R, r, blockIdx.x, lds, red, dlut, skey, soff, sstart, nsel);
+        return;
+    }
     const RepDev &rp = B.rep[r];
     const int cur = C.cur_next, cnt_live = C.cnt_next;  // the buffers the gather wrote
     Box b = rp.box;
@@ -1211,7 +1453,6 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
     // j owns [soff[j], soff[j+1])), so every thread tests one candidate at a time and the
     // loads of different candidates overlap; a block scan per chunk keeps the reference's
     // order (stencil order, then cell order).
-    __shared__ int skey[MAX_STENCIL], sstart[MAX_STENCIL], soff[MAX_STENCIL + 1], sflag[MAX_STENCIL];
     const int col = blockIdx.x;
     const double *cp = B.cpos + 6 * r;
     const double x[3] = {cp[col], cp[2 + col], cp[4 + col]};
@@ -1298,6 +1539,7 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
     if (threadIdx.x != 0) return;
     C.vmax2 = all;
     C.virt = forced ? 0 : 1;
+    C.def = 0;
     C.cur = cur;
     C.cnt = cnt_live;
     C.k_since = 0;
@@ -1519,6 +1761,10 @@ static FB<F> make_fb(DimerSim &s)
     b.memb = s.memb;
     b.key = s.f_key;
     b.perm = s.f_perm;
+    b.dkey = s.f_dkey;
+    b.cand = s.f_cand;
+    b.stage = reinterpret_cast<int4 *>(s.f_stage);
+    b.cand_cap = s.cand_cap;
     b.inv = s.f_inv;
     b.pair = reinterpret_cast<int2 *>(s.f_pair);
     b.rec = reinterpret_cast<typename FB<F>::Rec *>(s.f_rec);
@@ -1567,6 +1813,7 @@ static FP make_fp(const DimerSim &s)
     p.mass[1] = s.mass[1];
     p.nmax = s.nmax;
     p.n_stencil = s.n_stencil;
+    p.stencil_m = s.stencil_m;
     p.bulk = s.cfg.bulk_rmpcd ? 1 : 0;
     p.radius_sq = s.radius * s.radius;
     p.sc = make_lj(&s.lj_sc);
@@ -1788,6 +2035,19 @@ int fused_prepare(DimerSim &s)
         return rc;
     if ((rc = s.alloc((void **)&s.f_slot, s.ntot * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_perm, s.ntot * 4))) return rc;
+    // deferred extra sorts: the stencil's half width, the candidate buffers (the stencil
+    // box with one cell of margin, for both colloids, at four times the mean density)
+    {
+        std::vector<int32_t> st(3 * (size_t)s.n_stencil);
+        rmpcd_make_stencil(1.0, s.radius, st.data(), s.n_stencil);
+        s.stencil_m = 0;
+        for (int32_t o : st) s.stencil_m = std::max(s.stencil_m, std::abs((int)o));
+        const int64_t wm = 2 * (int64_t)s.stencil_m + 3;
+        s.cand_cap = 2 * wm * wm * wm * 4 * (int64_t)std::max(1, s.cfg.rho);
+    }
+    if ((rc = s.alloc((void **)&s.f_dkey, s.ntot * 4))) return rc;
+    if ((rc = s.alloc((void **)&s.f_cand, (size_t)s.R * s.cand_cap * 4))) return rc;
+    if ((rc = s.alloc(&s.f_stage, (size_t)s.R * 4 * s.nmax * 16))) return rc;
     if ((rc = s.alloc((void **)&s.f_inv, s.ntot * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_pair, s.ntot * 8))) return rc;
     if ((rc = s.alloc((void **)&s.lut, s.luttot * 4))) return rc;
@@ -1815,6 +2075,14 @@ int fused_prepare(DimerSim &s)
         rd[r].lut_off = rp.lut_off;
         rd[r].seed = rp.seed;
         rd[r].vec = (rp.n % 4 == 0 && rp.p_off % 4 == 0) ? 1 : 0;
+        {
+            // a deferred extra sort needs each stencil cell once (the box holds the
+            // stencil box) and the stencil box in LDS
+            const int wd = 2 * s.stencil_m + 1;
+            bool ok = wd * wd * wd <= DEFER_BOX;
+            for (int k = 0; k < 3; ++k) ok = ok && rp.box.L[k] >= wd;
+            rd[r].defer = ok ? 1 : 0;
+        }
         CtlDev &c = cd[r];
         memset(&c, 0, sizeof(c));
         c.cur = rp.cur;

@@ -26,14 +26,17 @@ struct RepDev {
     Box box;
     int64_t n, p_off, c_off, lut_off;
     uint64_t seed;
-    int32_t vec, pad_;  // vec: planes and membership bytes allow 16-byte accesses
+    int32_t vec;    // planes and membership bytes allow 16-byte accesses
+    int32_t defer;  // an extra sort may be deferred (the box holds the stencil box)
 };
 struct CtlDev {
     int32_t cur, fpar, cfpar, need_sort;
     int32_t need_exact, k_since, error, cnt;  // cnt: which cell_count array is live
     int32_t cur_next, cnt_next;               // a sort in progress: the buffers it writes
-    int32_t virt, pad_;  // virt: an extra sort since the last collision sort left the
-                         // particles in place (the reference order is in perm/inv)
+    int32_t virt, def;   // virt: an extra sort since the last collision sort left the
+                         // particles in place (the reference order is in perm/inv);
+                         // def: the last extra sort was deferred (its cells are in dkey)
+    int32_t kind, ccount;  // the sort in progress is deferred; its candidate count
     int64_t md_step, cycle, n_extra;
     double origin[3];
     double vmax2, list_max2, co_max, e1, e2;
@@ -68,6 +71,10 @@ struct DimerSim {
     int32_t *lut = nullptr;
     void *f_rec = nullptr;  // fused mode: staging records of a sort
     int32_t *f_perm = nullptr, *f_inv = nullptr, *f_pair = nullptr;  // fused mode: extra sorts
+    int32_t *f_dkey = nullptr, *f_cand = nullptr;  // fused mode: deferred extra sorts
+    void *f_stage = nullptr;
+    int64_t cand_cap = 0;
+    int stencil_m = 0;
     int32_t *f_key = nullptr, *f_slot = nullptr, *scan_sums = nullptr, *f_slocal = nullptr;
     int32_t *f_mslots = nullptr, *f_mcount = nullptr;
     int64_t *f_plan = nullptr;
