@@ -1549,6 +1549,21 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
 }
 
 // ---- collision: SRD (mpcd.f90:120-180) + refuel or bulk reaction, |v|max ---------------
+// refuel's test (single_dimer_pbc.f90:544-565): farther than far = (L(1)*0.45)**2, the
+// product in single precision, from both colloids
+template <class R>
+__device__ inline bool refuel_far(const R *pos, int64_t n, int64_t i, const double *cp, const Box &box)
+{
+    float farf = (float)box.L[0] * 0.45f;
+    farf = farf * farf;
+    const double far = (double)farf;
+    const double *L = box.edges;
+    const double y0 = (double)pos[i], y1 = (double)pos[n + i], y2 = (double)pos[2 * n + i];
+    const double a0 = min_image(cp[0] - y0, L[0]), a1 = min_image(cp[2] - y1, L[1]), a2 = min_image(cp[4] - y2, L[2]);
+    const double b0 = min_image(cp[1] - y0, L[0]), b1 = min_image(cp[3] - y1, L[1]), b2 = min_image(cp[5] - y2, L[2]);
+    return (a0 * a0 + a1 * a1) + a2 * a2 > far && (b0 * b0 + b1 * b1) + b2 * b2 > far;
+}
+
 template <class F>
 __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
 {
@@ -1654,7 +1669,8 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
                     }
                 }
             }
-            for (int i = 0; i < m; ++i) {
+            // with the tile in LDS, |v|max and refuel run over the tile's particles below
+            for (int i = 0; i < m && !in_lds; ++i) {
                 double a = V(0, i), b2 = V(1, i), c2 = V(2, i);
                 double q2 = (a * a + b2 * b2) + c2 * c2;
                 vm = q2 > vm ? q2 : vm;
@@ -1674,27 +1690,18 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
                     threefry2x64((uint64_t)C.cycle, (uint64_t)c, rp.seed, STREAM_REACTION, a0, a1);
                     if (u01(a0) < (1 - exp(-local_rate * P.tau)) && pick >= 0) spc[pick] = (S)1;
                 }
-            } else {
-                // refuel (single_dimer_pbc.f90:544-565): far = (L(1)*0.45)**2 in single precision
-                float farf = (float)rp.box.L[0] * 0.45f;
-                farf = farf * farf;
-                const double far = (double)farf;
-                const double *cp = B.cpos + 6 * r;
-                const double *L = rp.box.edges;
-                for (int i = 0; i < m; ++i) {
-                    if (Sp(i) != 2) continue;
-                    const double y0 = (double)pos[s + i], y1 = (double)pos[n + s + i], y2 = (double)pos[2 * n + s + i];
-                    double a0 = min_image(cp[0] - y0, L[0]), a1 = min_image(cp[2] - y1, L[1]),
-                           a2 = min_image(cp[4] - y2, L[2]);
-                    double b0 = min_image(cp[1] - y0, L[0]), b1 = min_image(cp[3] - y1, L[1]),
-                           b2 = min_image(cp[5] - y2, L[2]);
-                    if ((a0 * a0 + a1 * a1) + a2 * a2 > far && (b0 * b0 + b1 * b1) + b2 * b2 > far)
-                        spc[s + i] = (S)1;
-                }
+            } else if (!in_lds) {
+                for (int i = 0; i < m; ++i)
+                    if (Sp(i) == 2 && refuel_far(pos, n, s + i, B.cpos + 6 * r, rp.box)) spc[s + i] = (S)1;
             }
         }
         __syncthreads();
+        // the tile's particles in parallel: |v|max, refuel, write-back (all coalesced)
         for (int q = threadIdx.x; q < tcount; q += CT) {
+            const double a = lx[q], b2 = ly[q], c2 = lz[q];
+            const double q2 = (a * a + b2 * b2) + c2 * c2;
+            vm = q2 > vm ? q2 : vm;
+            if (!P.bulk && lsp[q] == 2 && refuel_far(pos, n, base + q, B.cpos + 6 * r, rp.box)) spc[base + q] = (S)1;
             vel[base + q] = lx[q];
             vel[n + base + q] = ly[q];
             vel[2 * n + base + q] = lz[q];
