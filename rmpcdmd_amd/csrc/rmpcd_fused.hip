@@ -38,7 +38,7 @@ constexpr int LAZY_STREAM_BLOCKS = 1024;  // stream-role blocks per replica in l
 constexpr int SORT_BLOCKS = 4096;     // blocks of the particle passes of a sort (all replicas), default
 constexpr int SCAN_BLOCKS = 1024;     // blocks of the cell-count tile scan (all replicas)
 constexpr int CT = 128;            // cells per block in the collision kernel
-constexpr int CTILE = 2048;        // staged particles per collision block
+constexpr int CTILE = 1536;  // staged particles per collision block (12 per cell; a block with more works from memory)
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
 constexpr int MAX_STENCIL = 2048;  // stencil cells of the fused list build
