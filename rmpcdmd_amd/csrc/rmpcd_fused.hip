@@ -4,10 +4,14 @@
 //
 // Per MD step (programs/single_dimer_pbc.f90:306-383), one kernel k_fz_step: the
 // neighbour-list members finish the previous step (forces, md_vel, catalytic flagging
-// and conversion) and stream; every other particle streams; the last block finishes the
-// colloids and takes the skin decision.  Then, only when the skin test fires, the sort:
-// key+count, scan, place, stable gather, list.  Per collision: the same sort with a
-// fresh grid shift, then SRD + refuel (or the bulk reaction) per cell.
+// and conversion) and stream; every other particle streams.  Then one block per replica
+// (k_fz_colloid) finishes the colloids and takes the skin decision, and, only when the
+// skin test fires, an extra sort: it computes the reference's new order without moving
+// the particles (key+count, scan, place, rank into perm/inv) or, one extra sort in two,
+// defers even that (the key pass keeps the cells, the list build ranks the particles
+// near the colloids).  Per collision: a sort with a fresh grid shift that moves the
+// particles (the key pass writes a staging record per particle, the gather writes them
+// out in cell order), then SRD + refuel (or the bulk reaction) per cell.
 //
 // Why member-only work is exact: forces only reach particles in a colloid's list, the
 // list keeps every particle within max_cut + skin, and the skin test rebuilds it before
@@ -876,48 +880,48 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
             }
         }
         if (!dfr) {
-        // the LDS hash after all of the thread's loads are issued
-        for (int q = threadIdx.x; q < KHASH; q += FT) {
-            hkey[q] = -1;
-            hcnt[q] = 0;
-        }
-        __syncthreads();
-        for (int v = 0; v < KPT; ++v) {
-            const int key = keys[v];
-            if (key < 0) continue;
-            // open addressing; the table has twice as many entries as a chunk has keys
-            unsigned h = ((unsigned)key * 2654435761u) >> (32 - KHASH_BITS);
-            for (int probe = 0; probe < KHASH; ++probe) {
-                const int prev = atomicCAS(&hkey[h], -1, key);
-                if (prev == -1 || prev == key) break;
-                h = (h + 1) & (KHASH - 1);
+            // the LDS hash after all of the thread's loads are issued
+            for (int q = threadIdx.x; q < KHASH; q += FT) {
+                hkey[q] = -1;
+                hcnt[q] = 0;
             }
-            ent[v] = (int)h;
-            loc[v] = atomicAdd(&hcnt[h], 1);
-        }
-        __syncthreads();
-        {
-            // the occupied entries compacted first, so that a thread waits for at most
-            // one global atomic per round (a chunk has a few hundred distinct keys)
-            constexpr int PER = KHASH / FT;
-            int c = 0;
-            for (int j = 0; j < PER; ++j) c += hkey[threadIdx.x * PER + j] >= 0 ? 1 : 0;
-            int tot;
-            int o = scan_block_exclusive(c, lds, &tot);
-            for (int j = 0; j < PER; ++j)
-                if (hkey[threadIdx.x * PER + j] >= 0) occ[o++] = threadIdx.x * PER + j;
             __syncthreads();
-            for (int e = threadIdx.x; e < tot; e += FT) {
-                const int q = occ[e];
-                hcnt[q] = atomicAdd(&cnt[hkey[q]], hcnt[q]);
+            for (int v = 0; v < KPT; ++v) {
+                const int key = keys[v];
+                if (key < 0) continue;
+                // open addressing; the table has twice as many entries as a chunk has keys
+                unsigned h = ((unsigned)key * 2654435761u) >> (32 - KHASH_BITS);
+                for (int probe = 0; probe < KHASH; ++probe) {
+                    const int prev = atomicCAS(&hkey[h], -1, key);
+                    if (prev == -1 || prev == key) break;
+                    h = (h + 1) & (KHASH - 1);
+                }
+                ent[v] = (int)h;
+                loc[v] = atomicAdd(&hcnt[h], 1);
             }
-        }
-        __syncthreads();
-        for (int v = 0; v < KPT; ++v) {
-            const int64_t i = c0 + v * FT + threadIdx.x;
-            if (keys[v] < 0) continue;
-            B.slot[rp.p_off + i] = hcnt[ent[v]] + loc[v];
-        }
+            __syncthreads();
+            {
+                // the occupied entries compacted first, so that a thread waits for at most
+                // one global atomic per round (a chunk has a few hundred distinct keys)
+                constexpr int PER = KHASH / FT;
+                int c = 0;
+                for (int j = 0; j < PER; ++j) c += hkey[threadIdx.x * PER + j] >= 0 ? 1 : 0;
+                int tot;
+                int o = scan_block_exclusive(c, lds, &tot);
+                for (int j = 0; j < PER; ++j)
+                    if (hkey[threadIdx.x * PER + j] >= 0) occ[o++] = threadIdx.x * PER + j;
+                __syncthreads();
+                for (int e = threadIdx.x; e < tot; e += FT) {
+                    const int q = occ[e];
+                    hcnt[q] = atomicAdd(&cnt[hkey[q]], hcnt[q]);
+                }
+            }
+            __syncthreads();
+            for (int v = 0; v < KPT; ++v) {
+                const int64_t i = c0 + v * FT + threadIdx.x;
+                if (keys[v] < 0) continue;
+                B.slot[rp.p_off + i] = hcnt[ent[v]] + loc[v];
+            }
         }
         __syncthreads();
         if (c0 == 0 && threadIdx.x == 0) {
@@ -1340,35 +1344,35 @@ __device__ void deferred_list(FB<F> &B, const FP &P, int R, int r, int col, int 
             for (int k = 0; k < 3; ++k) xv[u][k] = pos[k * n + iv[u]];
         }
         for (int u = 0; u < CU; ++u) {
-        if (e0 + u * blockDim.x >= ncand) break;
-        const int i = iv[u];
-        const int spc = sv[u];
-        if (spc <= 0) continue;
-        const typename F::real xi[3] = {xv[u][0], xv[u][1], xv[u][2]};
-        int q[3];
-        cell_of<typename F::real>(b, xi[0], xi[1], xi[2], q[0], q[1], q[2]);
-        int box = 0;
-        bool in = true;
-        for (int k = 0; k < 3; ++k) {
-            int o = (q[k] - pc[k]) % b.L[k];
-            o += o < 0 ? b.L[k] : 0;
-            o = o > b.L[k] / 2 ? o - b.L[k] : o;
-            in = in && o >= -m && o <= m;
-            box = box * wd + (o + m);
-        }
-        if (!in) continue;
-        const int j = (int)dlut[box];
-        if (j < 0) continue;
-        const double d0 = min_image(x[0] - (double)xi[0], L0);
-        const double d1 = min_image(x[1] - (double)xi[1], L1);
-        const double d2 = min_image(x[2] - (double)xi[2], L2);
-        const double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
-        if (!(rsq < P.radius_sq)) continue;
-        const int at = atomicAdd(&nsel, 1);
-        if (at < P.nmax) {
-            stg[at] = make_int4(i, j, virt_in ? inv[i] : i, 0);
-            atomicAdd(&cntj[j], 1);
-        }
+            if (e0 + u * blockDim.x >= ncand) break;
+            const int i = iv[u];
+            const int spc = sv[u];
+            if (spc <= 0) continue;
+            const typename F::real xi[3] = {xv[u][0], xv[u][1], xv[u][2]};
+            int q[3];
+            cell_of<typename F::real>(b, xi[0], xi[1], xi[2], q[0], q[1], q[2]);
+            int box = 0;
+            bool in = true;
+            for (int k = 0; k < 3; ++k) {
+                int o = (q[k] - pc[k]) % b.L[k];
+                o += o < 0 ? b.L[k] : 0;
+                o = o > b.L[k] / 2 ? o - b.L[k] : o;
+                in = in && o >= -m && o <= m;
+                box = box * wd + (o + m);
+            }
+            if (!in) continue;
+            const int j = (int)dlut[box];
+            if (j < 0) continue;
+            const double d0 = min_image(x[0] - (double)xi[0], L0);
+            const double d1 = min_image(x[1] - (double)xi[1], L1);
+            const double d2 = min_image(x[2] - (double)xi[2], L2);
+            const double rsq = (d0 * d0 + d1 * d1) + d2 * d2;
+            if (!(rsq < P.radius_sq)) continue;
+            const int at = atomicAdd(&nsel, 1);
+            if (at < P.nmax) {
+                stg[at] = make_int4(i, j, virt_in ? inv[i] : i, 0);
+                atomicAdd(&cntj[j], 1);
+            }
         }
     }
     __syncthreads();
