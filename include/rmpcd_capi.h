@@ -148,15 +148,31 @@ int rmpcd_cell_maximum_displacement(const rmpcd_box_t *box, const rmpcd_cells_t 
  * used[item] (uniforms consumed, -1 when the stream ran out) are device outputs */
 int rmpcd_rand_sphere(const double *u, int64_t stride, int64_t n_items, double *axes,
                       int32_t *used);
-/* simple_mpcd_step (src/mpcd.f90:71-183), SRD branch.  axes: NULL or device array of
- * 3 planes x ncells, the unit axis drawn for each cell (read only for cells that draw);
- * otherwise the axis comes from Threefry keyed by (seed, step, cell).  alpha_given = 0
- * gives the reference default (sin = 1, cos = 0).  thermostat != 0 is rejected
- * (the MPC-AT branch, mpcd.f90:144-160, is not part of this library). */
+/* simple_mpcd_step (src/mpcd.f90:71-183).  SRD branch: axes is NULL or a device array
+ * of 3 planes x ncells, the unit axis drawn for each cell (read only for cells that
+ * draw); otherwise the axis comes from Threefry keyed by (seed, step, cell).
+ * alpha_given = 0 gives the reference default (sin = 1, cos = 0).  thermostat != 0 runs
+ * the MPC-AT branch (mpcd.f90:144-160) with Threefry normals, as rmpcd_mpcat_step with
+ * normals = NULL (axes is not read). */
 int rmpcd_simple_mpcd_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
                            rmpcd_particles_t *p, const double *axes, uint64_t seed,
                            uint64_t step, int32_t alpha_given, double alpha,
                            int32_t thermostat, double T, int32_t hydro);
+/* The MPC-AT branch of simple_mpcd_step (src/mpcd.f90:144-160, thermostat = .true.):
+ * normals is NULL or a device array of 3 planes x n, the three normals drawn for the
+ * particle in each slot (read for species > 0 in cells that draw); otherwise they come
+ * from Threefry keyed by (seed, step, particle id).  hydro = 0 leaves the new
+ * velocities' cell mean as drawn (mpcd.f90:152-158). */
+int rmpcd_mpcat_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p,
+                     const double *normals, uint64_t seed, uint64_t step, double T, int32_t hydro);
+/* rescale_cells (src/mpcd.f90:748-801) */
+int rmpcd_rescale_cells(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p, double T);
+/* bulk_reaction_endothermic (src/mpcd.f90:646-700); draws: NULL or device array, one per
+ * cell (read for the cells that draw) */
+int rmpcd_bulk_reaction_endothermic(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                                    rmpcd_particles_t *p, int32_t from, int32_t to, double rate,
+                                    double tau, double delta_u, const double *draws, uint64_t seed,
+                                    uint64_t step);
 /* compute_temperature (src/mpcd.f90:342-407) */
 int rmpcd_compute_temperature(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
                               const rmpcd_particles_t *p, double *te);

@@ -210,6 +210,22 @@ def bulk_reaction(c, cnt, st, is_reac, species, frm, to, rate, tau, draws):
                             D(rate), D(tau), P(draws))
 
 
+def mpcat_step(c, cnt, st, vel, species, normals, T, hydro=True):
+    """the MPC-AT branch of simple_mpcd_step (mpcd.f90:144-160); normals: (n, 3) per slot"""
+    lib().orc_mpcat_step(C.byref(c), P(cnt), P(st), P(vel), P(species), P(normals), D(T), 1 if hydro else 0)
+
+
+def rescale_cells(c, cnt, st, vel, species, T):
+    """rescale_cells (mpcd.f90:748-801)"""
+    lib().orc_rescale_cells(C.byref(c), P(cnt), P(st), P(vel), P(species), D(T))
+
+
+def bulk_reaction_endothermic(c, cnt, st, is_reac, vel, species, frm, to, rate, tau, delta_u, draws):
+    """bulk_reaction_endothermic (mpcd.f90:646-700); draws: one per cell"""
+    lib().orc_bulk_reaction_endothermic(C.byref(c), P(cnt), P(st), P(is_reac), P(vel), P(species), int(frm),
+                                        int(to), D(rate), D(tau), D(delta_u), P(draws))
+
+
 def mpcd_stream_nogravity_zwall(c, pos, vel, force, flags, dt):
     lib().orc_mpcd_stream_nogravity_zwall(C.byref(c), I64(pos.shape[0]), P(pos), P(vel),
                                           P(force), P(flags), D(dt))

@@ -383,7 +383,7 @@ void orc_simple_mpcd_step(const orc_cells_t *c, const int32_t *cell_count,
                           const int32_t *cell_start, double *vel, const int32_t *species,
                           const double *axes, int alpha_given, double alpha)
 {
-    /* mpcd.f90:71-183, non-thermostat branch (the MPC-AT branch is not restated) */
+    /* mpcd.f90:71-183, SRD branch (the thermostat branch is orc_mpcat_step) */
     double sin_a, cos_a;
     if (alpha_given) {
         sin_a = sin(alpha);
@@ -507,6 +507,112 @@ void orc_bulk_reaction(const orc_cells_t *c, const int32_t *cell_count,
         }
         local_rate = local_rate * rate;
         if (draws[ci] < (1 - exp(-local_rate * tau))) species[pick] = to;
+    }
+}
+
+void orc_mpcat_step(const orc_cells_t *c, const int32_t *cell_count, const int32_t *cell_start,
+                    double *vel, const int32_t *species, const double *normals, double T, int hydro)
+{
+    /* mpcd.f90:144-160, the thermostat branch of simple_mpcd_step: normals holds the
+       three normals drawn for the particle in each slot (read for species > 0) */
+    double t_factor = sqrt(T);
+    ORC_PARALLEL_FOR
+    for (int ci = 0; ci < c->N; ++ci) {
+        if (cell_count[ci] <= 1) continue;
+        int start = cell_start[ci], n = cell_count[ci];
+        int n_eff = 0;
+        double lv[3] = {0, 0, 0};
+        for (int i = start; i < start + n; ++i) {
+            if (species[i] > 0) {
+                for (int k = 0; k < 3; ++k) lv[k] = lv[k] + vel[3 * i + k];
+                n_eff++;
+            }
+        }
+        if (n_eff <= 1) continue;
+        for (int k = 0; k < 3; ++k) lv[k] = lv[k] / n_eff;
+        double vv[3] = {0, 0, 0};
+        for (int i = start; i < start + n; ++i) {
+            if (species[i] <= 0) continue;
+            for (int k = 0; k < 3; ++k) {
+                vel[3 * i + k] = normals[3 * i + k] * t_factor;
+                vv[k] = vv[k] + vel[3 * i + k];
+            }
+        }
+        for (int k = 0; k < 3; ++k) vv[k] = lv[k] - vv[k] / n_eff;
+        if (hydro)
+            for (int i = start; i < start + n; ++i) {
+                if (species[i] <= 0) continue;
+                for (int k = 0; k < 3; ++k) vel[3 * i + k] = vel[3 * i + k] + vv[k];
+            }
+    }
+}
+
+void orc_rescale_cells(const orc_cells_t *c, const int32_t *cell_count, const int32_t *cell_start,
+                       double *vel, const int32_t *species, double T)
+{
+    /* mpcd.f90:748-801 */
+    ORC_PARALLEL_FOR
+    for (int ci = 0; ci < c->N; ++ci) {
+        if (cell_count[ci] <= 1) continue;
+        int start = cell_start[ci], n = cell_count[ci];
+        int n_eff = 0;
+        double lv[3] = {0, 0, 0};
+        for (int i = start; i < start + n; ++i) {
+            if (species[i] > 0) {
+                for (int k = 0; k < 3; ++k) lv[k] = lv[k] + vel[3 * i + k];
+                n_eff++;
+            }
+        }
+        if (n_eff <= 1) continue;
+        for (int k = 0; k < 3; ++k) lv[k] = lv[k] / n_eff;
+        double kin = 0;
+        for (int i = start; i < start + n; ++i) {
+            if (species[i] <= 0) continue;
+            double d0 = vel[3 * i] - lv[0], d1 = vel[3 * i + 1] - lv[1], d2 = vel[3 * i + 2] - lv[2];
+            kin = kin + ((d0 * d0 + d1 * d1) + d2 * d2);
+        }
+        double f = sqrt((double)(3 * (n_eff - 1)) * T / kin);
+        for (int i = start; i < start + n; ++i) {
+            if (species[i] <= 0) continue;
+            for (int k = 0; k < 3; ++k) vel[3 * i + k] = lv[k] + f * (vel[3 * i + k] - lv[k]);
+        }
+    }
+}
+
+void orc_bulk_reaction_endothermic(const orc_cells_t *c, const int32_t *cell_count,
+                                   const int32_t *cell_start, const uint8_t *is_reac, double *vel,
+                                   int32_t *species, int from, int to, double rate, double tau,
+                                   double delta_u, const double *draws)
+{
+    /* mpcd.f90:646-700; draws[cell] is read only by the cells that draw */
+    ORC_PARALLEL_FOR
+    for (int ci = 0; ci < c->N; ++ci) {
+        if (cell_count[ci] <= 1 || !is_reac[ci]) continue;
+        int start = cell_start[ci], n = cell_count[ci];
+        double local_rate = 0, com[3] = {0, 0, 0};
+        int pick = -1;
+        for (int i = start; i < start + n; ++i) {
+            if (species[i] == from) {
+                local_rate = local_rate + 1;
+                pick = i;
+            }
+            for (int k = 0; k < 3; ++k) com[k] = com[k] + vel[3 * i + k];
+        }
+        for (int k = 0; k < 3; ++k) com[k] = com[k] / n;
+        double kin = 0;
+        for (int i = start; i < start + n; ++i) {
+            double d0 = vel[3 * i] - com[0], d1 = vel[3 * i + 1] - com[1], d2 = vel[3 * i + 2] - com[2];
+            kin = kin + ((d0 * d0 + d1 * d1) + d2 * d2);
+        }
+        kin = kin / 2;
+        if (!(local_rate > 0 && kin > delta_u)) continue;
+        local_rate = local_rate * rate;
+        if (draws[ci] < (1 - exp(-local_rate * tau))) {
+            species[pick] = to;
+            double f = sqrt((kin - delta_u) / kin);
+            for (int i = start; i < start + n; ++i)
+                for (int k = 0; k < 3; ++k) vel[3 * i + k] = com[k] + f * (vel[3 * i + k] - com[k]);
+        }
     }
 }
 

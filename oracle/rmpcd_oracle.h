@@ -106,6 +106,15 @@ void orc_bulk_reaction(const orc_cells_t *c, const int32_t *cell_count,
                        const int32_t *cell_start, const uint8_t *is_reac,
                        int32_t *species, int from, int to, double rate, double tau,
                        const double *draws);                          /* [pinned:math] */
+void orc_mpcat_step(const orc_cells_t *c, const int32_t *cell_count, const int32_t *cell_start,
+                    double *vel, const int32_t *species, const double *normals, double T,
+                    int hydro);                                              /* [pinned:math] */
+void orc_rescale_cells(const orc_cells_t *c, const int32_t *cell_count, const int32_t *cell_start,
+                       double *vel, const int32_t *species, double T);       /* [pinned:math] */
+void orc_bulk_reaction_endothermic(const orc_cells_t *c, const int32_t *cell_count,
+                                   const int32_t *cell_start, const uint8_t *is_reac, double *vel,
+                                   int32_t *species, int from, int to, double rate, double tau,
+                                   double delta_u, const double *draws);     /* [pinned:math] */
 void orc_mpcd_stream_nogravity_zwall(const orc_cells_t *c, int64_t n, double *pos,
                                      double *vel, const double *force, int32_t *flags,
                                      double dt);                      /* [pinned:math] */

@@ -131,6 +131,9 @@ SIGNATURES = {
     "rmpcd_compute_temperature": [_BOXP, _CELLP, _PARTP, _D3],
     "rmpcd_mpcd_stream_periodic": [_BOXP, _PARTP, _D],
     "rmpcd_bulk_reaction": [_BOXP, _CELLP, _PARTP, _I, _I, _D, _D, _P, _U64, _U64],
+    "rmpcd_mpcat_step": [_BOXP, _CELLP, _PARTP, _P, _U64, _U64, _D, _I],
+    "rmpcd_rescale_cells": [_BOXP, _CELLP, _PARTP, _D],
+    "rmpcd_bulk_reaction_endothermic": [_BOXP, _CELLP, _PARTP, _I, _I, _D, _D, _D, _P, _U64, _U64],
     "rmpcd_mpcd_stream_nogravity_zwall": [_BOXP, _PARTP, _D],
     "rmpcd_md_pos": [_PARTP, _D],
     "rmpcd_md_vel": [_PARTP, _D],

@@ -258,7 +258,22 @@ enum : uint64_t {
     STREAM_SHIFT = 5,
     STREAM_INIT = 6,
     STREAM_PLACE = 7,
+    STREAM_THERMO = 8,
 };
+
+// three standard normals for item at step: Box-Muller on the four uniforms of the
+// counters (2 step, item) and (2 step + 1, item) of stream st
+__device__ inline void normal3_threefry(uint64_t seed, uint64_t st, uint64_t step, uint64_t item, double out[3])
+{
+    uint64_t a, b, c, d;
+    threefry2x64(2 * step, item, seed, st, a, b);
+    threefry2x64(2 * step + 1, item, seed, st, c, d);
+    const double u1 = 1.0 - u01(a), u2 = u01(b), u3 = 1.0 - u01(c), u4 = u01(d);
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    out[0] = r1 * cos(2 * M_PI * u2);
+    out[1] = r1 * sin(2 * M_PI * u2);
+    out[2] = r2 * cos(2 * M_PI * u4);
+}
 
 // Marsaglia's method on a stream of uniform pairs (src/mpcd.f90:43-62)
 struct Sphere {

@@ -1,5 +1,6 @@
-// rmpcd_mpcd.hip -- SRD collision, temperature, streaming and bulk reaction
-// (src/mpcd.f90:43-183, 342-407, 440-457, 605-643, 702-745).
+// rmpcd_mpcd.hip -- SRD collision and its MPC-AT thermostat branch, temperature,
+// streaming, bulk reactions (plain and endothermic) and the cell-wise kinetic-energy
+// rescaling (src/mpcd.f90:43-183, 342-407, 440-457, 605-700, 702-745, 748-801).
 //
 // Cell kernels work on the cell-sorted solvent: a workgroup owns a run of consecutive
 // cells, stages their velocities and species from HBM into LDS with coalesced loads,
@@ -157,6 +158,151 @@ __global__ void __launch_bounds__(CELL_THREADS)
     }
 }
 
+// The MPC-AT branch of simple_mpcd_step (mpcd.f90:144-160): in a cell with more than
+// one particle of species > 0, each of them gets sqrt(T) times three normals; with hydro
+// the cell's previous mean velocity is restored by adding (mean before - mean after).
+// Normals injected per particle slot (3 planes of n) or drawn from Threefry keyed by
+// (seed, step, particle id).
+template <class F>
+__global__ void __launch_bounds__(CELL_THREADS)
+    k_mpcat(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count, PView<F> p,
+            const double *__restrict__ normals, uint64_t seed, uint64_t step, double t_factor, int hydro)
+{
+    using R = typename F::real;
+    __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
+    __shared__ typename F::small lsp[TILE_CAP];
+    int c0 = blockIdx.x * CELL_THREADS;
+    int c1 = min(c0 + CELL_THREADS, ncells);
+    CellTile<F> t{lx, ly, lz, lsp, 0, 0};
+    stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
+    int c = c0 + threadIdx.x;
+    if (c < c1) {
+        int n = count[c];
+        if (n > 1) {
+            int64_t s = start[c];
+            bool in_lds = t.count > 0;
+            int64_t o = s - t.base;
+            auto V = [&](int k, int64_t i) -> R {
+                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
+                return p.vel[k * p.n + s + i];
+            };
+            auto W = [&](int k, int64_t i, R v) {
+                R *l = k == 0 ? lx : k == 1 ? ly : lz;
+                if (in_lds)
+                    l[o + i] = v;
+                else
+                    p.vel[k * p.n + s + i] = v;
+            };
+            auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
+            Acc3<R> acc;
+            int n_eff = 0;
+            for (int i = 0; i < n; ++i)
+                if (S(i) > 0) {
+                    acc.add(0, V(0, i));
+                    acc.add(1, V(1, i));
+                    acc.add(2, V(2, i));
+                    ++n_eff;
+                }
+            if (n_eff > 1) {
+                const R u[3] = {acc.s[0] / (R)n_eff, acc.s[1] / (R)n_eff, acc.s[2] / (R)n_eff};
+                Acc3<R> vir;
+                for (int i = 0; i < n; ++i) {
+                    if (S(i) <= 0) continue;
+                    double g[3];
+                    if (normals) {
+                        for (int k = 0; k < 3; ++k) g[k] = normals[k * p.n + s + i];
+                    } else {
+                        normal3_threefry(seed, STREAM_THERMO, step, (uint64_t)p.id[s + i], g);
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        const R v = (R)(g[k] * t_factor);
+                        W(k, i, v);
+                        vir.add(k, v);
+                    }
+                }
+                if (hydro) {
+                    const R d[3] = {u[0] - vir.s[0] / (R)n_eff, u[1] - vir.s[1] / (R)n_eff,
+                                    u[2] - vir.s[2] / (R)n_eff};
+                    for (int i = 0; i < n; ++i) {
+                        if (S(i) <= 0) continue;
+                        for (int k = 0; k < 3; ++k) W(k, i, V(k, i) + d[k]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < t.count; q += blockDim.x) {
+        p.vel[t.base + q] = lx[q];
+        p.vel[p.n + t.base + q] = ly[q];
+        p.vel[2 * p.n + t.base + q] = lz[q];
+    }
+}
+
+// rescale_cells (mpcd.f90:748-801): in a cell with more than one particle of species
+// > 0, their velocities about the cell mean are scaled so that their kinetic energy is
+// 3 (n_eff - 1) T / 2 (sums in FP64 in the reference's order)
+template <class F>
+__global__ void __launch_bounds__(CELL_THREADS)
+    k_rescale(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count, PView<F> p,
+              double T)
+{
+    using R = typename F::real;
+    __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
+    __shared__ typename F::small lsp[TILE_CAP];
+    int c0 = blockIdx.x * CELL_THREADS;
+    int c1 = min(c0 + CELL_THREADS, ncells);
+    CellTile<F> t{lx, ly, lz, lsp, 0, 0};
+    stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
+    int c = c0 + threadIdx.x;
+    if (c < c1) {
+        int n = count[c];
+        if (n > 1) {
+            int64_t s = start[c];
+            bool in_lds = t.count > 0;
+            int64_t o = s - t.base;
+            auto V = [&](int k, int64_t i) -> R {
+                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
+                return p.vel[k * p.n + s + i];
+            };
+            auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
+            double u[3] = {0, 0, 0};
+            int n_eff = 0;
+            for (int i = 0; i < n; ++i)
+                if (S(i) > 0) {
+                    for (int k = 0; k < 3; ++k) u[k] = u[k] + (double)V(k, i);
+                    ++n_eff;
+                }
+            if (n_eff > 1) {
+                for (int k = 0; k < 3; ++k) u[k] = u[k] / n_eff;
+                double kin = 0;
+                for (int i = 0; i < n; ++i)
+                    if (S(i) > 0) {
+                        const double d0 = V(0, i) - u[0], d1 = V(1, i) - u[1], d2 = V(2, i) - u[2];
+                        kin = kin + ((d0 * d0 + d1 * d1) + d2 * d2);
+                    }
+                const double f = sqrt((double)(3 * (n_eff - 1)) * T / kin);
+                for (int i = 0; i < n; ++i) {
+                    if (S(i) <= 0) continue;
+                    for (int k = 0; k < 3; ++k) {
+                        const R v = (R)(u[k] + f * ((double)V(k, i) - u[k]));
+                        if (in_lds)
+                            (k == 0 ? lx : k == 1 ? ly : lz)[o + i] = v;
+                        else
+                            p.vel[k * p.n + s + i] = v;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < t.count; q += blockDim.x) {
+        p.vel[t.base + q] = lx[q];
+        p.vel[p.n + t.base + q] = ly[q];
+        p.vel[2 * p.n + t.base + q] = lz[q];
+    }
+}
+
 // compute_temperature (mpcd.f90:342-407): per block partial sums of kin and of the
 // number of counted cells; the last block adds the partials in block order
 template <class F>
@@ -267,6 +413,59 @@ __global__ void k_bulk_reaction(int ncells, const int32_t *__restrict__ start,
     if (u < (1 - exp(-local_rate * tau)) && pick >= 0) p.species[pick] = (typename F::small)to;
 }
 
+// bulk_reaction_endothermic (mpcd.f90:646-700): in a reactive cell with a 'from'
+// particle whose kinetic energy about the cell's centre of mass exceeds delta_u, one draw;
+// on success the last 'from' converts and the velocities about the centre of mass are
+// scaled by sqrt((kin - delta_u) / kin), which removes delta_u
+template <class F>
+__global__ void k_bulk_endothermic(int ncells, const int32_t *__restrict__ start,
+                                   const int32_t *__restrict__ count, const uint8_t *__restrict__ is_reac,
+                                   PView<F> p, int from, int to, double rate, double tau, double delta_u,
+                                   const double *__restrict__ draws, uint64_t seed, uint64_t step)
+{
+    using R = typename F::real;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int n = count[c];
+    if (n <= 1 || !is_reac[c]) return;
+    int64_t s = start[c];
+    double local_rate = 0;
+    int64_t pick = -1;
+    double com[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        if ((int)p.species[s + i] == from) {
+            local_rate = local_rate + 1;
+            pick = s + i;
+        }
+        for (int k = 0; k < 3; ++k) com[k] = com[k] + (double)p.vel[k * p.n + s + i];
+    }
+    for (int k = 0; k < 3; ++k) com[k] = com[k] / n;
+    double kin = 0;
+    for (int i = 0; i < n; ++i) {
+        const double d0 = p.vel[s + i] - com[0], d1 = p.vel[p.n + s + i] - com[1],
+                     d2 = p.vel[2 * p.n + s + i] - com[2];
+        kin = kin + ((d0 * d0 + d1 * d1) + d2 * d2);
+    }
+    kin = kin / 2;
+    if (!(local_rate > 0 && kin > delta_u)) return;
+    local_rate = local_rate * rate;
+    double u;
+    if (draws) {
+        u = draws[c];
+    } else {
+        uint64_t a, b;
+        threefry2x64(step, (uint64_t)c, seed, STREAM_REACTION, a, b);
+        u = u01(a);
+    }
+    if (u < (1 - exp(-local_rate * tau))) {
+        p.species[pick] = (typename F::small)to;
+        const double f = sqrt((kin - delta_u) / kin);
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k)
+                p.vel[k * p.n + s + i] = (R)(com[k] + f * ((double)p.vel[k * p.n + s + i] - com[k]));
+    }
+}
+
 // mpcd_stream_periodic (mpcd.f90:440-457)
 template <class F>
 __global__ void k_stream_periodic(Box b, PView<F> p, double dt)
@@ -372,6 +571,39 @@ int srd_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles
 }
 
 template <class F>
+int mpcat_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *pp, const double *normals,
+               uint64_t seed, uint64_t step, double t_factor, int hydro)
+{
+    int nc = box->ncells;
+    k_mpcat<F><<<blocks_for(nc, CELL_THREADS), CELL_THREADS, 0, stream()>>>(
+        nc, cells->cell_start, cells->cell_count, view<F>(pp), normals, seed, step, t_factor, hydro);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
+int rescale_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *pp, double T)
+{
+    int nc = box->ncells;
+    k_rescale<F><<<blocks_for(nc, CELL_THREADS), CELL_THREADS, 0, stream()>>>(nc, cells->cell_start,
+                                                                               cells->cell_count, view<F>(pp), T);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
+int bulk_endo_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *pp, int from, int to,
+                   double rate, double tau, double delta_u, const double *draws, uint64_t seed, uint64_t step)
+{
+    int nc = box->ncells;
+    k_bulk_endothermic<F><<<blocks_for(nc, 256), 256, 0, stream()>>>(nc, cells->cell_start, cells->cell_count,
+                                                                      cells->is_reac, view<F>(pp), from, to, rate,
+                                                                      tau, delta_u, draws, seed, step);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
 int temperature_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
                      const rmpcd_particles_t *pp, double *te)
 {
@@ -443,7 +675,6 @@ int rmpcd_simple_mpcd_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells, r
                            const double *axes, uint64_t seed, uint64_t step, int32_t alpha_given,
                            double alpha, int32_t thermostat, double T, int32_t hydro)
 {
-    (void)T;
     int rc = check_box(box, "simple_mpcd_step");
     if (!rc) rc = check_particles(p, "simple_mpcd_step", false);
     if (rc) return rc;
@@ -451,7 +682,12 @@ int rmpcd_simple_mpcd_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells, r
     // mpcd.f90:107-109
     if (!hydro && !thermostat)
         return set_error(RMPCD_ERR_ARG, "requiring .not.do_hydro without thermostat in simple_mpcd_step");
-    RMPCD_CHECK(!thermostat, "simple_mpcd_step: the MPC-AT thermostat branch is not provided");
+    if (thermostat) {
+        // the MPC-AT branch (mpcd.f90:144-160), normals from Threefry
+        RMPCD_CHECK(T >= 0, "simple_mpcd_step: negative temperature");
+        RMPCD_CHECK(p->id != nullptr, "simple_mpcd_step: the thermostat draws by particle id (null id)");
+        return RMPCD_DISPATCH_FORM(p, mpcat_impl, box, cells, p, nullptr, seed, step, sqrt(T), hydro ? 1 : 0);
+    }
     double s = 1, c = 0;
     if (alpha_given) {
         s = sin(alpha);
@@ -499,6 +735,40 @@ int rmpcd_bulk_reaction(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpc
                 "bulk_reaction: null cell arrays");
     RMPCD_CHECK(p->form == RMPCD_FP64 || (to >= 0 && to < 256), "bulk_reaction: species out of range");
     return RMPCD_DISPATCH_FORM(p, bulk_impl, box, cells, p, from, to, rate, tau, draws, seed, step);
+}
+
+int rmpcd_mpcat_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p,
+                     const double *normals, uint64_t seed, uint64_t step, double T, int32_t hydro)
+{
+    int rc = check_box(box, "mpcat_step");
+    if (!rc) rc = check_particles(p, "mpcat_step", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start, "mpcat_step: null cell arrays");
+    RMPCD_CHECK(T >= 0, "mpcat_step: negative temperature");
+    RMPCD_CHECK(normals || p->id, "mpcat_step: the thermostat draws by particle id (null id)");
+    return RMPCD_DISPATCH_FORM(p, mpcat_impl, box, cells, p, normals, seed, step, sqrt(T), hydro ? 1 : 0);
+}
+
+int rmpcd_rescale_cells(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p, double T)
+{
+    int rc = check_box(box, "rescale_cells");
+    if (!rc) rc = check_particles(p, "rescale_cells", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start, "rescale_cells: null cell arrays");
+    return RMPCD_DISPATCH_FORM(p, rescale_impl, box, cells, p, T);
+}
+
+int rmpcd_bulk_reaction_endothermic(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p,
+                                    int32_t from, int32_t to, double rate, double tau, double delta_u,
+                                    const double *draws, uint64_t seed, uint64_t step)
+{
+    int rc = check_box(box, "bulk_reaction_endothermic");
+    if (!rc) rc = check_particles(p, "bulk_reaction_endothermic", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start && cells->is_reac,
+                "bulk_reaction_endothermic: null cell arrays");
+    RMPCD_CHECK(p->form == RMPCD_FP64 || (to >= 0 && to < 256), "bulk_reaction_endothermic: species out of range");
+    return RMPCD_DISPATCH_FORM(p, bulk_endo_impl, box, cells, p, from, to, rate, tau, delta_u, draws, seed, step);
 }
 
 int rmpcd_threefry_uniform(uint64_t seed, uint64_t stream_id, uint64_t step, int64_t n, double *out)

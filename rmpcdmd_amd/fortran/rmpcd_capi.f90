@@ -137,6 +137,24 @@ module rmpcd_capi
        type(c_ptr), value :: draws
        integer(c_int64_t), value :: seed, step
      end function rmpcd_bulk_reaction
+     integer(c_int) function rmpcd_bulk_reaction_endothermic(box, cells, p, from, to, rate, tau, delta_u, draws, &
+          seed, step) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t) :: p
+       integer(c_int32_t), value :: from, to
+       real(c_double), value :: rate, tau, delta_u
+       type(c_ptr), value :: draws
+       integer(c_int64_t), value :: seed, step
+     end function rmpcd_bulk_reaction_endothermic
+     integer(c_int) function rmpcd_rescale_cells(box, cells, p, T) bind(c)
+       import :: c_int, c_double, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: T
+     end function rmpcd_rescale_cells
      integer(c_int) function rmpcd_mpcd_stream_nogravity_zwall(box, p, dt) bind(c)
        import :: c_int, c_double, rmpcd_box_t, rmpcd_particles_t
        type(rmpcd_box_t), intent(in) :: box

@@ -1013,7 +1013,7 @@ module mpcd
   implicit none
   private
   public :: simple_mpcd_step, compute_temperature, mpcd_stream_periodic, bulk_reaction
-  public :: mpcd_stream_nogravity_zwall, rand_sphere
+  public :: mpcd_stream_nogravity_zwall, rand_sphere, rescale_cells, bulk_reaction_endothermic
 
 contains
 
@@ -1042,7 +1042,7 @@ contains
     state(1)%ctr(2) = state(1)%ctr(2) + 1
   end subroutine device_draw_key
 
-  !> simple_mpcd_step (src/mpcd.f90:71-183), SRD rotation
+  !> simple_mpcd_step (src/mpcd.f90:71-183): SRD rotation or, with thermostat, MPC-AT
   subroutine simple_mpcd_step(particles, cells, state, alpha, thermostat, T, hydro)
     class(particle_system_t), intent(inout) :: particles
     class(cell_system_t), intent(inout) :: cells
@@ -1073,7 +1073,11 @@ contains
        if (.not. hydro) hy = 0
     end if
     call particles%time_step%tic()
-    call particles%upload('vs')
+    if (th == 1) then
+       call particles%upload('vsd')  ! the thermostat's draws are keyed by particle id
+    else
+       call particles%upload('vs')
+    end if
     p = particles%view()
     call cells%to_device()
     call device_draw_key(state, seed, step)
@@ -1127,6 +1131,40 @@ contains
          c_null_ptr, seed, step))
     call p%download('s')
   end subroutine bulk_reaction
+
+  !> bulk_reaction_endothermic (src/mpcd.f90:646-700)
+  subroutine bulk_reaction_endothermic(p, c, from, to, rate, tau, state, delta_u)
+    type(particle_system_t), intent(inout) :: p
+    type(cell_system_t), intent(inout) :: c
+    integer, intent(in) :: from, to
+    double precision, intent(in) :: rate, tau, delta_u
+    type(threefry_rng_t), intent(inout) :: state(:)
+    type(rmpcd_particles_t) :: v
+    integer(c_int64_t) :: seed, step
+    call p%upload('vs')
+    v = p%view()
+    call c%to_device()
+    call device_draw_key(state, seed, step)
+    call check(rmpcd_bulk_reaction_endothermic(c%box(), c%view(), v, int(from, c_int32_t), int(to, c_int32_t), &
+         rate, tau, delta_u, c_null_ptr, seed, step))
+    call p%download('vs')
+  end subroutine bulk_reaction_endothermic
+
+  !> rescale_cells (src/mpcd.f90:748-801)
+  subroutine rescale_cells(particles, cells, state, T)
+    class(particle_system_t), intent(inout) :: particles
+    class(cell_system_t), intent(inout) :: cells
+    type(threefry_rng_t), intent(inout) :: state(:)
+    double precision, intent(in) :: T
+    type(rmpcd_particles_t) :: p
+    call particles%time_step%tic()
+    call particles%upload('vs')
+    p = particles%view()
+    call cells%to_device()
+    call check(rmpcd_rescale_cells(cells%box(), cells%view(), p, T))
+    call particles%download('v')
+    call particles%time_step%tac()
+  end subroutine rescale_cells
 
   !> mpcd_stream_nogravity_zwall (src/mpcd.f90:702-745)
   subroutine mpcd_stream_nogravity_zwall(particles, cells, dt)

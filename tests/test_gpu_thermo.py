@@ -1,0 +1,108 @@
+"""The thermostat and energy variants of the collision step on the GPU, through the C
+ABI, against the oracle on the same injected draws: the MPC-AT branch of
+simple_mpcd_step (mpcd.f90:144-160), rescale_cells (mpcd.f90:748-801) and
+bulk_reaction_endothermic (mpcd.f90:646-700).  FP64 form: species bit-exact, velocities
+within 1e-12 relative.  FP32-mixed form: the oracle starts from the same single-precision
+velocities, species bit-exact, velocities within 1e-6 relative."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from rmpcdmd_amd import capi
+
+from _helpers import cells_to_device, same_bits, sorted_state, to_device
+
+pytestmark = pytest.mark.gpu
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def close(a, b, rtol):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return a.shape == b.shape and bool(np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), 1.0)))
+
+
+def state(L, seed, form, frac2=0.0):
+    c, srt, cnt, st = sorted_state(L, 10, seed, frac2=frac2)
+    if form == capi.FP32:
+        srt.vel[:] = srt.vel.astype(np.float32)  # the oracle starts from the stored values
+    return c, srt, cnt, st
+
+
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+@pytest.mark.parametrize("hydro", [1, 0])
+@pytest.mark.parametrize("L,frac2", [((8, 5, 5), 0.0), ((20, 20, 20), 0.3)])
+def test_mpcat_step(gpu, L, frac2, hydro, form):
+    c, srt, cnt, st = state(L, 31, form, frac2)
+    srt.species[::7] = 0  # left out of the mean and of the thermostat
+    g = np.random.default_rng(32).normal(size=srt.vel.shape)
+    T = 0.8
+    want = srt.vel.copy()
+    O.mpcat_step(c, cnt, st, want, srt.species, g, T, hydro=bool(hydro))
+    d = to_device(srt, form=form)
+    cs = cells_to_device(c, cnt, st)
+    g_d = torch.as_tensor(np.ascontiguousarray(g.T)).cuda()
+    capi.call("rmpcd_mpcat_step", cs.boxp(), cs.cellsp(), d.struct(), ptr(g_d), 0, 0, T, hydro)
+    torch.cuda.synchronize()
+    assert close(d.host("vel"), want, 1e-12 if form == capi.FP64 else 1e-6)
+    assert (d.host("species").astype(np.int32) == srt.species).all()
+
+
+def test_mpcat_threefry(gpu):
+    """simple_mpcd_step with thermostat: Threefry normals keyed by particle id; per-cell
+    momentum of the species > 0 particles kept, temperature near T"""
+    c, srt, cnt, st = sorted_state((16, 16, 16), 10, 33)
+    d = to_device(srt)
+    cs = cells_to_device(c, cnt, st)
+    T = 0.6
+    capi.call("rmpcd_simple_mpcd_step", cs.boxp(), cs.cellsp(), d.struct(), None, 99, 3, 0, 0.0, 1, T, 1)
+    torch.cuda.synchronize()
+    v = d.host("vel")
+    for ci in np.nonzero(cnt > 1)[0][:200]:
+        s, n = st[ci], cnt[ci]
+        assert np.allclose(v[s:s + n].sum(axis=0), srt.vel[s:s + n].sum(axis=0), atol=1e-11)
+    assert abs(O.compute_temperature(c, cnt, st, np.ascontiguousarray(v), srt.species) / T - 1) < 0.03
+    # the draws follow the particle, not its slot: the same ids give the same velocities
+    d2 = to_device(srt)
+    capi.call("rmpcd_simple_mpcd_step", cs.boxp(), cs.cellsp(), d2.struct(), None, 99, 3, 0, 0.0, 1, T, 1)
+    torch.cuda.synchronize()
+    assert same_bits(d2.host("vel"), v)
+
+
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+@pytest.mark.parametrize("L", [(8, 5, 5), (24, 20, 16)])
+def test_rescale_cells(gpu, L, form):
+    c, srt, cnt, st = state(L, 34, form, 0.2)
+    srt.species[::9] = 0
+    want = srt.vel.copy()
+    O.rescale_cells(c, cnt, st, want, srt.species, 0.45)
+    d = to_device(srt, form=form)
+    cs = cells_to_device(c, cnt, st)
+    capi.call("rmpcd_rescale_cells", cs.boxp(), cs.cellsp(), d.struct(), 0.45)
+    torch.cuda.synchronize()
+    assert close(d.host("vel"), want, 1e-12 if form == capi.FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+def test_bulk_reaction_endothermic(gpu, form):
+    c, srt, cnt, st = state((16, 16, 16), 35, form, 0.5)
+    draws = np.random.default_rng(36).random(c.N)
+    is_reac = (np.random.default_rng(37).random(c.N) < 0.8).astype(np.uint8)
+    v_want, sp_want = srt.vel.copy(), srt.species.copy()
+    O.bulk_reaction_endothermic(c, cnt, st, is_reac, v_want, sp_want, 2, 1, 0.5, 1.0, 3.0, draws)
+    assert (sp_want != srt.species).sum() > 10
+    d = to_device(srt, form=form)
+    cs = cells_to_device(c, cnt, st)
+    cs.is_reac.copy_(torch.as_tensor(is_reac))
+    dr = torch.as_tensor(draws).cuda()
+    capi.call("rmpcd_bulk_reaction_endothermic", cs.boxp(), cs.cellsp(), d.struct(), 2, 1, 0.5, 1.0, 3.0,
+              ptr(dr), 0, 0)
+    torch.cuda.synchronize()
+    assert same_bits(d.host("species").astype(np.int32), sp_want)
+    assert close(d.host("vel"), v_want, 1e-12 if form == capi.FP64 else 1e-6)

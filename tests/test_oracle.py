@@ -307,6 +307,95 @@ def test_bulk_reaction_closed_form():
     assert (sp2 == srt.species).all()
 
 
+def test_mpcat_thermostat_invariants():
+    # MPC-AT (mpcd.f90:144-160): in a drawing cell the species > 0 velocities become
+    # sqrt(T) times the normals, shifted (hydro) so that the cell's species > 0 momentum is
+    # unchanged; species <= 0 particles and cells with fewer than two counted particles
+    # keep their velocities; without hydro the velocities are the scaled normals
+    c, srt, cnt, st = _cells_state((5, 4, 6), 10, 21)
+    ci = int(np.nonzero(cnt > 3)[0][0])
+    s0 = st[ci]
+    srt.species[s0] = 0
+    cj = int(np.nonzero(cnt == 2)[0][0]) if (cnt == 2).any() else None
+    if cj is not None:
+        srt.species[st[cj]] = 0  # a cell with a single counted particle draws nothing
+    rng = np.random.default_rng(22)
+    g = rng.normal(size=srt.vel.shape)
+    T = 1.7
+    v = srt.vel.copy()
+    O.mpcat_step(c, cnt, st, v, srt.species, g, T, hydro=True)
+    for ci2 in np.nonzero(cnt > 1)[0]:
+        s, n = st[ci2], cnt[ci2]
+        on = srt.species[s:s + n] > 0
+        if on.sum() <= 1:
+            assert (v[s:s + n] == srt.vel[s:s + n]).all()
+            continue
+        assert np.allclose(v[s:s + n][on].sum(axis=0), srt.vel[s:s + n][on].sum(axis=0), atol=1e-11)
+        d = v[s:s + n][on] - np.sqrt(T) * g[s:s + n][on]
+        assert np.allclose(d, d[0], atol=1e-12)  # one shift per cell
+        assert (v[s:s + n][~on] == srt.vel[s:s + n][~on]).all()
+    v2 = srt.vel.copy()
+    O.mpcat_step(c, cnt, st, v2, srt.species, g, T, hydro=False)
+    s, n = st[ci], cnt[ci]
+    on = srt.species[s:s + n] > 0
+    assert (v2[s:s + n][on] == np.sqrt(T) * g[s:s + n][on]).all()
+    # the bulk temperature follows T
+    c3, srt3, cnt3, st3 = _cells_state((8, 8, 8), 10, 23)
+    g3 = np.random.default_rng(24).normal(size=srt3.vel.shape)
+    O.mpcat_step(c3, cnt3, st3, srt3.vel, srt3.species, g3, 0.5, hydro=True)
+    assert abs(O.compute_temperature(c3, cnt3, st3, srt3.vel, srt3.species) / 0.5 - 1) < 0.03
+
+
+def test_rescale_cells_invariants():
+    # rescale_cells (mpcd.f90:748-801): per drawing cell the species > 0 momentum is kept,
+    # the directions about the mean are kept and their kinetic energy becomes
+    # 3 (n_eff - 1) T / 2; so compute_temperature returns T exactly for a cell set with
+    # species > 0 only
+    c, srt, cnt, st = _cells_state((5, 5, 4), 10, 25)
+    T = 0.37
+    v = srt.vel.copy()
+    O.rescale_cells(c, cnt, st, v, srt.species, T)
+    for ci in np.nonzero(cnt > 1)[0]:
+        s, n = st[ci], cnt[ci]
+        u0 = srt.vel[s:s + n].mean(axis=0)
+        assert np.allclose(v[s:s + n].mean(axis=0), u0, atol=1e-12)
+        w0, w1 = srt.vel[s:s + n] - u0, v[s:s + n] - u0
+        assert np.isclose((w1 * w1).sum(), 3 * (n - 1) * T, rtol=1e-12)
+        scale = np.sqrt((w1 * w1).sum() / (w0 * w0).sum())
+        assert np.allclose(w1, scale * w0, atol=1e-12)
+    assert np.isclose(O.compute_temperature(c, cnt, st, v, srt.species), T, rtol=1e-12)
+
+
+def test_bulk_reaction_endothermic_invariants():
+    # bulk_reaction_endothermic (mpcd.f90:646-700): a reacting cell converts its last
+    # 'from' particle and loses exactly delta_u of kinetic energy about its centre of
+    # mass, keeping its momentum; cells below delta_u or without 'from' do not react
+    c, srt, cnt, st = _cells_state((4, 5, 4), 10, 26, frac2=0.5)
+    is_reac = np.ones(c.N, np.uint8)
+    is_reac[0] = 0
+    du = 2.0
+    for draw, expect in ((0.0, True), (1.0, False)):
+        v, sp = srt.vel.copy(), srt.species.copy()
+        O.bulk_reaction_endothermic(c, cnt, st, is_reac, v, sp, 2, 1, 0.3, 0.5, du, np.full(c.N, draw))
+        reacted = 0
+        for ci in np.nonzero(cnt > 1)[0]:
+            s, n = st[ci], cnt[ci]
+            com = srt.vel[s:s + n].mean(axis=0)
+            k0 = 0.5 * ((srt.vel[s:s + n] - com) ** 2).sum()
+            nf = (srt.species[s:s + n] == 2).sum()
+            if expect and is_reac[ci] and nf > 0 and k0 > du:
+                reacted += 1
+                k1 = 0.5 * ((v[s:s + n] - com) ** 2).sum()
+                assert np.isclose(k0 - k1, du, rtol=1e-10, atol=1e-10)
+                assert np.allclose(v[s:s + n].sum(axis=0), srt.vel[s:s + n].sum(axis=0), atol=1e-11)
+                ch = np.nonzero(sp[s:s + n] != srt.species[s:s + n])[0]
+                assert list(ch) == [np.nonzero(srt.species[s:s + n] == 2)[0][-1]]
+                assert sp[s + ch[0]] == 1
+            else:
+                assert (v[s:s + n] == srt.vel[s:s + n]).all() and (sp[s:s + n] == srt.species[s:s + n]).all()
+        assert reacted > 0 or not expect
+
+
 def test_zwall_stream_reflection():
     c = O.cells_init((5, 5, 5), has_walls=True)
     pos = np.array([[1, 1, 0.1], [1, 1, 4.9], [1, 1, 2.0]])
