@@ -182,6 +182,21 @@ int rmpcd_mpcd_stream_periodic(const rmpcd_box_t *box, rmpcd_particles_t *p, dou
 int rmpcd_bulk_reaction(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
                         rmpcd_particles_t *p, int32_t from, int32_t to, double rate,
                         double tau, const double *draws, uint64_t seed, uint64_t step);
+/* wall_mpcd_step (src/mpcd.f90:193-337): the collision of a box with z walls.  walls_given
+ * = 1: wall_temperature[2] and wall_n[2] (host arrays) fill the first and last z layer's
+ * cells up to wall_n particles with virtual ones.  thermostat = 1: MPC-AT with
+ * bulk_temperature (keep_cell_v = 0 drops the cell-mean restoration; it needs the
+ * thermostat, as in the reference).  Injected draws, each NULL for Threefry: axes and
+ * vnorm (the virtual particles' three normals) as 3 planes x ncells, pnorm (the
+ * thermostat's normals) as 3 planes x n. */
+int rmpcd_wall_mpcd_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p,
+                         int32_t walls_given, const double *wall_temperature, const int32_t *wall_n,
+                         int32_t thermostat, double bulk_temperature, int32_t keep_cell_v,
+                         int32_t alpha_given, double alpha, const double *axes, const double *vnorm,
+                         const double *pnorm, uint64_t seed, uint64_t step);
+/* mpcd_stream_xforce_yzwall (src/mpcd.f90:459-511) with yzwall_collision (513-587): the
+ * boundary conditions of y and z are box->bc[1], box->bc[2] */
+int rmpcd_mpcd_stream_xforce_yzwall(const rmpcd_box_t *box, rmpcd_particles_t *p, double dt, double g);
 /* mpcd_stream_nogravity_zwall (src/mpcd.f90:702-745) */
 int rmpcd_mpcd_stream_nogravity_zwall(const rmpcd_box_t *box, rmpcd_particles_t *p,
                                       double dt);

@@ -60,7 +60,7 @@ def lib():
         _lib.orc_lj_energy.argtypes = [C.c_double] * 4
         _lib.orc_srd_axes_from_stream.restype = C.c_int64
         for name in ("orc_md_pos", "orc_md_vel", "orc_mpcd_stream_periodic",
-                     "orc_mpcd_stream_nogravity_zwall"):
+                     "orc_mpcd_stream_nogravity_zwall", "orc_mpcd_stream_xforce_yzwall", "orc_wall_mpcd_step"):
             getattr(_lib, name).restype = None
     return _lib
 
@@ -224,6 +224,23 @@ def bulk_reaction_endothermic(c, cnt, st, is_reac, vel, species, frm, to, rate, 
     """bulk_reaction_endothermic (mpcd.f90:646-700); draws: one per cell"""
     lib().orc_bulk_reaction_endothermic(C.byref(c), P(cnt), P(st), P(is_reac), P(vel), P(species), int(frm),
                                         int(to), D(rate), D(tau), D(delta_u), P(draws))
+
+
+def wall_mpcd_step(c, cnt, st, vel, axes=None, vnorm=None, pnorm=None, wall_temperature=None, wall_n=None,
+                   thermostat=False, bulk_temperature=1.0, keep_cell_v=True, alpha=None):
+    """wall_mpcd_step (mpcd.f90:193-337); axes, vnorm: (ncells, 3); pnorm: (n, 3)"""
+    given = wall_temperature is not None
+    wt = (C.c_double * 2)(*(wall_temperature if given else (0.0, 0.0)))
+    wn = (C.c_int32 * 2)(*(wall_n if given else (0, 0)))
+    lib().orc_wall_mpcd_step(C.byref(c), P(cnt), P(st), P(vel), None, 1 if given else 0, wt, wn,
+                             1 if thermostat else 0, D(bulk_temperature), 1 if keep_cell_v else 0,
+                             0 if alpha is None else 1, D(0.0 if alpha is None else alpha), P(axes), P(vnorm),
+                             P(pnorm))
+
+
+def mpcd_stream_xforce_yzwall(c, pos, vel, flags, dt, g):
+    """mpcd_stream_xforce_yzwall (mpcd.f90:459-511); y/z boundary conditions from c.bc"""
+    lib().orc_mpcd_stream_xforce_yzwall(C.byref(c), I64(pos.shape[0]), P(pos), P(vel), P(flags), D(dt), D(g))
 
 
 def mpcd_stream_nogravity_zwall(c, pos, vel, force, flags, dt):

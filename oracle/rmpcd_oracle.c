@@ -616,6 +616,145 @@ void orc_bulk_reaction_endothermic(const orc_cells_t *c, const int32_t *cell_cou
     }
 }
 
+void orc_wall_mpcd_step(const orc_cells_t *c, const int32_t *cell_count, const int32_t *cell_start,
+                        double *vel, const int32_t *id, int walls_given, const double *wall_temperature,
+                        const int32_t *wall_n, int thermostat, double bulk_temperature, int keep_cell_v,
+                        int alpha_given, double alpha, const double *axes, const double *vnorm,
+                        const double *pnorm)
+{
+    /* mpcd.f90:193-337.  axes and vnorm: 3 per cell; pnorm: 3 per particle slot */
+    (void)id;
+    double sin_a = alpha_given ? sin(alpha) : 1, cos_a = alpha_given ? cos(alpha) : 0;
+    double t_factor = thermostat ? sqrt(bulk_temperature) : 0;
+    ORC_PARALLEL_FOR
+    for (int ci = 0; ci < c->N; ++ci) {
+        if (cell_count[ci] <= 1) continue;
+        int start = cell_start[ci], n = cell_count[ci];
+        int cell[3];
+        orc_compact_h_to_p(ci, c->M, cell);
+        int w = -1;
+        if (walls_given && cell[2] == 0)
+            w = 0;
+        else if (walls_given && cell[2] == c->L[2] - 1)
+            w = 1;
+        int n_virtual = 0;
+        double vv[3] = {0, 0, 0};
+        if (w >= 0 && n < wall_n[w]) {
+            n_virtual = wall_n[w] - n;
+            double f = sqrt(n_virtual * wall_temperature[w]);
+            for (int k = 0; k < 3; ++k) vv[k] = vnorm[3 * ci + k] * f;
+        }
+        double lv[3] = {0, 0, 0};
+        for (int i = start; i < start + n; ++i)
+            for (int k = 0; k < 3; ++k) lv[k] = lv[k] + vel[3 * i + k];
+        for (int k = 0; k < 3; ++k) lv[k] = n_virtual > 0 ? (lv[k] + vv[k]) / wall_n[w] : lv[k] / n;
+        if (thermostat) {
+            double sv[3] = {0, 0, 0};
+            for (int i = start; i < start + n; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    vel[3 * i + k] = pnorm[3 * i + k] * t_factor;
+                    sv[k] = sv[k] + vel[3 * i + k];
+                }
+            if (keep_cell_v) {
+                for (int k = 0; k < 3; ++k) sv[k] = lv[k] - sv[k] / (double)n;
+                for (int i = start; i < start + n; ++i)
+                    for (int k = 0; k < 3; ++k) vel[3 * i + k] = vel[3 * i + k] + sv[k];
+            }
+        } else {
+            const double *v = axes + 3 * ci;
+            double omc = 1 - cos_a;
+            double om[3][3];
+            om[0][0] = cos_a + (v[0] * v[0]) * omc;
+            om[1][0] = (v[0] * v[1]) * omc - v[2] * sin_a;
+            om[2][0] = (v[0] * v[2]) * omc + v[1] * sin_a;
+            om[0][1] = (v[1] * v[0]) * omc + v[2] * sin_a;
+            om[1][1] = cos_a + (v[1] * v[1]) * omc;
+            om[2][1] = (v[1] * v[2]) * omc - v[0] * sin_a;
+            om[0][2] = (v[2] * v[0]) * omc - v[1] * sin_a;
+            om[1][2] = (v[2] * v[1]) * omc + v[0] * sin_a;
+            om[2][2] = cos_a + (v[2] * v[2]) * omc;
+            for (int i = start; i < start + n; ++i) {
+                double d[3], r[3];
+                for (int k = 0; k < 3; ++k) d[k] = vel[3 * i + k] - lv[k];
+                for (int rr = 0; rr < 3; ++rr) r[rr] = (om[rr][0] * d[0] + om[rr][1] * d[1]) + om[rr][2] * d[2];
+                for (int k = 0; k < 3; ++k) vel[3 * i + k] = lv[k] + r[k];
+            }
+        }
+    }
+}
+
+/* yzwall_collision (mpcd.f90:513-587); the alternating wall loop is capped at 64 */
+static void orc_yzwall_collision(const double x0[3], const double v0[3], double x[3], double v[3],
+                                 const double L[3], double t, const int bc[3], double g)
+{
+    double gv[3] = {g, 0, 0};
+    int im[3] = {0, 0, 0};
+    int cd = -1;
+    double tcol = 1.7976931348623157e308;
+    for (int i = 1; i < 3; ++i) {
+        double tt = v0[i] > 0 ? (L[i] - x0[i]) / v0[i] : -x0[i] / v0[i];
+        if (tt > 0 && tt < tcol) {
+            tcol = tt;
+            cd = i;
+        }
+    }
+    if (cd < 0) return;
+    double trem = t - tcol;
+    for (int k = 0; k < 3; ++k) {
+        x[k] = (x0[k] + v0[k] * tcol) + (gv[k] * (tcol * tcol)) / 2;
+        v[k] = v0[k] + gv[k] * tcol;
+    }
+    for (int it = -1; it < 64; ++it) {
+        if (it >= 0) {
+            cd = cd == 1 ? 2 : 1;
+            double tc = v[cd] > 0 ? (L[cd] - x[cd]) / v[cd] : -x[cd] / v[cd];
+            if (tc < 0 || tc > trem) break;
+            for (int k = 0; k < 3; ++k) {
+                x[k] = (x[k] + v[k] * tc) + (gv[k] * (tc * tc)) / 2;
+                v[k] = v[k] + gv[k] * tc;
+            }
+            trem = trem - tc;
+        }
+        if (bc[cd] == ORC_BOUNCE_BACK_BC) {
+            for (int k = 0; k < 3; ++k) v[k] = -v[k];
+        } else if (bc[cd] == ORC_SPECULAR_BC) {
+            v[cd] = -v[cd];
+        } else if (bc[cd] == ORC_PERIODIC_BC) {
+            int jump = (int)floor(x[cd] / L[cd]);
+            x[cd] = x[cd] - jump * L[cd];
+            im[cd] = im[cd] + jump;
+        }
+    }
+    double tc = trem;
+    for (int k = 0; k < 3; ++k) {
+        x[k] = ((x[k] + v[k] * tc) + (gv[k] * (tc * tc)) / 2) + im[k] * L[k];
+        v[k] = v[k] + gv[k] * tc;
+    }
+}
+
+void orc_mpcd_stream_xforce_yzwall(const orc_cells_t *c, int64_t n, double *pos, double *vel,
+                                   int32_t *flags, double dt, double g)
+{
+    /* mpcd.f90:459-511 */
+    const double *L = c->edges;
+    for (int64_t i = 0; i < n; ++i) {
+        double op[3], ov[3], np[3], nv[3], gv[3] = {g, 0, 0};
+        for (int k = 0; k < 3; ++k) {
+            op[k] = pos[3 * i + k];
+            ov[k] = vel[3 * i + k];
+            np[k] = (op[k] + ov[k] * dt) + (gv[k] * (dt * dt)) / 2;
+            nv[k] = ov[k];
+        }
+        int y_out = np[1] < 0 || np[1] > L[1], z_out = np[2] < 0 || np[2] > L[2];
+        if ((c->bc[1] == ORC_PERIODIC_BC && z_out) || (c->bc[1] != ORC_PERIODIC_BC && (y_out || z_out))) {
+            orc_yzwall_collision(op, ov, np, nv, L, dt, c->bc, g);
+            flags[i] |= 1 << ORC_WALL_BIT;
+            for (int k = 0; k < 3; ++k) vel[3 * i + k] = nv[k];
+        }
+        for (int k = 0; k < 3; ++k) pos[3 * i + k] = np[k];
+    }
+}
+
 void orc_mpcd_stream_nogravity_zwall(const orc_cells_t *c, int64_t n, double *pos,
                                      double *vel, const double *force, int32_t *flags,
                                      double dt)

@@ -115,6 +115,13 @@ void orc_bulk_reaction_endothermic(const orc_cells_t *c, const int32_t *cell_cou
                                    const int32_t *cell_start, const uint8_t *is_reac, double *vel,
                                    int32_t *species, int from, int to, double rate, double tau,
                                    double delta_u, const double *draws);     /* [pinned:math] */
+void orc_wall_mpcd_step(const orc_cells_t *c, const int32_t *cell_count, const int32_t *cell_start,
+                        double *vel, const int32_t *id, int walls_given, const double *wall_temperature,
+                        const int32_t *wall_n, int thermostat, double bulk_temperature, int keep_cell_v,
+                        int alpha_given, double alpha, const double *axes, const double *vnorm,
+                        const double *pnorm);                               /* [pinned:math] */
+void orc_mpcd_stream_xforce_yzwall(const orc_cells_t *c, int64_t n, double *pos, double *vel,
+                                   int32_t *flags, double dt, double g);   /* [pinned:math] */
 void orc_mpcd_stream_nogravity_zwall(const orc_cells_t *c, int64_t n, double *pos,
                                      double *vel, const double *force, int32_t *flags,
                                      double dt);                      /* [pinned:math] */

@@ -35,7 +35,7 @@ FORTRAN = os.path.join(HERE, "fortran")
 FBIN = os.path.join(FORTRAN, "bin")
 FLIB = os.path.join(FORTRAN, "librmpcd_fortran.so")
 AMDFLANG = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
-FPROGRAMS = ["fortran_cells", "fortran_fluid", "fortran_dimer"]
+FPROGRAMS = ["fortran_cells", "fortran_fluid", "fortran_dimer", "fortran_walls"]
 
 
 def _run(cmd, what):

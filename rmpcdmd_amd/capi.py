@@ -132,6 +132,8 @@ SIGNATURES = {
     "rmpcd_mpcd_stream_periodic": [_BOXP, _PARTP, _D],
     "rmpcd_bulk_reaction": [_BOXP, _CELLP, _PARTP, _I, _I, _D, _D, _P, _U64, _U64],
     "rmpcd_mpcat_step": [_BOXP, _CELLP, _PARTP, _P, _U64, _U64, _D, _I],
+    "rmpcd_wall_mpcd_step": [_BOXP, _CELLP, _PARTP, _I, _P, _P, _I, _D, _I, _I, _D, _P, _P, _P, _U64, _U64],
+    "rmpcd_mpcd_stream_xforce_yzwall": [_BOXP, _PARTP, _D, _D],
     "rmpcd_rescale_cells": [_BOXP, _CELLP, _PARTP, _D],
     "rmpcd_bulk_reaction_endothermic": [_BOXP, _CELLP, _PARTP, _I, _I, _D, _D, _D, _P, _U64, _U64],
     "rmpcd_mpcd_stream_nogravity_zwall": [_BOXP, _PARTP, _D],

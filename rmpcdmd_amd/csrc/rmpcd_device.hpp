@@ -259,6 +259,7 @@ enum : uint64_t {
     STREAM_INIT = 6,
     STREAM_PLACE = 7,
     STREAM_THERMO = 8,
+    STREAM_WALL = 9,
 };
 
 // three standard normals for item at step: Box-Muller on the four uniforms of the

@@ -1,12 +1,15 @@
-// rmpcd_mpcd.hip -- SRD collision and its MPC-AT thermostat branch, temperature,
-// streaming, bulk reactions (plain and endothermic) and the cell-wise kinetic-energy
-// rescaling (src/mpcd.f90:43-183, 342-407, 440-457, 605-700, 702-745, 748-801).
+// rmpcd_mpcd.hip -- SRD collision and its MPC-AT thermostat branch, the wall collision,
+// temperature, streaming (periodic, z walls, forced flow between y/z walls), bulk
+// reactions (plain and endothermic) and the cell-wise kinetic-energy rescaling
+// (src/mpcd.f90:43-337, 342-407, 440-602, 605-700, 702-745, 748-801).
 //
 // Cell kernels work on the cell-sorted solvent: a workgroup owns a run of consecutive
 // cells, stages their velocities and species from HBM into LDS with coalesced loads,
 // each thread then reduces one cell in the reference's summation order (FP32 form: with
 // Kahan compensation), and the rotated velocities go back to HBM coalesced.  Cells whose
 // particles do not fit the tile fall back to direct loads.
+#include <float.h>
+
 #include "rmpcd_internal.hpp"
 
 namespace rmpcd {
@@ -187,11 +190,16 @@ __global__ void __launch_bounds__(CELL_THREADS)
                 return p.vel[k * p.n + s + i];
             };
             auto W = [&](int k, int64_t i, R v) {
-                R *l = k == 0 ? lx : k == 1 ? ly : lz;
-                if (in_lds)
-                    l[o + i] = v;
-                else
+                // direct LDS stores (a pointer chosen among the arrays would be a generic
+                // one, and its stores are not ordered with the LDS loads of V)
+                if (!in_lds)
                     p.vel[k * p.n + s + i] = v;
+                else if (k == 0)
+                    lx[o + i] = v;
+                else if (k == 1)
+                    ly[o + i] = v;
+                else
+                    lz[o + i] = v;
             };
             auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
             Acc3<R> acc;
@@ -205,27 +213,33 @@ __global__ void __launch_bounds__(CELL_THREADS)
                 }
             if (n_eff > 1) {
                 const R u[3] = {acc.s[0] / (R)n_eff, acc.s[1] / (R)n_eff, acc.s[2] / (R)n_eff};
-                Acc3<R> vir;
-                for (int i = 0; i < n; ++i) {
-                    if (S(i) <= 0) continue;
-                    double g[3];
+                // the new velocities are recomputed in the second pass rather than read
+                // back, so no slot is read after it is written
+                auto draw = [&](int i, double g[3]) {
                     if (normals) {
                         for (int k = 0; k < 3; ++k) g[k] = normals[k * p.n + s + i];
                     } else {
                         normal3_threefry(seed, STREAM_THERMO, step, (uint64_t)p.id[s + i], g);
                     }
-                    for (int k = 0; k < 3; ++k) {
-                        const R v = (R)(g[k] * t_factor);
-                        W(k, i, v);
-                        vir.add(k, v);
-                    }
-                }
+                };
+                R d[3] = {0, 0, 0};
                 if (hydro) {
-                    const R d[3] = {u[0] - vir.s[0] / (R)n_eff, u[1] - vir.s[1] / (R)n_eff,
-                                    u[2] - vir.s[2] / (R)n_eff};
+                    Acc3<R> vir;
                     for (int i = 0; i < n; ++i) {
                         if (S(i) <= 0) continue;
-                        for (int k = 0; k < 3; ++k) W(k, i, V(k, i) + d[k]);
+                        double g[3];
+                        draw(i, g);
+                        for (int k = 0; k < 3; ++k) vir.add(k, (R)(g[k] * t_factor));
+                    }
+                    for (int k = 0; k < 3; ++k) d[k] = u[k] - vir.s[k] / (R)n_eff;
+                }
+                for (int i = 0; i < n; ++i) {
+                    if (S(i) <= 0) continue;
+                    double g[3];
+                    draw(i, g);
+                    for (int k = 0; k < 3; ++k) {
+                        const R v = (R)(g[k] * t_factor);
+                        W(k, i, hydro ? v + d[k] : v);
                     }
                 }
             }
@@ -291,6 +305,155 @@ __global__ void __launch_bounds__(CELL_THREADS)
                         else
                             p.vel[k * p.n + s + i] = v;
                     }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < t.count; q += blockDim.x) {
+        p.vel[t.base + q] = lx[q];
+        p.vel[p.n + t.base + q] = ly[q];
+        p.vel[2 * p.n + t.base + q] = lz[q];
+    }
+}
+
+// wall_mpcd_step (mpcd.f90:193-337): the collision of a box with walls at z = 0 and
+// z = L.  A wall cell (first or last z layer, when the wall parameters are given) with
+// fewer than wall_n particles is completed with wall_n - n virtual particles, whose total
+// velocity is sqrt((wall_n - n) T_wall) times three normals; the cell mean counts every
+// particle (no species filter).  Then SRD about that mean (axis injected or drawn) or,
+// with the thermostat, new velocities sqrt(T) times normals, shifted back to the cell
+// mean when keep_cell_v.  Injected draws: axes and vnorm per cell (3 planes of ncells),
+// pnorm per particle slot (3 planes of n); otherwise Threefry keyed by the cell (axis:
+// SRD stream; virtual normals: WALL stream) or the particle id (THERMO stream).
+struct WallParams {
+    int present;      // the wall parameters were given
+    double temp[2];   // wall temperatures (bottom, top)
+    int nw[2];        // particles per wall cell
+    int thermostat, keep_cell_v;
+    double t_factor;  // sqrt(bulk temperature)
+    double sin_a, cos_a;
+};
+
+template <class F>
+__global__ void __launch_bounds__(CELL_THREADS)
+    k_wall_srd(Box bx, const int32_t *__restrict__ start, const int32_t *__restrict__ count, PView<F> p,
+               WallParams w, const double *__restrict__ axes, const double *__restrict__ vnorm,
+               const double *__restrict__ pnorm, uint64_t seed, uint64_t step)
+{
+    using R = typename F::real;
+    __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
+    __shared__ typename F::small lsp[TILE_CAP];
+    const int ncells = bx.ncells;
+    int c0 = blockIdx.x * CELL_THREADS;
+    int c1 = min(c0 + CELL_THREADS, ncells);
+    CellTile<F> t{lx, ly, lz, lsp, 0, 0};
+    stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
+    int c = c0 + threadIdx.x;
+    if (c < c1) {
+        int n = count[c];
+        if (n > 1) {
+            int64_t s = start[c];
+            bool in_lds = t.count > 0;
+            int64_t o = s - t.base;
+            auto V = [&](int k, int64_t i) -> R {
+                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
+                return p.vel[k * p.n + s + i];
+            };
+            auto W = [&](int k, int64_t i, R v) {
+                // direct LDS stores (a pointer chosen among the arrays would be a generic
+                // one, and its stores are not ordered with the LDS loads of V)
+                if (!in_lds)
+                    p.vel[k * p.n + s + i] = v;
+                else if (k == 0)
+                    lx[o + i] = v;
+                else if (k == 1)
+                    ly[o + i] = v;
+                else
+                    lz[o + i] = v;
+            };
+            int q[3];
+            compact_point(c, bx.M[0], bx.M[1], bx.M[2], q);
+            const int wi = !w.present ? -1 : q[2] == 0 ? 0 : q[2] == bx.L[2] - 1 ? 1 : -1;
+            int n_virtual = 0;
+            double vv[3] = {0, 0, 0};
+            if (wi >= 0 && n < w.nw[wi]) {
+                n_virtual = w.nw[wi] - n;
+                if (vnorm) {
+                    for (int k = 0; k < 3; ++k) vv[k] = vnorm[k * ncells + c];
+                } else {
+                    normal3_threefry(seed, STREAM_WALL, step, (uint64_t)c, vv);
+                }
+                const double f = sqrt(n_virtual * w.temp[wi]);
+                for (int k = 0; k < 3; ++k) vv[k] = vv[k] * f;
+            }
+            Acc3<R> acc;
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < 3; ++k) acc.add(k, V(k, i));
+            R u[3];
+            for (int k = 0; k < 3; ++k)
+                u[k] = n_virtual > 0 ? (R)(((double)acc.s[k] + vv[k]) / w.nw[wi]) : acc.s[k] / (R)n;
+            if (w.thermostat) {
+                // the new velocities are recomputed in the second pass rather than read
+                // back, so no slot is read after it is written
+                auto draw = [&](int i, double g[3]) {
+                    if (pnorm) {
+                        for (int k = 0; k < 3; ++k) g[k] = pnorm[k * p.n + s + i];
+                    } else {
+                        normal3_threefry(seed, STREAM_THERMO, step, (uint64_t)p.id[s + i], g);
+                    }
+                };
+                R d[3] = {0, 0, 0};
+                if (w.keep_cell_v) {
+                    Acc3<R> sum;
+                    for (int i = 0; i < n; ++i) {
+                        double g[3];
+                        draw(i, g);
+                        for (int k = 0; k < 3; ++k) sum.add(k, (R)(g[k] * w.t_factor));
+                    }
+                    for (int k = 0; k < 3; ++k) d[k] = u[k] - sum.s[k] / (R)n;
+                }
+                for (int i = 0; i < n; ++i) {
+                    double g[3];
+                    draw(i, g);
+                    for (int k = 0; k < 3; ++k) {
+                        const R v = (R)(g[k] * w.t_factor);
+                        W(k, i, w.keep_cell_v ? v + d[k] : v);
+                    }
+                }
+            } else {
+                double v0, v1, v2;
+                if (axes) {
+                    v0 = axes[c];
+                    v1 = axes[ncells + c];
+                    v2 = axes[2 * ncells + c];
+                } else {
+                    Sphere sp = rand_sphere_threefry(seed, STREAM_SRD, step, (uint64_t)c);
+                    v0 = sp.n[0];
+                    v1 = sp.n[1];
+                    v2 = sp.n[2];
+                }
+                const double sin_a = w.sin_a, cos_a = w.cos_a, omc = 1 - cos_a;
+                R o00 = (R)(cos_a + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sin_a),
+                  o20 = (R)((v0 * v2) * omc + v1 * sin_a);
+                R o01 = (R)((v1 * v0) * omc + v2 * sin_a), o11 = (R)(cos_a + (v1 * v1) * omc),
+                  o21 = (R)((v1 * v2) * omc - v0 * sin_a);
+                R o02 = (R)((v2 * v0) * omc - v1 * sin_a), o12 = (R)((v2 * v1) * omc + v0 * sin_a),
+                  o22 = (R)(cos_a + (v2 * v2) * omc);
+                for (int i = 0; i < n; ++i) {
+                    R w0 = V(0, i) - u[0], w1 = V(1, i) - u[1], w2 = V(2, i) - u[2];
+                    R r0 = o00 * w0;
+                    r0 = r0 + o01 * w1;
+                    r0 = r0 + o02 * w2;
+                    R r1 = o10 * w0;
+                    r1 = r1 + o11 * w1;
+                    r1 = r1 + o12 * w2;
+                    R r2 = o20 * w0;
+                    r2 = r2 + o21 * w1;
+                    r2 = r2 + o22 * w2;
+                    W(0, i, u[0] + r0);
+                    W(1, i, u[1] + r1);
+                    W(2, i, u[2] + r2);
                 }
             }
         }
@@ -466,6 +629,91 @@ __global__ void k_bulk_endothermic(int ncells, const int32_t *__restrict__ start
     }
 }
 
+// yzwall_collision (mpcd.f90:513-587): the flight of a particle from x0 with velocity v0
+// over time t under the acceleration g along x, between the y and z walls, with the
+// boundary condition of each axis (bounce-back reverses the velocity, specular the
+// normal component, periodic wraps and counts the image, which is added back at the end).
+// The alternating y/z wall loop is capped at 64 crossings (the reference's loop only
+// ends by exhausting the time; a velocity component of exactly zero at a wall would keep
+// it going).
+__device__ inline void yzwall_collision(const double x0[3], const double v0[3], double x[3], double v[3],
+                                        const double L[3], double t, const int bc[3], double g)
+{
+    const double gv[3] = {g, 0, 0};
+    int im[3] = {0, 0, 0};
+    int cd = -1;
+    double tcol = DBL_MAX;
+    for (int i = 1; i < 3; ++i) {
+        const double tt = v0[i] > 0 ? (L[i] - x0[i]) / v0[i] : -x0[i] / v0[i];
+        if (tt > 0 && tt < tcol) {
+            tcol = tt;
+            cd = i;
+        }
+    }
+    if (cd < 0) return;
+    double trem = t - tcol;
+    for (int k = 0; k < 3; ++k) {
+        x[k] = (x0[k] + v0[k] * tcol) + (gv[k] * (tcol * tcol)) / 2;
+        v[k] = v0[k] + gv[k] * tcol;
+    }
+    auto wall = [&](int d) {
+        if (bc[d] == RMPCD_BOUNCE_BACK_BC) {
+            for (int k = 0; k < 3; ++k) v[k] = -v[k];
+        } else if (bc[d] == RMPCD_SPECULAR_BC) {
+            v[d] = -v[d];
+        } else if (bc[d] == RMPCD_PERIODIC_BC) {
+            const int jump = (int)floor(x[d] / L[d]);
+            x[d] = x[d] - jump * L[d];
+            im[d] = im[d] + jump;
+        }
+    };
+    wall(cd);
+    for (int it = 0; it < 64; ++it) {
+        cd = cd == 1 ? 2 : 1;
+        const double tc = v[cd] > 0 ? (L[cd] - x[cd]) / v[cd] : -x[cd] / v[cd];
+        if (tc < 0 || tc > trem) break;
+        for (int k = 0; k < 3; ++k) {
+            x[k] = (x[k] + v[k] * tc) + (gv[k] * (tc * tc)) / 2;
+            v[k] = v[k] + gv[k] * tc;
+        }
+        trem = trem - tc;
+        wall(cd);
+    }
+    const double tc = trem;
+    for (int k = 0; k < 3; ++k) {
+        x[k] = ((x[k] + v[k] * tc) + (gv[k] * (tc * tc)) / 2) + im[k] * L[k];
+        v[k] = v[k] + gv[k] * tc;
+    }
+}
+
+// mpcd_stream_xforce_yzwall (mpcd.f90:459-511): x += v dt + g dt^2/2 along x; a particle
+// that leaves through a wall (z, and y unless y is periodic) flies through
+// yzwall_collision, gets the WALL bit and its new velocity; the others keep theirs (the
+// acceleration reaches their velocity elsewhere in the reference's drivers)
+template <class F>
+__global__ void k_stream_xforce_yzwall(Box b, PView<F> p, double dt, double g)
+{
+    using R = typename F::real;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    double op[3], ov[3], np[3], nv[3];
+    const double gv[3] = {g, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        op[k] = (double)p.pos[k * p.n + i];
+        ov[k] = (double)p.vel[k * p.n + i];
+        np[k] = (op[k] + ov[k] * dt) + (gv[k] * (dt * dt)) / 2;
+        nv[k] = ov[k];
+    }
+    const double *L = b.edges;
+    const bool y_out = np[1] < 0 || np[1] > L[1], z_out = np[2] < 0 || np[2] > L[2];
+    if ((b.bc[1] == RMPCD_PERIODIC_BC && z_out) || (b.bc[1] != RMPCD_PERIODIC_BC && (y_out || z_out))) {
+        yzwall_collision(op, ov, np, nv, L, dt, b.bc, g);
+        p.flags[i] = (typename F::small)((int)p.flags[i] | WALL_MASK);
+        for (int k = 0; k < 3; ++k) p.vel[k * p.n + i] = (R)nv[k];
+    }
+    for (int k = 0; k < 3; ++k) p.pos[k * p.n + i] = (R)np[k];
+}
+
 // mpcd_stream_periodic (mpcd.f90:440-457)
 template <class F>
 __global__ void k_stream_periodic(Box b, PView<F> p, double dt)
@@ -599,6 +847,26 @@ int bulk_endo_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_par
     k_bulk_endothermic<F><<<blocks_for(nc, 256), 256, 0, stream()>>>(nc, cells->cell_start, cells->cell_count,
                                                                       cells->is_reac, view<F>(pp), from, to, rate,
                                                                       tau, delta_u, draws, seed, step);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
+int wall_srd_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *pp, WallParams w,
+                  const double *axes, const double *vnorm, const double *pnorm, uint64_t seed, uint64_t step)
+{
+    int nc = box->ncells;
+    k_wall_srd<F><<<blocks_for(nc, CELL_THREADS), CELL_THREADS, 0, stream()>>>(
+        make_box(box), cells->cell_start, cells->cell_count, view<F>(pp), w, axes, vnorm, pnorm, seed, step);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
+int stream_yzwall_impl(const rmpcd_box_t *box, rmpcd_particles_t *pp, double dt, double g)
+{
+    PView<F> p = view<F>(pp);
+    k_stream_xforce_yzwall<F><<<blocks_for(p.n, 256), 256, 0, stream()>>>(make_box(box), p, dt, g);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
@@ -769,6 +1037,48 @@ int rmpcd_bulk_reaction_endothermic(const rmpcd_box_t *box, const rmpcd_cells_t 
                 "bulk_reaction_endothermic: null cell arrays");
     RMPCD_CHECK(p->form == RMPCD_FP64 || (to >= 0 && to < 256), "bulk_reaction_endothermic: species out of range");
     return RMPCD_DISPATCH_FORM(p, bulk_endo_impl, box, cells, p, from, to, rate, tau, delta_u, draws, seed, step);
+}
+
+int rmpcd_wall_mpcd_step(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particles_t *p,
+                         int32_t walls_given, const double *wall_temperature, const int32_t *wall_n,
+                         int32_t thermostat, double bulk_temperature, int32_t keep_cell_v, int32_t alpha_given,
+                         double alpha, const double *axes, const double *vnorm, const double *pnorm, uint64_t seed,
+                         uint64_t step)
+{
+    int rc = check_box(box, "wall_mpcd_step");
+    if (!rc) rc = check_particles(p, "wall_mpcd_step", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start, "wall_mpcd_step: null cell arrays");
+    WallParams w{};
+    w.present = walls_given ? 1 : 0;
+    if (walls_given) {
+        RMPCD_CHECK(wall_temperature && wall_n, "wall parameters must be all present or all absent in wall_mpcd_step");
+        for (int k = 0; k < 2; ++k) {
+            w.temp[k] = wall_temperature[k];
+            w.nw[k] = wall_n[k];
+        }
+    }
+    w.thermostat = thermostat ? 1 : 0;
+    // mpcd.f90:247-251: keep_cell_v = .false. without the thermostat is refused
+    if (!keep_cell_v && !thermostat)
+        return set_error(RMPCD_ERR_ARG, "requiring .not. keep_cell_v and .not. thermostat in wall_mpcd_step");
+    w.keep_cell_v = keep_cell_v ? 1 : 0;
+    if (thermostat) {
+        RMPCD_CHECK(bulk_temperature >= 0, "wall_mpcd_step: negative temperature");
+        RMPCD_CHECK(pnorm || p->id, "wall_mpcd_step: the thermostat draws by particle id (null id)");
+        w.t_factor = sqrt(bulk_temperature);
+    }
+    w.sin_a = alpha_given ? sin(alpha) : 1;
+    w.cos_a = alpha_given ? cos(alpha) : 0;
+    return RMPCD_DISPATCH_FORM(p, wall_srd_impl, box, cells, p, w, axes, vnorm, pnorm, seed, step);
+}
+
+int rmpcd_mpcd_stream_xforce_yzwall(const rmpcd_box_t *box, rmpcd_particles_t *p, double dt, double g)
+{
+    int rc = check_box(box, "mpcd_stream_xforce_yzwall");
+    if (!rc) rc = check_particles(p, "mpcd_stream_xforce_yzwall", true);
+    if (rc) return rc;
+    return RMPCD_DISPATCH_FORM(p, stream_yzwall_impl, box, p, dt, g);
 }
 
 int rmpcd_threefry_uniform(uint64_t seed, uint64_t stream_id, uint64_t step, int64_t n, double *out)

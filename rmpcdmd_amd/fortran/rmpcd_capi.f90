@@ -155,6 +155,23 @@ module rmpcd_capi
        type(rmpcd_particles_t) :: p
        real(c_double), value :: T
      end function rmpcd_rescale_cells
+     integer(c_int) function rmpcd_wall_mpcd_step(box, cells, p, walls_given, wall_temperature, wall_n, &
+          thermostat, bulk_temperature, keep_cell_v, alpha_given, alpha, axes, vnorm, pnorm, seed, step) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t) :: p
+       integer(c_int32_t), value :: walls_given, thermostat, keep_cell_v, alpha_given
+       type(c_ptr), value :: wall_temperature, wall_n, axes, vnorm, pnorm
+       real(c_double), value :: bulk_temperature, alpha
+       integer(c_int64_t), value :: seed, step
+     end function rmpcd_wall_mpcd_step
+     integer(c_int) function rmpcd_mpcd_stream_xforce_yzwall(box, p, dt, g) bind(c)
+       import :: c_int, c_double, rmpcd_box_t, rmpcd_particles_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt, g
+     end function rmpcd_mpcd_stream_xforce_yzwall
      integer(c_int) function rmpcd_mpcd_stream_nogravity_zwall(box, p, dt) bind(c)
        import :: c_int, c_double, rmpcd_box_t, rmpcd_particles_t
        type(rmpcd_box_t), intent(in) :: box

@@ -1014,6 +1014,7 @@ module mpcd
   private
   public :: simple_mpcd_step, compute_temperature, mpcd_stream_periodic, bulk_reaction
   public :: mpcd_stream_nogravity_zwall, rand_sphere, rescale_cells, bulk_reaction_endothermic
+  public :: wall_mpcd_step, mpcd_stream_xforce_yzwall
 
 contains
 
@@ -1179,6 +1180,92 @@ contains
     call particles%download('pvg')
     call particles%time_stream%tac()
   end subroutine mpcd_stream_nogravity_zwall
+
+  !> wall_mpcd_step (src/mpcd.f90:193-337); wall_v is accepted and, as in the reference,
+  !! not used
+  subroutine wall_mpcd_step(particles, cells, state, wall_temperature, wall_v, wall_n, &
+       thermostat, bulk_temperature, alpha, keep_cell_v)
+    class(particle_system_t), intent(inout) :: particles
+    class(cell_system_t), intent(inout) :: cells
+    type(threefry_rng_t), intent(inout) :: state(:)
+    double precision, optional, intent(in) :: wall_temperature(2)
+    double precision, optional, intent(in) :: wall_v(3,2)
+    integer, optional, intent(in) :: wall_n(2)
+    logical, intent(in), optional :: thermostat
+    double precision, intent(in), optional :: bulk_temperature
+    double precision, intent(in), optional :: alpha
+    logical, intent(in), optional :: keep_cell_v
+    double precision, target :: wt(2)
+    integer(c_int32_t), target :: wn(2)
+    type(rmpcd_particles_t) :: p
+    integer(c_int64_t) :: seed, step
+    integer(c_int32_t) :: wg, th, kc, ag
+    double precision :: bt, a
+    logical :: all_present, all_absent
+    all_present = present(wall_temperature) .and. present(wall_v) .and. present(wall_n)
+    all_absent = .not. present(wall_temperature) .and. .not. present(wall_v) .and. .not. present(wall_n)
+    if ( .not. (all_present .or. all_absent) ) &
+         error stop 'wall parameters must be all present or all absent in wall_mpcd_step'
+    wg = 0
+    wt = 0
+    wn = 0
+    if (all_present) then
+       wg = 1
+       wt = wall_temperature
+       wn = int(wall_n, c_int32_t)
+    end if
+    th = 0
+    bt = 1
+    if (present(thermostat)) then
+       if (thermostat) then
+          th = 1
+          if (.not. present(bulk_temperature)) &
+               error stop 'thermostat requested but no temperature given in wall_mpcd_step'
+          bt = bulk_temperature
+       end if
+    end if
+    kc = 1
+    if (present(keep_cell_v)) then
+       if (.not. present(thermostat)) stop 'requiring keep_cell_v and not setting thermostat in wall_mpcd_step'
+       if ((.not. keep_cell_v) .and. th == 0) &
+            stop 'requiring .not. keep_cell_v and .not. thermostat in wall_mpcd_step'
+       if (.not. keep_cell_v) kc = 0
+    end if
+    ag = 0
+    a = 0
+    if (present(alpha)) then
+       ag = 1
+       a = alpha
+    end if
+    call particles%time_step%tic()
+    if (th == 1) then
+       call particles%upload('vsd')  ! the thermostat's draws are keyed by particle id
+    else
+       call particles%upload('vs')
+    end if
+    p = particles%view()
+    call cells%to_device()
+    call device_draw_key(state, seed, step)
+    call check(rmpcd_wall_mpcd_step(cells%box(), cells%view(), p, wg, c_loc(wt), c_loc(wn), th, bt, kc, ag, a, &
+         c_null_ptr, c_null_ptr, c_null_ptr, seed, step))
+    call particles%download('v')
+    call particles%time_step%tac()
+  end subroutine wall_mpcd_step
+
+  !> mpcd_stream_xforce_yzwall (src/mpcd.f90:459-511)
+  subroutine mpcd_stream_xforce_yzwall(particles, cells, dt, g)
+    type(particle_system_t), intent(inout) :: particles
+    type(cell_system_t), intent(in) :: cells
+    double precision, intent(in) :: dt
+    double precision, intent(in) :: g
+    type(rmpcd_particles_t) :: p
+    call particles%time_stream%tic()
+    call particles%upload('pvsg')
+    p = particles%view()
+    call check(rmpcd_mpcd_stream_xforce_yzwall(cells%box(), p, dt, g))
+    call particles%download('pvg')
+    call particles%time_stream%tac()
+  end subroutine mpcd_stream_xforce_yzwall
 end module mpcd
 
 module md

@@ -396,6 +396,110 @@ def test_bulk_reaction_endothermic_invariants():
         assert reacted > 0 or not expect
 
 
+def _flight(x0, v0, g, dt, L, bc):
+    """the forced flight between y/z walls restated from the physics (independent of the
+    oracle's code): straight lines in y and z, uniform acceleration along x, the earliest
+    wall crossing handled first; periodic axes are crossed freely (unwrapped)"""
+    x, v, t = np.array(x0, float), np.array(v0, float), dt
+    gv = np.array([g, 0.0, 0.0])
+    for _ in range(64):
+        best, dim = None, None
+        for d in (1, 2):
+            if bc[d] == O.PERIODIC_BC or v[d] == 0:
+                continue
+            tc = (L[d] - x[d]) / v[d] if v[d] > 0 else -x[d] / v[d]
+            if 0 < tc <= t and (best is None or tc < best):
+                best, dim = tc, d
+        if best is None:
+            break
+        x = x + v * best + gv * best * best / 2
+        v = v + gv * best
+        t -= best
+        if bc[dim] == O.BOUNCE_BACK_BC:
+            v = -v
+        else:
+            v[dim] = -v[dim]
+    return x + v * t + gv * t * t / 2, v + gv * t
+
+
+@pytest.mark.parametrize("bc", [(1, 2, 2), (1, 3, 3), (1, 1, 2), (1, 3, 2)])
+def test_yzwall_stream_against_the_physics(bc):
+    # mpcd_stream_xforce_yzwall (mpcd.f90:459-511): particles that stay inside move by
+    # v dt + g dt^2/2 and keep their velocity (bit for bit); the others follow the flight
+    # with reflections (bounce-back reverses v, specular the normal component) and get the
+    # WALL bit
+    L = (6.0, 5.0, 4.0)
+    c = O.cells_init((6, 5, 4), has_walls=True)
+    for k in range(3):
+        c.bc[k] = bc[k]
+    rng = np.random.default_rng(40)
+    n = 4000
+    pos = rng.random((n, 3)) * L
+    vel = rng.normal(size=(n, 3))
+    flags = np.zeros(n, np.int32)
+    dt, g = 0.3, 0.7
+    p1, v1 = pos.copy(), vel.copy()
+    O.mpcd_stream_xforce_yzwall(c, p1, v1, flags, dt, g)
+    hit = (flags & (1 << 3)) != 0
+    assert 50 < hit.sum() < n
+    free = ~hit
+    want = (pos[free] + vel[free] * dt) + np.array([g, 0, 0]) * (dt * dt) / 2
+    assert (p1[free] == want).all() and (v1[free] == vel[free]).all()
+    for i in np.nonzero(hit)[0]:
+        x, v = _flight(pos[i], vel[i], g, dt, L, bc)
+        assert np.allclose(p1[i], x, atol=1e-9) and np.allclose(v1[i], v, atol=1e-9), (i, p1[i], x, v1[i], v)
+
+
+def test_wall_mpcd_step_invariants():
+    # wall_mpcd_step (mpcd.f90:193-337): SRD about a cell mean that counts every particle,
+    # plus, in the first and last z layer, wall_n - n virtual particles of total velocity
+    # sqrt((wall_n - n) T_wall) g; the rotation keeps each velocity's distance from that
+    # mean and its axis component.  Thermostat: sqrt(T) times the normals, shifted to the
+    # mean (keep_cell_v) or not.
+    Lc = (6, 5, 10)
+    c = O.cells_init(Lc, has_walls=True)
+    O.set_origin(c, (-0.3, -0.6, -0.5))  # half-filled wall layers
+    rng = np.random.default_rng(41)
+    n = 10 * 6 * 5 * 10
+    ps = O.PState(n)
+    ps.pos[:] = rng.random((n, 3)) * (6, 5, 10)
+    ps.vel[:] = rng.normal(size=(n, 3))
+    ps.id[:] = np.arange(n, dtype=np.int32)
+    srt, _, cnt, st = O.sort(c, ps)
+    axes = rng.normal(size=(c.N, 3))
+    axes /= np.linalg.norm(axes, axis=1)[:, None]
+    vnorm = rng.normal(size=(c.N, 3))
+    pnorm = rng.normal(size=(n, 3))
+    wt, wn = (0.8, 1.3), (14, 12)
+    v = srt.vel.copy()
+    O.wall_mpcd_step(c, cnt, st, v, axes=axes, vnorm=vnorm, wall_temperature=wt, wall_n=wn)
+    layers = set()
+    for ci in np.nonzero(cnt > 1)[0]:
+        s, m = st[ci], cnt[ci]
+        pz = O.compact_h_to_p(int(ci), list(c.M))[2]
+        w = 0 if pz == 0 else 1 if pz == c.L[2] - 1 else -1
+        u = srt.vel[s:s + m].sum(axis=0)
+        if w >= 0 and m < wn[w]:
+            layers.add(w)
+            u = (u + vnorm[ci] * np.sqrt((wn[w] - m) * wt[w])) / wn[w]
+        else:
+            u = u / m
+        d0, d1 = srt.vel[s:s + m] - u, v[s:s + m] - u
+        assert np.allclose(np.linalg.norm(d1, axis=1), np.linalg.norm(d0, axis=1), atol=1e-12)
+        assert np.allclose(d1 @ axes[ci], d0 @ axes[ci], atol=1e-12)
+    assert layers == {0, 1}
+    v2 = srt.vel.copy()
+    O.wall_mpcd_step(c, cnt, st, v2, vnorm=vnorm, pnorm=pnorm, wall_temperature=wt, wall_n=wn, thermostat=True,
+                     bulk_temperature=0.6)
+    v3 = srt.vel.copy()
+    O.wall_mpcd_step(c, cnt, st, v3, pnorm=pnorm, thermostat=True, bulk_temperature=0.6, keep_cell_v=False)
+    for ci in np.nonzero(cnt > 1)[0][:300]:
+        s, m = st[ci], cnt[ci]
+        d = v2[s:s + m] - np.sqrt(0.6) * pnorm[s:s + m]
+        assert np.allclose(d, d[0], atol=1e-12)
+        assert (v3[s:s + m] == np.sqrt(0.6) * pnorm[s:s + m]).all()
+
+
 def test_zwall_stream_reflection():
     c = O.cells_init((5, 5, 5), has_walls=True)
     pos = np.array([[1, 1, 0.1], [1, 1, 4.9], [1, 1, 2.0]])
