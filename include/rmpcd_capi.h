@@ -220,6 +220,52 @@ int rmpcd_rattle_dimer_pos(rmpcd_particles_t *colloids, const double *pos_rattle
                            const double edges[3]);
 int rmpcd_rattle_dimer_vel(rmpcd_particles_t *colloids, const double *mass, int32_t n_mass,
                            double d, double dt, const double edges[3]);
+/* rattle_body_pos (md.f90:321-408) / rattle_body_vel (md.f90:410-480): iterative
+ * RATTLE over a link list, links updated one after the other in list order, at most
+ * 1000 sweeps.  links: host, 2*n_links zero-based particle indices (pairs); distances:
+ * host, n_links (the library keeps a level schedule of the list on the device and
+ * rebuilds it when the contents change); pos_rattle: device, 3 planes of p->n, unfolded
+ * around particle 0 in place as the reference does; mass: host, by species-1.
+ * *iterations (may be NULL, then the call does not wait for the device) receives the
+ * sweep count, 1001 when the precision was not reached. */
+int rmpcd_rattle_body_pos(rmpcd_particles_t *p, double *pos_rattle, const int32_t *links,
+                          const double *distances, int32_t n_links, const double *mass,
+                          int32_t n_mass, double dt, const double edges[3], double precision,
+                          int32_t *iterations);
+int rmpcd_rattle_body_vel(rmpcd_particles_t *p, const int32_t *links, const double *distances,
+                          int32_t n_links, const double *mass, int32_t n_mass, double dt,
+                          const double edges[3], double precision, int32_t *iterations);
+/* elastic_network (md.f90:516-544): harmonic links added to p->force in link order;
+ * links/distances: host, as for the rattles; the energy comes back in *energy (host) */
+int rmpcd_elastic_network(rmpcd_particles_t *p, const int32_t *links, const double *distances,
+                          int32_t n_links, double k, const double edges[3], double *energy);
+
+/* rigid_body_t (md.f90:39-57) in device memory; quaternions scalar first.  pos_body:
+ * device, 3*(i_stop-i_start+1), body-frame position of particle i at 3*(i-i_start).
+ * i_start/i_stop are zero-based and inclusive. */
+typedef struct rmpcd_rigid_body {
+    int32_t i_start, i_stop;
+    double mass;
+    double pos[3], vel[3], force[3];
+    double q[4];
+    double L[3], torque[3];
+    double I_body[3], omega_body[3];
+    double *pos_body;
+} rmpcd_rigid_body_t;
+/* rigid_body_init (md.f90:547-588): body (device) and pos_body (device) are written */
+int rmpcd_rigid_body_init(rmpcd_rigid_body_t *body, const rmpcd_particles_t *p, const double *mass,
+                          int32_t n_mass, const double edges[3], int32_t i_start, int32_t i_stop,
+                          double *pos_body);
+/* rigid_body_compute_force_torque (md.f90:591-607) */
+int rmpcd_rigid_body_compute_force_torque(rmpcd_rigid_body_t *body, const rmpcd_particles_t *p);
+/* rigid_body_vv1 (md.f90:613-673): the half-step quaternion is iterated until it moves
+ * less than threshold (at most 1000 rounds, the reference has no cap); *iterations (may
+ * be NULL, then the call does not wait for the device) receives the round count, 1001
+ * when the threshold was not reached */
+int rmpcd_rigid_body_vv1(rmpcd_rigid_body_t *body, rmpcd_particles_t *p, double dt,
+                         double threshold, const double edges[3], int32_t *iterations);
+/* rigid_body_vv2 (md.f90:679-700) */
+int rmpcd_rigid_body_vv2(rmpcd_rigid_body_t *body, rmpcd_particles_t *p, double dt);
 /* colloid Verlet written inline in the drivers (programs/single_dimer_pbc.f90:312-315
  * and :356-359); mass_by_particle: device, one per colloid */
 int rmpcd_colloid_md_pos(rmpcd_particles_t *colloids, const double *mass_by_particle,

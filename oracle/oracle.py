@@ -59,6 +59,7 @@ def lib():
         _lib.orc_lj_energy.restype = C.c_double
         _lib.orc_lj_energy.argtypes = [C.c_double] * 4
         _lib.orc_srd_axes_from_stream.restype = C.c_int64
+        _lib.orc_elastic_network.restype = C.c_double
         for name in ("orc_md_pos", "orc_md_vel", "orc_mpcd_stream_periodic",
                      "orc_mpcd_stream_nogravity_zwall", "orc_mpcd_stream_xforce_yzwall", "orc_wall_mpcd_step"):
             getattr(_lib, name).restype = None
@@ -285,6 +286,63 @@ def rattle_dimer_pos(pos, vel, pos_rattle, species, mass, d, dt, edges):
 def rattle_dimer_vel(pos, vel, species, mass, d, dt, edges):
     ea = (C.c_double * 3)(*[float(v) for v in edges])
     lib().orc_rattle_dimer_vel(P(pos), P(vel), P(species), P(mass), D(d), D(dt), ea)
+
+
+class Body(C.Structure):
+    """rigid_body_t (md.f90:39-57) with zero-based inclusive i_start/i_stop; pos_body
+    points into the numpy array kept in .body_rows"""
+    _fields_ = [("i_start", C.c_int32), ("i_stop", C.c_int32), ("mass", C.c_double),
+                ("pos", C.c_double * 3), ("vel", C.c_double * 3), ("force", C.c_double * 3),
+                ("q", C.c_double * 4), ("L", C.c_double * 3), ("torque", C.c_double * 3),
+                ("I_body", C.c_double * 3), ("omega_body", C.c_double * 3),
+                ("pos_body", C.POINTER(C.c_double))]
+
+
+def _e3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def rattle_body_pos(pos, vel, pos_rattle, species, mass, links, distances, dt, edges, precision):
+    """md.f90:321-408 on (n, 3) arrays in place; links (n_links, 2) zero-based int32;
+    returns the sweep counter at exit"""
+    return lib().orc_rattle_body_pos(I64(pos.shape[0]), P(pos), P(vel), P(pos_rattle), P(species), P(mass),
+                                     P(links), P(distances), int(distances.shape[0]), D(dt), _e3(edges),
+                                     D(precision))
+
+
+def rattle_body_vel(pos, vel, species, mass, links, distances, dt, edges, precision):
+    """md.f90:410-480 on (n, 3) arrays in place; returns the sweep counter at exit"""
+    return lib().orc_rattle_body_vel(I64(pos.shape[0]), P(pos), P(vel), P(species), P(mass), P(links),
+                                     P(distances), int(distances.shape[0]), D(dt), _e3(edges), D(precision))
+
+
+def elastic_network(pos, force, links, distances, k, edges):
+    """md.f90:516-544: adds to force in place, returns the energy"""
+    return lib().orc_elastic_network(P(pos), P(force), P(links), P(distances), int(distances.shape[0]), D(k),
+                                     _e3(edges))
+
+
+def rigid_body_init(pos, vel, image, species, mass, edges, i_start, i_stop):
+    """md.f90:547-588; returns a Body"""
+    b = Body()
+    b.body_rows = np.zeros((i_stop - i_start + 1, 3))
+    b.pos_body = b.body_rows.ctypes.data_as(C.POINTER(C.c_double))
+    lib().orc_rigid_body_init(C.byref(b), P(pos), P(vel), P(image), P(species), P(mass), _e3(edges),
+                              int(i_start), int(i_stop))
+    return b
+
+
+def rigid_body_compute_force_torque(b, force):
+    lib().orc_rigid_body_compute_force_torque(C.byref(b), P(force))
+
+
+def rigid_body_vv1(b, pos, dt, threshold, edges):
+    """md.f90:613-673; returns the iteration counter at exit"""
+    return lib().orc_rigid_body_vv1(C.byref(b), P(pos), D(dt), D(threshold), _e3(edges))
+
+
+def rigid_body_vv2(b, vel, dt):
+    lib().orc_rigid_body_vv2(C.byref(b), P(vel), D(dt))
 
 
 def colloid_md_pos(pos, vel, force, mass_by_particle, dt):

@@ -972,6 +972,306 @@ void orc_colloid_md_vel(int64_t n, double *vel, const double *force,
                              (dt * (force[3 * k + c] + force_old[3 * k + c])) / (2 * mass_by_particle[k]);
 }
 
+/* ---- linked bodies (md.f90:321-480, 516-544) ---------------------------------
+ * Positions are (n, 3) rows; links are zero-based pairs; mass[s - 1] is the mass of
+ * species s.  Each function returns the reference's loop counter at exit (the sweep
+ * that met the precision, rattle_max + 1 when none did). */
+static void body_unfold(int64_t n, double *x, const double edges[3], int32_t *im)
+{
+    /* md.f90:343-347: every particle brought next to particle 1 (index 0 here) */
+    for (int64_t i = 1; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            int j = (int)floor(((x[3 * i + k] - x[k]) / edges[k]) + 0.5);
+            if (im) im[3 * i + k] = j;
+            x[3 * i + k] = x[3 * i + k] - j * edges[k];
+        }
+}
+
+static void body_refold(int64_t n, double *x, const double edges[3], const int32_t *im)
+{
+    /* md.f90:400-402 */
+    for (int64_t i = 1; i < n; ++i)
+        for (int k = 0; k < 3; ++k) x[3 * i + k] = x[3 * i + k] + im[3 * i + k] * edges[k];
+}
+
+int orc_rattle_body_pos(int64_t n, double *pos, double *vel, double *pos_rattle,
+                        const int32_t *species, const double *mass, const int32_t *links,
+                        const double *distances, int n_links, double dt, const double edges[3],
+                        double precision)
+{
+    /* md.f90:321-408 */
+    int32_t *im = (int32_t *)calloc((size_t)(3 * n), sizeof(int32_t));
+    body_unfold(n, pos, edges, im);
+    body_unfold(n, pos_rattle, edges, NULL);
+    int sweep;
+    for (sweep = 1; sweep <= 1000; ++sweep) {
+        for (int l = 0; l < n_links; ++l) {
+            const int a = links[2 * l], b = links[2 * l + 1];
+            const double d = distances[l];
+            const double ia = 1 / mass[species[a] - 1], ib = 1 / mass[species[b] - 1];
+            double r[3], s[3];
+            for (int k = 0; k < 3; ++k) {
+                r[k] = pos_rattle[3 * a + k] - pos_rattle[3 * b + k];
+                s[k] = pos[3 * a + k] - pos[3 * b + k];
+            }
+            const double rsq = dot3(r, r), ssq = dot3(s, s), rs = dot3(r, s);
+            double g = rs - sqrt(rs * rs - rsq * (ssq - d * d));
+            g = g / ((dt * (ia + ib)) * rsq);
+            for (int k = 0; k < 3; ++k) {
+                pos[3 * a + k] -= ((g * dt) * r[k]) * ia;
+                pos[3 * b + k] += ((g * dt) * r[k]) * ib;
+                vel[3 * a + k] -= (g * r[k]) * ia;
+                vel[3 * b + k] += (g * r[k]) * ib;
+            }
+        }
+        double worst = 0;
+        for (int l = 0; l < n_links; ++l) {
+            const int a = links[2 * l], b = links[2 * l + 1];
+            double r[3];
+            for (int k = 0; k < 3; ++k) r[k] = pos[3 * a + k] - pos[3 * b + k];
+            const double e = fabs(sqrt(dot3(r, r)) - distances[l]);
+            if (e > worst) worst = e;
+        }
+        if (worst < precision) break;
+    }
+    body_refold(n, pos, edges, im);
+    free(im);
+    return sweep;
+}
+
+static double link_k(const double *pos, const double *vel, int a, int b, double d, double inv_mass)
+{
+    double s[3], dv[3];
+    for (int k = 0; k < 3; ++k) {
+        s[k] = pos[3 * a + k] - pos[3 * b + k];
+        dv[k] = vel[3 * a + k] - vel[3 * b + k];
+    }
+    return dot3(dv, s) / ((d * d) * inv_mass);
+}
+
+int orc_rattle_body_vel(int64_t n, double *pos, double *vel, const int32_t *species,
+                        const double *mass, const int32_t *links, const double *distances,
+                        int n_links, double dt, const double edges[3], double precision)
+{
+    /* md.f90:410-480 */
+    (void)dt;
+    int32_t *im = (int32_t *)calloc((size_t)(3 * n), sizeof(int32_t));
+    body_unfold(n, pos, edges, im);
+    int sweep;
+    for (sweep = 1; sweep <= 1000; ++sweep) {
+        for (int l = 0; l < n_links; ++l) {
+            const int a = links[2 * l], b = links[2 * l + 1];
+            const double ia = 1 / mass[species[a] - 1], ib = 1 / mass[species[b] - 1];
+            const double kk = link_k(pos, vel, a, b, distances[l], ia + ib);
+            for (int k = 0; k < 3; ++k) {
+                const double s = pos[3 * a + k] - pos[3 * b + k];
+                vel[3 * a + k] -= (kk * s) * ia;
+                vel[3 * b + k] += (kk * s) * ib;
+            }
+        }
+        double worst = 0;
+        for (int l = 0; l < n_links; ++l) {
+            const int a = links[2 * l], b = links[2 * l + 1];
+            const double ia = 1 / mass[species[a] - 1], ib = 1 / mass[species[b] - 1];
+            const double e = fabs(link_k(pos, vel, a, b, distances[l], ia + ib));
+            if (e > worst) worst = e;
+        }
+        if (worst < precision) break;
+    }
+    body_refold(n, pos, edges, im);
+    free(im);
+    return sweep;
+}
+
+double orc_elastic_network(const double *pos, double *force, const int32_t *links,
+                           const double *distances, int n_links, double k, const double edges[3])
+{
+    /* md.f90:516-544 */
+    double e = 0;
+    for (int l = 0; l < n_links; ++l) {
+        const int a = links[2 * l], b = links[2 * l + 1];
+        double x[3];
+        orc_rel_pos(pos + 3 * a, pos + 3 * b, edges, x);
+        const double r = sqrt(dot3(x, x)), r0 = distances[l];
+        e = e + (k * ((r - r0) * (r - r0))) / 2;
+        for (int c = 0; c < 3; ++c) {
+            const double f = ((-k) * (r - r0)) * x[c] / r;
+            force[3 * a + c] += f;
+            force[3 * b + c] -= f;
+        }
+    }
+    return e;
+}
+
+/* ---- rigid body (md.f90:547-700) ----------------------------------------------
+ * Quaternions (s, x, y, z): the reference takes qmul/qconj/qrot/qnormalize from an
+ * external quaternion module; these are the textbook Hamilton product and the
+ * rotation v -> q (0, v) q*. */
+static void oq_mul(const double a[4], const double b[4], double r[4])
+{
+    double t[4];
+    t[0] = a[0] * b[0] - ((a[1] * b[1] + a[2] * b[2]) + a[3] * b[3]);
+    t[1] = (a[0] * b[1] + b[0] * a[1]) + (a[2] * b[3] - a[3] * b[2]);
+    t[2] = (a[0] * b[2] + b[0] * a[2]) + (a[3] * b[1] - a[1] * b[3]);
+    t[3] = (a[0] * b[3] + b[0] * a[3]) + (a[1] * b[2] - a[2] * b[1]);
+    memcpy(r, t, sizeof t);
+}
+static void oq_conj(const double q[4], double r[4])
+{
+    r[0] = q[0];
+    for (int k = 1; k < 4; ++k) r[k] = -q[k];
+}
+static void oq_rot(const double q[4], const double v[3], double r[3])
+{
+    double p[4] = {0, v[0], v[1], v[2]}, c[4], t[4];
+    oq_conj(q, c);
+    oq_mul(q, p, t);
+    oq_mul(t, c, t);
+    for (int k = 0; k < 3; ++k) r[k] = t[k + 1];
+}
+static double oq_norm(const double q[4]) { return sqrt(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]); }
+static void oq_normalize(double q[4])
+{
+    double n = oq_norm(q);
+    for (int k = 0; k < 4; ++k) q[k] = q[k] / n;
+}
+static void ocross(const double a[3], const double b[3], double r[3])
+{
+    double t[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    memcpy(r, t, sizeof t);
+}
+
+void orc_rigid_body_init(orc_body_t *b, const double *pos, const double *vel, const int32_t *image,
+                         const int32_t *species, const double *mass, const double edges[3],
+                         int i_start, int i_stop)
+{
+    /* md.f90:547-588 */
+    b->i_start = i_start;
+    b->i_stop = i_stop;
+    b->mass = 0;
+    for (int k = 0; k < 3; ++k) b->pos[k] = b->vel[k] = 0;
+    for (int i = i_start; i <= i_stop; ++i) {
+        const double m = mass[species[i] - 1];
+        b->mass = b->mass + m;
+        for (int k = 0; k < 3; ++k) {
+            b->pos[k] = b->pos[k] + m * (pos[3 * i + k] + image[3 * i + k] * edges[k]);
+            b->vel[k] = b->vel[k] + m * vel[3 * i + k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        b->pos[k] = b->pos[k] / b->mass;
+        b->vel[k] = b->vel[k] / b->mass;
+        b->I_body[k] = 0;
+    }
+    for (int i = i_start; i <= i_stop; ++i) {
+        const double m = mass[species[i] - 1];
+        double *x = b->pos_body + 3 * (i - i_start);
+        for (int k = 0; k < 3; ++k) x[k] = (pos[3 * i + k] + image[3 * i + k] * edges[k]) - b->pos[k];
+        const double x2 = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
+        for (int k = 0; k < 3; ++k) b->I_body[k] = b->I_body[k] + m * (x2 - x[k] * x[k]);
+    }
+    for (int k = 0; k < 3; ++k) b->force[k] = b->torque[k] = b->L[k] = b->omega_body[k] = 0;
+    b->q[0] = 1;
+    b->q[1] = b->q[2] = b->q[3] = 0;
+}
+
+void orc_rigid_body_compute_force_torque(orc_body_t *b, const double *force)
+{
+    /* md.f90:591-607 */
+    for (int k = 0; k < 3; ++k) b->force[k] = b->torque[k] = 0;
+    for (int i = b->i_start; i <= b->i_stop; ++i) {
+        const double *f = force + 3 * i;
+        double x[3], t[3];
+        oq_rot(b->q, b->pos_body + 3 * (i - b->i_start), x);
+        ocross(x, f, t);
+        for (int k = 0; k < 3; ++k) {
+            b->force[k] = b->force[k] + f[k];
+            b->torque[k] = b->torque[k] + t[k];
+        }
+    }
+}
+
+int orc_rigid_body_vv1(orc_body_t *b, double *pos, double dt, double threshold, const double edges[3])
+{
+    /* md.f90:613-673 (the iteration capped at 1000 rounds) */
+    double I_inv[3], L[3], Lb[3], tb[3], ob[3], Lbd[3], c[3], q[4], qc[4], qd[4], w[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 3; ++k) I_inv[k] = 1 / b->I_body[k];
+    memcpy(L, b->L, sizeof L);
+    memcpy(q, b->q, sizeof q);
+    oq_conj(q, qc);
+    oq_rot(qc, L, Lb);
+    oq_rot(qc, b->torque, tb);
+    for (int k = 0; k < 3; ++k) ob[k] = Lb[k] * I_inv[k];
+    ocross(ob, Lb, c);
+    for (int k = 0; k < 3; ++k) Lbd[k] = tb[k] - c[k];
+    for (int k = 0; k < 3; ++k) {
+        Lb[k] = Lb[k] + (dt * Lbd[k]) / 2;
+        ob[k] = Lb[k] * I_inv[k];
+        w[k + 1] = ob[k];
+    }
+    oq_mul(q, w, qd);
+    for (int k = 0; k < 4; ++k) {
+        qd[k] = qd[k] / 2;
+        q[k] = q[k] + (qd[k] * dt) / 2;
+    }
+    oq_normalize(q);
+    for (int k = 0; k < 3; ++k) L[k] = b->L[k] + (b->torque[k] * dt) / 2;
+    double q_old[4];
+    memcpy(q_old, q, sizeof q);
+    int round;
+    for (round = 1; round <= 1000; ++round) {
+        oq_conj(q, qc);
+        oq_rot(qc, L, Lb);
+        for (int k = 0; k < 3; ++k) w[k + 1] = I_inv[k] * Lb[k];
+        oq_mul(q, w, qd);
+        for (int k = 0; k < 4; ++k) {
+            qd[k] = qd[k] / 2;
+            q[k] = b->q[k] + (qd[k] * dt) / 2;
+        }
+        oq_normalize(q);
+        double dq[4];
+        for (int k = 0; k < 4; ++k) dq[k] = q[k] - q_old[k];
+        if (oq_norm(dq) < threshold) break;
+        memcpy(q_old, q, sizeof q);
+    }
+    for (int k = 0; k < 4; ++k) b->q[k] = b->q[k] + dt * qd[k];
+    oq_normalize(b->q);
+    for (int k = 0; k < 3; ++k)
+        b->pos[k] = (b->pos[k] + b->vel[k] * dt) + (b->force[k] * (dt * dt)) / (2 * b->mass);
+    for (int i = b->i_start; i <= b->i_stop; ++i) {
+        double x[3];
+        oq_rot(b->q, b->pos_body + 3 * (i - b->i_start), x);
+        for (int k = 0; k < 3; ++k) {
+            const double nx = b->pos[k] + x[k];
+            const double delta = (nx - pos[3 * i + k]) + edges[k] / 2;
+            pos[3 * i + k] = nx - floor(delta / edges[k]) * edges[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        b->L[k] = b->L[k] + (b->torque[k] * dt) / 2;
+        b->vel[k] = b->vel[k] + (b->force[k] * dt) / (2 * b->mass);
+    }
+    return round;
+}
+
+void orc_rigid_body_vv2(orc_body_t *b, double *vel, double dt)
+{
+    /* md.f90:679-700 */
+    double qc[4], Lb[3], om[3];
+    for (int k = 0; k < 3; ++k) b->L[k] = b->L[k] + (b->torque[k] * dt) / 2;
+    oq_conj(b->q, qc);
+    oq_rot(qc, b->L, Lb);
+    for (int k = 0; k < 3; ++k) b->omega_body[k] = Lb[k] / b->I_body[k];
+    oq_rot(b->q, b->omega_body, om);
+    for (int k = 0; k < 3; ++k) b->vel[k] = b->vel[k] + (b->force[k] * dt) / (2 * b->mass);
+    for (int i = b->i_start; i <= b->i_stop; ++i) {
+        double x[3], c[3];
+        oq_rot(b->q, b->pos_body + 3 * (i - b->i_start), x);
+        ocross(om, x, c);
+        for (int k = 0; k < 3; ++k) vel[3 * i + k] = b->vel[k] + c[k];
+    }
+}
+
 /* ======================= interaction.f90 ======================================= */
 /* amdflang evaluates x**6 as ((((x*x)*x)*x)*x)*x: checked bit for bit against the
  * reference's interaction module built from source (tests/golden/interaction_ref.npz) */

@@ -158,6 +158,36 @@ void orc_colloid_md_vel(int64_t n, double *vel, const double *force,
 
 /* ---- common.f90 / interaction.f90 ------------------------------------------ */
 void orc_rel_pos(const double x[3], const double y[3], const double L[3], double r[3]);
+/* ---- linked bodies (md.f90:321-480, 516-544), positions (n, 3), links zero-based pairs;
+ * the rattles return the sweep counter at exit (1001 when the precision was not met) ---- */
+int orc_rattle_body_pos(int64_t n, double *pos, double *vel, double *pos_rattle,
+                        const int32_t *species, const double *mass, const int32_t *links,
+                        const double *distances, int n_links, double dt, const double edges[3],
+                        double precision);                                 /* [pinned:math] */
+int orc_rattle_body_vel(int64_t n, double *pos, double *vel, const int32_t *species,
+                        const double *mass, const int32_t *links, const double *distances,
+                        int n_links, double dt, const double edges[3], double precision); /* [pinned:math] */
+double orc_elastic_network(const double *pos, double *force, const int32_t *links,
+                           const double *distances, int n_links, double k,
+                           const double edges[3]);                         /* [pinned:math] */
+/* rigid_body_t (md.f90:39-57), the layout of the library's rmpcd_rigid_body_t;
+ * i_start/i_stop zero-based, inclusive; pos_body (n_body, 3) host rows */
+typedef struct orc_body {
+    int32_t i_start, i_stop;
+    double mass;
+    double pos[3], vel[3], force[3];
+    double q[4];
+    double L[3], torque[3];
+    double I_body[3], omega_body[3];
+    double *pos_body;
+} orc_body_t;
+void orc_rigid_body_init(orc_body_t *b, const double *pos, const double *vel, const int32_t *image,
+                         const int32_t *species, const double *mass, const double edges[3],
+                         int i_start, int i_stop);                         /* [pinned:math] */
+void orc_rigid_body_compute_force_torque(orc_body_t *b, const double *force); /* [pinned:math] */
+int orc_rigid_body_vv1(orc_body_t *b, double *pos, double dt, double threshold,
+                       const double edges[3]);                             /* [pinned:math] */
+void orc_rigid_body_vv2(orc_body_t *b, double *vel, double dt);            /* [pinned:math] */
 typedef struct {
     int n1, n2;                      /* (n1 solvent species, n2 colloid species) */
     double eps[16], sigma[16], cut[16], cut_sq[16], cut_energy[16];  /* [s1-1 + n1*(s2-1)] */

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librmpcd_amd.so")
 SOURCES = ["rmpcd_runtime.hip", "rmpcd_sort.hip", "rmpcd_mpcd.hip", "rmpcd_md.hip",
-           "rmpcd_nlist.hip", "rmpcd_sim.hip", "rmpcd_fused.hip", "rmpcd_programs.hip"]
+           "rmpcd_nlist.hip", "rmpcd_sim.hip", "rmpcd_fused.hip", "rmpcd_programs.hip",
+           "rmpcd_bodies.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -35,7 +36,7 @@ FORTRAN = os.path.join(HERE, "fortran")
 FBIN = os.path.join(FORTRAN, "bin")
 FLIB = os.path.join(FORTRAN, "librmpcd_fortran.so")
 AMDFLANG = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
-FPROGRAMS = ["fortran_cells", "fortran_fluid", "fortran_dimer", "fortran_walls"]
+FPROGRAMS = ["fortran_cells", "fortran_fluid", "fortran_dimer", "fortran_walls", "fortran_bodies"]
 
 
 def _run(cmd, what):

@@ -88,6 +88,15 @@ _LJP = C.POINTER(LJ)
 PROG_FLUID, PROG_SPHERE, PROG_COLLOIDS = 0, 1, 2
 
 
+class RigidBody(C.Structure):
+    """rmpcd_rigid_body_t: rigid_body_t (md.f90:39-57), zero-based inclusive range;
+    lives in device memory, pos_body is a device pointer"""
+    _fields_ = [("i_start", C.c_int32), ("i_stop", C.c_int32), ("mass", C.c_double),
+                ("pos", C.c_double * 3), ("vel", C.c_double * 3), ("force", C.c_double * 3),
+                ("q", C.c_double * 4), ("L", C.c_double * 3), ("torque", C.c_double * 3),
+                ("I_body", C.c_double * 3), ("omega_body", C.c_double * 3), ("pos_body", C.c_void_p)]
+
+
 class ProgramConfig(C.Structure):
     _fields_ = [("kind", C.c_int32), ("L", C.c_int32 * 3), ("rho", C.c_int32), ("n_md", C.c_int32),
                 ("form", C.c_int32), ("n_colloids", C.c_int32), ("nmax_list", C.c_int32), ("pad_", C.c_int32),
@@ -145,6 +154,13 @@ SIGNATURES = {
     "rmpcd_cell_md_pos_wall": [_BOXP, _CELLP, _PARTP, _D, _I, _I],
     "rmpcd_rattle_dimer_pos": [_PARTP, _P, _D3, _I, _D, _D, _D3],
     "rmpcd_rattle_dimer_vel": [_PARTP, _D3, _I, _D, _D, _D3],
+    "rmpcd_rattle_body_pos": [_PARTP, _P, _P, _P, _I, _D3, _I, _D, _D3, _D, C.POINTER(_I)],
+    "rmpcd_rattle_body_vel": [_PARTP, _P, _P, _I, _D3, _I, _D, _D3, _D, C.POINTER(_I)],
+    "rmpcd_elastic_network": [_PARTP, _P, _P, _I, _D, _D3, _D3],
+    "rmpcd_rigid_body_init": [_P, _PARTP, _D3, _I, _D3, _I, _I, _P],
+    "rmpcd_rigid_body_compute_force_torque": [_P, _PARTP],
+    "rmpcd_rigid_body_vv1": [_P, _PARTP, _D, _D, _D3, C.POINTER(_I)],
+    "rmpcd_rigid_body_vv2": [_P, _PARTP, _D],
     "rmpcd_colloid_md_pos": [_PARTP, _P, _D],
     "rmpcd_colloid_md_vel": [_PARTP, _P, _D],
     "rmpcd_lj_params_init": [_LJP, _I, _I, _D3, _D3, _D3],

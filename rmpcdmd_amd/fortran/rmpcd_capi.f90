@@ -27,6 +27,14 @@ module rmpcd_capi
      type(c_ptr) :: is_md = c_null_ptr, is_reac = c_null_ptr
   end type rmpcd_cells_t
 
+  !> rmpcd_rigid_body_t, in device memory
+  type, bind(c) :: rmpcd_rigid_body_t
+     integer(c_int32_t) :: i_start, i_stop
+     real(c_double) :: mass
+     real(c_double) :: pos(3), vel(3), force(3), q(4), L(3), torque(3), I_body(3), omega_body(3)
+     type(c_ptr) :: pos_body
+  end type rmpcd_rigid_body_t
+
   type, bind(c) :: rmpcd_lj_t
      integer(c_int32_t) :: n1, n2
      real(c_double) :: eps(16), sigma(16), cut(16), cut_sq(16), cut_energy(16)
@@ -234,6 +242,65 @@ module rmpcd_capi
        real(c_double), value :: d, dt
        real(c_double), intent(in) :: edges(3)
      end function rmpcd_rattle_dimer_vel
+     integer(c_int) function rmpcd_rattle_body_pos(p, pos_rattle, links, distances, n_links, mass, n_mass, &
+          dt, edges, precision, iterations) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, rmpcd_particles_t
+       type(rmpcd_particles_t) :: p
+       type(c_ptr), value :: pos_rattle
+       integer(c_int32_t), intent(in) :: links(*)
+       real(c_double), intent(in) :: distances(*), mass(*)
+       integer(c_int32_t), value :: n_links, n_mass
+       real(c_double), value :: dt, precision
+       real(c_double), intent(in) :: edges(3)
+       integer(c_int32_t) :: iterations
+     end function rmpcd_rattle_body_pos
+     integer(c_int) function rmpcd_rattle_body_vel(p, links, distances, n_links, mass, n_mass, dt, edges, &
+          precision, iterations) bind(c)
+       import :: c_int, c_double, c_int32_t, rmpcd_particles_t
+       type(rmpcd_particles_t) :: p
+       integer(c_int32_t), intent(in) :: links(*)
+       real(c_double), intent(in) :: distances(*), mass(*)
+       integer(c_int32_t), value :: n_links, n_mass
+       real(c_double), value :: dt, precision
+       real(c_double), intent(in) :: edges(3)
+       integer(c_int32_t) :: iterations
+     end function rmpcd_rattle_body_vel
+     integer(c_int) function rmpcd_elastic_network(p, links, distances, n_links, k, edges, energy) bind(c)
+       import :: c_int, c_double, c_int32_t, rmpcd_particles_t
+       type(rmpcd_particles_t) :: p
+       integer(c_int32_t), intent(in) :: links(*)
+       real(c_double), intent(in) :: distances(*)
+       integer(c_int32_t), value :: n_links
+       real(c_double), value :: k
+       real(c_double), intent(in) :: edges(3)
+       real(c_double) :: energy
+     end function rmpcd_elastic_network
+     integer(c_int) function rmpcd_rigid_body_init(body, p, mass, n_mass, edges, i_start, i_stop, pos_body) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, rmpcd_particles_t
+       type(c_ptr), value :: body, pos_body
+       type(rmpcd_particles_t), intent(in) :: p
+       real(c_double), intent(in) :: mass(*), edges(3)
+       integer(c_int32_t), value :: n_mass, i_start, i_stop
+     end function rmpcd_rigid_body_init
+     integer(c_int) function rmpcd_rigid_body_compute_force_torque(body, p) bind(c)
+       import :: c_int, c_ptr, rmpcd_particles_t
+       type(c_ptr), value :: body
+       type(rmpcd_particles_t), intent(in) :: p
+     end function rmpcd_rigid_body_compute_force_torque
+     integer(c_int) function rmpcd_rigid_body_vv1(body, p, dt, threshold, edges, iterations) bind(c)
+       import :: c_int, c_ptr, c_double, c_int32_t, rmpcd_particles_t
+       type(c_ptr), value :: body
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt, threshold
+       real(c_double), intent(in) :: edges(3)
+       integer(c_int32_t) :: iterations
+     end function rmpcd_rigid_body_vv1
+     integer(c_int) function rmpcd_rigid_body_vv2(body, p, dt) bind(c)
+       import :: c_int, c_ptr, c_double, rmpcd_particles_t
+       type(c_ptr), value :: body
+       type(rmpcd_particles_t) :: p
+       real(c_double), value :: dt
+     end function rmpcd_rigid_body_vv2
      integer(c_int) function rmpcd_lj_params_init(lj, n1, n2, eps, sigma, cut) bind(c)
        import :: c_int, c_double, c_int32_t, rmpcd_lj_t
        type(rmpcd_lj_t) :: lj

@@ -1279,6 +1279,30 @@ module md
   private
   public :: md_pos, md_vel, apply_pbc, cell_md_pos, cell_md_vel, cell_md_pos_ywall, cell_md_pos_zwall
   public :: rattle_dimer_pos, rattle_dimer_vel
+  public :: rattle_body_pos, rattle_body_vel, elastic_network, rigid_body_t
+
+  !> rigid_body_t (src/md.f90:39-57); the fields are the host copy of the device state,
+  !! written back after every call and read before it
+  type rigid_body_t
+     integer :: i_start
+     integer :: i_stop
+     double precision :: mass
+     double precision :: pos(3)
+     double precision, allocatable :: pos_body(:,:)
+     double precision :: vel(3)
+     double precision :: force(3)
+     double precision :: q(4)
+     double precision :: L(3)
+     double precision :: torque(3)
+     double precision :: I_body(3)
+     double precision :: omega_body(3)
+     type(dbuf_t), private :: d_body, d_pos_body
+   contains
+     procedure :: init => rigid_body_init
+     procedure :: compute_force_torque => rigid_body_compute_force_torque
+     procedure :: vv1 => rigid_body_vv1
+     procedure :: vv2 => rigid_body_vv2
+  end type rigid_body_t
 
 contains
 
@@ -1416,6 +1440,173 @@ contains
     call p%download('v')
     call p%time_rattle_vel%tac()
   end subroutine rattle_dimer_vel
+
+  !> the link list with zero-based indices, as the C ABI takes it
+  function links0(links) result(l0)
+    integer, intent(in) :: links(:,:)
+    integer(c_int32_t), allocatable :: l0(:)
+    l0 = int(reshape(links, [size(links)]) - 1, c_int32_t)
+  end function links0
+
+  !> rattle_body_pos (src/md.f90:321-408); pos_rattle comes back unfolded as in the reference
+  subroutine rattle_body_pos(p, links, distances, dt, edges, precision)
+    type(particle_system_t), intent(inout) :: p
+    integer, intent(in) :: links(:,:)
+    double precision, intent(in) :: distances(:)
+    double precision, intent(in) :: dt, edges(3), precision
+    type(rmpcd_particles_t) :: v
+    integer(c_int32_t) :: it
+    call p%time_rattle_pos%tic()
+    call p%upload('pvs')
+    v = p%view()
+    call put_vec(p%d_aux, p%pos_rattle)
+    call check(rmpcd_rattle_body_pos(v, p%d_aux%p, links0(links), distances, int(size(distances), c_int32_t), &
+         p%mass, int(size(p%mass), c_int32_t), dt, edges, precision, it))
+    call p%download('pv')
+    call get_vec(p%d_aux, p%pos_rattle)
+    call p%time_rattle_pos%tac()
+    if (it >= 1000) write(*,*) 'rattle_max reached in rattle_body_pos'
+  end subroutine rattle_body_pos
+
+  !> rattle_body_vel (src/md.f90:410-480)
+  subroutine rattle_body_vel(p, links, distances, dt, edges, precision)
+    type(particle_system_t), intent(inout) :: p
+    integer, intent(in) :: links(:,:)
+    double precision, intent(in) :: distances(:)
+    double precision, intent(in) :: dt, edges(3), precision
+    type(rmpcd_particles_t) :: v
+    integer(c_int32_t) :: it
+    call p%time_rattle_vel%tic()
+    call p%upload('pvs')
+    v = p%view()
+    call check(rmpcd_rattle_body_vel(v, links0(links), distances, int(size(distances), c_int32_t), &
+         p%mass, int(size(p%mass), c_int32_t), dt, edges, precision, it))
+    call p%download('pv')
+    call p%time_rattle_vel%tac()
+    if (it >= 1000) write(*,*) 'rattle_max reached in rattle_body_vel'
+  end subroutine rattle_body_vel
+
+  !> elastic_network (src/md.f90:516-544)
+  function elastic_network(p, links, distances, k, edges) result(e)
+    type(particle_system_t), intent(inout) :: p
+    integer, intent(in) :: links(:,:)
+    double precision, intent(in) :: distances(:)
+    double precision, intent(in) :: k
+    double precision, intent(in) :: edges(3)
+    double precision :: e
+    type(rmpcd_particles_t) :: v
+    call p%time_elastic%tic()
+    call p%upload('pfo')
+    v = p%view()
+    call check(rmpcd_elastic_network(v, links0(links), distances, int(size(distances), c_int32_t), k, edges, e))
+    call p%download('f')
+    call p%time_elastic%tac()
+  end function elastic_network
+
+  subroutine body_push(this)
+    class(rigid_body_t), intent(inout) :: this
+    type(rmpcd_rigid_body_t), target :: h
+    h%i_start = this%i_start - 1
+    h%i_stop = this%i_stop - 1
+    h%mass = this%mass
+    h%pos = this%pos
+    h%vel = this%vel
+    h%force = this%force
+    h%q = this%q
+    h%L = this%L
+    h%torque = this%torque
+    h%I_body = this%I_body
+    h%omega_body = this%omega_body
+    h%pos_body = this%d_pos_body%p
+    call check(rmpcd_memcpy(this%d_body%p, c_loc(h), int(c_sizeof(h), c_size_t), H2D))
+  end subroutine body_push
+
+  subroutine body_pull(this)
+    class(rigid_body_t), intent(inout) :: this
+    type(rmpcd_rigid_body_t), target :: h
+    call check(rmpcd_memcpy(c_loc(h), this%d_body%p, int(c_sizeof(h), c_size_t), D2H))
+    this%mass = h%mass
+    this%pos = h%pos
+    this%vel = h%vel
+    this%force = h%force
+    this%q = h%q
+    this%L = h%L
+    this%torque = h%torque
+    this%I_body = h%I_body
+    this%omega_body = h%omega_body
+  end subroutine body_pull
+
+  !> rigid_body_init (src/md.f90:547-588); pos_body(:, i - i_start + 1) is particle i
+  subroutine rigid_body_init(this, ps, i_start, i_stop, edges)
+    class(rigid_body_t), intent(inout) :: this
+    type(particle_system_t), intent(inout) :: ps
+    integer, intent(in) :: i_start
+    integer, intent(in) :: i_stop
+    double precision, intent(in) :: edges(3)
+    type(rmpcd_particles_t) :: v
+    type(rmpcd_rigid_body_t) :: h
+    this%i_start = i_start
+    this%i_stop = i_stop
+    call ensure(this%d_body, int(c_sizeof(h), c_size_t))
+    call ensure(this%d_pos_body, int(3*(i_stop - i_start + 1), c_size_t)*8)
+    call ps%upload('pvsi')
+    v = ps%view()
+    call check(rmpcd_rigid_body_init(this%d_body%p, v, ps%mass, int(size(ps%mass), c_int32_t), edges, &
+         int(i_start - 1, c_int32_t), int(i_stop - 1, c_int32_t), this%d_pos_body%p))
+    call body_pull(this)
+    if (allocated(this%pos_body)) deallocate(this%pos_body)
+    allocate(this%pos_body(3, i_stop - i_start + 1))
+    call get_rows(this%d_pos_body, this%pos_body)
+  end subroutine rigid_body_init
+
+  !> body-frame rows (3, n) stored contiguously on the device
+  subroutine get_rows(b, a)
+    type(dbuf_t), intent(in) :: b
+    double precision, intent(inout), target :: a(:,:)
+    call check(rmpcd_memcpy(c_loc(a), b%p, int(size(a), c_size_t)*8, D2H))
+  end subroutine get_rows
+
+  !> rigid_body_compute_force_torque (src/md.f90:591-607)
+  subroutine rigid_body_compute_force_torque(this, ps)
+    class(rigid_body_t), intent(inout) :: this
+    type(particle_system_t), intent(inout) :: ps
+    type(rmpcd_particles_t) :: v
+    call ps%upload('fo')
+    v = ps%view()
+    call body_push(this)
+    call check(rmpcd_rigid_body_compute_force_torque(this%d_body%p, v))
+    call body_pull(this)
+  end subroutine rigid_body_compute_force_torque
+
+  !> rigid_body_vv1 (src/md.f90:613-673)
+  subroutine rigid_body_vv1(this, ps, dt, treshold, edges)
+    class(rigid_body_t), intent(inout) :: this
+    type(particle_system_t), intent(inout) :: ps
+    double precision, intent(in) :: dt
+    double precision, intent(in) :: treshold
+    double precision, intent(in) :: edges(3)
+    type(rmpcd_particles_t) :: v
+    integer(c_int32_t) :: it
+    call ps%upload('p')
+    v = ps%view()
+    call body_push(this)
+    call check(rmpcd_rigid_body_vv1(this%d_body%p, v, dt, treshold, edges, it))
+    call body_pull(this)
+    call ps%download('p')
+  end subroutine rigid_body_vv1
+
+  !> rigid_body_vv2 (src/md.f90:679-700)
+  subroutine rigid_body_vv2(this, ps, dt)
+    class(rigid_body_t), intent(inout) :: this
+    type(particle_system_t), intent(inout) :: ps
+    double precision, intent(in) :: dt
+    type(rmpcd_particles_t) :: v
+    v = ps%view()
+    call body_push(this)
+    call check(rmpcd_rigid_body_vv2(this%d_body%p, v, dt))
+    call body_pull(this)
+    call ps%download('v')
+  end subroutine rigid_body_vv2
 end module md
 
 !> the umbrella module the drivers use (src/rmpcdmd_module.f90.in)

@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rmpcdmd_amd", "fortran", "bin")
 
 
-@pytest.mark.parametrize("prog", ["fortran_cells", "fortran_fluid", "fortran_dimer", "fortran_walls"])
+@pytest.mark.parametrize("prog", ["fortran_cells", "fortran_fluid", "fortran_dimer", "fortran_walls",
+                                  "fortran_bodies"])
 def test_fortran_program(gpu, prog):
     exe = os.path.join(BIN, prog)
     assert os.path.exists(exe), "Fortran programs are not built (run __graft_entry__.build())"
