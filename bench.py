@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-single", action="store_true", help="skip the single-replica run")
     ap.add_argument("--no-lazy", action="store_true", help="skip the lazy-streaming run")
+    ap.add_argument("--no-bodies", action="store_true", help="skip the linked-colloid routines")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the 128-replica run with L drawn from {24, 32, 40, 48}")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -153,6 +154,104 @@ def measured_traffic(form, L, replicas):
     return None
 
 
+def linked_bodies(with_cpu):
+    """the linked-colloid routines on a Janus-like shell: 300 beads on a sphere of radius
+    4.5, linked to the beads closer than 1.8 mean spacings (i < j, by i then j, as the
+    reference's single_body driver builds them; 1202 links in 155 levels); one MD step of
+    dt = 0.01 from the constrained shape with unit thermal velocities, RATTLE to 1e-8.
+    GPU times are per call, wall clock, with the device state reset between calls."""
+    import numpy as np
+    import torch
+
+    from rmpcdmd_amd import capi
+    from rmpcdmd_amd.system import ParticleSet
+    n, R = 300, 4.5
+    k = np.arange(n) + 0.5
+    th, ph = np.arccos(1 - 2 * k / n), np.pi * (1 + 5 ** 0.5) * k
+    r = R * np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)], 1)
+    i, j = np.triu_indices(n, 1)
+    near = np.linalg.norm(r[i] - r[j], axis=1) < 1.8 * np.sqrt(4 * np.pi * R * R / n)
+    links = np.ascontiguousarray(np.stack([i[near], j[near]], 1).astype(np.int32))
+    d = np.linalg.norm(r[links[:, 0]] - r[links[:, 1]], axis=1)
+    rng = np.random.default_rng(5)
+    edges = np.array([40.0] * 3)
+    pr = r + 20.0
+    vel = rng.normal(size=(n, 3))
+    vel -= vel.mean(0)
+    pos = pr + 0.01 * vel
+    mass = np.array([1.0])
+    g = ParticleSet(n, form=0, device="cuda")
+    g.load(pos=pos, vel=vel, force=rng.normal(size=(n, 3)), force_old=np.zeros((n, 3)), image=np.zeros((n, 3), np.int32),
+           id=np.arange(n, dtype=np.int32), species=np.ones(n, np.int32), flags=np.zeros(n, np.int32))
+    p0, v0 = g.pos.clone(), g.vel.clone()
+    rt0 = torch.as_tensor(np.ascontiguousarray(pr.T)).cuda()
+    rt = rt0.clone()
+    lk, dd = links.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p)
+    m = (C.c_double * 1)(1.0)
+    e3 = capi.d3(edges)
+    it = C.c_int32(0)
+
+    def timed(fn, reps=5):
+        best = 1e30
+        for _ in range(reps):
+            g.pos.copy_(p0)
+            g.vel.copy_(v0)
+            rt.copy_(rt0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best * 1e3
+
+    def rpos():
+        capi.call("rmpcd_rattle_body_pos", g.struct(), C.c_void_p(rt.data_ptr()), lk, dd, len(d), m, 1, 0.01, e3,
+                  1e-8, C.byref(it))
+
+    ms_pos = timed(rpos)
+    sweeps_pos = it.value
+    rpos()
+    p1, v1 = g.pos.clone(), g.vel.clone()
+
+    def rvel():
+        g.pos.copy_(p1)
+        g.vel.copy_(v1)
+        capi.call("rmpcd_rattle_body_vel", g.struct(), lk, dd, len(d), m, 1, 0.01, e3, 1e-8, C.byref(it))
+
+    ms_vel = timed(rvel)
+    sweeps_vel = it.value
+    e = C.c_double(0)
+    ms_el = timed(lambda: capi.call("rmpcd_elastic_network", g.struct(), lk, dd, len(d), 30.0, e3, C.byref(e)))
+    body = torch.zeros(C.sizeof(capi.RigidBody), dtype=torch.uint8, device="cuda")
+    rows = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    capi.call("rmpcd_rigid_body_init", C.c_void_p(body.data_ptr()), g.struct(), m, 1, e3, 0, n - 1,
+              C.c_void_p(rows.data_ptr()))
+    bp = C.c_void_p(body.data_ptr())
+
+    def rigid_step():
+        capi.call("rmpcd_rigid_body_compute_force_torque", bp, g.struct())
+        capi.call("rmpcd_rigid_body_vv1", bp, g.struct(), 0.01, 1e-12, e3, None)
+        capi.call("rmpcd_rigid_body_vv2", bp, g.struct(), 0.01)
+
+    ms_rigid = timed(rigid_step)
+    out = {"beads": n, "links": int(len(d)), "rattle_pos": {"ms": ms_pos, "sweeps": sweeps_pos},
+           "rattle_vel": {"ms": ms_vel, "sweeps": sweeps_vel}, "elastic_network_ms": ms_el,
+           "rigid_body_step_ms": ms_rigid}
+    if with_cpu:
+        from oracle import oracle as O
+        sp = np.ones(n, np.int32)
+        t = time.perf_counter()
+        w_pos, w_vel, w_pr = pos.copy(), vel.copy(), pr.copy()
+        O.rattle_body_pos(w_pos, w_vel, w_pr, sp, mass, links, d, 0.01, edges, 1e-8)
+        cpu_pos = (time.perf_counter() - t) * 1e3
+        t = time.perf_counter()
+        O.rattle_body_vel(w_pos, w_vel, sp, mass, links, d, 0.01, edges, 1e-8)
+        cpu_vel = (time.perf_counter() - t) * 1e3
+        out["cpu_baseline"] = {"kind": "port", "cores": 1, "rattle_pos_ms": cpu_pos, "rattle_vel_ms": cpu_vel,
+                               "sample": "the oracle (C, one thread) on the same shell and step"}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -241,6 +340,8 @@ def main():
     line["step_bytes_per_particle_step"] = 9 * w + n_sorts * (2 * S + 16 / 10)
     line["step_effective_GBs"] = line["step_bytes_per_particle_step"] * main_run["ps_per_s"] / 1e9
     line["step_frac_of_hbm_peak"] = line["step_effective_GBs"] / HBM_PEAK_GBS
+    if not a.no_bodies:
+        line["linked_bodies"] = linked_bodies(not a.no_cpu)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.L, a.n_md)
     print(json.dumps(line))

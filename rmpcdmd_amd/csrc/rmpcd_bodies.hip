@@ -3,14 +3,23 @@
 // quaternion velocity Verlet (547-700).
 //
 // The reference updates the links one after the other, each correction seeing the ones
-// before it.  Here the links are grouped into levels on the host: a link's level is one
-// more than the highest level of the earlier links that share a particle with it.  Links
-// of one level touch disjoint particles, and every earlier link that touches the same
-// particle sits in a lower level, so one workgroup updating a level at a time, with a
-// barrier between levels, reads and writes every particle in exactly the order of the
-// sequential loop: the results are the reference's, bit for bit, with a level's links
-// done in parallel.  The body (positions, velocities, inverse masses, the schedule)
-// lives in LDS while the sweeps run, or in global memory when it does not fit.
+// before it, sweep after sweep until the constraints hold.  That order is kept exactly.
+// On the host the link list becomes a schedule of levels: an update's level is one more
+// than the highest level of the earlier updates that share a particle with it.  Updates
+// of one level touch disjoint particles and every earlier update of the same particle
+// sits in a lower level, so a workgroup running the levels in turn, with a barrier in
+// between, reads and writes every particle in the order of the sequential loops: the
+// results are the reference's bit for bit.
+//
+// Sweeps overlap.  The schedule covers a block of K consecutive sweeps as one unrolled
+// sequence, so the first links of sweep s + 1 start while the last ones of sweep s are
+// still running (on a 300-bead shell one sweep alone takes 155 levels, each further
+// sweep of the block about 16).  The stopping test of a sweep needs the state at its end:
+// the update that is a particle's last in a sweep copies the particle to that sweep's
+// snapshot, the block's tests then run on the snapshots, and the state of the first sweep
+// that passes is put back before the block's later sweeps are forgotten.  Body, links
+// and schedule live in LDS (up to 160 KiB on gfx950); bodies too large for it run the
+// one-sweep schedule from global memory.
 #include <math.h>
 #include <string.h>
 
@@ -20,9 +29,9 @@
 
 namespace rmpcd {
 
-constexpr int BT = 256;                // threads of the one workgroup
-constexpr size_t BODY_LDS = 64 * 1024; // LDS budget of the staged body
-constexpr int RATTLE_MAX = 1000;       // md.f90:341, :432
+constexpr int BT = 256;           // threads of the one workgroup
+constexpr int RATTLE_MAX = 1000;  // md.f90:341, :432
+constexpr int KMAX = 12;          // sweeps per block at most (4 bits of the entry)
 
 struct BodyMass {
     double m[16];    // particle_system%mass by species (species s at s - 1)
@@ -31,25 +40,56 @@ struct BodyMass {
 
 __device__ inline double dot3b(const double a[3], const double b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 
-// ---- the link schedule ----------------------------------------------------------------
-// Built on the host from the caller's link list; kept on the device while the list is
+// ---- the link schedules -----------------------------------------------------------------
+// Built on the host from the caller's link list and kept on the device while the list is
 // unchanged (compared by content on every call).
+struct BlockSched {
+    int K = 0, n_glev = 0;
+    uint2 *d_ent = nullptr;   // by level: {a | b << 16, l | s << 24 | last-of-a << 28 | last-of-b << 29}
+    int32_t *d_gs = nullptr;  // level starts, n_glev + 1
+    void *block = nullptr;
+};
+
 struct Schedule {
     std::vector<int32_t> links;  // the caller's list, for the comparison
     std::vector<double> dist;
     int64_t n = 0;
     int n_links = 0, n_levels = 0;
-    int32_t *d_i1 = nullptr, *d_i2 = nullptr, *d_start = nullptr;  // links by level
-    int32_t *d_order = nullptr;  // the caller's index of each scheduled link
+    // one sweep by level (the global-memory rattle)
+    int32_t *d_i1 = nullptr, *d_i2 = nullptr, *d_start = nullptr;
     double *d_d = nullptr;
+    // by link: a, b, a | b << 16 (when n < 65536), distance; per particle: the links
+    // touching it in link order (bit 31 set where it is the second particle)
+    int32_t *d_la = nullptr, *d_lb = nullptr;
+    uint32_t *d_lab = nullptr;
+    double *d_dist = nullptr;
+    int32_t *d_cstart = nullptr, *d_csr = nullptr;
     void *block = nullptr;
+    BlockSched blocks[KMAX + 1];  // by K, built on first use
 };
 
 static Schedule g_sched[4];
 static int g_sched_next = 0;
 
+static void free_schedule(Schedule &s)
+{
+    for (auto &b : s.blocks) {
+        if (b.block) (void)hipFree(b.block);
+        b = BlockSched();
+    }
+    if (s.block) (void)hipFree(s.block);
+    s.block = nullptr;
+}
+
+template <class T>
+static int upload(T *dst, const std::vector<T> &src)
+{
+    if (!src.empty()) RMPCD_HIP(hipMemcpy(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return RMPCD_OK;
+}
+
 static int get_schedule(const int32_t *links, const double *dist, int n_links, int64_t n, const char *who,
-                        const Schedule **out)
+                        Schedule **out)
 {
     for (auto &s : g_sched)
         if (s.block && s.n == n && s.n_links == n_links && n_links > 0 &&
@@ -70,9 +110,10 @@ static int get_schedule(const int32_t *links, const double *dist, int n_links, i
         last[a] = last[b] = v + 1;
         if (v + 1 > n_levels) n_levels = v + 1;
     }
-    std::vector<int32_t> start((size_t)n_levels + 1, 0), i1((size_t)n_links), i2((size_t)n_links),
-        order((size_t)n_links);
-    std::vector<double> dd((size_t)n_links);
+    const size_t nl = (size_t)n_links;
+    std::vector<int32_t> start((size_t)n_levels + 1, 0), i1(nl), i2(nl), la(nl), lb(nl);
+    std::vector<uint32_t> lab(nl);
+    std::vector<double> dd(nl);
     for (int l = 0; l < n_links; ++l) ++start[lvl[l] + 1];
     for (int v = 0; v < n_levels; ++v) start[v + 1] += start[v];
     std::vector<int32_t> fill(start.begin(), start.end() - 1);
@@ -81,32 +122,49 @@ static int get_schedule(const int32_t *links, const double *dist, int n_links, i
         i1[k] = links[2 * l];
         i2[k] = links[2 * l + 1];
         dd[k] = dist[l];
-        order[k] = l;
+        la[l] = links[2 * l];
+        lb[l] = links[2 * l + 1];
+        lab[l] = (uint32_t)links[2 * l] | ((uint32_t)links[2 * l + 1] << 16);
+    }
+    std::vector<int32_t> cstart((size_t)n + 1, 0), csr(2 * nl);
+    for (int l = 0; l < n_links; ++l) {
+        ++cstart[links[2 * l] + 1];
+        ++cstart[links[2 * l + 1] + 1];
+    }
+    for (int64_t p = 0; p < n; ++p) cstart[p + 1] += cstart[p];
+    std::vector<int32_t> cf(cstart.begin(), cstart.end() - 1);
+    for (int l = 0; l < n_links; ++l) {
+        csr[cf[links[2 * l]]++] = l;
+        csr[cf[links[2 * l + 1]]++] = (int32_t)((uint32_t)l | 0x80000000u);
     }
     Schedule &s = g_sched[g_sched_next];
     g_sched_next = (g_sched_next + 1) % 4;
     if (s.block) {
         RMPCD_HIP(hipStreamSynchronize(stream()));  // a kernel may still read it
-        RMPCD_HIP(hipFree(s.block));
-        s.block = nullptr;
+        free_schedule(s);
     }
-    const size_t nl = (size_t)(n_links > 0 ? n_links : 1);
-    const size_t bytes = nl * 8 + 3 * nl * 4 + ((size_t)n_levels + 1) * 4 + 64;
+    const size_t nl1 = nl > 0 ? nl : 1;
+    const size_t bytes = nl1 * 16 + nl1 * 4 * 5 + ((size_t)n_levels + 1) * 4 + ((size_t)n + 1) * 4 + 2 * nl1 * 4 + 256;
     RMPCD_HIP(hipMalloc(&s.block, bytes));
     s.d_d = (double *)s.block;
-    s.d_i1 = (int32_t *)(s.d_d + nl);
-    s.d_i2 = s.d_i1 + nl;
-    s.d_order = s.d_i2 + nl;
-    s.d_start = s.d_order + nl;
-    if (n_links > 0) {
-        RMPCD_HIP(hipMemcpy(s.d_d, dd.data(), sizeof(double) * n_links, hipMemcpyHostToDevice));
-        RMPCD_HIP(hipMemcpy(s.d_i1, i1.data(), sizeof(int32_t) * n_links, hipMemcpyHostToDevice));
-        RMPCD_HIP(hipMemcpy(s.d_i2, i2.data(), sizeof(int32_t) * n_links, hipMemcpyHostToDevice));
-        RMPCD_HIP(hipMemcpy(s.d_order, order.data(), sizeof(int32_t) * n_links, hipMemcpyHostToDevice));
-    }
-    RMPCD_HIP(hipMemcpy(s.d_start, start.data(), sizeof(int32_t) * (n_levels + 1), hipMemcpyHostToDevice));
-    s.links.assign(links, links + 2 * (size_t)n_links);
-    s.dist.assign(dist, dist + n_links);
+    s.d_dist = s.d_d + nl1;
+    s.d_i1 = (int32_t *)(s.d_dist + nl1);
+    s.d_i2 = s.d_i1 + nl1;
+    s.d_la = s.d_i2 + nl1;
+    s.d_lb = s.d_la + nl1;
+    s.d_lab = (uint32_t *)(s.d_lb + nl1);
+    s.d_csr = (int32_t *)(s.d_lab + nl1);
+    s.d_cstart = s.d_csr + 2 * nl1;
+    s.d_start = s.d_cstart + n + 1;
+    std::vector<double> dist_v(dist, dist + n_links);
+    int rc;
+    if ((rc = upload(s.d_d, dd)) || (rc = upload(s.d_dist, dist_v)) || (rc = upload(s.d_i1, i1)) ||
+        (rc = upload(s.d_i2, i2)) || (rc = upload(s.d_la, la)) || (rc = upload(s.d_lb, lb)) ||
+        (rc = upload(s.d_lab, lab)) || (rc = upload(s.d_csr, csr)) || (rc = upload(s.d_cstart, cstart)) ||
+        (rc = upload(s.d_start, start)))
+        return rc;
+    s.links.assign(links, links + 2 * nl);
+    s.dist = dist_v;
     s.n = n;
     s.n_links = n_links;
     s.n_levels = n_levels;
@@ -114,18 +172,288 @@ static int get_schedule(const int32_t *links, const double *dist, int n_links, i
     return RMPCD_OK;
 }
 
-struct SchedView {
-    const int32_t *i1, *i2, *start, *order;
-    const double *d;
-    int n_links, n_levels;
+// the unrolled schedule of K sweeps, built once per list and K
+static int get_block(Schedule &s, int K, const BlockSched **out)
+{
+    BlockSched &b = s.blocks[K];
+    if (b.block) {
+        *out = &b;
+        return RMPCD_OK;
+    }
+    const int nl = s.n_links;
+    const int32_t *links = s.links.data();
+    std::vector<int32_t> last((size_t)s.n, 0), lastlink((size_t)s.n, -1), lvl((size_t)K * nl);
+    for (int l = 0; l < nl; ++l) lastlink[links[2 * l]] = lastlink[links[2 * l + 1]] = l;
+    int n_glev = 0;
+    for (int q = 0; q < K; ++q)
+        for (int l = 0; l < nl; ++l) {
+            const int a = links[2 * l], c = links[2 * l + 1];
+            const int v = last[a] > last[c] ? last[a] : last[c];
+            lvl[(size_t)q * nl + l] = v;
+            last[a] = last[c] = v + 1;
+            if (v + 1 > n_glev) n_glev = v + 1;
+        }
+    std::vector<int32_t> gs((size_t)n_glev + 1, 0);
+    for (size_t k = 0; k < lvl.size(); ++k) ++gs[lvl[k] + 1];
+    for (int v = 0; v < n_glev; ++v) gs[v + 1] += gs[v];
+    std::vector<int32_t> fill(gs.begin(), gs.end() - 1);
+    std::vector<uint2> ent(lvl.size());
+    for (int q = 0; q < K; ++q)
+        for (int l = 0; l < nl; ++l) {
+            const int a = links[2 * l], c = links[2 * l + 1];
+            uint2 e;
+            e.x = (uint32_t)a | ((uint32_t)c << 16);
+            e.y = (uint32_t)l | ((uint32_t)q << 24) | ((lastlink[a] == l ? 1u : 0u) << 28) |
+                  ((lastlink[c] == l ? 1u : 0u) << 29);
+            ent[fill[lvl[(size_t)q * nl + l]]++] = e;
+        }
+    RMPCD_HIP(hipMalloc(&b.block, ent.size() * sizeof(uint2) + gs.size() * 4 + 256));
+    b.d_ent = (uint2 *)b.block;
+    b.d_gs = (int32_t *)(b.d_ent + ent.size());
+    int rc;
+    if ((rc = upload(b.d_ent, ent)) || (rc = upload(b.d_gs, gs))) return rc;
+    b.K = K;
+    b.n_glev = n_glev;
+    *out = &b;
+    return RMPCD_OK;
+}
+
+// bytes of LDS the pipelined rattle takes: particles (P, V and for positions R, four
+// doubles each), links (distance, packed pair), the entries and level starts, flags
+static size_t pipe_lds_bytes(int64_t n, int n_links, int K, int n_glev, bool with_pr)
+{
+    return (size_t)n * 32 * (with_pr ? 3 : 2) + (size_t)n_links * 12 + (size_t)K * n_links * 8 +
+           ((size_t)n_glev + 1) * 4 + 16 * 4 + 64;
+}
+
+static int lds_limit()
+{
+    static int limit = 0;
+    if (!limit) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        limit = v;
+    }
+    return limit;
+}
+
+// a workgroup barrier that orders LDS only: global stores stay in flight
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+struct PipeView {
+    const uint2 *ent;
+    const int32_t *gs;
+    const uint32_t *lab;
+    const double *dist;
+    int n_links, n_ent, n_glev, K;
 };
 
-// bytes of LDS a staged body takes
-static size_t body_lds_bytes(int64_t n, int n_links, int n_levels, bool with_pr, bool with_vel)
+// ---- rattle_body_pos (kPos) / rattle_body_vel, pipelined -------------------------------
+// Staging: every particle but the first is unfolded next to particle 0 (md.f90:343-353,
+// :427-431) and folded back at the end (md.f90:400-402, :474-476); pos_rattle is left
+// unfolded, as the reference leaves it.
+template <bool kPos>
+__global__ void __launch_bounds__(BT) k_rattle_pipe(PView<F64> p, double *__restrict__ gpr, PipeView pv, BodyMass bm,
+                                                    double dt, double e0, double e1, double e2, double precision,
+                                                    int32_t *__restrict__ out, int32_t *__restrict__ gim,
+                                                    double *__restrict__ snap)
 {
-    size_t dbl = (size_t)n * ((with_pr ? 6 : 3) + (with_vel ? 3 : 0) + 1) + n_links;
-    size_t ints = 3 * (size_t)n + 2 * (size_t)n_links + n_levels + 1;
-    return dbl * 8 + ints * 4 + 64 * 8;  // + the reduction scratch
+    extern __shared__ double smem[];
+    const int64_t n = p.n;
+    const int tid = threadIdx.x;
+    const int nl = pv.n_links, K = pv.K;
+    const double edges[3] = {e0, e1, e2};
+    double *P = smem;          // {x, y, z, 1/m}
+    double *V = P + 4 * n;     // {vx, vy, vz, linked}
+    double *R = V + 4 * n;     // {x, y, z, -} (positions only)
+    double *D = kPos ? R + 4 * n : R;
+    uint2 *E = (uint2 *)(D + nl);
+    uint32_t *LAB = (uint32_t *)(E + pv.n_ent);
+    int32_t *GS = (int32_t *)(LAB + nl);
+    int32_t *NC = GS + pv.n_glev + 1;
+    for (int64_t i = tid; i < n; i += BT) {
+        for (int k = 0; k < 3; ++k) {
+            const double x0 = p.pos[k * n], x = p.pos[k * n + i];
+            const int j = i == 0 ? 0 : (int)floor(((x - x0) / edges[k]) + 0.5);
+            gim[k * n + i] = j;
+            P[4 * i + k] = i == 0 ? x : x - j * edges[k];
+            V[4 * i + k] = p.vel[k * n + i];
+            if (kPos) {
+                const double r0 = gpr[k * n], r = gpr[k * n + i];
+                const int jr = i == 0 ? 0 : (int)floor(((r - r0) / edges[k]) + 0.5);
+                R[4 * i + k] = i == 0 ? r : r - jr * edges[k];
+            }
+        }
+        P[4 * i + 3] = bm.inv[((int)p.species[i] - 1) & 15];
+        V[4 * i + 3] = 0;
+    }
+    for (int l = tid; l < nl; l += BT) {
+        D[l] = pv.dist[l];
+        LAB[l] = pv.lab[l];
+    }
+    for (int x = tid; x < pv.n_ent; x += BT) E[x] = pv.ent[x];
+    for (int g = tid; g <= pv.n_glev; g += BT) GS[g] = pv.gs[g];
+    __syncthreads();
+    for (int l = tid; l < nl; l += BT) {
+        V[4 * (LAB[l] & 0xFFFF) + 3] = 1;
+        V[4 * (LAB[l] >> 16) + 3] = 1;
+    }
+    __syncthreads();
+    int it = RATTLE_MAX + 1;
+    // one link update of the schedule (md.f90:365-383, :435-452)
+    auto update = [&](const uint2 e, const int ns) {
+        const int q = (e.y >> 24) & 15;
+        if (q >= ns) return;
+        const int a = e.x & 0xFFFF, b = e.x >> 16, l = e.y & 0xFFFFFF;
+        const double pa[3] = {P[4 * a], P[4 * a + 1], P[4 * a + 2]};
+        const double pb[3] = {P[4 * b], P[4 * b + 1], P[4 * b + 2]};
+        const double va[3] = {V[4 * a], V[4 * a + 1], V[4 * a + 2]};
+        const double vb[3] = {V[4 * b], V[4 * b + 1], V[4 * b + 2]};
+        const double ma = P[4 * a + 3], mb = P[4 * b + 3], d = D[l];
+        double s[3], na[3], nb[3], wa[3], wb[3];
+        for (int k = 0; k < 3; ++k) s[k] = pa[k] - pb[k];
+        if (kPos) {
+            double r[3];
+            for (int k = 0; k < 3; ++k) r[k] = R[4 * a + k] - R[4 * b + k];
+            const double rsq = dot3b(r, r), ssq = dot3b(s, s), rs = dot3b(r, s);
+            double gg = rs - sqrt(rs * rs - rsq * (ssq - d * d));
+            gg = gg / ((dt * (ma + mb)) * rsq);
+            // a link from a particle to itself reads its own update, as the reference does
+            for (int k = 0; k < 3; ++k) {
+                na[k] = pa[k] - ((gg * dt) * r[k]) * ma;
+                nb[k] = (a == b ? na[k] : pb[k]) + ((gg * dt) * r[k]) * mb;
+                wa[k] = va[k] - (gg * r[k]) * ma;
+                wb[k] = (a == b ? wa[k] : vb[k]) + (gg * r[k]) * mb;
+            }
+            for (int k = 0; k < 3; ++k) {
+                P[4 * a + k] = na[k];
+                V[4 * a + k] = wa[k];
+            }
+            for (int k = 0; k < 3; ++k) {
+                P[4 * b + k] = nb[k];
+                V[4 * b + k] = wb[k];
+            }
+        } else {
+            double dv[3];
+            for (int k = 0; k < 3; ++k) dv[k] = va[k] - vb[k];
+            const double kk = dot3b(dv, s) / ((d * d) * (ma + mb));
+            for (int k = 0; k < 3; ++k) {
+                na[k] = pa[k];
+                nb[k] = pb[k];
+                wa[k] = va[k] - (kk * s[k]) * ma;
+                wb[k] = (a == b ? wa[k] : vb[k]) + (kk * s[k]) * mb;
+            }
+            for (int k = 0; k < 3; ++k) V[4 * a + k] = wa[k];
+            for (int k = 0; k < 3; ++k) V[4 * b + k] = wb[k];
+        }
+        // the last update of a particle in a sweep leaves its end-of-sweep state
+        if (e.y & (1u << 28)) {
+            double *o = snap + ((size_t)q * n + a) * 8;
+            for (int k = 0; k < 3; ++k) {
+                o[k] = na[k];
+                o[3 + k] = wa[k];
+            }
+        }
+        if (e.y & (1u << 29)) {
+            double *o = snap + ((size_t)q * n + b) * 8;
+            for (int k = 0; k < 3; ++k) {
+                o[k] = nb[k];
+                o[3 + k] = wb[k];
+            }
+        }
+    };
+    const int last_ent = pv.n_ent > 0 ? pv.n_ent - 1 : 0;
+    for (int base = 0; base < RATTLE_MAX; base += K) {
+        const int ns = K < RATTLE_MAX - base ? K : RATTLE_MAX - base;
+        // each level's first entry of this thread is read one level ahead; the barrier
+        // between levels orders LDS only (the snapshots are read after the full barrier
+        // that ends the block)
+        int x0 = GS[0], x1 = pv.n_glev > 0 ? GS[1] : 0;
+        uint2 cur = E[min(x0 + tid, last_ent)];
+        for (int g = 0; g < pv.n_glev; ++g) {
+            const uint2 next = E[min(x1 + tid, last_ent)];
+            const int y1 = GS[g + 2 <= pv.n_glev ? g + 2 : pv.n_glev];
+            if (x0 + tid < x1) update(cur, ns);
+            for (int x = x0 + tid + BT; x < x1; x += BT) update(E[x], ns);
+            lds_barrier();
+            x0 = x1;
+            x1 = y1;
+            cur = next;
+        }
+        __syncthreads();
+        // the stopping test of each sweep of the block (md.f90:385-392, :454-465): a sweep
+        // passes when no link reaches the precision (a NaN never does).  Items run sweep
+        // fastest, four at a time, and skip sweeps already failed.
+        if (tid < K) NC[tid] = 0 < precision ? 0 : 1;
+        __syncthreads();
+        const int n_items = ns * nl;
+        for (int x0c = tid; x0c < n_items; x0c += 4 * BT) {
+            double err[4];
+            int qs[4];
+            bool live[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int x = x0c + u * BT;
+                const int q = x % ns, l = x / ns;
+                qs[u] = q;
+                live[u] = x < n_items && !NC[q];
+                err[u] = 0;
+                if (!live[u]) continue;
+                const int a = LAB[l] & 0xFFFF, b = LAB[l] >> 16;
+                const double *sa = snap + ((size_t)q * n + a) * 8, *sb = snap + ((size_t)q * n + b) * 8;
+                if (kPos) {
+                    double r[3];
+                    for (int k = 0; k < 3; ++k) r[k] = sa[k] - sb[k];
+                    err[u] = fabs(sqrt(dot3b(r, r)) - D[l]);
+                } else {
+                    double sv[3], dv[3];
+                    for (int k = 0; k < 3; ++k) {
+                        sv[k] = P[4 * a + k] - P[4 * b + k];
+                        dv[k] = sa[3 + k] - sb[3 + k];
+                    }
+                    err[u] = fabs(dot3b(dv, sv) / ((D[l] * D[l]) * (P[4 * a + 3] + P[4 * b + 3])));
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (live[u] && err[u] >= precision) NC[qs[u]] = 1;
+        }
+        __syncthreads();
+        int conv = -1;
+        for (int q = 0; q < ns; ++q)
+            if (!NC[q]) {
+                conv = q;
+                break;
+            }
+        if (conv >= 0) {
+            it = base + conv + 1;
+            for (int64_t i = tid; i < n; i += BT)
+                if (V[4 * i + 3] != 0) {
+                    const double *o = snap + ((size_t)conv * n + i) * 8;
+                    for (int k = 0; k < 3; ++k) {
+                        P[4 * i + k] = o[k];
+                        V[4 * i + k] = o[3 + k];
+                    }
+                }
+            __syncthreads();
+            break;
+        }
+        __syncthreads();
+    }
+    for (int64_t i = tid; i < n; i += BT)
+        for (int k = 0; k < 3; ++k) {
+            p.pos[k * n + i] = i == 0 ? P[4 * i + k] : P[4 * i + k] + gim[k * n + i] * edges[k];
+            p.vel[k * n + i] = V[4 * i + k];
+            if (kPos) gpr[k * n + i] = R[4 * i + k];
+        }
+    if (tid == 0) out[0] = it;
 }
 
 __device__ inline double body_max(double v, double *red)
@@ -141,69 +469,37 @@ __device__ inline double body_max(double v, double *red)
     return r;
 }
 
-// ---- rattle_body_pos (kPos) / rattle_body_vel ----------------------------------------
-// Staging: every particle but the first is unfolded next to particle 0
-// (md.f90:343-353, :427-431), and folded back at the end (md.f90:400-402, :474-476).
-template <bool kPos, bool kLds>
-__global__ void __launch_bounds__(BT) k_rattle_body(PView<F64> p, double *__restrict__ gpr, SchedView sv, BodyMass bm,
-                                                    double dt, double e0, double e1, double e2, double precision,
-                                                    int32_t *__restrict__ out, double *__restrict__ gminv,
-                                                    int32_t *__restrict__ gim)
+struct SchedView {
+    const int32_t *i1, *i2, *start;
+    const double *d;
+    int n_links, n_levels;
+};
+
+// ---- rattle_body_pos (kPos) / rattle_body_vel from global memory ----------------------
+// For bodies whose links and schedule do not fit in LDS: one sweep at a time, its levels
+// in turn, the particles updated in place.
+template <bool kPos>
+__global__ void __launch_bounds__(BT) k_rattle_global(PView<F64> p, double *__restrict__ pr, SchedView sv,
+                                                      BodyMass bm, double dt, double e0, double e1, double e2,
+                                                      double precision, int32_t *__restrict__ out,
+                                                      double *__restrict__ minv, int32_t *__restrict__ im)
 {
-    extern __shared__ double smem[];
+    __shared__ double red[64];
     const int64_t n = p.n;
     const int tid = threadIdx.x;
     const double edges[3] = {e0, e1, e2};
-    double *pos, *vel, *pr = nullptr, *minv, *d, *red;
-    int32_t *im, *i1, *i2, *start;
-    int64_t st;
-    if (kLds) {
-        st = n;
-        pos = smem;
-        vel = pos + 3 * n;
-        double *q = vel + 3 * n;
-        if (kPos) {
-            pr = q;
-            q += 3 * n;
-        }
-        minv = q;
-        d = minv + n;
-        red = d + sv.n_links;
-        im = (int32_t *)(red + 64);
-        i1 = im + 3 * n;
-        i2 = i1 + sv.n_links;
-        start = i2 + sv.n_links;
-        for (int l = tid; l < sv.n_links; l += BT) {
-            i1[l] = sv.i1[l];
-            i2[l] = sv.i2[l];
-            d[l] = sv.d[l];
-        }
-        for (int l = tid; l <= sv.n_levels; l += BT) start[l] = sv.start[l];
-    } else {
-        st = n;
-        pos = p.pos;
-        vel = p.vel;
-        pr = gpr;
-        minv = gminv;
-        d = (double *)sv.d;
-        red = smem;
-        im = gim;
-        i1 = (int32_t *)sv.i1;
-        i2 = (int32_t *)sv.i2;
-        start = (int32_t *)sv.start;
-    }
+    double *pos = p.pos, *vel = p.vel;
     for (int64_t i = tid; i < n; i += BT) {
         minv[i] = bm.inv[((int)p.species[i] - 1) & 15];
         for (int k = 0; k < 3; ++k) {
-            const double x0 = p.pos[k * n], x = p.pos[k * n + i];
+            const double x0 = pos[k * n], x = pos[k * n + i];
             const int j = i == 0 ? 0 : (int)floor(((x - x0) / edges[k]) + 0.5);
-            im[k * st + i] = j;
-            pos[k * st + i] = i == 0 ? x : x - j * edges[k];
-            if (kLds) vel[k * st + i] = p.vel[k * n + i];
+            im[k * n + i] = j;
+            pos[k * n + i] = i == 0 ? x : x - j * edges[k];
             if (kPos) {
-                const double r0 = gpr[k * n], r = gpr[k * n + i];
+                const double r0 = pr[k * n], r = pr[k * n + i];
                 const int jr = i == 0 ? 0 : (int)floor(((r - r0) / edges[k]) + 0.5);
-                pr[k * st + i] = i == 0 ? r : r - jr * edges[k];
+                pr[k * n + i] = i == 0 ? r : r - jr * edges[k];
             }
         }
     }
@@ -211,30 +507,30 @@ __global__ void __launch_bounds__(BT) k_rattle_body(PView<F64> p, double *__rest
     int it;
     for (it = 1; it <= RATTLE_MAX; ++it) {
         for (int v = 0; v < sv.n_levels; ++v) {
-            for (int l = start[v] + tid; l < start[v + 1]; l += BT) {
-                const int a = i1[l], b = i2[l];
-                const double ma = minv[a], mb = minv[b], dl = d[l];
+            for (int l = sv.start[v] + tid; l < sv.start[v + 1]; l += BT) {
+                const int a = sv.i1[l], b = sv.i2[l];
+                const double ma = minv[a], mb = minv[b], dl = sv.d[l];
                 double s[3];
-                for (int k = 0; k < 3; ++k) s[k] = pos[k * st + a] - pos[k * st + b];
+                for (int k = 0; k < 3; ++k) s[k] = pos[k * n + a] - pos[k * n + b];
                 if (kPos) {
                     double r[3];
-                    for (int k = 0; k < 3; ++k) r[k] = pr[k * st + a] - pr[k * st + b];
+                    for (int k = 0; k < 3; ++k) r[k] = pr[k * n + a] - pr[k * n + b];
                     const double rsq = dot3b(r, r), ssq = dot3b(s, s), rs = dot3b(r, s);
                     double g = rs - sqrt(rs * rs - rsq * (ssq - dl * dl));
                     g = g / ((dt * (ma + mb)) * rsq);
                     for (int k = 0; k < 3; ++k) {
-                        pos[k * st + a] = pos[k * st + a] - ((g * dt) * r[k]) * ma;
-                        pos[k * st + b] = pos[k * st + b] + ((g * dt) * r[k]) * mb;
-                        vel[k * st + a] = vel[k * st + a] - (g * r[k]) * ma;
-                        vel[k * st + b] = vel[k * st + b] + (g * r[k]) * mb;
+                        pos[k * n + a] = pos[k * n + a] - ((g * dt) * r[k]) * ma;
+                        pos[k * n + b] = pos[k * n + b] + ((g * dt) * r[k]) * mb;
+                        vel[k * n + a] = vel[k * n + a] - (g * r[k]) * ma;
+                        vel[k * n + b] = vel[k * n + b] + (g * r[k]) * mb;
                     }
                 } else {
                     double dv[3];
-                    for (int k = 0; k < 3; ++k) dv[k] = vel[k * st + a] - vel[k * st + b];
+                    for (int k = 0; k < 3; ++k) dv[k] = vel[k * n + a] - vel[k * n + b];
                     const double kk = dot3b(dv, s) / ((dl * dl) * (ma + mb));
                     for (int k = 0; k < 3; ++k) {
-                        vel[k * st + a] = vel[k * st + a] - (kk * s[k]) * ma;
-                        vel[k * st + b] = vel[k * st + b] + (kk * s[k]) * mb;
+                        vel[k * n + a] = vel[k * n + a] - (kk * s[k]) * ma;
+                        vel[k * n + b] = vel[k * n + b] + (kk * s[k]) * mb;
                     }
                 }
             }
@@ -242,16 +538,16 @@ __global__ void __launch_bounds__(BT) k_rattle_body(PView<F64> p, double *__rest
         }
         double err = 0;
         for (int l = tid; l < sv.n_links; l += BT) {
-            const int a = i1[l], b = i2[l];
+            const int a = sv.i1[l], b = sv.i2[l];
             double s[3];
-            for (int k = 0; k < 3; ++k) s[k] = pos[k * st + a] - pos[k * st + b];
+            for (int k = 0; k < 3; ++k) s[k] = pos[k * n + a] - pos[k * n + b];
             double e;
             if (kPos) {
-                e = fabs(sqrt(dot3b(s, s)) - d[l]);
+                e = fabs(sqrt(dot3b(s, s)) - sv.d[l]);
             } else {
                 double dv[3];
-                for (int k = 0; k < 3; ++k) dv[k] = vel[k * st + a] - vel[k * st + b];
-                e = fabs(dot3b(dv, s) / ((d[l] * d[l]) * (minv[a] + minv[b])));
+                for (int k = 0; k < 3; ++k) dv[k] = vel[k * n + a] - vel[k * n + b];
+                e = fabs(dot3b(dv, s) / ((sv.d[l] * sv.d[l]) * (minv[a] + minv[b])));
             }
             if (e > err) err = e;
         }
@@ -259,44 +555,57 @@ __global__ void __launch_bounds__(BT) k_rattle_body(PView<F64> p, double *__rest
         if (err < precision) break;
     }
     __syncthreads();
-    for (int64_t i = tid; i < n; i += BT)
-        for (int k = 0; k < 3; ++k) {
-            p.pos[k * n + i] = i == 0 ? pos[k * st] : pos[k * st + i] + im[k * st + i] * edges[k];
-            if (kLds) p.vel[k * n + i] = vel[k * st + i];
-            if (kPos && kLds) gpr[k * n + i] = pr[k * st + i];
-        }
+    for (int64_t i = tid + 1; i < n; i += BT)
+        for (int k = 0; k < 3; ++k) pos[k * n + i] = pos[k * n + i] + im[k * n + i] * edges[k];
     if (tid == 0) out[0] = it;
 }
 
-// elastic_network (md.f90:516-544): harmonic links, each particle's force receiving the
-// link terms in link order (the level schedule), the energy summed in link order
-__global__ void __launch_bounds__(BT) k_elastic_network(PView<F64> p, SchedView sv, double kk, double e0, double e1,
-                                                        double e2, double *__restrict__ terms,
+// elastic_network (md.f90:516-544).  The link forces are independent: made in parallel
+// first.  Each particle then adds the forces of its links in link order (the per-particle
+// lists of the schedule), which is the order of the reference's loop, and the energy
+// terms are summed in link order.  kLds: the terms and forces of the links in LDS.
+template <bool kLds>
+__global__ void __launch_bounds__(BT) k_elastic_network(PView<F64> p, const int32_t *__restrict__ la,
+                                                        const int32_t *__restrict__ lb, const double *__restrict__ dist,
+                                                        const int32_t *__restrict__ cstart,
+                                                        const int32_t *__restrict__ csr, int n_links, double kk,
+                                                        double e0, double e1, double e2, double *__restrict__ gterms,
                                                         double *__restrict__ out)
 {
+    extern __shared__ double smem[];
     const int64_t n = p.n;
     const int tid = threadIdx.x;
     const double L[3] = {e0, e1, e2};
-    for (int v = 0; v < sv.n_levels; ++v) {
-        for (int l = sv.start[v] + tid; l < sv.start[v + 1]; l += BT) {
-            const int a = sv.i1[l], b = sv.i2[l];
-            const double r0 = sv.d[l];
-            double x[3];
-            for (int k = 0; k < 3; ++k) x[k] = min_image(p.pos[k * n + a] - p.pos[k * n + b], L[k]);
-            const double r = sqrt(dot3b(x, x));
-            terms[sv.order[l]] = (kk * ((r - r0) * (r - r0))) / 2;
-            for (int k = 0; k < 3; ++k) {
-                const double f = ((-kk) * (r - r0)) * x[k] / r;
-                p.force[k * n + a] = p.force[k * n + a] + f;
-                p.force[k * n + b] = p.force[k * n + b] - f;
-            }
+    double *T = kLds ? smem : gterms;
+    double *F = T + n_links;
+    for (int l = tid; l < n_links; l += BT) {
+        const int a = la[l], b = lb[l];
+        const double r0 = dist[l];
+        double x[3];
+        for (int k = 0; k < 3; ++k) x[k] = min_image(p.pos[k * n + a] - p.pos[k * n + b], L[k]);
+        const double r = sqrt(dot3b(x, x));
+        T[l] = (kk * ((r - r0) * (r - r0))) / 2;
+        for (int k = 0; k < 3; ++k) F[3 * l + k] = ((-kk) * (r - r0)) * x[k] / r;
+    }
+    __syncthreads();
+    for (int64_t i = tid; i < n; i += BT) {
+        const int j0 = cstart[i], j1 = cstart[i + 1];
+        if (j0 == j1) continue;
+        double f[3] = {p.force[i], p.force[n + i], p.force[2 * n + i]};
+        for (int j = j0; j < j1; ++j) {
+            const uint32_t c = (uint32_t)csr[j];
+            const int l = (int)(c & 0x7FFFFFFFu);
+            if (c >> 31)
+                for (int k = 0; k < 3; ++k) f[k] = f[k] - F[3 * l + k];
+            else
+                for (int k = 0; k < 3; ++k) f[k] = f[k] + F[3 * l + k];
         }
-        __syncthreads();
+        for (int k = 0; k < 3; ++k) p.force[k * n + i] = f[k];
     }
     if (tid == 0) {
         double e = 0;
 #pragma unroll 8
-        for (int l = 0; l < sv.n_links; ++l) e = e + terms[l];
+        for (int l = 0; l < n_links; ++l) e = e + T[l];
         out[0] = e;
     }
 }
@@ -610,9 +919,28 @@ static int fetch_i32(const int32_t *dev, int32_t *host_out)
     return RMPCD_OK;
 }
 
-static SchedView sched_view(const Schedule *s)
+// which rattle runs: the pipelined one with the largest K whose body fits in LDS, else
+// the global one (K = 0)
+static int pick_k(Schedule &s, bool with_pr, const BlockSched **blk, size_t *lds)
 {
-    return SchedView{s->d_i1, s->d_i2, s->d_start, s->d_order, s->d_d, s->n_links, s->n_levels};
+    *blk = nullptr;
+    *lds = 0;
+    if (s.n >= 65536 || s.n_links >= (1 << 24) || s.n_links == 0) return RMPCD_OK;
+    const size_t limit = (size_t)lds_limit();
+    for (int K = KMAX; K >= 1; --K) {
+        // a cheap bound first: every entry could be its own level
+        if (pipe_lds_bytes(s.n, s.n_links, K, 0, with_pr) > limit) continue;
+        const BlockSched *b;
+        int rc = get_block(s, K, &b);
+        if (rc) return rc;
+        const size_t need = pipe_lds_bytes(s.n, s.n_links, K, b->n_glev, with_pr);
+        if (need <= limit) {
+            *blk = b;
+            *lds = need;
+            return RMPCD_OK;
+        }
+    }
+    return RMPCD_OK;
 }
 
 template <bool kPos>
@@ -625,21 +953,35 @@ static int rattle_body(rmpcd_particles_t *p, double *pos_rattle, const int32_t *
     RMPCD_CHECK(edges && (!kPos || pos_rattle), "%s: null argument", who);
     BodyMass bm;
     if ((rc = body_mass(mass, n_mass, bm))) return rc;
-    const Schedule *s;
-    if ((rc = get_schedule(links, distances, n_links, p->n, who, &s))) return rc;
     const int64_t n = p->n;
-    const size_t lds = body_lds_bytes(n, n_links, s->n_levels, kPos, true);
     int32_t *out = (int32_t *)workspace(WS_FLAG, 64);
     RMPCD_CHECK(out != nullptr, "%s: workspace allocation failed", who);
-    if (lds <= BODY_LDS) {
-        k_rattle_body<kPos, true><<<1, BT, lds, stream()>>>(view<F64>(p), pos_rattle, sched_view(s), bm, dt, edges[0],
-                                                             edges[1], edges[2], precision, out, nullptr, nullptr);
+    // no links: nothing to correct, the first sweep meets any positive precision, and
+    // the positions still go through the reference's unfold and fold
+    Schedule *s = nullptr;
+    if (n_links > 0 && (rc = get_schedule(links, distances, n_links, n, who, &s))) return rc;
+    const BlockSched *blk = nullptr;
+    size_t lds = 0;
+    if (s && (rc = pick_k(*s, kPos, &blk, &lds))) return rc;
+    if (blk) {
+        char *w = (char *)workspace(WS_AUX, 3 * (size_t)n * 4 + (size_t)blk->K * n * 64 + 512);
+        RMPCD_CHECK(w != nullptr, "%s: workspace allocation failed", who);
+        int32_t *gim = (int32_t *)w;
+        double *snap = (double *)(w + ((3 * (size_t)n * 4 + 255) & ~(size_t)255));
+        if (lds > 64 * 1024)
+            RMPCD_HIP(hipFuncSetAttribute((const void *)k_rattle_pipe<kPos>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        PipeView pv{blk->d_ent, blk->d_gs, s->d_lab, s->d_dist, s->n_links, blk->K * s->n_links, blk->n_glev, blk->K};
+        k_rattle_pipe<kPos><<<1, BT, lds, stream()>>>(view<F64>(p), pos_rattle, pv, bm, dt, edges[0], edges[1],
+                                                       edges[2], precision, out, gim, snap);
     } else {
         char *w = (char *)workspace(WS_AUX, (size_t)n * 8 + 3 * (size_t)n * 4 + 256);
         RMPCD_CHECK(w != nullptr, "%s: workspace allocation failed", who);
-        k_rattle_body<kPos, false><<<1, BT, 64 * 8, stream()>>>(view<F64>(p), pos_rattle, sched_view(s), bm, dt,
-                                                                 edges[0], edges[1], edges[2], precision, out,
-                                                                 (double *)w, (int32_t *)(w + (size_t)n * 8));
+        SchedView sv{nullptr, nullptr, nullptr, nullptr, 0, 0};
+        if (s) sv = SchedView{s->d_i1, s->d_i2, s->d_start, s->d_d, s->n_links, s->n_levels};
+        k_rattle_global<kPos><<<1, BT, 0, stream()>>>(view<F64>(p), pos_rattle, sv, bm, dt, edges[0], edges[1],
+                                                       edges[2], precision, out, (double *)w,
+                                                       (int32_t *)(w + (size_t)n * 8));
     }
     RMPCD_LAUNCHED();
     if (iterations) return fetch_i32(out, iterations);
@@ -675,16 +1017,28 @@ int rmpcd_elastic_network(rmpcd_particles_t *p, const int32_t *links, const doub
     int rc = check_body(p, links, distances, n_links, who, true);
     if (rc) return rc;
     RMPCD_CHECK(edges && energy, "%s: null argument", who);
-    const Schedule *s;
+    if (n_links == 0) {
+        *energy = 0;
+        return RMPCD_OK;
+    }
+    Schedule *s;
     if ((rc = get_schedule(links, distances, n_links, p->n, who, &s))) return rc;
-    // the energy terms land at their link's place in the caller's order
-    const size_t nl = (size_t)(n_links > 0 ? n_links : 1);
-    double *terms = (double *)workspace(WS_AUX, nl * 8 + 256);
+    const size_t tbytes = (size_t)n_links * 32;  // a term and a force per link
+    double *w = (double *)workspace(WS_AUX, tbytes + 256);
     double *h = (double *)pinned(64);
-    RMPCD_CHECK(terms && h, "%s: workspace allocation failed", who);
-    double *out = terms + nl;
-    k_elastic_network<<<1, BT, 0, stream()>>>(view<F64>(p), sched_view(s), k, edges[0], edges[1], edges[2], terms,
-                                               out);
+    RMPCD_CHECK(w && h, "%s: workspace allocation failed", who);
+    double *out = w + 4 * (size_t)n_links;
+    if (tbytes <= (size_t)lds_limit()) {
+        if (tbytes > 64 * 1024)
+            RMPCD_HIP(hipFuncSetAttribute((const void *)k_elastic_network<true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)tbytes));
+        k_elastic_network<true><<<1, BT, tbytes, stream()>>>(view<F64>(p), s->d_la, s->d_lb, s->d_dist, s->d_cstart,
+                                                             s->d_csr, n_links, k, edges[0], edges[1], edges[2], w,
+                                                             out);
+    } else {
+        k_elastic_network<false><<<1, BT, 0, stream()>>>(view<F64>(p), s->d_la, s->d_lb, s->d_dist, s->d_cstart,
+                                                          s->d_csr, n_links, k, edges[0], edges[1], edges[2], w, out);
+    }
     RMPCD_LAUNCHED();
     RMPCD_HIP(hipMemcpyAsync(h, out, sizeof(double), hipMemcpyDeviceToHost, stream()));
     RMPCD_HIP(hipStreamSynchronize(stream()));

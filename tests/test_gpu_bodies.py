@@ -56,12 +56,14 @@ def fold(x, edges=EDGES):
     return x - np.floor(x / edges) * edges
 
 
-# 12 and 300 beads run from LDS, 900 beads (3721 links) from global memory
+# up to 900 beads the body and its schedule fit in LDS (12 sweeps overlapped for 300
+# beads, 1 or 2 for 900); 1400 beads (5800 links) run from global memory
 @pytest.mark.parametrize("n,radius,centre,edges", [
     (12, 2.0, (8.0, 6.0, 10.0), EDGES),
     (60, 2.0, (15.5, 0.3, 19.6), EDGES),
     (300, 4.5, (3.0, 6.0, 10.0), EDGES * 2),
     (900, 8.5, (39.0, 20.0, 1.0), np.array([40.0, 40.0, 40.0])),
+    (1400, 10.5, (1.0, 20.0, 39.5), np.array([44.0, 44.0, 44.0])),
 ])
 def test_rattle_body_pos_vel(gpu, n, radius, centre, edges):
     ps, links, d, rng = sphere_body(n, radius, centre, 11 + n)
