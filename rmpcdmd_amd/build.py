@@ -36,7 +36,8 @@ FORTRAN = os.path.join(HERE, "fortran")
 FBIN = os.path.join(FORTRAN, "bin")
 FLIB = os.path.join(FORTRAN, "librmpcd_fortran.so")
 AMDFLANG = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
-FPROGRAMS = ["fortran_cells", "fortran_fluid", "fortran_dimer", "fortran_walls", "fortran_bodies"]
+FPROGRAMS = ["fortran_cells", "fortran_fluid", "fortran_dimer", "fortran_walls", "fortran_bodies",
+             "fortran_params", "single_dimer_pbc"]
 
 
 def _run(cmd, what):
@@ -51,7 +52,7 @@ def build_fortran(verbose=False):
     os.makedirs(mod, exist_ok=True)
     os.makedirs(FBIN, exist_ok=True)
     objs = []
-    for src in ("rmpcd_capi.f90", "rmpcdmd.f90"):
+    for src in ("rmpcd_capi.f90", "rmpcdmd.f90", "parsetext.f90"):
         out = os.path.join(OBJ, src.replace(".f90", ".o"))
         _run([AMDFLANG, "-c", "-O2", "-fPIC", "-J", mod, os.path.join(FORTRAN, src), "-o", out], src)
         objs.append(out)

@@ -27,6 +27,28 @@ module rmpcd_capi
      type(c_ptr) :: is_md = c_null_ptr, is_reac = c_null_ptr
   end type rmpcd_cells_t
 
+  integer(c_int32_t), parameter :: RMPCD_SIM_REFERENCE = 0, RMPCD_SIM_FUSED = 1
+
+  !> rmpcd_dimer_config_t: the device-resident single-dimer program
+  type, bind(c) :: rmpcd_dimer_config_t
+     integer(c_int32_t) :: L(3) = 0, rho = 0, n_md = 0, form = 0, mode = 1, bulk_rmpcd = 0
+     real(c_double) :: T = 0, tau = 0, sigma_C = 0, sigma_N = 0, d = 0
+     real(c_double) :: epsilon_C(2) = 0, epsilon_N(2) = 0
+     real(c_double) :: epsilon_CC = 0, epsilon_NC = 0, epsilon_NN = 0
+     real(c_double) :: prob = 0, bulk_rate = 0, skin = 0
+     integer(c_int32_t) :: nmax_list = 0, n_replicas = 0
+     integer(c_int64_t) :: seed = 0
+     type(c_ptr) :: replica_L = c_null_ptr
+     integer(c_int32_t) :: stream_blocks = 0, sort_blocks = 0, lazy = 0, pad_ = 0
+  end type rmpcd_dimer_config_t
+
+  type, bind(c) :: rmpcd_dimer_stats_t
+     integer(c_int64_t) :: n_solvent, md_steps, cycles, n_extra_sorting, n_replicas, n_solvent_total
+     real(c_double) :: e1, e2, dt, mass(2)
+     integer(c_int32_t) :: nmax, pad_
+     type(c_ptr) :: list, nlist, pos_old, colloid_pos_old
+  end type rmpcd_dimer_stats_t
+
   !> rmpcd_rigid_body_t, in device memory
   type, bind(c) :: rmpcd_rigid_body_t
      integer(c_int32_t) :: i_start, i_stop
@@ -301,6 +323,29 @@ module rmpcd_capi
        type(rmpcd_particles_t) :: p
        real(c_double), value :: dt
      end function rmpcd_rigid_body_vv2
+     integer(c_int) function rmpcd_dimer_create(cfg, sim) bind(c)
+       import :: c_int, c_ptr, rmpcd_dimer_config_t
+       type(rmpcd_dimer_config_t), intent(in) :: cfg
+       type(c_ptr) :: sim
+     end function rmpcd_dimer_create
+     integer(c_int) function rmpcd_dimer_run(sim, n_cycles) bind(c)
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: sim
+       integer(c_int32_t), value :: n_cycles
+     end function rmpcd_dimer_run
+     integer(c_int) function rmpcd_dimer_state(sim, replica, solvent, colloids, box, cells, stats) bind(c)
+       import :: c_int, c_ptr, c_int32_t, rmpcd_particles_t, rmpcd_box_t, rmpcd_cells_t, rmpcd_dimer_stats_t
+       type(c_ptr), value :: sim
+       integer(c_int32_t), value :: replica
+       type(rmpcd_particles_t) :: solvent, colloids
+       type(rmpcd_box_t) :: box
+       type(rmpcd_cells_t) :: cells
+       type(rmpcd_dimer_stats_t) :: stats
+     end function rmpcd_dimer_state
+     integer(c_int) function rmpcd_dimer_destroy(sim) bind(c)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: sim
+     end function rmpcd_dimer_destroy
      integer(c_int) function rmpcd_lj_params_init(lj, n1, n2, eps, sigma, cut) bind(c)
        import :: c_int, c_double, c_int32_t, rmpcd_lj_t
        type(rmpcd_lj_t) :: lj

@@ -5,6 +5,8 @@ and bond length)."""
 import os
 import subprocess
 
+import numpy as np
+
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -20,3 +22,29 @@ def test_fortran_program(gpu, prog):
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "%s ok" % prog in r.stdout
+
+
+def test_single_dimer_pbc_driver(gpu, tmp_path):
+    """the single_dimer_pbc driver on the reference's test/dimer_test.parameters (L=24,
+    N_MD=125, 20 + 200 loops): the observables table has a row per sampled loop, the
+    temperature holds at T, the centre-of-mass velocity stays at its start, and the
+    colloid mass is the reference's printed 1130.9733867645264 (doc/run.rst:103; the
+    driver's single-precision pi)"""
+    exe = os.path.join(BIN, "single_dimer_pbc")
+    fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dimer_test.parameters")
+    out = tmp_path / "dimer.txt"
+    print("single_dimer_pbc on dimer_test.parameters", flush=True)
+    r = subprocess.run([exe, fixture, str(out), "42"], capture_output=True, text=True, timeout=100)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    head = [ln for ln in out.read_text().splitlines() if ln.startswith("#")]
+    mass = [float(x) for ln in head if ln.startswith("# mass") for x in ln.split()[2:]]
+    assert mass and all(abs(m / 1130.9733867645264 - 1) < 1e-13 for m in mass)
+    rows = np.loadtxt(out)
+    assert rows.shape == (201, 11)
+    assert (rows[:, 0] == np.arange(20, 221)).all()
+    T = rows[:, 1]
+    assert abs(T.mean() - 1) < 0.03 and np.all(np.abs(T - 1) < 0.1)
+    v_com = rows[:, 5:8]
+    assert np.max(np.abs(v_com - v_com[0])) < 1e-10
+    assert np.max(np.abs(v_com)) < 1e-3
