@@ -358,7 +358,9 @@ typedef struct rmpcd_dimer_config {
                                   moved only when read (at a sort or an exact skin test), by
                                   replaying their per-step additions; the same trajectory
                                   bit for bit, without the per-step pass over all solvent */
-    int32_t pad_;
+    int32_t extra_sort_blocks; /* fused mode launch shape, 0: sort_blocks: workgroups of the
+                                  key, place and rank passes of the sorts that may follow
+                                  each MD step (they return at once when none is due) */
 } rmpcd_dimer_config_t;
 typedef struct rmpcd_dimer_stats {
     int64_t n_solvent, md_steps, cycles, n_extra_sorting;  /* of the replica asked for */

@@ -4,7 +4,8 @@ For a workload (box size, replica count or the mixed-size batch, form) it times 
 collision cycles for each candidate launch shape and keeps the fastest, one parameter
 at a time (coordinate descent, two passes):
     stream_blocks  stream-role workgroups per replica of the step kernel
-    sort_blocks    workgroups of the key, place and gather passes of a sort
+    sort_blocks    workgroups of the key, place and gather passes of the collision sort
+    extra_sort_blocks  the same for the sorts that may follow each MD step
 The result is stored in a JSON table keyed by the workload; bench.py applies the entry
 of its workload when the table exists.
 
@@ -20,7 +21,8 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-CANDIDATES = {"stream_blocks": [2048, 4096, 8192, 16384], "sort_blocks": [2048, 4096, 8192, 16384, 32768]}
+CANDIDATES = {"stream_blocks": [2048, 4096, 8192, 16384], "sort_blocks": [2048, 4096, 8192, 16384, 32768],
+              "extra_sort_blocks": [256, 512, 1024, 2048, 4096]}
 
 
 def workload_key(form, L, replicas, mixed=False):

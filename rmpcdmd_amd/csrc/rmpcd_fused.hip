@@ -1852,14 +1852,17 @@ template <class F>
 static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int forced)
 {
     const unsigned Rr = (unsigned)s.R;
+    // the sorts that may follow each MD step get their own grid: when none is due their
+    // blocks only read the plan and return, and fewer blocks return sooner
+    const int nb = forced ? s.sort_blocks : s.extra_blocks;
     k_fz_plan<F><<<1, FT, 0, st>>>(b, forced, s.R);
-    k_fz_key<F><<<s.sort_blocks, FT, 0, st>>>(b, p, forced, s.R);
+    k_fz_key<F><<<nb, FT, 0, st>>>(b, p, forced, s.R);
     k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, s.R);
-    k_fz_place<F><<<s.sort_blocks, FT, 0, st>>>(b, forced, s.R);
+    k_fz_place<F><<<nb, FT, 0, st>>>(b, forced, s.R);
     if (forced)
-        k_fz_gather<F><<<s.sort_blocks, FT, 0, st>>>(b, p, s.R);
+        k_fz_gather<F><<<nb, FT, 0, st>>>(b, p, s.R);
     else
-        k_fz_rank<F><<<s.sort_blocks, FT, 0, st>>>(b, s.R);
+        k_fz_rank<F><<<nb, FT, 0, st>>>(b, s.R);
     k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, s.R);
 }
 
@@ -2032,8 +2035,9 @@ int fused_prepare(DimerSim &s)
     // every per-block partial of a replica must fit its PS slots
     s.stream_blocks = s.cfg.stream_blocks > 0 ? s.cfg.stream_blocks : STREAM_BLOCKS;
     s.sort_blocks = s.cfg.sort_blocks > 0 ? s.cfg.sort_blocks : SORT_BLOCKS;
-    RMPCD_CHECK(s.stream_blocks <= STREAM_BLOCKS && s.sort_blocks <= 65536,
-                "dimer: stream_blocks must be at most 16384 and sort_blocks at most 65536");
+    s.extra_blocks = s.cfg.extra_sort_blocks > 0 ? s.cfg.extra_sort_blocks : s.sort_blocks;
+    RMPCD_CHECK(s.stream_blocks <= STREAM_BLOCKS && s.sort_blocks <= 65536 && s.extra_blocks <= 65536,
+                "dimer: stream_blocks must be at most 16384 and sort_blocks, extra_sort_blocks at most 65536");
     RMPCD_CHECK((s.ncmax_rep + CT - 1) / CT <= PS && (int64_t)member_blocks(s) * MPV + STREAM_BLOCKS <= PS &&
                     member_blocks(s) <= MAX_MEMBER_BLOCKS && s.n_stencil <= MAX_STENCIL &&
                     (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE <= MAX_TILES,

@@ -192,16 +192,22 @@ def test_many_extra_sorts_per_cycle(gpu, form):
     ref = SingleDimer(mode=capi.SIM_REFERENCE, **kw)
     fus = SingleDimer(mode=capi.SIM_FUSED, **kw)
     lazy = SingleDimer(mode=capi.SIM_FUSED, lazy=1, **kw)
+    # odd grids for the sort passes (units are split over whatever grid there is): the
+    # same trajectory bit for bit
+    odd = SingleDimer(mode=capi.SIM_FUSED, sort_blocks=96, extra_sort_blocks=7, **kw)
     for cycle in range(2):
         ref.run(1)
         fus.run(1)
         lazy.run(1)
+        odd.run(1)
         for r in range(len(Ls)):
-            a, b, c = ref.download(r), fus.download(r), lazy.download(r)
+            a, b, c, d = ref.download(r), fus.download(r), lazy.download(r), odd.download(r)
             compare(a, b, "replica %d cycle %d" % (r, cycle), 1e-12 if form == capi.FP64 else 1e-5)
             compare(b, c, "lazy replica %d cycle %d" % (r, cycle), rtol=0.0)
+            compare(b, d, "odd grids replica %d cycle %d" % (r, cycle), rtol=0.0)
     for r in range(len(Ls)):
         assert fus.download(r)["n_extra_sorting"] >= 6, "replica %d" % r
     ref.close()
     fus.close()
     lazy.close()
+    odd.close()
