@@ -92,6 +92,28 @@ def test_rattle_body_pos_vel(gpu, n, radius, centre, edges):
     assert same_bits(g.host("vel"), want.vel)
 
 
+def test_rattle_body_without_links(gpu):
+    """no links: the first sweep passes, and the positions still go through the
+    reference's unfold next to particle 1 and fold back (md.f90:343-353, 400-402)"""
+    ps, _, _, rng = sphere_body(12, 2.0, (15.9, 0.1, 10.0), 7)
+    ps.pos[:] = fold(ps.pos)
+    links = np.zeros((0, 2), np.int32)
+    d = np.zeros(0)
+    mass = np.array([1.0, 2.0])
+    want = ps.copy()
+    assert O.rattle_body_vel(want.pos, want.vel, want.species, mass, links, d, 0.01, EDGES, 1e-12) == 1
+    g = to_device(ps)
+    it = C.c_int32(0)
+    m = (C.c_double * 2)(*mass)
+    capi.call("rmpcd_rattle_body_vel", g.struct(), None, None, 0, m, 2, 0.01, capi.d3(EDGES), 1e-12, C.byref(it))
+    assert it.value == 1
+    assert same_bits(g.host("pos"), want.pos)
+    assert same_bits(g.host("vel"), want.vel)
+    e = C.c_double(1.0)
+    capi.call("rmpcd_elastic_network", g.struct(), None, None, 0, 3.0, capi.d3(EDGES), C.byref(e))
+    assert e.value == 0.0
+
+
 def test_body_link_out_of_range(gpu):
     ps, links, d, _ = sphere_body(12, 2.0, (8.0, 6.0, 10.0), 5)
     links[3, 1] = 12
