@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-single", action="store_true", help="skip the single-replica run")
     ap.add_argument("--no-lazy", action="store_true", help="skip the lazy-streaming run")
     ap.add_argument("--no-bodies", action="store_true", help="skip the linked-colloid routines")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the wall, forced-flow, thermostat and energy routines")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the 128-replica run with L drawn from {24, 32, 40, 48}")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -252,6 +254,100 @@ def linked_bodies(with_cpu):
     return out
 
 
+def mpcd_variants(with_cpu, L=64, reps=20):
+    """the wall, forced-flow, thermostat and energy routines (mpcd.f90:144-160, 193-337,
+    459-511, 646-700, 748-801) on one L^3 box at rho = 10 (walls: z walls, periodic x, y),
+    each form: mean time per call over `reps` calls (HIP events on the library's stream),
+    algorithmic bytes per call and the fraction of the HBM peak.  Per particle:
+    SRD-like cell passes read and write the velocities and read the species (6w + s),
+    MPC-AT also reads the id (Threefry normals keyed by particle), the endothermic
+    reaction also writes species, the forced-flow stream reads position and velocity and
+    writes the position (9w); per cell: count and start (8 B)."""
+    import numpy as np
+    import torch
+
+    from rmpcdmd_amd import capi
+    from rmpcdmd_amd.system import CellSet, ParticleSet
+    n = 10 * L ** 3
+    out = {"L": L, "n": n}
+    for name, form in (("fp32", capi.FP32), ("fp64", capi.FP64)):
+        w, sb = (4, 1) if form == capi.FP32 else (8, 4)
+        gen = torch.Generator(device="cuda").manual_seed(11)
+        res = {}
+        for walls in (False, True):
+            src, dst = ParticleSet(n, form), ParticleSet(n, form)
+            src.pos.copy_((torch.rand(3, n, generator=gen, device="cuda", dtype=torch.float64) * L).to(src.pos.dtype))
+            src.vel.copy_(torch.randn(3, n, generator=gen, device="cuda", dtype=torch.float64).to(src.vel.dtype))
+            sp = 1 + (torch.rand(n, generator=gen, device="cuda") < 0.3).to(src.species.dtype)
+            src.species.copy_(sp)
+            cs = CellSet((L, L, L), 1.0, walls)
+            if walls:
+                cs.box.bc[0] = 1  # periodic x and y, bounce-back z (RMPCD_*_BC)
+                cs.box.bc[1] = 1
+                cs.box.bc[2] = 2
+                cs.set_origin((-0.3, -0.6, 0.0))
+            else:
+                cs.set_origin((-0.3, -0.6, -0.9))
+            i2i = torch.zeros(n, dtype=torch.int32, device="cuda")
+            capi.call("rmpcd_sort", cs.boxp(), src.struct(), dst.struct(), C.c_void_p(i2i.data_ptr()), cs.cellsp(), 0)
+            cs.is_reac.fill_(1)
+            nc = cs.box.ncells
+            if walls:
+                calls = {
+                    "wall_mpcd_step": (lambda k: capi.call("rmpcd_wall_mpcd_step", cs.boxp(), cs.cellsp(), dst.struct(),
+                                                           0, None, None, 0, 1.0, 1, 0, 0.0, None, None, None, 7, k),
+                                       (6 * w + sb) * n + 8 * nc),
+                    "mpcd_stream_xforce_yzwall": (lambda k: capi.call("rmpcd_mpcd_stream_xforce_yzwall", cs.boxp(),
+                                                                      dst.struct(), 1e-4, 0.01), 9 * w * n),
+                }
+            else:
+                calls = {
+                    "mpcat_step": (lambda k: capi.call("rmpcd_mpcat_step", cs.boxp(), cs.cellsp(), dst.struct(), None, 7,
+                                                       k, 1.0, 1), (6 * w + sb + 4) * n + 8 * nc),
+                    "rescale_cells": (lambda k: capi.call("rmpcd_rescale_cells", cs.boxp(), cs.cellsp(), dst.struct(),
+                                                          1.0), (6 * w + sb) * n + 8 * nc),
+                    "bulk_reaction_endothermic": (lambda k: capi.call("rmpcd_bulk_reaction_endothermic", cs.boxp(),
+                                                                      cs.cellsp(), dst.struct(), 2, 1, 1e-3, 0.5, 0.01,
+                                                                      None, 7, k), (6 * w + 2 * sb) * n + 9 * nc),
+                }
+            for key, (fn, alg) in calls.items():
+                fn(0)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for k in range(reps):
+                    fn(k + 1)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                gbs = alg / (ms * 1e-3) / 1e9
+                res[key] = {"ms": ms, "algorithmic_bytes": alg, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+            del src, dst, cs
+        out[name] = res
+    if with_cpu:
+        from oracle import oracle as O
+        Lc = 32
+        rng = np.random.default_rng(3)
+        c = O.cells_init((Lc, Lc, Lc))
+        O.set_origin(c, (-0.3, -0.6, -0.9))
+        ps = O.PState(10 * Lc ** 3)
+        ps.pos[:] = rng.random((ps.n, 3)) * Lc
+        ps.vel[:] = rng.normal(size=(ps.n, 3))
+        ps.species[:] = 1 + (rng.random(ps.n) < 0.3)
+        srt, _, cnt, st = O.sort(c, ps)
+        cpu = {}
+        t = time.perf_counter()
+        O.rescale_cells(c, cnt, st, srt.vel, srt.species, 1.0)
+        cpu["rescale_cells_ns_per_particle"] = (time.perf_counter() - t) / ps.n * 1e9
+        g = rng.normal(size=srt.vel.shape)
+        t = time.perf_counter()
+        O.mpcat_step(c, cnt, st, srt.vel, srt.species, g, 1.0, hydro=True)
+        cpu["mpcat_step_ns_per_particle"] = (time.perf_counter() - t) / ps.n * 1e9
+        out["cpu_baseline"] = {"kind": "port", "cores": 1, "sample": "the oracle (C, one thread) on an L=32 box "
+                               "(327,680 particles), one call each", **cpu}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -342,6 +438,8 @@ def main():
     line["step_frac_of_hbm_peak"] = line["step_effective_GBs"] / HBM_PEAK_GBS
     if not a.no_bodies:
         line["linked_bodies"] = linked_bodies(not a.no_cpu)
+    if not a.no_variants:
+        line["mpcd_variants"] = mpcd_variants(not a.no_cpu)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(a.L, a.n_md)
     print(json.dumps(line))

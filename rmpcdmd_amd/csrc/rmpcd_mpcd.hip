@@ -60,7 +60,34 @@ __device__ inline void stage_tile(const PView<F> &p, const int32_t *start, const
     __syncthreads();
     t.base = b;
     t.count = (e - b <= TILE_CAP) ? (int)(e - b) : 0;
-    for (int q = threadIdx.x; q < t.count; q += blockDim.x) {
+    // four slots per thread at a time: their loads are all in flight before the first
+    // LDS store waits for one
+    const int nt = blockDim.x;
+    int q = threadIdx.x;
+    for (; q + 3 * nt < t.count; q += 4 * nt) {
+        typename F::real x[4], y[4], z[4];
+        typename F::small sp[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t g = b + q + u * nt;
+            if (load_vel) {
+                x[u] = p.vel[g];
+                y[u] = p.vel[p.n + g];
+                z[u] = p.vel[2 * p.n + g];
+            }
+            sp[u] = p.species[g];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (load_vel) {
+                t.vx[q + u * nt] = x[u];
+                t.vy[q + u * nt] = y[u];
+                t.vz[q + u * nt] = z[u];
+            }
+            t.sp[q + u * nt] = sp[u];
+        }
+    }
+    for (; q < t.count; q += nt) {
         if (load_vel) {
             t.vx[q] = p.vel[b + q];
             t.vy[q] = p.vel[p.n + b + q];
@@ -73,6 +100,59 @@ __device__ inline void stage_tile(const PView<F> &p, const int32_t *start, const
 
 // SRD rotation (mpcd.f90:120-180, default or given alpha); axes injected per cell or
 // drawn from Threefry keyed by (seed, step, cell)
+// one cell of the SRD collision on arrays vx, vy, vz, sp (the LDS tile or global memory)
+template <class R, class S>
+__device__ inline void srd_cell(R *vx, R *vy, R *vz, const S *sp, int n, int c, int ncells,
+                                const double *axes, uint64_t seed, uint64_t step, double sin_a, double cos_a)
+{
+    Acc3<R> acc;
+    int n_eff = 0;
+    for (int i = 0; i < n; ++i) {
+        if ((int)sp[i] > 0) {
+            acc.add(0, vx[i]);
+            acc.add(1, vy[i]);
+            acc.add(2, vz[i]);
+            ++n_eff;
+        }
+    }
+    if (n_eff <= 1) return;
+    const R u0 = acc.s[0] / (R)n_eff, u1 = acc.s[1] / (R)n_eff, u2 = acc.s[2] / (R)n_eff;
+    double v0, v1, v2;
+    if (axes) {
+        v0 = axes[c];
+        v1 = axes[ncells + c];
+        v2 = axes[2 * ncells + c];
+    } else {
+        Sphere sph = rand_sphere_threefry(seed, STREAM_SRD, step, (uint64_t)c);
+        v0 = sph.n[0];
+        v1 = sph.n[1];
+        v2 = sph.n[2];
+    }
+    const double omc = 1 - cos_a;
+    // omega(r,k) as filled column-major by mpcd.f90:163-171
+    const R o00 = (R)(cos_a + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sin_a),
+            o20 = (R)((v0 * v2) * omc + v1 * sin_a);
+    const R o01 = (R)((v1 * v0) * omc + v2 * sin_a), o11 = (R)(cos_a + (v1 * v1) * omc),
+            o21 = (R)((v1 * v2) * omc - v0 * sin_a);
+    const R o02 = (R)((v2 * v0) * omc - v1 * sin_a), o12 = (R)((v2 * v1) * omc + v0 * sin_a),
+            o22 = (R)(cos_a + (v2 * v2) * omc);
+    for (int i = 0; i < n; ++i) {
+        const R w0 = vx[i] - u0, w1 = vy[i] - u1, w2 = vz[i] - u2;
+        R r0 = o00 * w0;
+        r0 = r0 + o01 * w1;
+        r0 = r0 + o02 * w2;
+        R r1 = o10 * w0;
+        r1 = r1 + o11 * w1;
+        r1 = r1 + o12 * w2;
+        R r2 = o20 * w0;
+        r2 = r2 + o21 * w1;
+        r2 = r2 + o22 * w2;
+        vx[i] = u0 + r0;
+        vy[i] = u1 + r1;
+        vz[i] = u2 + r2;
+    }
+}
+
 template <class F>
 __global__ void __launch_bounds__(CELL_THREADS)
     k_srd(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
@@ -87,70 +167,16 @@ __global__ void __launch_bounds__(CELL_THREADS)
     CellTile<F> t{lx, ly, lz, lsp, 0, 0};
     stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
     int c = c0 + threadIdx.x;
-    if (c < c1) {
-        int n = count[c];
-        if (n > 1) {
-            int64_t s = start[c];
-            bool in_lds = t.count > 0;
-            int64_t o = s - t.base;
-            auto V = [&](int k, int64_t i) -> R {
-                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
-                return p.vel[k * p.n + s + i];
-            };
-            auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
-            Acc3<R> acc;
-            int n_eff = 0;
-            for (int i = 0; i < n; ++i) {
-                if (S(i) > 0) {
-                    acc.add(0, V(0, i));
-                    acc.add(1, V(1, i));
-                    acc.add(2, V(2, i));
-                    ++n_eff;
-                }
-            }
-            if (n_eff > 1) {
-                R u0 = acc.s[0] / (R)n_eff, u1 = acc.s[1] / (R)n_eff, u2 = acc.s[2] / (R)n_eff;
-                double v0, v1, v2;
-                if (axes) {
-                    v0 = axes[c];
-                    v1 = axes[ncells + c];
-                    v2 = axes[2 * ncells + c];
-                } else {
-                    Sphere sp = rand_sphere_threefry(seed, STREAM_SRD, step, (uint64_t)c);
-                    v0 = sp.n[0];
-                    v1 = sp.n[1];
-                    v2 = sp.n[2];
-                }
-                double omc = 1 - cos_a;
-                // omega(r,k) as filled column-major by mpcd.f90:163-171
-                R o00 = (R)(cos_a + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sin_a),
-                  o20 = (R)((v0 * v2) * omc + v1 * sin_a);
-                R o01 = (R)((v1 * v0) * omc + v2 * sin_a), o11 = (R)(cos_a + (v1 * v1) * omc),
-                  o21 = (R)((v1 * v2) * omc - v0 * sin_a);
-                R o02 = (R)((v2 * v0) * omc - v1 * sin_a), o12 = (R)((v2 * v1) * omc + v0 * sin_a),
-                  o22 = (R)(cos_a + (v2 * v2) * omc);
-                for (int i = 0; i < n; ++i) {
-                    R w0 = V(0, i) - u0, w1 = V(1, i) - u1, w2 = V(2, i) - u2;
-                    R r0 = o00 * w0;
-                    r0 = r0 + o01 * w1;
-                    r0 = r0 + o02 * w2;
-                    R r1 = o10 * w0;
-                    r1 = r1 + o11 * w1;
-                    r1 = r1 + o12 * w2;
-                    R r2 = o20 * w0;
-                    r2 = r2 + o21 * w1;
-                    r2 = r2 + o22 * w2;
-                    if (in_lds) {
-                        lx[o + i] = u0 + r0;
-                        ly[o + i] = u1 + r1;
-                        lz[o + i] = u2 + r2;
-                    } else {
-                        p.vel[s + i] = u0 + r0;
-                        p.vel[p.n + s + i] = u1 + r1;
-                        p.vel[2 * p.n + s + i] = u2 + r2;
-                    }
-                }
-            }
+    // the LDS tile and global memory get separate calls, so each is addressed directly
+    if (c < c1 && count[c] > 1) {
+        const int n = count[c];
+        const int64_t s = start[c];
+        if (t.count > 0) {
+            const int o = (int)(s - t.base);
+            srd_cell(lx + o, ly + o, lz + o, lsp + o, n, c, ncells, axes, seed, step, sin_a, cos_a);
+        } else {
+            srd_cell(p.vel + s, p.vel + p.n + s, p.vel + 2 * p.n + s, p.species + s, n, c, ncells, axes, seed, step,
+                     sin_a, cos_a);
         }
     }
     __syncthreads();
@@ -166,96 +192,169 @@ __global__ void __launch_bounds__(CELL_THREADS)
 // the cell's previous mean velocity is restored by adding (mean before - mean after).
 // Normals injected per particle slot (3 planes of n) or drawn from Threefry keyed by
 // (seed, step, particle id).
+constexpr int MPCAT_THREADS = 512;  // a workgroup of k_mpcat: CELL_THREADS cells, the draws over all threads
+
 template <class F>
-__global__ void __launch_bounds__(CELL_THREADS)
+__global__ void __launch_bounds__(MPCAT_THREADS)
     k_mpcat(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count, PView<F> p,
             const double *__restrict__ normals, uint64_t seed, uint64_t step, double t_factor, int hydro)
 {
     using R = typename F::real;
     __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
     __shared__ typename F::small lsp[TILE_CAP];
-    int c0 = blockIdx.x * CELL_THREADS;
-    int c1 = min(c0 + CELL_THREADS, ncells);
+    __shared__ uint8_t lcell[TILE_CAP];  // cell (in the workgroup) of each staged slot
+    __shared__ R lu[3][CELL_THREADS];    // cell means
+    __shared__ int lneff[CELL_THREADS];  // n_eff of a cell that gets new velocities, else 0
+    const int tid = threadIdx.x;
+    const int c0 = blockIdx.x * CELL_THREADS;
+    const int c1 = min(c0 + CELL_THREADS, ncells);
     CellTile<F> t{lx, ly, lz, lsp, 0, 0};
     stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
-    int c = c0 + threadIdx.x;
-    if (c < c1) {
-        int n = count[c];
-        if (n > 1) {
-            int64_t s = start[c];
-            bool in_lds = t.count > 0;
-            int64_t o = s - t.base;
-            auto V = [&](int k, int64_t i) -> R {
-                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
-                return p.vel[k * p.n + s + i];
-            };
-            auto W = [&](int k, int64_t i, R v) {
-                // direct LDS stores (a pointer chosen among the arrays would be a generic
-                // one, and its stores are not ordered with the LDS loads of V)
-                if (!in_lds)
-                    p.vel[k * p.n + s + i] = v;
-                else if (k == 0)
-                    lx[o + i] = v;
-                else if (k == 1)
-                    ly[o + i] = v;
-                else
-                    lz[o + i] = v;
-            };
-            auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
-            Acc3<R> acc;
-            int n_eff = 0;
-            for (int i = 0; i < n; ++i)
-                if (S(i) > 0) {
-                    acc.add(0, V(0, i));
-                    acc.add(1, V(1, i));
-                    acc.add(2, V(2, i));
-                    ++n_eff;
-                }
-            if (n_eff > 1) {
-                const R u[3] = {acc.s[0] / (R)n_eff, acc.s[1] / (R)n_eff, acc.s[2] / (R)n_eff};
-                // the new velocities are recomputed in the second pass rather than read
-                // back, so no slot is read after it is written
-                auto draw = [&](int i, double g[3]) {
-                    if (normals) {
-                        for (int k = 0; k < 3; ++k) g[k] = normals[k * p.n + s + i];
-                    } else {
-                        normal3_threefry(seed, STREAM_THERMO, step, (uint64_t)p.id[s + i], g);
-                    }
-                };
-                R d[3] = {0, 0, 0};
-                if (hydro) {
-                    Acc3<R> vir;
-                    for (int i = 0; i < n; ++i) {
-                        if (S(i) <= 0) continue;
-                        double g[3];
-                        draw(i, g);
-                        for (int k = 0; k < 3; ++k) vir.add(k, (R)(g[k] * t_factor));
-                    }
-                    for (int k = 0; k < 3; ++k) d[k] = u[k] - vir.s[k] / (R)n_eff;
-                }
-                for (int i = 0; i < n; ++i) {
-                    if (S(i) <= 0) continue;
-                    double g[3];
-                    draw(i, g);
-                    for (int k = 0; k < 3; ++k) {
-                        const R v = (R)(g[k] * t_factor);
-                        W(k, i, hydro ? v + d[k] : v);
+    const int c = c0 + tid;
+    auto draw = [&](int64_t slot, double g[3]) {
+        if (normals) {
+            for (int k = 0; k < 3; ++k) g[k] = normals[k * p.n + slot];
+        } else {
+            normal3_threefry(seed, STREAM_THERMO, step, (uint64_t)p.id[slot], g);
+        }
+    };
+    if (t.count > 0) {
+        // 1: each cell's mean over species > 0, in slot order (LDS)
+        if (tid < CELL_THREADS) {
+            lneff[tid] = 0;
+            if (c < c1) {
+                const int n = count[c];
+                const int o = (int)(start[c] - t.base);
+                for (int i = 0; i < n; ++i) lcell[o + i] = (uint8_t)tid;
+                if (n > 1) {
+                    Acc3<R> acc;
+                    int n_eff = 0;
+                    for (int i = 0; i < n; ++i)
+                        if ((int)lsp[o + i] > 0) {
+                            acc.add(0, lx[o + i]);
+                            acc.add(1, ly[o + i]);
+                            acc.add(2, lz[o + i]);
+                            ++n_eff;
+                        }
+                    if (n_eff > 1) {
+                        for (int k = 0; k < 3; ++k) lu[k][tid] = acc.s[k] / (R)n_eff;
+                        lneff[tid] = n_eff;
                     }
                 }
             }
         }
+        __syncthreads();
+        // 2: every particle of such a cell with species > 0 draws its three normals (all
+        // threads, one slot each); sqrt(T) g replaces its velocity in LDS
+        for (int q = tid; q < t.count; q += blockDim.x)
+            if (lneff[lcell[q]] > 0 && (int)lsp[q] > 0) {
+                double g[3];
+                draw(t.base + q, g);
+                lx[q] = (R)(g[0] * t_factor);
+                ly[q] = (R)(g[1] * t_factor);
+                lz[q] = (R)(g[2] * t_factor);
+            }
+        __syncthreads();
+        // 3: with hydro, the cell's mean is put back: v + (mean before - mean of the new)
+        if (hydro && tid < CELL_THREADS && c < c1 && lneff[tid] > 0) {
+            const int n = count[c];
+            const int o = (int)(start[c] - t.base);
+            const R n_eff = (R)lneff[tid];
+            Acc3<R> vir;
+            for (int i = 0; i < n; ++i)
+                if ((int)lsp[o + i] > 0) {
+                    vir.add(0, lx[o + i]);
+                    vir.add(1, ly[o + i]);
+                    vir.add(2, lz[o + i]);
+                }
+            const R d[3] = {lu[0][tid] - vir.s[0] / n_eff, lu[1][tid] - vir.s[1] / n_eff,
+                            lu[2][tid] - vir.s[2] / n_eff};
+            for (int i = 0; i < n; ++i)
+                if ((int)lsp[o + i] > 0) {
+                    lx[o + i] = lx[o + i] + d[0];
+                    ly[o + i] = ly[o + i] + d[1];
+                    lz[o + i] = lz[o + i] + d[2];
+                }
+        }
+        __syncthreads();
+        for (int q = tid; q < t.count; q += blockDim.x) {
+            p.vel[t.base + q] = lx[q];
+            p.vel[p.n + t.base + q] = ly[q];
+            p.vel[2 * p.n + t.base + q] = lz[q];
+        }
+        return;
     }
-    __syncthreads();
-    for (int q = threadIdx.x; q < t.count; q += blockDim.x) {
-        p.vel[t.base + q] = lx[q];
-        p.vel[p.n + t.base + q] = ly[q];
-        p.vel[2 * p.n + t.base + q] = lz[q];
+    // the cells of this workgroup hold more particles than the tile: one thread per cell
+    // from global memory, each new velocity drawn again where it is written (so no slot
+    // is read after it is written)
+    if (tid >= CELL_THREADS || c >= c1) return;
+    const int n = count[c];
+    if (n <= 1) return;
+    const int64_t s = start[c];
+    Acc3<R> acc;
+    int n_eff = 0;
+    for (int i = 0; i < n; ++i)
+        if ((int)p.species[s + i] > 0) {
+            for (int k = 0; k < 3; ++k) acc.add(k, p.vel[k * p.n + s + i]);
+            ++n_eff;
+        }
+    if (n_eff <= 1) return;
+    const R u[3] = {acc.s[0] / (R)n_eff, acc.s[1] / (R)n_eff, acc.s[2] / (R)n_eff};
+    R d[3] = {0, 0, 0};
+    if (hydro) {
+        Acc3<R> vir;
+        for (int i = 0; i < n; ++i) {
+            if ((int)p.species[s + i] <= 0) continue;
+            double g[3];
+            draw(s + i, g);
+            for (int k = 0; k < 3; ++k) vir.add(k, (R)(g[k] * t_factor));
+        }
+        for (int k = 0; k < 3; ++k) d[k] = u[k] - vir.s[k] / (R)n_eff;
+    }
+    for (int i = 0; i < n; ++i) {
+        if ((int)p.species[s + i] <= 0) continue;
+        double g[3];
+        draw(s + i, g);
+        for (int k = 0; k < 3; ++k) {
+            const R v = (R)(g[k] * t_factor);
+            p.vel[k * p.n + s + i] = hydro ? v + d[k] : v;
+        }
     }
 }
 
 // rescale_cells (mpcd.f90:748-801): in a cell with more than one particle of species
 // > 0, their velocities about the cell mean are scaled so that their kinetic energy is
 // 3 (n_eff - 1) T / 2 (sums in FP64 in the reference's order)
+// one cell of rescale_cells on arrays vx, vy, vz, sp (the LDS tile or global memory)
+template <class R, class S>
+__device__ inline void rescale_cell(R *vx, R *vy, R *vz, const S *sp, int n, double T)
+{
+    double u[3] = {0, 0, 0};
+    int n_eff = 0;
+    for (int i = 0; i < n; ++i)
+        if ((int)sp[i] > 0) {
+            u[0] = u[0] + (double)vx[i];
+            u[1] = u[1] + (double)vy[i];
+            u[2] = u[2] + (double)vz[i];
+            ++n_eff;
+        }
+    if (n_eff <= 1) return;
+    for (int k = 0; k < 3; ++k) u[k] = u[k] / n_eff;
+    double kin = 0;
+    for (int i = 0; i < n; ++i)
+        if ((int)sp[i] > 0) {
+            const double d0 = vx[i] - u[0], d1 = vy[i] - u[1], d2 = vz[i] - u[2];
+            kin = kin + ((d0 * d0 + d1 * d1) + d2 * d2);
+        }
+    const double f = sqrt((double)(3 * (n_eff - 1)) * T / kin);
+    for (int i = 0; i < n; ++i) {
+        if ((int)sp[i] <= 0) continue;
+        vx[i] = (R)(u[0] + f * ((double)vx[i] - u[0]));
+        vy[i] = (R)(u[1] + f * ((double)vy[i] - u[1]));
+        vz[i] = (R)(u[2] + f * ((double)vz[i] - u[2]));
+    }
+}
+
 template <class F>
 __global__ void __launch_bounds__(CELL_THREADS)
     k_rescale(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count, PView<F> p,
@@ -269,44 +368,15 @@ __global__ void __launch_bounds__(CELL_THREADS)
     CellTile<F> t{lx, ly, lz, lsp, 0, 0};
     stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
     int c = c0 + threadIdx.x;
-    if (c < c1) {
-        int n = count[c];
-        if (n > 1) {
-            int64_t s = start[c];
-            bool in_lds = t.count > 0;
-            int64_t o = s - t.base;
-            auto V = [&](int k, int64_t i) -> R {
-                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
-                return p.vel[k * p.n + s + i];
-            };
-            auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
-            double u[3] = {0, 0, 0};
-            int n_eff = 0;
-            for (int i = 0; i < n; ++i)
-                if (S(i) > 0) {
-                    for (int k = 0; k < 3; ++k) u[k] = u[k] + (double)V(k, i);
-                    ++n_eff;
-                }
-            if (n_eff > 1) {
-                for (int k = 0; k < 3; ++k) u[k] = u[k] / n_eff;
-                double kin = 0;
-                for (int i = 0; i < n; ++i)
-                    if (S(i) > 0) {
-                        const double d0 = V(0, i) - u[0], d1 = V(1, i) - u[1], d2 = V(2, i) - u[2];
-                        kin = kin + ((d0 * d0 + d1 * d1) + d2 * d2);
-                    }
-                const double f = sqrt((double)(3 * (n_eff - 1)) * T / kin);
-                for (int i = 0; i < n; ++i) {
-                    if (S(i) <= 0) continue;
-                    for (int k = 0; k < 3; ++k) {
-                        const R v = (R)(u[k] + f * ((double)V(k, i) - u[k]));
-                        if (in_lds)
-                            (k == 0 ? lx : k == 1 ? ly : lz)[o + i] = v;
-                        else
-                            p.vel[k * p.n + s + i] = v;
-                    }
-                }
-            }
+    // the LDS tile and global memory get separate calls, so each is addressed directly
+    if (c < c1 && count[c] > 1) {
+        const int n = count[c];
+        const int64_t s = start[c];
+        if (t.count > 0) {
+            const int o = (int)(s - t.base);
+            rescale_cell(lx + o, ly + o, lz + o, lsp + o, n, T);
+        } else {
+            rescale_cell(p.vel + s, p.vel + p.n + s, p.vel + 2 * p.n + s, p.species + s, n, T);
         }
     }
     __syncthreads();
@@ -335,6 +405,104 @@ struct WallParams {
     double sin_a, cos_a;
 };
 
+// one cell of wall_mpcd_step on arrays vx, vy, vz (the LDS tile or global memory); s is
+// the cell's first slot in the particle arrays (normals and ids are read there).  With
+// the thermostat each particle draws once: its new velocity is written, and with
+// keep_cell_v the difference of the means is added in a second pass over the written
+// values (the same arrays, addressed directly).
+template <class R>
+__device__ inline void wall_cell(R *vx, R *vy, R *vz, int64_t s, int n, int c, const Box &bx, const WallParams &w,
+                                 const double *axes, const double *vnorm, const double *pnorm,
+                                 const int32_t *id, int64_t np, uint64_t seed, uint64_t step)
+{
+    const int ncells = bx.ncells;
+    int q[3];
+    compact_point(c, bx.M[0], bx.M[1], bx.M[2], q);
+    const int wi = !w.present ? -1 : q[2] == 0 ? 0 : q[2] == bx.L[2] - 1 ? 1 : -1;
+    int n_virtual = 0;
+    double vv[3] = {0, 0, 0};
+    if (wi >= 0 && n < w.nw[wi]) {
+        n_virtual = w.nw[wi] - n;
+        if (vnorm) {
+            for (int k = 0; k < 3; ++k) vv[k] = vnorm[k * ncells + c];
+        } else {
+            normal3_threefry(seed, STREAM_WALL, step, (uint64_t)c, vv);
+        }
+        const double f = sqrt(n_virtual * w.temp[wi]);
+        for (int k = 0; k < 3; ++k) vv[k] = vv[k] * f;
+    }
+    Acc3<R> acc;
+    for (int i = 0; i < n; ++i) {
+        acc.add(0, vx[i]);
+        acc.add(1, vy[i]);
+        acc.add(2, vz[i]);
+    }
+    R u[3];
+    for (int k = 0; k < 3; ++k)
+        u[k] = n_virtual > 0 ? (R)(((double)acc.s[k] + vv[k]) / w.nw[wi]) : acc.s[k] / (R)n;
+    if (w.thermostat) {
+        for (int i = 0; i < n; ++i) {
+            double g[3];
+            if (pnorm) {
+                for (int k = 0; k < 3; ++k) g[k] = pnorm[k * np + s + i];
+            } else {
+                normal3_threefry(seed, STREAM_THERMO, step, (uint64_t)id[s + i], g);
+            }
+            vx[i] = (R)(g[0] * w.t_factor);
+            vy[i] = (R)(g[1] * w.t_factor);
+            vz[i] = (R)(g[2] * w.t_factor);
+        }
+        if (w.keep_cell_v) {
+            Acc3<R> sum;
+            for (int i = 0; i < n; ++i) {
+                sum.add(0, vx[i]);
+                sum.add(1, vy[i]);
+                sum.add(2, vz[i]);
+            }
+            const R d[3] = {u[0] - sum.s[0] / (R)n, u[1] - sum.s[1] / (R)n, u[2] - sum.s[2] / (R)n};
+            for (int i = 0; i < n; ++i) {
+                vx[i] = vx[i] + d[0];
+                vy[i] = vy[i] + d[1];
+                vz[i] = vz[i] + d[2];
+            }
+        }
+        return;
+    }
+    double v0, v1, v2;
+    if (axes) {
+        v0 = axes[c];
+        v1 = axes[ncells + c];
+        v2 = axes[2 * ncells + c];
+    } else {
+        Sphere sph = rand_sphere_threefry(seed, STREAM_SRD, step, (uint64_t)c);
+        v0 = sph.n[0];
+        v1 = sph.n[1];
+        v2 = sph.n[2];
+    }
+    const double sin_a = w.sin_a, cos_a = w.cos_a, omc = 1 - cos_a;
+    const R o00 = (R)(cos_a + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sin_a),
+            o20 = (R)((v0 * v2) * omc + v1 * sin_a);
+    const R o01 = (R)((v1 * v0) * omc + v2 * sin_a), o11 = (R)(cos_a + (v1 * v1) * omc),
+            o21 = (R)((v1 * v2) * omc - v0 * sin_a);
+    const R o02 = (R)((v2 * v0) * omc - v1 * sin_a), o12 = (R)((v2 * v1) * omc + v0 * sin_a),
+            o22 = (R)(cos_a + (v2 * v2) * omc);
+    for (int i = 0; i < n; ++i) {
+        const R w0 = vx[i] - u[0], w1 = vy[i] - u[1], w2 = vz[i] - u[2];
+        R r0 = o00 * w0;
+        r0 = r0 + o01 * w1;
+        r0 = r0 + o02 * w2;
+        R r1 = o10 * w0;
+        r1 = r1 + o11 * w1;
+        r1 = r1 + o12 * w2;
+        R r2 = o20 * w0;
+        r2 = r2 + o21 * w1;
+        r2 = r2 + o22 * w2;
+        vx[i] = u[0] + r0;
+        vy[i] = u[1] + r1;
+        vz[i] = u[2] + r2;
+    }
+}
+
 template <class F>
 __global__ void __launch_bounds__(CELL_THREADS)
     k_wall_srd(Box bx, const int32_t *__restrict__ start, const int32_t *__restrict__ count, PView<F> p,
@@ -350,112 +518,16 @@ __global__ void __launch_bounds__(CELL_THREADS)
     CellTile<F> t{lx, ly, lz, lsp, 0, 0};
     stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
     int c = c0 + threadIdx.x;
-    if (c < c1) {
-        int n = count[c];
-        if (n > 1) {
-            int64_t s = start[c];
-            bool in_lds = t.count > 0;
-            int64_t o = s - t.base;
-            auto V = [&](int k, int64_t i) -> R {
-                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
-                return p.vel[k * p.n + s + i];
-            };
-            auto W = [&](int k, int64_t i, R v) {
-                // direct LDS stores (a pointer chosen among the arrays would be a generic
-                // one, and its stores are not ordered with the LDS loads of V)
-                if (!in_lds)
-                    p.vel[k * p.n + s + i] = v;
-                else if (k == 0)
-                    lx[o + i] = v;
-                else if (k == 1)
-                    ly[o + i] = v;
-                else
-                    lz[o + i] = v;
-            };
-            int q[3];
-            compact_point(c, bx.M[0], bx.M[1], bx.M[2], q);
-            const int wi = !w.present ? -1 : q[2] == 0 ? 0 : q[2] == bx.L[2] - 1 ? 1 : -1;
-            int n_virtual = 0;
-            double vv[3] = {0, 0, 0};
-            if (wi >= 0 && n < w.nw[wi]) {
-                n_virtual = w.nw[wi] - n;
-                if (vnorm) {
-                    for (int k = 0; k < 3; ++k) vv[k] = vnorm[k * ncells + c];
-                } else {
-                    normal3_threefry(seed, STREAM_WALL, step, (uint64_t)c, vv);
-                }
-                const double f = sqrt(n_virtual * w.temp[wi]);
-                for (int k = 0; k < 3; ++k) vv[k] = vv[k] * f;
-            }
-            Acc3<R> acc;
-            for (int i = 0; i < n; ++i)
-                for (int k = 0; k < 3; ++k) acc.add(k, V(k, i));
-            R u[3];
-            for (int k = 0; k < 3; ++k)
-                u[k] = n_virtual > 0 ? (R)(((double)acc.s[k] + vv[k]) / w.nw[wi]) : acc.s[k] / (R)n;
-            if (w.thermostat) {
-                // the new velocities are recomputed in the second pass rather than read
-                // back, so no slot is read after it is written
-                auto draw = [&](int i, double g[3]) {
-                    if (pnorm) {
-                        for (int k = 0; k < 3; ++k) g[k] = pnorm[k * p.n + s + i];
-                    } else {
-                        normal3_threefry(seed, STREAM_THERMO, step, (uint64_t)p.id[s + i], g);
-                    }
-                };
-                R d[3] = {0, 0, 0};
-                if (w.keep_cell_v) {
-                    Acc3<R> sum;
-                    for (int i = 0; i < n; ++i) {
-                        double g[3];
-                        draw(i, g);
-                        for (int k = 0; k < 3; ++k) sum.add(k, (R)(g[k] * w.t_factor));
-                    }
-                    for (int k = 0; k < 3; ++k) d[k] = u[k] - sum.s[k] / (R)n;
-                }
-                for (int i = 0; i < n; ++i) {
-                    double g[3];
-                    draw(i, g);
-                    for (int k = 0; k < 3; ++k) {
-                        const R v = (R)(g[k] * w.t_factor);
-                        W(k, i, w.keep_cell_v ? v + d[k] : v);
-                    }
-                }
-            } else {
-                double v0, v1, v2;
-                if (axes) {
-                    v0 = axes[c];
-                    v1 = axes[ncells + c];
-                    v2 = axes[2 * ncells + c];
-                } else {
-                    Sphere sp = rand_sphere_threefry(seed, STREAM_SRD, step, (uint64_t)c);
-                    v0 = sp.n[0];
-                    v1 = sp.n[1];
-                    v2 = sp.n[2];
-                }
-                const double sin_a = w.sin_a, cos_a = w.cos_a, omc = 1 - cos_a;
-                R o00 = (R)(cos_a + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sin_a),
-                  o20 = (R)((v0 * v2) * omc + v1 * sin_a);
-                R o01 = (R)((v1 * v0) * omc + v2 * sin_a), o11 = (R)(cos_a + (v1 * v1) * omc),
-                  o21 = (R)((v1 * v2) * omc - v0 * sin_a);
-                R o02 = (R)((v2 * v0) * omc - v1 * sin_a), o12 = (R)((v2 * v1) * omc + v0 * sin_a),
-                  o22 = (R)(cos_a + (v2 * v2) * omc);
-                for (int i = 0; i < n; ++i) {
-                    R w0 = V(0, i) - u[0], w1 = V(1, i) - u[1], w2 = V(2, i) - u[2];
-                    R r0 = o00 * w0;
-                    r0 = r0 + o01 * w1;
-                    r0 = r0 + o02 * w2;
-                    R r1 = o10 * w0;
-                    r1 = r1 + o11 * w1;
-                    r1 = r1 + o12 * w2;
-                    R r2 = o20 * w0;
-                    r2 = r2 + o21 * w1;
-                    r2 = r2 + o22 * w2;
-                    W(0, i, u[0] + r0);
-                    W(1, i, u[1] + r1);
-                    W(2, i, u[2] + r2);
-                }
-            }
+    // the LDS tile and global memory get separate calls, so each is addressed directly
+    if (c < c1 && count[c] > 1) {
+        const int n = count[c];
+        const int64_t s = start[c];
+        if (t.count > 0) {
+            const int o = (int)(s - t.base);
+            wall_cell(lx + o, ly + o, lz + o, s, n, c, bx, w, axes, vnorm, pnorm, p.id, p.n, seed, step);
+        } else {
+            wall_cell(p.vel + s, p.vel + p.n + s, p.vel + 2 * p.n + s, s, n, c, bx, w, axes, vnorm, pnorm, p.id, p.n,
+                      seed, step);
         }
     }
     __syncthreads();
@@ -468,56 +540,59 @@ __global__ void __launch_bounds__(CELL_THREADS)
 
 // compute_temperature (mpcd.f90:342-407): per block partial sums of kin and of the
 // number of counted cells; the last block adds the partials in block order
+// one cell of compute_temperature on arrays vx, vy, vz, sp (the LDS tile or global
+// memory): its kinetic term, 0 when n_eff <= 1
+template <class R, class S>
+__device__ inline double temperature_cell(const R *vx, const R *vy, const R *vz, const S *sp, int n)
+{
+    double u0 = 0, u1 = 0, u2 = 0;
+    int n_eff = 0;
+    for (int i = 0; i < n; ++i)
+        if ((int)sp[i] > 0) {
+            u0 = u0 + vx[i];
+            u1 = u1 + vy[i];
+            u2 = u2 + vz[i];
+            ++n_eff;
+        }
+    if (n_eff <= 1) return 0;
+    u0 = u0 / n;  // divided by n (mpcd.f90:388)
+    u1 = u1 / n;
+    u2 = u2 / n;
+    double k = 0;
+    for (int i = 0; i < n; ++i)
+        if ((int)sp[i] > 0) {
+            const double d0 = vx[i] - u0, d1 = vy[i] - u1, d2 = vz[i] - u2;
+            k = k + ((d0 * d0 + d1 * d1) + d2 * d2);
+        }
+    return k / (double)(3 * (n_eff - 1));
+}
+
+// per workgroup: the sum of the cells' kinetic terms and the number of cells with more
+// than one particle (the count goes up before the n_eff test, mpcd.f90:374-386)
 template <class F>
 __global__ void __launch_bounds__(CELL_THREADS)
     k_temperature(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
-                  PView<F> p, double *__restrict__ part, unsigned *__restrict__ done,
-                  double *__restrict__ out)
+                  PView<F> p, double *__restrict__ part)
 {
     using R = typename F::real;
     __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
     __shared__ typename F::small lsp[TILE_CAP];
     __shared__ double red[16];
-    __shared__ bool s_last;
     int c0 = blockIdx.x * CELL_THREADS;
     int c1 = min(c0 + CELL_THREADS, ncells);
     CellTile<F> t{lx, ly, lz, lsp, 0, 0};
     stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
     double kin = 0, cnt = 0;
     int c = c0 + threadIdx.x;
-    if (c < c1) {
-        int n = count[c];
-        if (n > 1) {
-            cnt = 1;
-            int64_t s = start[c];
-            bool in_lds = t.count > 0;
-            int64_t o = s - t.base;
-            auto V = [&](int k, int64_t i) -> double {
-                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
-                return p.vel[k * p.n + s + i];
-            };
-            auto S = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)p.species[s + i]; };
-            double u0 = 0, u1 = 0, u2 = 0;
-            int n_eff = 0;
-            for (int i = 0; i < n; ++i)
-                if (S(i) > 0) {
-                    u0 = u0 + V(0, i);
-                    u1 = u1 + V(1, i);
-                    u2 = u2 + V(2, i);
-                    ++n_eff;
-                }
-            if (n_eff > 1) {
-                u0 = u0 / n;  // divided by n (mpcd.f90:388)
-                u1 = u1 / n;
-                u2 = u2 / n;
-                double k = 0;
-                for (int i = 0; i < n; ++i)
-                    if (S(i) > 0) {
-                        double d0 = V(0, i) - u0, d1 = V(1, i) - u1, d2 = V(2, i) - u2;
-                        k = k + ((d0 * d0 + d1 * d1) + d2 * d2);
-                    }
-                kin = k / (double)(3 * (n_eff - 1));
-            }
+    if (c < c1 && count[c] > 1) {
+        const int n = count[c];
+        const int64_t s = start[c];
+        cnt = 1;
+        if (t.count > 0) {
+            const int o = (int)(s - t.base);
+            kin = temperature_cell(lx + o, ly + o, lz + o, lsp + o, n);
+        } else {
+            kin = temperature_cell(p.vel + s, p.vel + p.n + s, p.vel + 2 * p.n + s, p.species + s, n);
         }
     }
     double bk = block_sum(kin, red);
@@ -525,24 +600,23 @@ __global__ void __launch_bounds__(CELL_THREADS)
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = bk;
         part[2 * blockIdx.x + 1] = bc;
-        __threadfence();
-        s_last = atomicAdd(done, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        double a = 0, b = 0;
-        for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
-            a += part[2 * q];
-            b += part[2 * q + 1];
-        }
-        a = block_sum(a, red);
-        b = block_sum(b, red);
-        if (threadIdx.x == 0) {
-            out[0] = a / b;
-            *done = 0;
-        }
+}
+
+// the workgroups' partial sums, added in a fixed order by one workgroup after the kernel
+// boundary (a last-workgroup pattern would need a device-scope fence in every workgroup)
+__global__ void __launch_bounds__(CELL_THREADS) k_temperature_final(const double *__restrict__ part, unsigned nb,
+                                                                    double *__restrict__ out)
+{
+    __shared__ double red[16];
+    double a = 0, b = 0;
+    for (unsigned q = threadIdx.x; q < nb; q += blockDim.x) {
+        a += part[2 * q];
+        b += part[2 * q + 1];
     }
+    a = block_sum(a, red);
+    b = block_sum(b, red);
+    if (threadIdx.x == 0) out[0] = a / b;
 }
 
 // bulk_reaction (mpcd.f90:605-643): one draw per eligible cell, last 'from' converts
@@ -580,37 +654,33 @@ __global__ void k_bulk_reaction(int ncells, const int32_t *__restrict__ start,
 // particle whose kinetic energy about the cell's centre of mass exceeds delta_u, one draw;
 // on success the last 'from' converts and the velocities about the centre of mass are
 // scaled by sqrt((kin - delta_u) / kin), which removes delta_u
-template <class F>
-__global__ void k_bulk_endothermic(int ncells, const int32_t *__restrict__ start,
-                                   const int32_t *__restrict__ count, const uint8_t *__restrict__ is_reac,
-                                   PView<F> p, int from, int to, double rate, double tau, double delta_u,
-                                   const double *__restrict__ draws, uint64_t seed, uint64_t step)
+// one cell of bulk_reaction_endothermic on arrays vx, vy, vz, sp (the LDS tile or global
+// memory); returns the converting particle (its index in the cell) or -1
+template <class R, class S>
+__device__ inline int endothermic_cell(R *vx, R *vy, R *vz, const S *sp, int n, int c, int from, double rate,
+                                       double tau, double delta_u, const double *draws, uint64_t seed,
+                                       uint64_t step)
 {
-    using R = typename F::real;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    int n = count[c];
-    if (n <= 1 || !is_reac[c]) return;
-    int64_t s = start[c];
     double local_rate = 0;
-    int64_t pick = -1;
+    int pick = -1;
     double com[3] = {0, 0, 0};
     for (int i = 0; i < n; ++i) {
-        if ((int)p.species[s + i] == from) {
+        if ((int)sp[i] == from) {
             local_rate = local_rate + 1;
-            pick = s + i;
+            pick = i;
         }
-        for (int k = 0; k < 3; ++k) com[k] = com[k] + (double)p.vel[k * p.n + s + i];
+        com[0] = com[0] + (double)vx[i];
+        com[1] = com[1] + (double)vy[i];
+        com[2] = com[2] + (double)vz[i];
     }
     for (int k = 0; k < 3; ++k) com[k] = com[k] / n;
     double kin = 0;
     for (int i = 0; i < n; ++i) {
-        const double d0 = p.vel[s + i] - com[0], d1 = p.vel[p.n + s + i] - com[1],
-                     d2 = p.vel[2 * p.n + s + i] - com[2];
+        const double d0 = vx[i] - com[0], d1 = vy[i] - com[1], d2 = vz[i] - com[2];
         kin = kin + ((d0 * d0 + d1 * d1) + d2 * d2);
     }
     kin = kin / 2;
-    if (!(local_rate > 0 && kin > delta_u)) return;
+    if (!(local_rate > 0 && kin > delta_u)) return -1;
     local_rate = local_rate * rate;
     double u;
     if (draws) {
@@ -620,12 +690,50 @@ __global__ void k_bulk_endothermic(int ncells, const int32_t *__restrict__ start
         threefry2x64(step, (uint64_t)c, seed, STREAM_REACTION, a, b);
         u = u01(a);
     }
-    if (u < (1 - exp(-local_rate * tau))) {
-        p.species[pick] = (typename F::small)to;
-        const double f = sqrt((kin - delta_u) / kin);
-        for (int i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k)
-                p.vel[k * p.n + s + i] = (R)(com[k] + f * ((double)p.vel[k * p.n + s + i] - com[k]));
+    if (!(u < (1 - exp(-local_rate * tau)))) return -1;
+    const double f = sqrt((kin - delta_u) / kin);
+    for (int i = 0; i < n; ++i) {
+        vx[i] = (R)(com[0] + f * ((double)vx[i] - com[0]));
+        vy[i] = (R)(com[1] + f * ((double)vy[i] - com[1]));
+        vz[i] = (R)(com[2] + f * ((double)vz[i] - com[2]));
+    }
+    return pick;
+}
+
+template <class F>
+__global__ void __launch_bounds__(CELL_THREADS)
+    k_bulk_endothermic(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+                       const uint8_t *__restrict__ is_reac, PView<F> p, int from, int to, double rate, double tau,
+                       double delta_u, const double *__restrict__ draws, uint64_t seed, uint64_t step)
+{
+    using R = typename F::real;
+    __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
+    __shared__ typename F::small lsp[TILE_CAP];
+    int c0 = blockIdx.x * CELL_THREADS;
+    int c1 = min(c0 + CELL_THREADS, ncells);
+    CellTile<F> t{lx, ly, lz, lsp, 0, 0};
+    stage_tile<F>(p, start, count, ncells, c0, c1, t, true);
+    int c = c0 + threadIdx.x;
+    // the LDS tile and global memory get separate calls, so each is addressed directly
+    if (c < c1 && count[c] > 1 && is_reac[c]) {
+        const int n = count[c];
+        const int64_t s = start[c];
+        int pick;
+        if (t.count > 0) {
+            const int o = (int)(s - t.base);
+            pick = endothermic_cell(lx + o, ly + o, lz + o, lsp + o, n, c, from, rate, tau, delta_u, draws, seed,
+                                    step);
+        } else {
+            pick = endothermic_cell(p.vel + s, p.vel + p.n + s, p.vel + 2 * p.n + s, p.species + s, n, c, from, rate,
+                                    tau, delta_u, draws, seed, step);
+        }
+        if (pick >= 0) p.species[s + pick] = (typename F::small)to;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < t.count; q += blockDim.x) {
+        p.vel[t.base + q] = lx[q];
+        p.vel[p.n + t.base + q] = ly[q];
+        p.vel[2 * p.n + t.base + q] = lz[q];
     }
 }
 
@@ -823,7 +931,7 @@ int mpcat_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_particl
                uint64_t seed, uint64_t step, double t_factor, int hydro)
 {
     int nc = box->ncells;
-    k_mpcat<F><<<blocks_for(nc, CELL_THREADS), CELL_THREADS, 0, stream()>>>(
+    k_mpcat<F><<<blocks_for(nc, CELL_THREADS), MPCAT_THREADS, 0, stream()>>>(
         nc, cells->cell_start, cells->cell_count, view<F>(pp), normals, seed, step, t_factor, hydro);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
@@ -844,7 +952,7 @@ int bulk_endo_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, rmpcd_par
                    double rate, double tau, double delta_u, const double *draws, uint64_t seed, uint64_t step)
 {
     int nc = box->ncells;
-    k_bulk_endothermic<F><<<blocks_for(nc, 256), 256, 0, stream()>>>(nc, cells->cell_start, cells->cell_count,
+    k_bulk_endothermic<F><<<blocks_for(nc, CELL_THREADS), CELL_THREADS, 0, stream()>>>(nc, cells->cell_start, cells->cell_count,
                                                                       cells->is_reac, view<F>(pp), from, to, rate,
                                                                       tau, delta_u, draws, seed, step);
     RMPCD_LAUNCHED();
@@ -879,14 +987,12 @@ int temperature_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
     unsigned nb = blocks_for(nc, CELL_THREADS);
     double *part = (double *)workspace(WS_PART, sizeof(double) * 2 * nb);
     double *red = (double *)workspace(WS_RED, 64);
-    unsigned *done = (unsigned *)workspace(WS_FLAG, 64);
     double *host = (double *)pinned(64);
-    if (!part || !red || !done || !host)
-        return set_error(RMPCD_ERR_HIP, "compute_temperature: workspace allocation failed");
+    if (!part || !red || !host) return set_error(RMPCD_ERR_HIP, "compute_temperature: workspace allocation failed");
     hipStream_t s = stream();
-    RMPCD_HIP(hipMemsetAsync(done, 0, sizeof(unsigned), s));
-    k_temperature<F><<<nb, CELL_THREADS, 0, s>>>(nc, cells->cell_start, cells->cell_count,
-                                                 view<F>(pp), part, done, red);
+    k_temperature<F><<<nb, CELL_THREADS, 0, s>>>(nc, cells->cell_start, cells->cell_count, view<F>(pp), part);
+    RMPCD_LAUNCHED();
+    k_temperature_final<<<1, CELL_THREADS, 0, s>>>(part, nb, red);
     RMPCD_LAUNCHED();
     RMPCD_HIP(hipMemcpyAsync(host, red, sizeof(double), hipMemcpyDeviceToHost, s));
     RMPCD_HIP(hipStreamSynchronize(s));

@@ -28,8 +28,8 @@ def close(a, b, rtol):
     return a.shape == b.shape and bool(np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), 1.0)))
 
 
-def state(L, seed, form, frac2=0.0):
-    c, srt, cnt, st = sorted_state(L, 10, seed, frac2=frac2)
+def state(L, seed, form, frac2=0.0, rho=10):
+    c, srt, cnt, st = sorted_state(L, rho, seed, frac2=frac2)
     if form == capi.FP32:
         srt.vel[:] = srt.vel.astype(np.float32)  # the oracle starts from the stored values
     return c, srt, cnt, st
@@ -37,9 +37,10 @@ def state(L, seed, form, frac2=0.0):
 
 @pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
 @pytest.mark.parametrize("hydro", [1, 0])
-@pytest.mark.parametrize("L,frac2", [((8, 5, 5), 0.0), ((20, 20, 20), 0.3)])
-def test_mpcat_step(gpu, L, frac2, hydro, form):
-    c, srt, cnt, st = state(L, 31, form, frac2)
+# rho = 25: 128 cells hold more particles than a workgroup stages (the global-memory path)
+@pytest.mark.parametrize("L,frac2,rho", [((8, 5, 5), 0.0, 10), ((20, 20, 20), 0.3, 10), ((16, 12, 10), 0.3, 25)])
+def test_mpcat_step(gpu, L, frac2, rho, hydro, form):
+    c, srt, cnt, st = state(L, 31, form, frac2, rho)
     srt.species[::7] = 0  # left out of the mean and of the thermostat
     g = np.random.default_rng(32).normal(size=srt.vel.shape)
     T = 0.8
