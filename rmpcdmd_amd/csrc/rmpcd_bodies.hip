@@ -53,6 +53,7 @@ struct BlockSched {
 struct Schedule {
     std::vector<int32_t> links;  // the caller's list, for the comparison
     std::vector<double> dist;
+    int device = -1;             // the device whose memory holds it
     int64_t n = 0;
     int n_links = 0, n_levels = 0;
     // one sweep by level (the global-memory rattle)
@@ -91,8 +92,10 @@ static int upload(T *dst, const std::vector<T> &src)
 static int get_schedule(const int32_t *links, const double *dist, int n_links, int64_t n, const char *who,
                         Schedule **out)
 {
+    int device = 0;
+    RMPCD_HIP(hipGetDevice(&device));
     for (auto &s : g_sched)
-        if (s.block && s.n == n && s.n_links == n_links && n_links > 0 &&
+        if (s.block && s.device == device && s.n == n && s.n_links == n_links && n_links > 0 &&
             !memcmp(s.links.data(), links, sizeof(int32_t) * 2 * (size_t)n_links) &&
             !memcmp(s.dist.data(), dist, sizeof(double) * (size_t)n_links)) {
             *out = &s;
@@ -140,8 +143,12 @@ static int get_schedule(const int32_t *links, const double *dist, int n_links, i
     Schedule &s = g_sched[g_sched_next];
     g_sched_next = (g_sched_next + 1) % 4;
     if (s.block) {
-        RMPCD_HIP(hipStreamSynchronize(stream()));  // a kernel may still read it
+        // a kernel may still read it; its memory belongs to the device it was made on
+        int cur = device;
+        RMPCD_HIP(hipSetDevice(s.device));
+        RMPCD_HIP(hipDeviceSynchronize());
         free_schedule(s);
+        RMPCD_HIP(hipSetDevice(cur));
     }
     const size_t nl1 = nl > 0 ? nl : 1;
     const size_t bytes = nl1 * 16 + nl1 * 4 * 5 + ((size_t)n_levels + 1) * 4 + ((size_t)n + 1) * 4 + 2 * nl1 * 4 + 256;
@@ -165,6 +172,7 @@ static int get_schedule(const int32_t *links, const double *dist, int n_links, i
         return rc;
     s.links.assign(links, links + 2 * nl);
     s.dist = dist_v;
+    s.device = device;
     s.n = n;
     s.n_links = n_links;
     s.n_levels = n_levels;
