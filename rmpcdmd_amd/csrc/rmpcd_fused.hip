@@ -1568,6 +1568,56 @@ __device__ inline bool refuel_far(const R *pos, int64_t n, int64_t i, const doub
     return (a0 * a0 + a1 * a1) + a2 * a2 > far && (b0 * b0 + b1 * b1) + b2 * b2 > far;
 }
 
+// the SRD rotation of one cell (mpcd.f90:120-180 with the default alpha) on arrays
+// vx, vy, vz, sp: the LDS tile or global memory, each addressed directly
+template <class R, class S>
+__device__ inline void fz_srd_cell(R *vx, R *vy, R *vz, const S *sp, int m, uint64_t seed, uint64_t cycle, int c)
+{
+    R ss[3] = {0, 0, 0}, cc[3] = {0, 0, 0};
+    int n_eff = 0;
+    for (int i = 0; i < m; ++i) {
+        if ((int)sp[i] > 0) {
+            const R v3[3] = {vx[i], vy[i], vz[i]};
+            for (int k = 0; k < 3; ++k) {
+                const R v = v3[k];
+                if constexpr (sizeof(R) == 4) {
+                    R y = v - cc[k];
+                    R t = ss[k] + y;
+                    cc[k] = (t - ss[k]) - y;
+                    ss[k] = t;
+                } else {
+                    ss[k] = ss[k] + v;
+                }
+            }
+            ++n_eff;
+        }
+    }
+    if (n_eff <= 1) return;
+    const R u0 = ss[0] / (R)n_eff, u1 = ss[1] / (R)n_eff, u2 = ss[2] / (R)n_eff;
+    Sphere ax = rand_sphere_threefry(seed, STREAM_SRD, cycle, (uint64_t)c);
+    const double v0 = ax.n[0], v1 = ax.n[1], v2 = ax.n[2];
+    // default alpha: sin = 1, cos = 0 (mpcd.f90:114-117)
+    const double sa = 1, ca = 0, omc = 1 - ca;
+    const R o00 = (R)(ca + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sa), o20 = (R)((v0 * v2) * omc + v1 * sa);
+    const R o01 = (R)((v1 * v0) * omc + v2 * sa), o11 = (R)(ca + (v1 * v1) * omc), o21 = (R)((v1 * v2) * omc - v0 * sa);
+    const R o02 = (R)((v2 * v0) * omc - v1 * sa), o12 = (R)((v2 * v1) * omc + v0 * sa), o22 = (R)(ca + (v2 * v2) * omc);
+    for (int i = 0; i < m; ++i) {
+        const R w0 = vx[i] - u0, w1 = vy[i] - u1, w2 = vz[i] - u2;
+        R r0 = o00 * w0;
+        r0 = r0 + o01 * w1;
+        r0 = r0 + o02 * w2;
+        R r1 = o10 * w0;
+        r1 = r1 + o11 * w1;
+        r1 = r1 + o12 * w2;
+        R r2 = o20 * w0;
+        r2 = r2 + o21 * w1;
+        r2 = r2 + o22 * w2;
+        vx[i] = u0 + r0;
+        vy[i] = u1 + r1;
+        vz[i] = u2 + r2;
+    }
+}
+
 template <class F>
 __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
 {
@@ -1598,11 +1648,33 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
         __syncthreads();
         const int64_t base = s_range[0], end = s_range[1];
         const int tcount = (end - base <= CTILE) ? (int)(end - base) : 0;
-        for (int q = threadIdx.x; q < tcount; q += CT) {
-            lx[q] = vel[base + q];
-            ly[q] = vel[n + base + q];
-            lz[q] = vel[2 * n + base + q];
-            lsp[q] = spc[base + q];
+        // four slots per thread at a time: their loads are all in flight before the first
+        // LDS store waits for one
+        int q0 = threadIdx.x;
+        for (; q0 + 3 * CT < tcount; q0 += 4 * CT) {
+            R x[4], y[4], z[4];
+            S sp4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t g = base + q0 + u * CT;
+                x[u] = vel[g];
+                y[u] = vel[n + g];
+                z[u] = vel[2 * n + g];
+                sp4[u] = spc[g];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                lx[q0 + u * CT] = x[u];
+                ly[q0 + u * CT] = y[u];
+                lz[q0 + u * CT] = z[u];
+                lsp[q0 + u * CT] = sp4[u];
+            }
+        }
+        for (; q0 < tcount; q0 += CT) {
+            lx[q0] = vel[base + q0];
+            ly[q0] = vel[n + base + q0];
+            lz[q0] = vel[2 * n + base + q0];
+            lsp[q0] = spc[base + q0];
         }
         __syncthreads();
         const int c = c0 + threadIdx.x;
@@ -1610,85 +1682,39 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
             const int m = cnt[c];
             const int64_t s = start[c];
             const bool in_lds = tcount > 0;
-            const int64_t o = s - base;
-            auto V = [&](int k, int64_t i) -> R {
-                if (in_lds) return k == 0 ? lx[o + i] : k == 1 ? ly[o + i] : lz[o + i];
-                return vel[k * n + s + i];
-            };
-            auto Wr = [&](int64_t i, R a, R b2, R c2) {
-                if (in_lds) {
-                    lx[o + i] = a;
-                    ly[o + i] = b2;
-                    lz[o + i] = c2;
-                } else {
-                    vel[s + i] = a;
-                    vel[n + s + i] = b2;
-                    vel[2 * n + s + i] = c2;
-                }
-            };
-            auto Sp = [&](int64_t i) -> int { return in_lds ? (int)lsp[o + i] : (int)spc[s + i]; };
+            const int o = (int)(s - base);
+            // the LDS tile and global memory get separate calls, so each is addressed directly
             if (m > 1) {
-                R ss[3] = {0, 0, 0}, cc[3] = {0, 0, 0};
-                int n_eff = 0;
-                for (int i = 0; i < m; ++i) {
-                    if (Sp(i) > 0) {
-                        for (int k = 0; k < 3; ++k) {
-                            R v = V(k, i);
-                            if constexpr (sizeof(R) == 4) {
-                                R y = v - cc[k];
-                                R t = ss[k] + y;
-                                cc[k] = (t - ss[k]) - y;
-                                ss[k] = t;
-                            } else {
-                                ss[k] = ss[k] + v;
-                            }
-                        }
-                        ++n_eff;
-                    }
-                }
-                if (n_eff > 1) {
-                    R u0 = ss[0] / (R)n_eff, u1 = ss[1] / (R)n_eff, u2 = ss[2] / (R)n_eff;
-                    Sphere ax = rand_sphere_threefry(rp.seed, STREAM_SRD, (uint64_t)C.cycle, (uint64_t)c);
-                    const double v0 = ax.n[0], v1 = ax.n[1], v2 = ax.n[2];
-                    // default alpha: sin = 1, cos = 0 (mpcd.f90:114-117)
-                    const double sa = 1, ca = 0, omc = 1 - ca;
-                    R o00 = (R)(ca + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sa),
-                      o20 = (R)((v0 * v2) * omc + v1 * sa);
-                    R o01 = (R)((v1 * v0) * omc + v2 * sa), o11 = (R)(ca + (v1 * v1) * omc),
-                      o21 = (R)((v1 * v2) * omc - v0 * sa);
-                    R o02 = (R)((v2 * v0) * omc - v1 * sa), o12 = (R)((v2 * v1) * omc + v0 * sa),
-                      o22 = (R)(ca + (v2 * v2) * omc);
-                    for (int i = 0; i < m; ++i) {
-                        R w0 = V(0, i) - u0, w1 = V(1, i) - u1, w2 = V(2, i) - u2;
-                        R r0 = o00 * w0;
-                        r0 = r0 + o01 * w1;
-                        r0 = r0 + o02 * w2;
-                        R r1 = o10 * w0;
-                        r1 = r1 + o11 * w1;
-                        r1 = r1 + o12 * w2;
-                        R r2 = o20 * w0;
-                        r2 = r2 + o21 * w1;
-                        r2 = r2 + o22 * w2;
-                        Wr(i, u0 + r0, u1 + r1, u2 + r2);
-                    }
-                }
+                if (in_lds)
+                    fz_srd_cell(lx + o, ly + o, lz + o, lsp + o, m, rp.seed, (uint64_t)C.cycle, c);
+                else
+                    fz_srd_cell(vel + s, vel + n + s, vel + 2 * n + s, spc + s, m, rp.seed, (uint64_t)C.cycle, c);
             }
             // with the tile in LDS, |v|max and refuel run over the tile's particles below
-            for (int i = 0; i < m && !in_lds; ++i) {
-                double a = V(0, i), b2 = V(1, i), c2 = V(2, i);
-                double q2 = (a * a + b2 * b2) + c2 * c2;
-                vm = q2 > vm ? q2 : vm;
-            }
+            if (!in_lds)
+                for (int i = 0; i < m; ++i) {
+                    const double a = vel[s + i], b2 = vel[n + s + i], c2 = vel[2 * n + s + i];
+                    const double q2 = (a * a + b2 * b2) + c2 * c2;
+                    vm = q2 > vm ? q2 : vm;
+                }
             if (P.bulk) {
                 // bulk_reaction B -> A (mpcd.f90:605-643), one draw threefry((cycle, cell), (seed, REACTION))
                 if (m > 1 && B.is_reac[rp.c_off + c]) {
                     double local_rate = 0;
                     int64_t pick = -1;
-                    for (int i = 0; i < m; ++i)
-                        if (Sp(i) == 2) {
-                            local_rate = local_rate + 1;
-                            pick = s + i;
-                        }
+                    if (in_lds) {
+                        for (int i = 0; i < m; ++i)
+                            if ((int)lsp[o + i] == 2) {
+                                local_rate = local_rate + 1;
+                                pick = s + i;
+                            }
+                    } else {
+                        for (int i = 0; i < m; ++i)
+                            if ((int)spc[s + i] == 2) {
+                                local_rate = local_rate + 1;
+                                pick = s + i;
+                            }
+                    }
                     local_rate = local_rate * P.bulk_rate;
                     uint64_t a0, a1;
                     threefry2x64((uint64_t)C.cycle, (uint64_t)c, rp.seed, STREAM_REACTION, a0, a1);
@@ -1696,7 +1722,7 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
                 }
             } else if (!in_lds) {
                 for (int i = 0; i < m; ++i)
-                    if (Sp(i) == 2 && refuel_far(pos, n, s + i, B.cpos + 6 * r, rp.box)) spc[s + i] = (S)1;
+                    if ((int)spc[s + i] == 2 && refuel_far(pos, n, s + i, B.cpos + 6 * r, rp.box)) spc[s + i] = (S)1;
             }
         }
         __syncthreads();
