@@ -46,7 +46,7 @@ constexpr int CTILE = 1536;  // staged particles per collision block (12 per cel
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
 constexpr int MAX_STENCIL = 2048;  // stencil cells of the fused list build
-constexpr int GPT = 4, GT = FT * GPT;  // destination slots per thread and per unit of the gather
+constexpr int GPT = 2, GT = FT * GPT;  // destination slots per thread and per unit of the gather
 constexpr int GHALO = 64;              // slots staged on either side of a gather or rank unit
 constexpr int KPT = 4, KCHUNK = FT * KPT;  // particles per thread and per chunk of the key pass
 constexpr int KHASH_BITS = 11, KHASH = 1 << KHASH_BITS;  // LDS hash entries, 2 x KCHUNK
@@ -703,9 +703,10 @@ __device__ inline UnitRange unit_range(const int64_t *plan, int R, int kind)
 // then one global atomic per distinct key reserves the chunk's slots in that cell
 // (arrival order within a cell is made stable by the gather).  The block with a
 // replica's first chunk records the grid shift and wraps the colloids.
-template <class F>
-__global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
+template <class F, int kForced>
+__device__ inline void key_pass(const FB<F> &B, const FP &P, int R)
 {
+    constexpr int forced = kForced;
     using Rl = typename F::real;
     __shared__ int hkey[KHASH], hcnt[KHASH], occ[KHASH];
     __shared__ int lds[FT / 64];
@@ -942,6 +943,19 @@ __global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int forced, int R)
     }
 }
 
+// the collision sort's key pass and the extra sorts' one are separate kernels, so that
+// each is given the registers of its own path (the collision sort's fits six waves per SIMD)
+template <class F>
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6))) k_fz_key_collision(FB<F> B, FP P, int R)
+{
+    key_pass<F, 1>(B, P, R);
+}
+template <class F>
+__global__ void __launch_bounds__(FT) k_fz_key(FB<F> B, FP P, int R)
+{
+    key_pass<F, 0>(B, P, R);
+}
+
 // S2: exclusive scan of the new cell counts inside tiles of SCAN_TILE cells (tile-local
 // offsets into slocal, tile totals into scan_sums); a unit is a tile
 template <class F>
@@ -1129,7 +1143,7 @@ __device__ inline int rank_in_cell(const int32_t *lv, const int2 *pair, int64_t 
 // one (MD/PAST_MD bits); every other destination force slot is already zero (see
 // k_fz_place).
 template <class F>
-__global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
+__device__ inline void gather_pass(const FB<F> &B, const FP &P, int R)
 {
     using Rl = typename F::real;
     using S = typename F::small;
@@ -1220,6 +1234,19 @@ __global__ void __launch_bounds__(FT) k_fz_gather(FB<F> B, FP P, int R)
         const double m = block_max(vm, red);
         if (threadIdx.x == 0) B.gpart[u] = m;
     }
+}
+
+// FP32: six waves per SIMD without spills (two slots per thread); FP64 records are too
+// large for that and keep the compiler's choice
+template <class F>
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6))) k_fz_gather(FB<F> B, FP P, int R)
+{
+    gather_pass<F>(B, P, R);
+}
+template <class F>
+__global__ void __launch_bounds__(FT) k_fz_gather_wide(FB<F> B, FP P, int R)
+{
+    gather_pass<F>(B, P, R);
 }
 
 // S4 of an extra sort: the particles stay where they are and only the order is computed.
@@ -1882,11 +1909,17 @@ static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t s
     // blocks only read the plan and return, and fewer blocks return sooner
     const int nb = forced ? s.sort_blocks : s.extra_blocks;
     k_fz_plan<F><<<1, FT, 0, st>>>(b, forced, s.R);
-    k_fz_key<F><<<nb, FT, 0, st>>>(b, p, forced, s.R);
+    if (forced)
+        k_fz_key_collision<F><<<nb, FT, 0, st>>>(b, p, s.R);
+    else
+        k_fz_key<F><<<nb, FT, 0, st>>>(b, p, s.R);
     k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, s.R);
     k_fz_place<F><<<nb, FT, 0, st>>>(b, forced, s.R);
     if (forced)
-        k_fz_gather<F><<<nb, FT, 0, st>>>(b, p, s.R);
+        if constexpr (std::is_same<F, F32>::value)
+            k_fz_gather<F><<<nb, FT, 0, st>>>(b, p, s.R);
+        else
+            k_fz_gather_wide<F><<<nb, FT, 0, st>>>(b, p, s.R);
     else
         k_fz_rank<F><<<nb, FT, 0, st>>>(b, s.R);
     k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, s.R);
