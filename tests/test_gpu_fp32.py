@@ -55,9 +55,12 @@ def temperature(sim):
     return te.value
 
 
-def test_fp32_form_against_fp64_form(gpu):
-    """same initial state and draws in both forms; the per-observable FP32 error"""
-    kw = dict(L=(24, 24, 24), n_md=10, tau=0.1, seed=9, mode=capi.SIM_FUSED)
+@pytest.mark.parametrize("L,n_md,tau,cycles", [(24, 10, 0.1, 4), (64, 50, 0.5, 2)],
+                         ids=["L24", "L64-headline"])
+def test_fp32_form_against_fp64_form(gpu, L, n_md, tau, cycles):
+    """same initial state and draws in both forms; the per-observable FP32 error (the second
+    case is one system of the headline workload: L=64, rho=10, tau=0.5, N_MD=50)"""
+    kw = dict(L=(L, L, L), n_md=n_md, tau=tau, seed=9, mode=capi.SIM_FUSED)
     s64 = SingleDimer(form=capi.FP64, **kw)
     s32 = SingleDimer(form=capi.FP32, **kw)
     m = np.array(s64.state()[4].mass[:])
@@ -69,7 +72,7 @@ def test_fp32_form_against_fp64_form(gpu):
     a0 = s32.download()
     p0 = momentum(a0)
     errors = {}
-    for cycle in range(4):
+    for cycle in range(cycles):
         s64.run(1)
         s32.run(1)
         t64, t32 = temperature(s64), temperature(s32)
@@ -81,7 +84,7 @@ def test_fp32_form_against_fp64_form(gpu):
     errors["kinetic_energy"] = abs(ke32 - ke64) / ke64
     errors["dimer_position"] = float(np.abs(a["colloid_pos"] - b["colloid_pos"]).max())
     errors["n_product"] = int(abs((a["species"] == 2).sum() - (b["species"] == 2).sum()))
-    print("FP32-mixed vs FP64 form after 4 cycles:", errors)
+    print("FP32-mixed vs FP64 form, L=%d, after %d cycles:" % (L, cycles), errors)
     assert max(errors["temperature"]) < 1e-3
     assert errors["kinetic_energy"] < 1e-3
     assert errors["dimer_position"] < 1e-3
