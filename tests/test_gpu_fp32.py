@@ -30,8 +30,9 @@ def cell_sums(v, cnt, st):
     return out, keep
 
 
-def test_fp32_srd_conserves_cell_momentum(gpu):
-    c, srt, cnt, st = sorted_state((16, 16, 16), 10, 31)
+@pytest.mark.parametrize("L", [16, 64])
+def test_fp32_srd_conserves_cell_momentum(gpu, L):
+    c, srt, cnt, st = sorted_state((L, L, L), 10, 31)
     d = to_device(srt, form=capi.FP32)
     cs = cells_to_device(c, cnt, st)
     v0 = d.host("vel").astype(np.float64)
@@ -40,8 +41,15 @@ def test_fp32_srd_conserves_cell_momentum(gpu):
     assert not np.allclose(v0, v1)
     p0, keep = cell_sums(v0, cnt, st)
     p1, _ = cell_sums(v1, cnt, st)
-    scale, _ = cell_sums(np.abs(v0), cnt, st)
-    assert (np.abs(p1 - p0) <= 1e-6 * scale).all()
+    # relative to the cell's momentum scale, the sum of its particles' speeds (the same for
+    # the three components: the rotation moves velocity between components, so a component
+    # whose own speeds are small in a cell can gain the rounding of the large ones)
+    speed, _ = cell_sums(np.sqrt((v0 * v0).sum(axis=1)), cnt, st)
+    rel = np.abs(p1 - p0) / speed[:, None]
+    comp, _ = cell_sums(np.abs(v0), cnt, st)
+    print("L=%d, %d cells: max |dp| / sum |v| %.2e (per component: %.2e)"
+          % (L, keep.size, rel.max(), (np.abs(p1 - p0) / comp).max()))
+    assert rel.max() <= 1e-6
     # the rotation keeps each cell's kinetic energy
     e0, _ = cell_sums((v0 * v0).sum(axis=1), cnt, st)
     e1, _ = cell_sums((v1 * v1).sum(axis=1), cnt, st)
