@@ -46,8 +46,8 @@ constexpr int CTILE = 1536;  // staged particles per collision block (12 per cel
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
 constexpr int MAX_STENCIL = 2048;  // stencil cells of the fused list build
-constexpr int GPT = 2, GT = FT * GPT;  // destination slots per thread and per unit of the gather
-constexpr int GHALO = 64;              // slots staged on either side of a gather or rank unit
+constexpr int GPT = 1, GT = FT * GPT;  // destination slots per thread and per unit of the gather
+constexpr int GHALO = 32;              // slots staged on either side of a gather or rank unit
 constexpr int KPT = 4, KCHUNK = FT * KPT;  // particles per thread and per chunk of the key pass
 constexpr int KHASH_BITS = 11, KHASH = 1 << KHASH_BITS;  // LDS hash entries, 2 x KCHUNK
 
@@ -1236,15 +1236,9 @@ __device__ inline void gather_pass(const FB<F> &B, const FP &P, int R)
     }
 }
 
-// FP32: six waves per SIMD without spills (two slots per thread); FP64 records are too
-// large for that and keep the compiler's choice
+// eight waves per SIMD without spills in both forms (one slot per thread)
 template <class F>
-__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6))) k_fz_gather(FB<F> B, FP P, int R)
-{
-    gather_pass<F>(B, P, R);
-}
-template <class F>
-__global__ void __launch_bounds__(FT) k_fz_gather_wide(FB<F> B, FP P, int R)
+__global__ void __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(8))) k_fz_gather(FB<F> B, FP P, int R)
 {
     gather_pass<F>(B, P, R);
 }
@@ -1916,10 +1910,7 @@ static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t s
     k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, s.R);
     k_fz_place<F><<<nb, FT, 0, st>>>(b, forced, s.R);
     if (forced)
-        if constexpr (std::is_same<F, F32>::value)
-            k_fz_gather<F><<<nb, FT, 0, st>>>(b, p, s.R);
-        else
-            k_fz_gather_wide<F><<<nb, FT, 0, st>>>(b, p, s.R);
+        k_fz_gather<F><<<nb, FT, 0, st>>>(b, p, s.R);
     else
         k_fz_rank<F><<<nb, FT, 0, st>>>(b, s.R);
     k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, s.R);

@@ -55,6 +55,29 @@ def test_fused_equals_reference_sequence(gpu, form):
     fus.close()
 
 
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+def test_fused_dense_cells(gpu, form):
+    """rho = 40: cells hold more particles than the gather and rank passes stage on either
+    side of their tiles, so the ranks of those cells read the rest from memory, and the
+    collision blocks overflow their LDS tiles; the fused program must still equal the
+    reference call sequence, and its odd-grid run the tuned one bit for bit"""
+    kw = dict(L=(12, 12, 12), rho=40, n_md=10, tau=0.1, form=form, seed=123)
+    ref = SingleDimer(mode=capi.SIM_REFERENCE, **kw)
+    fus = SingleDimer(mode=capi.SIM_FUSED, **kw)
+    odd = SingleDimer(mode=capi.SIM_FUSED, sort_blocks=96, extra_sort_blocks=7, **kw)
+    for cycle in range(2):
+        ref.run(1)
+        fus.run(1)
+        odd.run(1)
+        a, b, d = ref.download(), fus.download(), odd.download()
+        assert (a["cell_count"] > 32).sum() > 100  # cells larger than the 32-slot halo
+        compare(a, b, "after cycle %d" % cycle, 1e-12 if form == capi.FP64 else 1e-5)
+        compare(b, d, "odd grids after cycle %d" % cycle, rtol=0.0)
+    ref.close()
+    fus.close()
+    odd.close()
+
+
 def test_fused_replicas_of_uneven_size(gpu):
     Ls = [(12, 12, 12), (16, 12, 14), (12, 16, 12)]
     kw = dict(n_md=10, tau=0.1, form=capi.FP64, seed=5, replica_L=Ls)
