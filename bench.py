@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--no-bodies", action="store_true", help="skip the linked-colloid routines")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the wall, forced-flow, thermostat and energy routines")
-    ap.add_argument("--no-c4", action="store_true",
+    ap.add_argument("--no-mixed", action="store_true",
                     help="skip the 128-replica run with L drawn from {24, 32, 40, 48}")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device to device once (the reference dispatch for "
@@ -60,7 +60,7 @@ def parse():
     return ap.parse_args()
 
 
-def c4_sizes():
+def mixed_sizes():
     """128 replicas with L drawn uniformly from {24, 32, 40, 48} (seeded)"""
     import numpy as np
     return [int(v) for v in np.array([24, 32, 40, 48])[np.random.default_rng(20261202).integers(0, 4, 128)]]
@@ -429,11 +429,11 @@ def main():
         lz = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, a.replicas, profile=False, lazy=1)
         line["lazy_streaming"] = {"value": lz["ps_per_s"], "ms_per_step": lz["ms_per_cycle"],
                                   "over_eager": lz["ps_per_s"] / main_run["ps_per_s"]}
-    if not a.no_c4:
-        sizes = c4_sizes()
-        c4 = run_gpu(a.form, a.mode, 0, a.n_md, max(2, a.steps // 2), 1, 0, profile=False, sizes=sizes)
-        line["batched_c4"] = {"value": c4["ps_per_s"], "ms_per_step": c4["ms_per_cycle"], "n_solvent": c4["n"],
-                              "replicas": len(sizes), "L": "128 replicas, L uniform in {24,32,40,48}, seeded"}
+    if not a.no_mixed:
+        sizes = mixed_sizes()
+        mx = run_gpu(a.form, a.mode, 0, a.n_md, max(2, a.steps // 2), 1, 0, profile=False, sizes=sizes)
+        line["batched_mixed_L"] = {"value": mx["ps_per_s"], "ms_per_step": mx["ms_per_cycle"], "n_solvent": mx["n"],
+                                   "replicas": len(sizes), "L": "128 replicas, L uniform in {24,32,40,48}, seeded"}
     if not a.no_single and a.replicas > 1:
         single = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, 1, profile=False)
         line["single_replica"] = {"value": single["ps_per_s"], "ms_per_step": single["ms_per_cycle"],

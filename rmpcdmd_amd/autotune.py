@@ -9,7 +9,7 @@ at a time (coordinate descent, two passes):
 The result is stored in a JSON table keyed by the workload; bench.py applies the entry
 of its workload when the table exists.
 
-    python -m rmpcdmd_amd.autotune [--L 64] [--replicas 16] [--c4] [--form fp32]
+    python -m rmpcdmd_amd.autotune [--L 64] [--replicas 16] [--mixed-L] [--form fp32]
                                    [--out rmpcdmd_amd/tuning.json]
 """
 from __future__ import annotations
@@ -77,7 +77,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--replicas", type=int, default=16)
-    ap.add_argument("--c4", action="store_true", help="128 replicas with L from {24, 32, 40, 48}")
+    ap.add_argument("--mixed-L", action="store_true", help="128 replicas with L from {24, 32, 40, 48}")
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--n-md", type=int, default=50)
     ap.add_argument("--out", default=TABLE)
@@ -89,7 +89,7 @@ def main():
     torch.zeros(1, device="cuda")
     capi.call("rmpcd_init", 0)
     form = capi.FP32 if a.form == "fp32" else capi.FP64
-    if a.c4:
+    if a.mixed_L:
         import numpy as np
         sizes = [int(v) for v in np.array([24, 32, 40, 48])[np.random.default_rng(20261202).integers(0, 4, 128)]]
 

@@ -11,13 +11,14 @@ from . import capi
 from .capi import FP32, FP64, PROG_COLLOIDS, PROG_FLUID, PROG_SPHERE
 
 DEFAULTS = {
-    # plain SRD fluid
+    # setup_simple_fluid: plain SRD fluid, L=32, rho=10
     PROG_FLUID: dict(L=(32, 32, 32), rho=10, n_md=1, T=1.0 / 3.0, tau=0.1, sigma=1.0, epsilon=1.0,
                      epsilon_colloids=1.0, n_colloids=0, prob=1.0, bulk_rate=0.0, skin=0.0),
-    # config 2: one catalytic sphere sigma=4 at the box centre, bulk B->A at 0.01
+    # setup_single_catalytic_fixed_sphere: one catalytic sphere sigma=4 at the box centre,
+    # bulk B->A at 0.01
     PROG_SPHERE: dict(L=(32, 32, 32), rho=10, n_md=10, T=1.0 / 3.0, tau=0.1, sigma=4.0, epsilon=1.0,
                       epsilon_colloids=1.0, n_colloids=1, prob=1.0, bulk_rate=0.01, skin=1.5),
-    # config 5: many colloids sigma=2.5 in a periodic box
+    # n_colloids_pbc: 1000 colloids sigma=2.5 in a periodic L=128 box
     PROG_COLLOIDS: dict(L=(128, 128, 128), rho=10, n_md=50, T=1.0, tau=0.5, sigma=2.5, epsilon=1.0,
                         epsilon_colloids=1.0, n_colloids=1000, prob=1.0, bulk_rate=0.0, skin=0.8),
 }

@@ -163,7 +163,7 @@ def test_sequence_matches_oracle(gpu, bulk, mode):
     sim.close()
 
 
-def test_batched_replicas_of_the_c4_shape(gpu):
+def test_batched_replicas_of_mixed_sizes(gpu):
     """replicas of unequal L (the batched configuration draws L from {24, 32, 40, 48}),
     FP64, tau=0.5 and N_MD=50 as in the dimer runs: the fused batch against the reference
     call sequence replica by replica, and replica 0 of the batch against the same system
