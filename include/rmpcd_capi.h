@@ -99,6 +99,11 @@ typedef struct rmpcd_lj {
 /* ---- runtime -------------------------------------------------------------------- */
 const char *rmpcd_last_error(void);
 int rmpcd_init(int device);
+/* The library orders all its work on ONE stream, and its scratch buffers (sort keys and
+ * slots, partial sums, the Hilbert table) are shared by every entry point.  Switching the
+ * stream first waits for the work queued on the previous one, so nothing issued on the
+ * new stream can overwrite scratch memory that earlier kernels still read.  The library
+ * is not thread-safe: one host thread calls it at a time. */
 int rmpcd_set_stream(void *hip_stream);
 int rmpcd_synchronize(void);
 const char *rmpcd_version(void);
@@ -368,7 +373,10 @@ typedef struct rmpcd_dimer_stats {
     double e1, e2;        /* last compute_force / compute_force_n2 energies */
     double dt;
     double mass[2];
-    int32_t nmax, pad_;
+    int32_t nmax;
+    int32_t error;             /* 0, or the status this replica stopped with: a list
+                                  longer than nmax stops it at that MD step, as the
+                                  reference's error stop does (RMPCD_ERR_OVERFLOW) */
     int32_t *list, *nlist;     /* device: the colloids' neighbour lists */
     void *pos_old;             /* device: solvent positions at the last sort */
     double *colloid_pos_old;   /* device */

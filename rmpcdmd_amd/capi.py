@@ -71,7 +71,7 @@ class DimerStats(C.Structure):
     _fields_ = [("n_solvent", C.c_int64), ("md_steps", C.c_int64), ("cycles", C.c_int64),
                 ("n_extra_sorting", C.c_int64), ("n_replicas", C.c_int64),
                 ("n_solvent_total", C.c_int64), ("e1", C.c_double), ("e2", C.c_double),
-                ("dt", C.c_double), ("mass", C.c_double * 2), ("nmax", C.c_int32), ("pad_", C.c_int32),
+                ("dt", C.c_double), ("mass", C.c_double * 2), ("nmax", C.c_int32), ("error", C.c_int32),
                 ("list", C.c_void_p), ("nlist", C.c_void_p), ("pos_old", C.c_void_p),
                 ("colloid_pos_old", C.c_void_p)]
 

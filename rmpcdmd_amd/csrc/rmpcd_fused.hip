@@ -50,6 +50,10 @@ constexpr int GPT = 1, GT = FT * GPT;  // destination slots per thread and per u
 constexpr int GHALO = 32;              // slots staged on either side of a gather or rank unit
 constexpr int KPT = 4, KCHUNK = FT * KPT;  // particles per thread and per chunk of the key pass
 constexpr int KHASH_BITS = 11, KHASH = 1 << KHASH_BITS;  // LDS hash entries, 2 x KCHUNK
+// a replica's error flag: a list longer than its capacity (the reference's error stop),
+// or more particles near the colloids than a deferred extra sort can collect; every
+// kernel returns at once for a replica with the flag set, so it stops at that step
+constexpr int ERR_CANDIDATES = -100;
 
 
 // The staging record of a sort: a particle's wrapped state, written by the key pass in
@@ -204,6 +208,7 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
     const int r = blockIdx.y;
     const RepDev &rp = B.rep[r];
     CtlDev &C = B.ctl[r];
+    if (C.error) return;  // the replica stopped at a failed list build
     const int cur = C.cur, fp = C.fpar;
     const bool exact = C.need_exact != 0;
     const int64_t n = rp.n, po3 = 3 * rp.p_off;
@@ -433,6 +438,7 @@ __global__ void __launch_bounds__(1024) k_fz_colloid(FB<F> B, FP P, int mode, in
     const int r = blockIdx.y;
     const RepDev &rp = B.rep[r];
     CtlDev &C = B.ctl[r];
+    if (C.error) return;
     const bool exact = C.need_exact != 0;
     const double *part = B.part + (int64_t)r * PS;
     const int moff = nbm * MPV;
@@ -625,7 +631,7 @@ __global__ void __launch_bounds__(FT) k_fz_plan(FB<F> B, int forced, int R)
     int64_t *act = plan + 1 + U_KINDS, *amap = act + R;
     // the common case first: no replica sorts
     bool any = forced != 0;
-    for (int r = threadIdx.x; r < R && !any; r += blockDim.x) any = B.ctl[r].need_sort != 0;
+    for (int r = threadIdx.x; r < R && !any; r += blockDim.x) any = B.ctl[r].need_sort != 0 && !B.ctl[r].error;
     if (!__syncthreads_or(any)) {
         if (threadIdx.x <= U_KINDS) plan[threadIdx.x] = 0;
         return;
@@ -634,7 +640,8 @@ __global__ void __launch_bounds__(FT) k_fz_plan(FB<F> B, int forced, int R)
     int64_t csum[U_KINDS] = {0, 0, 0, 0};
     for (int base = 0; base < R; base += blockDim.x) {
         const int r = base + threadIdx.x;
-        const bool on = r < R && sort_active(B.ctl[r], forced);
+        // a replica whose list build failed has stopped (the reference's error stop)
+        const bool on = r < R && !B.ctl[r].error && sort_active(B.ctl[r], forced);
         // an extra sort is deferred when the previous one was not and the box allows it:
         // only the key pass and the list build run for it
         const bool dfr = on && !forced && B.ctl[r].def == 0 && B.rep[r].defer != 0;
@@ -1427,7 +1434,7 @@ __device__ void deferred_list(FB<F> &B, const FP &P, int R, int r, int col, int 
     if (threadIdx.x == 0) {
         B.nlist[2 * r + col] = tl;
         B.mcount[(r * 2 + cur) * 2 + col] = tl;
-        if (over) C.error = RMPCD_ERR_OVERFLOW;
+        if (over) C.error = C.ccount > B.cand_cap ? ERR_CANDIDATES : RMPCD_ERR_OVERFLOW;
     }
     if (col != 0) return;
     // |v|max: the previous bound raised to the list members' speeds (the key pass's units)
@@ -1457,6 +1464,10 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
     const int r = blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (!sort_active(C, forced)) return;
+    // replicas left out of this sort by the plan (stopped at an earlier failed list
+    // build; with no replica sorting the plan holds only its zero count); both blocks
+    // read the same plan, so they agree even while one of them sets the error flag
+    if (B.plan[0] == 0 || plan_amap(B.plan, R)[r] < 0) return;
     __shared__ int skey[MAX_STENCIL], sstart[MAX_STENCIL], soff[MAX_STENCIL + 1], sflag[MAX_STENCIL];
     if (!forced && C.kind == 1) {
         __shared__ short dlut[DEFER_BOX];
@@ -1651,6 +1662,7 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
     const int r = blockIdx.y;
     const RepDev &rp = B.rep[r];
     CtlDev &C = B.ctl[r];
+    if (C.error) return;
     const int64_t n = rp.n;
     const int nc = rp.box.ncells;
     const int c0 = blockIdx.x * CT;
@@ -1769,6 +1781,7 @@ __global__ void __launch_bounds__(1024) k_fz_collide_commit(FB<F> B, FP P, int n
     __shared__ double red[16];
     const int r = blockIdx.y;
     CtlDev &C = B.ctl[r];
+    if (C.error) return;
     double all = 0;
     for (int q = threadIdx.x; q < ncollide; q += blockDim.x) all = fmax(all, B.part[(int64_t)r * PS + q]);
     all = block_max(all, red);
@@ -2211,9 +2224,13 @@ int DimerSim::sync_state()
         rp.e1 = c.e1;
         rp.e2 = c.e2;
         for (int k = 0; k < 3; ++k) rp.box.origin[k] = c.origin[k];
+        rp.error = c.error == ERR_CANDIDATES ? RMPCD_ERR_OVERFLOW : c.error;
         if (c.error) err = c.error;
     }
     if (err == RMPCD_ERR_OVERFLOW) return set_error(RMPCD_ERR_OVERFLOW, "maximum of neighbor list reached");
+    if (err == ERR_CANDIDATES)
+        return set_error(RMPCD_ERR_OVERFLOW, "dimer: more particles near the colloids than the deferred extra "
+                                             "sort's candidate buffer holds (%lld)", (long long)cand_cap);
     if (err) return set_error(err, "dimer: device error %d", err);
     return RMPCD_OK;
 }

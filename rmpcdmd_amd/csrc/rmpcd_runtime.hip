@@ -148,6 +148,9 @@ int rmpcd_init(int device)
 
 int rmpcd_set_stream(void *s)
 {
+    // the scratch buffers are shared by all entry points: the kernels queued on the
+    // previous stream must be done with them before the new stream can reuse them
+    if ((hipStream_t)s != g_stream) RMPCD_HIP(hipStreamSynchronize(g_stream));
     g_stream = (hipStream_t)s;
     return RMPCD_OK;
 }

@@ -425,8 +425,10 @@ int rmpcd_dimer_state(rmpcd_dimer_t *sim, int32_t replica, rmpcd_particles_t *so
     RMPCD_CHECK(sim, "rmpcd_dimer_state: null simulation");
     DimerSim *s = (DimerSim *)sim;
     RMPCD_CHECK(replica >= 0 && replica < s->R, "rmpcd_dimer_state: replica out of range");
+    // a replica stopped by a failed list build still shows its state; the status it
+    // stopped with is in stats->error (rmpcd_dimer_run returned it)
     int rc = s->sync_state();
-    if (rc) return rc;
+    if (rc == RMPCD_ERR_HIP || rc == RMPCD_ERR_ARG) return rc;
     int r = replica;
     const Replica &rp = s->rep[r];
     if (solvent) *solvent = s->solvent_view(r);
@@ -442,6 +444,7 @@ int rmpcd_dimer_state(rmpcd_dimer_t *sim, int32_t replica, rmpcd_particles_t *so
         stats->e1 = rp.e1;
         stats->e2 = rp.e2;
         stats->nmax = s->nmax;
+        stats->error = rp.error;
         stats->list = s->list + 2 * (size_t)r * s->nmax;
         stats->nlist = s->nlist + 2 * r;
         stats->pos_old = s->cfg.mode == RMPCD_SIM_FUSED

@@ -19,6 +19,7 @@ struct Replica {
     int cur = 0, fpar = 0, cfpar = 0, cnt = 0;  // buffer parities, live cell_count array
     int64_t md_steps = 0, cycles = 0, n_extra = 0;
     double e1 = 0, e2 = 0;
+    int error = 0;  // fused mode: the status the replica stopped with (0: running)
 };
 
 // per-replica constants and mutable control for the fused kernels (device memory)

@@ -303,6 +303,12 @@ int rmpcd_sort(const rmpcd_box_t *box, const rmpcd_particles_t *src, rmpcd_parti
     RMPCD_CHECK(src->image && dst->image && src->id && dst->id && id_to_idx,
                 "rmpcd_sort: null image/id/id_to_idx");
     RMPCD_CHECK(cells && cells->cell_count && cells->cell_start, "rmpcd_sort: null cell arrays");
+    // the gather reads every source slot while it writes the destination: the two sets
+    // must be different buffers (the reference sorts into its second buffer and swaps)
+    RMPCD_CHECK(src->pos != dst->pos && src->vel != dst->vel && src->force != dst->force &&
+                    src->force_old != dst->force_old && src->image != dst->image &&
+                    src->id != dst->id && src->species != dst->species && src->flags != dst->flags,
+                "rmpcd_sort: source and destination share an array (the sort is out of place)");
     return RMPCD_DISPATCH_FORM(src, sort_impl, box, src, dst, id_to_idx, cells, apply_pbc ? 1 : 0);
 }
 

@@ -45,7 +45,7 @@ module rmpcd_capi
   type, bind(c) :: rmpcd_dimer_stats_t
      integer(c_int64_t) :: n_solvent, md_steps, cycles, n_extra_sorting, n_replicas, n_solvent_total
      real(c_double) :: e1, e2, dt, mass(2)
-     integer(c_int32_t) :: nmax, pad_
+     integer(c_int32_t) :: nmax, error
      type(c_ptr) :: list, nlist, pos_old, colloid_pos_old
   end type rmpcd_dimer_stats_t
 

@@ -91,7 +91,7 @@ class SingleDimer:
         out["list"] = get(st.list, 2 * st.nmax, np.int32).reshape(2, st.nmax)
         out["nlist"] = get(st.nlist, 2, np.int32)
         out["origin"] = np.array(box.origin[:])
-        for k in ("md_steps", "cycles", "n_extra_sorting", "e1", "e2", "n_solvent", "nmax", "dt"):
+        for k in ("md_steps", "cycles", "n_extra_sorting", "e1", "e2", "n_solvent", "nmax", "dt", "error"):
             out[k] = getattr(st, k)
         out["mass"] = np.array(st.mass[:])
         return out

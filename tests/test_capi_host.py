@@ -92,3 +92,28 @@ def test_threefry_known_answer():
     out = (C.c_uint64 * 3)()
     capi.lib().rmpcd_threefry2x64_20((C.c_uint64 * 2)(0, 0), (C.c_uint64 * 2)(0, 0), out)
     assert (out[0], out[1]) == (0xC2B6E3A8C2C69865, 0x6F81ED42F350084D)
+
+
+def test_sort_refuses_aliased_buffers():
+    # rmpcd_sort is out of place: a destination that shares any array with the source is
+    # refused on the host, before anything reaches the device (no GPU needed)
+    b = capi.Box()
+    capi.call("rmpcd_box_init", C.byref(b), capi.i3((4, 4, 4)), 1.0, 0)
+    names = ("pos", "vel", "force", "force_old", "image", "id", "species", "flags")
+
+    def fake(base):
+        p = capi.Particles()
+        p.n, p.form = 640, 0
+        for i, k in enumerate(names):
+            setattr(p, k, base + 0x10000 * i)
+        return p
+
+    cells = capi.Cells(0x900000, 0xA00000, 0xB00000, 0xC00000)
+    i2i = C.c_void_p(0xD00000)
+    src = fake(0x100000)
+    for k in names:
+        dst = fake(0x500000)
+        setattr(dst, k, getattr(src, k))
+        with pytest.raises(capi.RmpcdError) as e:
+            capi.call("rmpcd_sort", C.byref(b), C.byref(src), C.byref(dst), i2i, C.byref(cells), 0)
+        assert e.value.code == capi.RMPCD_ERR_ARG and "share an array" in e.value.msg

@@ -26,10 +26,10 @@ def test_fortran_program(gpu, prog):
 
 def test_single_dimer_pbc_driver(gpu, tmp_path):
     """the single_dimer_pbc driver on the reference's test/dimer_test.parameters (L=24,
-    N_MD=125, 20 + 200 loops): the observables table has a row per sampled loop, the
-    temperature holds at T, the centre-of-mass velocity stays at its start, and the
-    colloid mass is the reference's printed 1130.9733867645264 (doc/run.rst:103; the
-    driver's single-precision pi)"""
+    N_MD=125, 20 + 200 loops), FP64 form: the observables table has a row per sampled
+    loop, the temperature holds at T, the reference's CI bars on internal energy and
+    centre-of-mass velocity hold, and the colloid mass is the reference's printed
+    1130.9733867645264 (doc/run.rst:103; the driver's single-precision pi)"""
     exe = os.path.join(BIN, "single_dimer_pbc")
     fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dimer_test.parameters")
     out = tmp_path / "dimer.txt"
@@ -45,6 +45,13 @@ def test_single_dimer_pbc_driver(gpu, tmp_path):
     assert (rows[:, 0] == np.arange(20, 221)).all()
     T = rows[:, 1]
     assert abs(T.mean() - 1) < 0.03 and np.all(np.abs(T - 1) < 0.1)
-    v_com = rows[:, 5:8]
-    assert np.max(np.abs(v_com - v_com[0])) < 1e-10
-    assert np.max(np.abs(v_com)) < 1e-3
+    # the reference's own CI bars on this run (test/dimer_test.py:13-22): the internal
+    # energy (column e_total) fluctuates with a standard deviation below 0.1, and the
+    # centre-of-mass velocity stays within 1e-15 of its mean
+    e_int = rows[:, 4] - rows[:, 4].mean()
+    print("internal energy std", e_int.std())
+    assert e_int.std() < 0.1
+    v_com = rows[:, 5:8] - rows[:, 5:8].mean(axis=0)
+    print("max |v_com - mean|", np.max(np.abs(v_com)))
+    assert np.max(np.abs(v_com)) < 1e-15
+    assert np.max(np.abs(rows[:, 5:8])) < 1e-3

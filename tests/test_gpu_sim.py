@@ -56,6 +56,43 @@ def test_fused_equals_reference_sequence(gpu, form):
 
 
 @pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+def test_fused_list_overflow_stops(gpu, form):
+    """a list longer than its capacity stops the replica at that MD step with the
+    reference's message (neighbor_list.f90:135-137) instead of running on with a
+    truncated list: the state stays readable, later runs change nothing, and the other
+    replica of the batch keeps going (unless it overflows too).  The capacity is the
+    longest set-up list, so one of the rebuilds that the small skin asks for exceeds it."""
+    kw = dict(L=(16, 16, 16), n_md=10, tau=0.1, form=form, seed=91, skin=0.6)
+    probe = SingleDimer(mode=capi.SIM_FUSED, n_replicas=2, **kw)
+    n0 = max(int(probe.download(r)["nlist"].max()) for r in range(2))
+    probe.close()
+    print("list capacity", n0, flush=True)
+    sim = SingleDimer(mode=capi.SIM_FUSED, nmax_list=n0, n_replicas=2, **kw)
+    with pytest.raises(capi.RmpcdError) as e:
+        sim.run(8)
+    assert e.value.code == capi.RMPCD_ERR_OVERFLOW
+    assert "maximum of neighbor list reached" in e.value.msg
+    d = [sim.download(r) for r in range(2)]
+    stopped = [r for r in range(2) if d[r]["error"] == capi.RMPCD_ERR_OVERFLOW]
+    assert stopped, [x["error"] for x in d]
+    for r in range(2):
+        assert np.isfinite(d[r]["pos"]).all() and np.isfinite(d[r]["vel"]).all()
+        assert (d[r]["nlist"] <= n0).all()
+        if r in stopped:
+            assert d[r]["md_steps"] < 8 * 10
+        else:
+            assert d[r]["md_steps"] == 8 * 10 and d[r]["cycles"] == 8
+    with pytest.raises(capi.RmpcdError):
+        sim.run(1)
+    again = [sim.download(r) for r in range(2)]
+    for r in stopped:
+        for k in ("pos", "vel", "species", "id"):
+            assert again[r][k].tobytes() == d[r][k].tobytes(), k
+        assert again[r]["md_steps"] == d[r]["md_steps"]
+    sim.close()
+
+
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
 def test_fused_dense_cells(gpu, form):
     """rho = 40: cells hold more particles than the gather and rank passes stage on either
     side of their tiles, so the ranks of those cells read the rest from memory, and the

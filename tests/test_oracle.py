@@ -633,6 +633,7 @@ def test_force_newton_and_energy():
     assert np.allclose(f1.sum(axis=0) + f2.sum(axis=0), 0, atol=1e-9 * np.abs(f2).sum())
     # brute-force energy and flag bits over all pairs
     e_bf = 0.0
+    near = np.zeros((2, srt.n), bool)
     for i in range(2):
         for j in range(srt.n):
             d = O.rel_pos(col.pos[i], srt.pos[j], L)
@@ -640,8 +641,11 @@ def test_force_newton_and_energy():
             p = i  # (solvent species 1, colloid species i+1)
             if r2 <= lj.cut_sq[p]:
                 e_bf += O.lj_energy(r2, lj.eps[p], lj.sigma[p], lj.cut_energy[p])
-                assert fl[j] & (1 << O.MD_BIT)
-                assert bool(fl[j] & (1 << O.CATALYZED_BIT)) == bool(fl[j] & (1 << O.CATALYZED_BIT))
+                near[i, j] = True
+    # MD on every particle inside a cut-off; CATALYZED iff inside the species-1 colloid's
+    assert near[0].any() and near[1].any()
+    assert ((fl >> O.MD_BIT & 1) == (near[0] | near[1])).all()
+    assert ((fl >> O.CATALYZED_BIT & 1) == near[0]).all()
     assert abs(e - e_bf) < 1e-10 * max(1, abs(e_bf))
 
 
