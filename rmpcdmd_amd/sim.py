@@ -15,6 +15,13 @@ DIMER_DEFAULTS = dict(L=(48, 48, 48), rho=10, n_md=50, T=1.0, tau=0.5, sigma_C=2
                       replica_L=None, stream_blocks=0, sort_blocks=0, lazy=0, extra_sort_blocks=0)
 
 
+def mixed_L_sizes(n=128, seed=20261202):
+    """box sides of the batched single-dimer workload: n replicas with L drawn uniformly
+    from {24, 32, 40, 48} by a seeded generator"""
+    import numpy as np
+    return [int(v) for v in np.array([24, 32, 40, 48])[np.random.default_rng(seed).integers(0, 4, n)]]
+
+
 def make_config(**kw):
     p = dict(DIMER_DEFAULTS)
     unknown = set(kw) - set(p)
