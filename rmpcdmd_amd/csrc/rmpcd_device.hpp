@@ -307,6 +307,89 @@ __device__ inline Sphere rand_sphere_threefry(uint64_t seed, uint64_t stream, ui
 }
 
 // ---------------------------------------------------------------------------------
+// SRD cell arithmetic shared by the collision kernels
+
+// running sum of three components.  FP32: compensated, each addition split exactly into
+// its rounded sum and its rounding error by TwoSum (Knuth), the errors gathered in c, so
+// that s + c is as accurate as a sum in twice the precision (Ogita, Rump and Oishi's Sum2:
+// error below eps |sum| + (n eps)^2 sum |x|).  FP64: plain, in the reference's order.
+template <class R>
+struct Acc3 {
+    R s[3] = {0, 0, 0};
+    R c[3] = {0, 0, 0};
+    __device__ inline void add(int k, R v)
+    {
+        if constexpr (sizeof(R) == 4) {
+            const R t = s[k] + v;
+            const R bp = t - s[k];
+            c[k] = c[k] + ((s[k] - (t - bp)) + (v - bp));
+            s[k] = t;
+        } else {
+            s[k] = s[k] + v;
+        }
+    }
+    __device__ inline R sum(int k) const
+    {
+        if constexpr (sizeof(R) == 4)
+            return s[k] + c[k];
+        else
+            return s[k];
+    }
+};
+
+// v <- u + omega (v - u) for the m particles of a cell (mpcd.f90:173-175), omega given
+// row-major as o[3 r + k].  In FP32, when every particle of the cell entered the mean
+// (then exact arithmetic keeps the cell's momentum; old holds the compensated sums of
+// the old velocities), the rounding of the m new velocities is compensated: the
+// difference between the compensated sums before and after (s + c each, the s parts
+// subtracted first) is added, per component, to the particle whose new component is
+// smallest in magnitude, so the cell's momentum is kept to the rounding of that one
+// addition.  FP64 is left as the reference computes it.
+template <class R>
+__device__ inline void srd_rotate(R *vx, R *vy, R *vz, int m, const R *o, R u0, R u1, R u2, const Acc3<R> &old,
+                                  bool conserve)
+{
+    Acc3<R> nw;
+    int jmin[3] = {0, 0, 0};
+    R amin[3] = {INFINITY, INFINITY, INFINITY};
+    for (int i = 0; i < m; ++i) {
+        const R w0 = vx[i] - u0, w1 = vy[i] - u1, w2 = vz[i] - u2;
+        R r0 = o[0] * w0;
+        r0 = r0 + o[1] * w1;
+        r0 = r0 + o[2] * w2;
+        R r1 = o[3] * w0;
+        r1 = r1 + o[4] * w1;
+        r1 = r1 + o[5] * w2;
+        R r2 = o[6] * w0;
+        r2 = r2 + o[7] * w1;
+        r2 = r2 + o[8] * w2;
+        const R n3[3] = {u0 + r0, u1 + r1, u2 + r2};
+        vx[i] = n3[0];
+        vy[i] = n3[1];
+        vz[i] = n3[2];
+        if constexpr (sizeof(R) == 4) {
+            for (int k = 0; k < 3; ++k) {
+                nw.add(k, n3[k]);
+                const R a = fabsf(n3[k]);
+                if (a < amin[k]) {
+                    amin[k] = a;
+                    jmin[k] = i;
+                }
+            }
+        }
+    }
+    if constexpr (sizeof(R) == 4) {
+        if (conserve) {
+            R *vk[3] = {vx, vy, vz};
+            for (int k = 0; k < 3; ++k) {
+                const R d = (old.s[k] - nw.s[k]) + (old.c[k] - nw.c[k]);
+                vk[k][jmin[k]] = vk[k][jmin[k]] + d;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // wave64 primitives.  DPP row shifts build scans inside rows of 16 lanes; row
 // broadcasts 15 and 31 carry row totals across the wave.
 __device__ inline int wave_inclusive_scan(int v)

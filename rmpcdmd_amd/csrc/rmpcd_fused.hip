@@ -1605,49 +1605,26 @@ __device__ inline bool refuel_far(const R *pos, int64_t n, int64_t i, const doub
 template <class R, class S>
 __device__ inline void fz_srd_cell(R *vx, R *vy, R *vz, const S *sp, int m, uint64_t seed, uint64_t cycle, int c)
 {
-    R ss[3] = {0, 0, 0}, cc[3] = {0, 0, 0};
+    Acc3<R> acc;
     int n_eff = 0;
     for (int i = 0; i < m; ++i) {
         if ((int)sp[i] > 0) {
-            const R v3[3] = {vx[i], vy[i], vz[i]};
-            for (int k = 0; k < 3; ++k) {
-                const R v = v3[k];
-                if constexpr (sizeof(R) == 4) {
-                    R y = v - cc[k];
-                    R t = ss[k] + y;
-                    cc[k] = (t - ss[k]) - y;
-                    ss[k] = t;
-                } else {
-                    ss[k] = ss[k] + v;
-                }
-            }
+            acc.add(0, vx[i]);
+            acc.add(1, vy[i]);
+            acc.add(2, vz[i]);
             ++n_eff;
         }
     }
     if (n_eff <= 1) return;
-    const R u0 = ss[0] / (R)n_eff, u1 = ss[1] / (R)n_eff, u2 = ss[2] / (R)n_eff;
+    const R u0 = acc.sum(0) / (R)n_eff, u1 = acc.sum(1) / (R)n_eff, u2 = acc.sum(2) / (R)n_eff;
     Sphere ax = rand_sphere_threefry(seed, STREAM_SRD, cycle, (uint64_t)c);
     const double v0 = ax.n[0], v1 = ax.n[1], v2 = ax.n[2];
     // default alpha: sin = 1, cos = 0 (mpcd.f90:114-117)
     const double sa = 1, ca = 0, omc = 1 - ca;
-    const R o00 = (R)(ca + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sa), o20 = (R)((v0 * v2) * omc + v1 * sa);
-    const R o01 = (R)((v1 * v0) * omc + v2 * sa), o11 = (R)(ca + (v1 * v1) * omc), o21 = (R)((v1 * v2) * omc - v0 * sa);
-    const R o02 = (R)((v2 * v0) * omc - v1 * sa), o12 = (R)((v2 * v1) * omc + v0 * sa), o22 = (R)(ca + (v2 * v2) * omc);
-    for (int i = 0; i < m; ++i) {
-        const R w0 = vx[i] - u0, w1 = vy[i] - u1, w2 = vz[i] - u2;
-        R r0 = o00 * w0;
-        r0 = r0 + o01 * w1;
-        r0 = r0 + o02 * w2;
-        R r1 = o10 * w0;
-        r1 = r1 + o11 * w1;
-        r1 = r1 + o12 * w2;
-        R r2 = o20 * w0;
-        r2 = r2 + o21 * w1;
-        r2 = r2 + o22 * w2;
-        vx[i] = u0 + r0;
-        vy[i] = u1 + r1;
-        vz[i] = u2 + r2;
-    }
+    const R o[9] = {(R)(ca + (v0 * v0) * omc), (R)((v1 * v0) * omc + v2 * sa), (R)((v2 * v0) * omc - v1 * sa),
+                    (R)((v0 * v1) * omc - v2 * sa), (R)(ca + (v1 * v1) * omc), (R)((v2 * v1) * omc + v0 * sa),
+                    (R)((v0 * v2) * omc + v1 * sa), (R)((v1 * v2) * omc - v0 * sa), (R)(ca + (v2 * v2) * omc)};
+    srd_rotate(vx, vy, vz, m, o, u0, u1, u2, acc, n_eff == m);
 }
 
 template <class F>
