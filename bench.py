@@ -62,8 +62,8 @@ def parse():
 
 def mixed_sizes():
     """128 replicas with L drawn uniformly from {24, 32, 40, 48} (seeded)"""
-    import numpy as np
-    return [int(v) for v in np.array([24, 32, 40, 48])[np.random.default_rng(20261202).integers(0, 4, 128)]]
+    from rmpcdmd_amd.sim import mixed_L_sizes
+    return mixed_L_sizes()
 
 
 def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile, sizes=None, lazy=0):
@@ -121,31 +121,75 @@ def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile, sizes=None, l
     return out
 
 
-def cpu_baseline(L, n_md, min_seconds=10.0, max_cycles=40):
-    """the oracle on this host's cores: collision cycles of the same workload, repeated until
-    about min_seconds of CPU work have run"""
-    from oracle import oracle as O
-    from oracle.dimer import NumpyDraws, synthetic_dimer
+def host_cpu():
+    """the host's CPU model and the physical cores this process may run on (lscpu), and the
+    threads the CPU baseline uses: one per physical core, capped at OMP_NUM_THREADS when
+    the environment sets it (a host shared by several GPUs grants each job a share)"""
+    import subprocess
     try:
-        cores = len(os.sched_getaffinity(0))
+        allowed = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    O.lib().orc_set_threads(cores)
+        allowed = list(range(os.cpu_count() or 1))
+    model, cores = "unknown", None
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        m = re.search(r"^Model name:\s*(.+)$", txt, re.M)
+        model = m.group(1).strip() if m else model
+        topo = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+        core_of = {}
+        for ln in topo.splitlines():
+            if ln and not ln.startswith("#"):
+                c, core, sock = ln.split(",")[:3]
+                core_of[int(c)] = (core, sock)
+        cores = len({core_of[c] for c in allowed if c in core_of}) or None
+        host_cores = len(set(core_of.values()))
+    except (OSError, subprocess.SubprocessError, ValueError):
+        host_cores = None
+    cores = cores or len(allowed)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(cores, int(share)) if share.isdigit() and int(share) > 0 else cores
+    return {"model": model, "physical_cores_allowed": cores, "physical_cores_host": host_cores,
+            "threads": threads, "omp_num_threads_env": share or None}
+
+
+def cpu_baseline(L, n_md, min_seconds=10.0):
+    """the oracle (FP64 C with OpenMP) on this host's cores, in a child process with one
+    thread per physical core bound close (OMP_PROC_BIND=close, OMP_PLACES=cores):
+    collision cycles of one system of the same workload, repeated until about
+    min_seconds of CPU work have run"""
+    import subprocess
+    cpu = host_cpu()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cpu["threads"]), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", str(L), str(n_md),
+                        str(cpu["threads"]), str(min_seconds)], env=env, capture_output=True, text=True,
+                       timeout=600)
+    if r.returncode != 0:
+        return {"error": "CPU baseline child exited with %d: %s" % (r.returncode, r.stderr[-400:])}
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out.update({"cores": cpu["threads"], "cpu_model": cpu["model"],
+                "physical_cores_allowed": cpu["physical_cores_allowed"],
+                "physical_cores_host": cpu["physical_cores_host"], "omp_proc_bind": "close",
+                "omp_places": "cores"})
+    return out
+
+
+def cpu_baseline_child(L, n_md, threads, min_seconds, max_cycles=40):
+    from oracle import oracle as O
+    from oracle.dimer import TimingDraws, synthetic_dimer
+    O.lib().orc_set_threads(threads)
     drv = synthetic_dimer((L, L, L), seed=1, tau=0.5, n_md=n_md)
-    draws = NumpyDraws(2)
+    draws = TimingDraws(2)
     t0 = time.perf_counter()
     cycles = 0
     while cycles < max_cycles and (cycles == 0 or time.perf_counter() - t0 < min_seconds):
         drv.cycle(draws)
         cycles += 1
     sec = time.perf_counter() - t0
-    O.lib().orc_set_threads(1)
-    return {"value": drv.s.n * n_md * cycles / sec, "unit": "solvent particle-steps/s", "cores": cores,
-            "kind": "port",
-            "sample": "%d collision cycles (each %d MD steps + grid shift, sort, list rebuild, SRD) of the L=%d "
-                      "dimer workload, %d solvent, on the FP64 C/OpenMP oracle in %.1f s"
-                      % (cycles, n_md, L, drv.s.n, sec)}
+    print(json.dumps({
+        "value": drv.s.n * n_md * cycles / sec, "unit": "solvent particle-steps/s", "kind": "port",
+        "sample": "%d collision cycles (each %d MD steps + grid shift, sort, list rebuild, SRD) of the L=%d "
+                  "dimer workload, %d solvent, on the FP64 C/OpenMP oracle in %.1f s; change_species draws "
+                  "only for the particles the reference draws for" % (cycles, n_md, L, drv.s.n, sec)}))
 
 
 def measured_traffic(form, L, replicas):
@@ -454,4 +498,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) == 6 and sys.argv[1] == "--cpu-baseline-child":
+        cpu_baseline_child(int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]), float(sys.argv[5]))
+    else:
+        main()

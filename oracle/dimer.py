@@ -40,6 +40,16 @@ class NumpyDraws:
         return self.rng.random(ncells)
 
 
+class TimingDraws(NumpyDraws):
+    """for timing loops: as NumpyDraws, except that change_species draws inline, and only
+    for the particles the reference draws for (single_dimer_pbc.f90:533), from a
+    counter-based stream (``change_seed``) instead of a dense array per MD step"""
+
+    def __init__(self, seed):
+        super().__init__(seed)
+        self.change_seed = int(seed) * 0x9E3779B97F4A7C15 % (1 << 64)
+
+
 class DimerOracle:
     """state and loop of single_dimer_pbc on the oracle (FP64, 0-based)"""
 
@@ -116,8 +126,20 @@ class DimerOracle:
         O.rattle_dimer_vel(col.pos, col.vel, col.species, self.mass, self.d, dt, self.edges)
         O.flag_particles_nl(self.lst[0], self.nl[0], col.pos[0], s.pos, s.species, s.flags, self.edges,
                             self.lj_sc.cut_sq[0])
-        O.change_species(col.pos, s.pos, s.species, s.flags, self.edges, self.lj_sc.cut_sq[0],
-                         self.lj_sc.cut_sq[2], self.prob, np.ascontiguousarray(draws.change(self.md_steps, s.id)))
+        if hasattr(draws, "change_seed"):
+            O.change_species_seeded(col.pos, s.pos, s.species, s.flags, self.edges, self.lj_sc.cut_sq[0],
+                                    self.lj_sc.cut_sq[2], self.prob, draws.change_seed, self.md_steps, s.id)
+        else:
+            if getattr(draws, "per_id", False):
+                # a draw that depends on (md step, id) alone: made for the flagged particles only
+                u = np.zeros(s.n)
+                cand = np.flatnonzero(s.flags & (1 << O.REAC_BIT))
+                if cand.size:
+                    u[cand] = draws.change(self.md_steps, s.id[cand])
+            else:
+                u = np.ascontiguousarray(draws.change(self.md_steps, s.id))
+            O.change_species(col.pos, s.pos, s.species, s.flags, self.edges, self.lj_sc.cut_sq[0],
+                             self.lj_sc.cut_sq[2], self.prob, u)
         self.md_steps += 1
 
     def collide(self, draws):

@@ -1506,6 +1506,40 @@ void orc_change_species(int64_t n, const double *cpos, const double *pos, int32_
     }
 }
 
+/* splitmix64 (Steele, Lea and Flood 2014): one 64-bit output per counter value */
+static uint64_t orc_splitmix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+void orc_change_species_seeded(int64_t n, const double *cpos, const double *pos,
+                               int32_t *species, int32_t *flags, const double edges[3],
+                               double cut_sq_c, double cut_sq_n, double prob, uint64_t seed,
+                               int64_t step, const int32_t *id)
+{
+    /* programs/single_dimer_pbc.f90:511-542 with the draw made where the reference makes
+       it (:533): only for a flagged particle outside both cutoffs, from a counter-based
+       stream keyed by (seed, step, particle id), so the loop draws nothing for the others */
+    ORC_PARALLEL_FOR
+    for (int64_t m = 0; m < n; ++m) {
+        if (!(flags[m] & (1 << ORC_REAC_BIT))) continue;
+        double x[3];
+        orc_rel_pos(cpos, pos + 3 * m, edges, x);
+        double dc = dot3(x, x);
+        orc_rel_pos(cpos + 3, pos + 3 * m, edges, x);
+        double dn = dot3(x, x);
+        if (dc > cut_sq_c && dn > cut_sq_n) {
+            uint64_t r = orc_splitmix64(seed ^ orc_splitmix64(((uint64_t)step << 32) ^ (uint32_t)id[m]));
+            double u = (double)(r >> 11) * 0x1.0p-53;
+            if (u <= prob) species[m] = 2;
+            flags[m] &= ~(1 << ORC_REAC_BIT);
+        }
+    }
+}
+
 void orc_refuel(int64_t n, const double *cpos, const double *pos, int32_t *species,
                 const double edges[3], int L1)
 {

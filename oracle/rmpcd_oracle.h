@@ -224,6 +224,12 @@ void orc_change_species(int64_t n, const double *cpos, const double *pos,
                         int32_t *species, int32_t *flags, const double edges[3],
                         double cut_sq_c, double cut_sq_n, double prob,
                         const double *draws);
+/* the same with the draw made inline, only for the particles that convert or not (the
+   reference's order of draws is not kept: for timing loops, not for parity) */
+void orc_change_species_seeded(int64_t n, const double *cpos, const double *pos,
+                               int32_t *species, int32_t *flags, const double edges[3],
+                               double cut_sq_c, double cut_sq_n, double prob, uint64_t seed,
+                               int64_t step, const int32_t *id);
 void orc_refuel(int64_t n, const double *cpos, const double *pos, int32_t *species,
                 const double edges[3], int L1);
 /* the fixed-sphere driver's helpers (test/setup_single_catalytic_fixed_sphere.f90:311-349) */
