@@ -181,6 +181,13 @@ int rmpcd_bulk_reaction_endothermic(const rmpcd_box_t *box, const rmpcd_cells_t 
 /* compute_temperature (src/mpcd.f90:342-407) */
 int rmpcd_compute_temperature(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
                               const rmpcd_particles_t *p, double *te);
+/* compute_temperature with its optional z profile tz (src/mpcd.f90:342-407, the binning at
+ * :395-398 through profile_t%bin, src/common.f90:196-208): each cell's kinetic term is
+ * added to tz_data[floor((z - xmin) / dx)] and counted in tz_count (device arrays of n_bins,
+ * accumulated, not cleared), z = origin(3) + (cell(3) + 1/2) a */
+int rmpcd_compute_temperature_profile(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                                      const rmpcd_particles_t *p, double *te, double xmin, double dx,
+                                      int32_t n_bins, double *tz_data, int32_t *tz_count);
 /* mpcd_stream_periodic (src/mpcd.f90:440-457) */
 int rmpcd_mpcd_stream_periodic(const rmpcd_box_t *box, rmpcd_particles_t *p, double dt);
 /* bulk_reaction (src/mpcd.f90:605-643); draws: NULL or device array, one per cell */
@@ -403,9 +410,12 @@ int rmpcd_dimer_stream_time(rmpcd_dimer_t *sim, double *avg_ms, int32_t *samples
  * RMPCD_PROG_SPHERE    test/setup_single_catalytic_fixed_sphere.f90:89-253: a fixed
  *                      catalytic sphere, solvent Verlet with neighbour-list forces,
  *                      flag/change at the sphere, SRD and the bulk B->A reaction
- * RMPCD_PROG_COLLOIDS  programs/n_colloids_pbc.f90:123-346 (production loop, no
- *                      thermostat phase): many LJ colloids, lazy cell-wise streaming
- *                      (cell_md_pos/vel), colloid-colloid n^2 forces, SRD
+ * RMPCD_PROG_COLLOIDS  programs/n_colloids_pbc.f90:123-346: many LJ colloids, lazy
+ *                      cell-wise streaming (cell_md_pos/vel), colloid-colloid n^2
+ *                      forces; the first n_thermo cycles are the thermostat phase
+ *                      (:227-280: MPC-AT at T, rescale_cells every 10th), the later
+ *                      ones the production loop (:286-346: SRD, or MPC-AT with
+ *                      thermostat, at T ramped to T_final over n_loop cycles)
  * Each runs the driver's call sequence through the entry points above. */
 #define RMPCD_PROG_FLUID 0
 #define RMPCD_PROG_SPHERE 1
@@ -418,7 +428,7 @@ typedef struct rmpcd_program_config {
     int32_t form;             /* RMPCD_FP64 or RMPCD_FP32 for the solvent */
     int32_t n_colloids;       /* COLLOIDS */
     int32_t nmax_list;        /* 0: the driver's capacity */
-    int32_t pad_;
+    int32_t n_thermo;         /* COLLOIDS: thermostat-phase cycles (N_thermo_loop) */
     double T, tau;
     double sigma, epsilon;    /* solvent-colloid LJ (SPHERE, COLLOIDS) */
     double epsilon_colloids;  /* COLLOIDS: colloid-colloid LJ, sigma 2 sigma, cut 6 sigma */
@@ -426,6 +436,14 @@ typedef struct rmpcd_program_config {
     double bulk_rate;         /* SPHERE: bulk B->A rate */
     double skin;              /* 0: the driver's value (SPHERE 1.5, COLLOIDS 0.8) */
     uint64_t seed;
+    /* COLLOIDS production loop: T = T + (i - 1) (T_final - T) / (n_loop - 1) at loop i
+     * (n_loop < 2: T throughout); thermostat: MPC-AT instead of SRD; hydro: the MPC-AT
+     * keeps each cell's mean velocity (both phases) */
+    double T_final;
+    int32_t n_loop;
+    int32_t thermostat;
+    int32_t hydro;
+    int32_t pad2_;
 } rmpcd_program_config_t;
 typedef struct rmpcd_program_stats {
     int64_t n_solvent, n_colloids, md_steps, cycles, n_extra_sorting;

@@ -439,6 +439,12 @@ def change_species(cpos, pos, species, flags, edges, cut_sq_c, cut_sq_n, prob, d
                              D(cut_sq_c), D(cut_sq_n), D(prob), P(draws))
 
 
+def change_species_seeded(cpos, pos, species, flags, edges, cut_sq_c, cut_sq_n, prob, seed, step, ids):
+    ea = (C.c_double * 3)(*[float(v) for v in edges])
+    lib().orc_change_species_seeded(I64(pos.shape[0]), P(cpos), P(pos), P(species), P(flags), ea,
+                                    D(cut_sq_c), D(cut_sq_n), D(prob), C.c_uint64(seed), I64(step), P(ids))
+
+
 def refuel(cpos, pos, species, edges, L1):
     ea = (C.c_double * 3)(*[float(v) for v in edges])
     lib().orc_refuel(I64(pos.shape[0]), P(cpos), P(pos), P(species), ea, int(L1))

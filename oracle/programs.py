@@ -7,7 +7,7 @@ Each oracle starts from a state downloaded from the device program
 Threefry draws (oracle.threefry), so that both advance the same trajectory:
     FluidOracle     test/setup_simple_fluid.f90:108-118
     SphereOracle    test/setup_single_catalytic_fixed_sphere.f90:188-253, helpers :311-349
-    ColloidsOracle  programs/n_colloids_pbc.f90:286-346
+    ColloidsOracle  programs/n_colloids_pbc.f90:227-346 (thermostat phase and production)
 """
 from __future__ import annotations
 
@@ -155,7 +155,11 @@ class SphereOracle(_WithColloids):
 class ColloidsOracle(_WithColloids):
     """many colloids with lazy streaming (n_colloids_pbc.f90:286-346)"""
 
-    def __init__(self, st, L, tau, n_md, seed, sigma=2.5, epsilon=1.0, epsilon_colloids=1.0, skin=0.8):
+    def __init__(self, st, L, tau, n_md, seed, sigma=2.5, epsilon=1.0, epsilon_colloids=1.0, skin=0.8, T=1.0,
+                 n_thermo=0, T_final=None, n_loop=0, thermostat=False, hydro=True):
+        self.T0, self.n_thermo, self.n_loop = T, n_thermo, n_loop
+        self.T_final = T if T_final is None else T_final
+        self.thermostat, self.hydro = thermostat, hydro
         sc = sigma * 2 ** (1 / 6)
         lj = O.lj_params([[epsilon]], [[sigma]], [[sc]])
         self.lj_cc = O.lj_params([[epsilon_colloids]], [[2 * sigma]], [[2 * (3 * sigma)]])
@@ -199,12 +203,29 @@ class ColloidsOracle(_WithColloids):
         O.colloid_md_vel(col.vel, col.force, col.force_old, self.mass, dt)
         self.md_steps += 1
 
+    def mpcat(self, temp):
+        normals = T.normals3(self.seed, self.cycles, self.s.id)
+        O.mpcat_step(self.c, self.cnt, self.st, self.s.vel, self.s.species, np.ascontiguousarray(normals), temp,
+                     self.hydro)
+
     def collide(self):
         self.shift(self.cycles)
         self.catch_up(self.md_steps - self.last_sort)
         self.resort()
         self.last_sort = self.md_steps
-        self.srd()
+        if self.cycles < self.n_thermo:
+            # thermostat phase (n_colloids_pbc.f90:276-279)
+            self.mpcat(self.T0)
+            if (self.cycles + 1) % 10 == 0:
+                O.rescale_cells(self.c, self.cnt, self.st, self.s.vel, self.s.species, self.T0)
+        else:
+            # production loop i with the temperature ramp (:334-336)
+            i = self.cycles - self.n_thermo + 1
+            temp = self.T0 + (i - 1) * (self.T_final - self.T0) / (self.n_loop - 1) if self.n_loop > 1 else self.T0
+            if self.thermostat:
+                self.mpcat(temp)
+            else:
+                self.srd()
         self.temperature = O.compute_temperature(self.c, self.cnt, self.st, self.s.vel, self.s.species)
         self.cycles += 1
 

@@ -11,6 +11,7 @@ this module restates it so that the oracle's time loop can consume the same draw
     key = (seed + replica, stream)
     grid shift of collision s    : ctr (s, 0) -> u0 = u01(a), u1 = u01(b); ctr (s, 1) -> u2 = u01(a)
     SRD axis of cell c, cycle s  : Marsaglia pairs k = 0, 1, ... from ctr (s, c + k 2^40)
+    MPC-AT normals, id, cycle s  : Box-Muller on ctr (2s, id) and (2s + 1, id), stream THERMO
     change_species, md step t    : ctr (t, id) -> u01(a)
     bulk_reaction, cell c        : ctr (s, c) -> u01(a)
 """
@@ -22,7 +23,7 @@ MASK = (1 << 64) - 1
 ROT = (16, 42, 12, 31, 16, 32, 24, 21)
 PARITY = 0x1BD11BDAA9FC1A22
 
-SRD, REACTION, CHANGE, UNIFORM, SHIFT, INIT, PLACE = 1, 2, 3, 4, 5, 6, 7
+SRD, REACTION, CHANGE, UNIFORM, SHIFT, INIT, PLACE, THERMO = 1, 2, 3, 4, 5, 6, 7, 8
 
 
 def _rotl(x, r):
@@ -60,6 +61,18 @@ def shift_uniforms(seed, s):
     return np.array([u01(a), u01(b), u01(c)], np.float64)
 
 
+def normals3(seed, step, ids):
+    """the three normals of each particle id for the MPC-AT draw of step: Box-Muller on
+    the uniforms of counters (2 step, id) and (2 step + 1, id), stream THERMO"""
+    ids = np.asarray(ids).astype(np.uint64)
+    a, b = threefry2x64(2 * step, ids, seed, THERMO)
+    c, d = threefry2x64(2 * step + 1, ids, seed, THERMO)
+    u1, u2, u3, u4 = 1.0 - u01(a), u01(b), 1.0 - u01(c), u01(d)
+    r1 = np.sqrt(-2.0 * np.log(u1))
+    r2 = np.sqrt(-2.0 * np.log(u3))
+    return np.stack([r1 * np.cos(2 * np.pi * u2), r1 * np.sin(2 * np.pi * u2), r2 * np.cos(2 * np.pi * u4)], axis=1)
+
+
 def srd_axes(seed, step, ncells):
     """dense (ncells, 3) unit vectors: Marsaglia (1972) from threefry pairs"""
     cells = np.arange(ncells, dtype=np.uint64)
@@ -82,6 +95,8 @@ def srd_axes(seed, step, ncells):
 
 class ThreefryDraws:
     """the device program's draws, for the oracle's time loop (oracle.dimer.DimerOracle)"""
+
+    per_id = True  # change(md_step, ids) draws from (md_step, id) alone
 
     def __init__(self, seed):
         self.seed = int(seed)

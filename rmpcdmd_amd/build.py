@@ -37,7 +37,7 @@ FBIN = os.path.join(FORTRAN, "bin")
 FLIB = os.path.join(FORTRAN, "librmpcd_fortran.so")
 AMDFLANG = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
 FPROGRAMS = ["fortran_cells", "fortran_fluid", "fortran_dimer", "fortran_walls", "fortran_bodies",
-             "fortran_params", "single_dimer_pbc"]
+             "fortran_params", "single_dimer_pbc", "fortran_shims"]
 
 
 def _run(cmd, what):
@@ -52,8 +52,9 @@ def build_fortran(verbose=False):
     os.makedirs(mod, exist_ok=True)
     os.makedirs(FBIN, exist_ok=True)
     objs = []
-    for src in ("rmpcd_capi.f90", "rmpcdmd.f90", "parsetext.f90"):
-        out = os.path.join(OBJ, src.replace(".f90", ".o"))
+    for src in ("rmpcd_capi.f90", "shims/hdf5.f90", "shims/h5md_module.f90", "shims/tester.f90", "rmpcdmd.f90",
+                "particle_system_io.f90", "rmpcdmd_module.f90", "parsetext.f90"):
+        out = os.path.join(OBJ, os.path.basename(src).replace(".f90", ".o"))
         _run([AMDFLANG, "-c", "-O2", "-fPIC", "-J", mod, os.path.join(FORTRAN, src), "-o", out], src)
         objs.append(out)
     _run([AMDFLANG, "-shared", "-fPIC", "-o", FLIB, *objs, "-L" + HERE, "-lrmpcd_amd",

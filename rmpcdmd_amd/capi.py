@@ -99,10 +99,11 @@ class RigidBody(C.Structure):
 
 class ProgramConfig(C.Structure):
     _fields_ = [("kind", C.c_int32), ("L", C.c_int32 * 3), ("rho", C.c_int32), ("n_md", C.c_int32),
-                ("form", C.c_int32), ("n_colloids", C.c_int32), ("nmax_list", C.c_int32), ("pad_", C.c_int32),
+                ("form", C.c_int32), ("n_colloids", C.c_int32), ("nmax_list", C.c_int32), ("n_thermo", C.c_int32),
                 ("T", C.c_double), ("tau", C.c_double), ("sigma", C.c_double), ("epsilon", C.c_double),
                 ("epsilon_colloids", C.c_double), ("prob", C.c_double), ("bulk_rate", C.c_double),
-                ("skin", C.c_double), ("seed", C.c_uint64)]
+                ("skin", C.c_double), ("seed", C.c_uint64), ("T_final", C.c_double), ("n_loop", C.c_int32),
+                ("thermostat", C.c_int32), ("hydro", C.c_int32), ("pad2_", C.c_int32)]
 
 
 class ProgramStats(C.Structure):
@@ -138,6 +139,7 @@ SIGNATURES = {
     "rmpcd_rand_sphere": [_P, _I64, _I64, _P, _P],
     "rmpcd_simple_mpcd_step": [_BOXP, _CELLP, _PARTP, _P, _U64, _U64, _I, _D, _I, _D, _I],
     "rmpcd_compute_temperature": [_BOXP, _CELLP, _PARTP, _D3],
+    "rmpcd_compute_temperature_profile": [_BOXP, _CELLP, _PARTP, _D3, C.c_double, C.c_double, _I, _P, _P],
     "rmpcd_mpcd_stream_periodic": [_BOXP, _PARTP, _D],
     "rmpcd_bulk_reaction": [_BOXP, _CELLP, _PARTP, _I, _I, _D, _D, _P, _U64, _U64],
     "rmpcd_mpcat_step": [_BOXP, _CELLP, _PARTP, _P, _U64, _U64, _D, _I],

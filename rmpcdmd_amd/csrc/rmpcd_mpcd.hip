@@ -6,7 +6,7 @@
 // Cell kernels work on the cell-sorted solvent: a workgroup owns a run of consecutive
 // cells, stages their velocities and species from HBM into LDS with coalesced loads,
 // each thread then reduces one cell in the reference's summation order (FP32 form: with
-// Kahan compensation), and the rotated velocities go back to HBM coalesced.  Cells whose
+// TwoSum compensation), and the rotated velocities go back to HBM coalesced.  Cells whose
 // particles do not fit the tile fall back to direct loads.
 #include <float.h>
 
@@ -16,23 +16,6 @@ namespace rmpcd {
 
 constexpr int CELL_THREADS = 128;          // cells per workgroup
 constexpr int TILE_CAP = 2048;             // staged particles per workgroup
-
-template <class R>
-struct Acc3 {  // running sum; compensated in FP32
-    R s[3] = {0, 0, 0};
-    R c[3] = {0, 0, 0};
-    __device__ inline void add(int k, R v)
-    {
-        if constexpr (sizeof(R) == 4) {
-            R y = v - c[k];
-            R t = s[k] + y;
-            c[k] = (t - s[k]) - y;
-            s[k] = t;
-        } else {
-            s[k] = s[k] + v;
-        }
-    }
-};
 
 template <class F>
 struct CellTile {
@@ -116,7 +99,7 @@ __device__ inline void srd_cell(R *vx, R *vy, R *vz, const S *sp, int n, int c, 
         }
     }
     if (n_eff <= 1) return;
-    const R u0 = acc.s[0] / (R)n_eff, u1 = acc.s[1] / (R)n_eff, u2 = acc.s[2] / (R)n_eff;
+    const R u0 = acc.sum(0) / (R)n_eff, u1 = acc.sum(1) / (R)n_eff, u2 = acc.sum(2) / (R)n_eff;
     double v0, v1, v2;
     if (axes) {
         v0 = axes[c];
@@ -130,27 +113,10 @@ __device__ inline void srd_cell(R *vx, R *vy, R *vz, const S *sp, int n, int c, 
     }
     const double omc = 1 - cos_a;
     // omega(r,k) as filled column-major by mpcd.f90:163-171
-    const R o00 = (R)(cos_a + (v0 * v0) * omc), o10 = (R)((v0 * v1) * omc - v2 * sin_a),
-            o20 = (R)((v0 * v2) * omc + v1 * sin_a);
-    const R o01 = (R)((v1 * v0) * omc + v2 * sin_a), o11 = (R)(cos_a + (v1 * v1) * omc),
-            o21 = (R)((v1 * v2) * omc - v0 * sin_a);
-    const R o02 = (R)((v2 * v0) * omc - v1 * sin_a), o12 = (R)((v2 * v1) * omc + v0 * sin_a),
-            o22 = (R)(cos_a + (v2 * v2) * omc);
-    for (int i = 0; i < n; ++i) {
-        const R w0 = vx[i] - u0, w1 = vy[i] - u1, w2 = vz[i] - u2;
-        R r0 = o00 * w0;
-        r0 = r0 + o01 * w1;
-        r0 = r0 + o02 * w2;
-        R r1 = o10 * w0;
-        r1 = r1 + o11 * w1;
-        r1 = r1 + o12 * w2;
-        R r2 = o20 * w0;
-        r2 = r2 + o21 * w1;
-        r2 = r2 + o22 * w2;
-        vx[i] = u0 + r0;
-        vy[i] = u1 + r1;
-        vz[i] = u2 + r2;
-    }
+    const R o[9] = {(R)(cos_a + (v0 * v0) * omc), (R)((v1 * v0) * omc + v2 * sin_a), (R)((v2 * v0) * omc - v1 * sin_a),
+                    (R)((v0 * v1) * omc - v2 * sin_a), (R)(cos_a + (v1 * v1) * omc), (R)((v2 * v1) * omc + v0 * sin_a),
+                    (R)((v0 * v2) * omc + v1 * sin_a), (R)((v1 * v2) * omc - v0 * sin_a), (R)(cos_a + (v2 * v2) * omc)};
+    srd_rotate(vx, vy, vz, n, o, u0, u1, u2, acc, n_eff == n);
 }
 
 template <class F>
@@ -237,7 +203,7 @@ __global__ void __launch_bounds__(MPCAT_THREADS)
                             ++n_eff;
                         }
                     if (n_eff > 1) {
-                        for (int k = 0; k < 3; ++k) lu[k][tid] = acc.s[k] / (R)n_eff;
+                        for (int k = 0; k < 3; ++k) lu[k][tid] = acc.sum(k) / (R)n_eff;
                         lneff[tid] = n_eff;
                     }
                 }
@@ -267,8 +233,8 @@ __global__ void __launch_bounds__(MPCAT_THREADS)
                     vir.add(1, ly[o + i]);
                     vir.add(2, lz[o + i]);
                 }
-            const R d[3] = {lu[0][tid] - vir.s[0] / n_eff, lu[1][tid] - vir.s[1] / n_eff,
-                            lu[2][tid] - vir.s[2] / n_eff};
+            const R d[3] = {lu[0][tid] - vir.sum(0) / n_eff, lu[1][tid] - vir.sum(1) / n_eff,
+                            lu[2][tid] - vir.sum(2) / n_eff};
             for (int i = 0; i < n; ++i)
                 if ((int)lsp[o + i] > 0) {
                     lx[o + i] = lx[o + i] + d[0];
@@ -299,7 +265,7 @@ __global__ void __launch_bounds__(MPCAT_THREADS)
             ++n_eff;
         }
     if (n_eff <= 1) return;
-    const R u[3] = {acc.s[0] / (R)n_eff, acc.s[1] / (R)n_eff, acc.s[2] / (R)n_eff};
+    const R u[3] = {acc.sum(0) / (R)n_eff, acc.sum(1) / (R)n_eff, acc.sum(2) / (R)n_eff};
     R d[3] = {0, 0, 0};
     if (hydro) {
         Acc3<R> vir;
@@ -309,7 +275,7 @@ __global__ void __launch_bounds__(MPCAT_THREADS)
             draw(s + i, g);
             for (int k = 0; k < 3; ++k) vir.add(k, (R)(g[k] * t_factor));
         }
-        for (int k = 0; k < 3; ++k) d[k] = u[k] - vir.s[k] / (R)n_eff;
+        for (int k = 0; k < 3; ++k) d[k] = u[k] - vir.sum(k) / (R)n_eff;
     }
     for (int i = 0; i < n; ++i) {
         if ((int)p.species[s + i] <= 0) continue;
@@ -439,7 +405,7 @@ __device__ inline void wall_cell(R *vx, R *vy, R *vz, int64_t s, int n, int c, c
     }
     R u[3];
     for (int k = 0; k < 3; ++k)
-        u[k] = n_virtual > 0 ? (R)(((double)acc.s[k] + vv[k]) / w.nw[wi]) : acc.s[k] / (R)n;
+        u[k] = n_virtual > 0 ? (R)(((double)acc.sum(k) + vv[k]) / w.nw[wi]) : acc.sum(k) / (R)n;
     if (w.thermostat) {
         for (int i = 0; i < n; ++i) {
             double g[3];
@@ -459,7 +425,7 @@ __device__ inline void wall_cell(R *vx, R *vy, R *vz, int64_t s, int n, int c, c
                 sum.add(1, vy[i]);
                 sum.add(2, vz[i]);
             }
-            const R d[3] = {u[0] - sum.s[0] / (R)n, u[1] - sum.s[1] / (R)n, u[2] - sum.s[2] / (R)n};
+            const R d[3] = {u[0] - sum.sum(0) / (R)n, u[1] - sum.sum(1) / (R)n, u[2] - sum.sum(2) / (R)n};
             for (int i = 0; i < n; ++i) {
                 vx[i] = vx[i] + d[0];
                 vy[i] = vy[i] + d[1];
@@ -554,7 +520,7 @@ __device__ inline double temperature_cell(const R *vx, const R *vy, const R *vz,
             u2 = u2 + vz[i];
             ++n_eff;
         }
-    if (n_eff <= 1) return 0;
+    if (n_eff <= 1) return -1;  // the cell counts, but adds nothing (mpcd.f90:385)
     u0 = u0 / n;  // divided by n (mpcd.f90:388)
     u1 = u1 / n;
     u2 = u2 / n;
@@ -567,12 +533,22 @@ __device__ inline double temperature_cell(const R *vx, const R *vy, const R *vz,
     return k / (double)(3 * (n_eff - 1));
 }
 
+// the optional z profile of compute_temperature (mpcd.f90:354-358, 395-398): a cell's
+// kinetic term is added to bin floor((z - xmin) / dx) of data and counted in count, with z
+// the height of the cell's centre, origin(3) + (cell(3) + 1/2) a
+struct TzProfile {
+    double xmin, dx;
+    int n;
+    double *data;
+    int32_t *count;
+};
+
 // per workgroup: the sum of the cells' kinetic terms and the number of cells with more
 // than one particle (the count goes up before the n_eff test, mpcd.f90:374-386)
 template <class F>
 __global__ void __launch_bounds__(CELL_THREADS)
     k_temperature(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
-                  PView<F> p, double *__restrict__ part)
+                  PView<F> p, double *__restrict__ part, Box b, TzProfile tz)
 {
     using R = typename F::real;
     __shared__ R lx[TILE_CAP], ly[TILE_CAP], lz[TILE_CAP];
@@ -593,6 +569,18 @@ __global__ void __launch_bounds__(CELL_THREADS)
             kin = temperature_cell(lx + o, ly + o, lz + o, lsp + o, n);
         } else {
             kin = temperature_cell(p.vel + s, p.vel + p.n + s, p.vel + 2 * p.n + s, p.species + s, n);
+        }
+        if (kin < 0) {
+            kin = 0;  // n_eff <= 1
+        } else if (tz.data) {
+            int q[3];
+            compact_point(c, b.M[0], b.M[1], b.M[2], q);
+            const double z = b.origin[2] + (q[2] + 0.5) * b.a;
+            const double f = floor((z - tz.xmin) / tz.dx);
+            if (f >= 0 && f < tz.n) {
+                atomicAdd(&tz.data[(int)f], kin);
+                atomicAdd(&tz.count[(int)f], 1);
+            }
         }
     }
     double bk = block_sum(kin, red);
@@ -981,7 +969,7 @@ int stream_yzwall_impl(const rmpcd_box_t *box, rmpcd_particles_t *pp, double dt,
 
 template <class F>
 int temperature_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
-                     const rmpcd_particles_t *pp, double *te)
+                     const rmpcd_particles_t *pp, double *te, TzProfile tz)
 {
     int nc = box->ncells;
     unsigned nb = blocks_for(nc, CELL_THREADS);
@@ -990,7 +978,8 @@ int temperature_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
     double *host = (double *)pinned(64);
     if (!part || !red || !host) return set_error(RMPCD_ERR_HIP, "compute_temperature: workspace allocation failed");
     hipStream_t s = stream();
-    k_temperature<F><<<nb, CELL_THREADS, 0, s>>>(nc, cells->cell_start, cells->cell_count, view<F>(pp), part);
+    k_temperature<F><<<nb, CELL_THREADS, 0, s>>>(nc, cells->cell_start, cells->cell_count, view<F>(pp), part,
+                                                 make_box(box), tz);
     RMPCD_LAUNCHED();
     k_temperature_final<<<1, CELL_THREADS, 0, s>>>(part, nb, red);
     RMPCD_LAUNCHED();
@@ -1078,7 +1067,23 @@ int rmpcd_compute_temperature(const rmpcd_box_t *box, const rmpcd_cells_t *cells
     if (rc) return rc;
     RMPCD_CHECK(cells && cells->cell_count && cells->cell_start && te,
                 "compute_temperature: null argument");
-    return RMPCD_DISPATCH_FORM(p, temperature_impl, box, cells, p, te);
+    TzProfile tz{0, 0, 0, nullptr, nullptr};
+    return RMPCD_DISPATCH_FORM(p, temperature_impl, box, cells, p, te, tz);
+}
+
+int rmpcd_compute_temperature_profile(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                                      const rmpcd_particles_t *p, double *te, double xmin, double dx,
+                                      int32_t n_bins, double *tz_data, int32_t *tz_count)
+{
+    int rc = check_box(box, "compute_temperature");
+    if (!rc) rc = check_particles(p, "compute_temperature", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start && te,
+                "compute_temperature: null argument");
+    RMPCD_CHECK(tz_data && tz_count && n_bins > 0, "profile_t: data not allocated");
+    RMPCD_CHECK(dx > 0, "negative step in profile_init");
+    TzProfile tz{xmin, dx, n_bins, tz_data, tz_count};
+    return RMPCD_DISPATCH_FORM(p, temperature_impl, box, cells, p, te, tz);
 }
 
 int rmpcd_mpcd_stream_periodic(const rmpcd_box_t *box, rmpcd_particles_t *p, double dt)

@@ -2,8 +2,9 @@
 // in the reference's call order through this library's own entry points:
 //   FLUID     test/setup_simple_fluid.f90:45-127
 //   SPHERE    test/setup_single_catalytic_fixed_sphere.f90:89-253 and its helpers :311-349
-//   COLLOIDS  programs/n_colloids_pbc.f90:123-346 (production loop; the thermostat phase
-//             needs the MPC-AT collision, which this library does not provide)
+//   COLLOIDS  programs/n_colloids_pbc.f90:123-346: the thermostat phase (:227-280) for
+//             the first n_thermo cycles, then the production loop (:286-346) with its
+//             temperature ramp
 // Random numbers come from Threefry-2x64-20 keyed by (seed, stream), with the counters
 // given next to each use, so that a test can reproduce every draw:
 //   grid shift s        threefry((s, 0)) a, b and threefry((s, 1)) a, stream SHIFT
@@ -11,6 +12,8 @@
 //   bulk reaction       threefry((cycle, cell)) a, stream REACTION
 //   sphere flag draw    threefry((md_step, id)) a, stream CHANGE
 //   colloid placement   draw k: threefry((k, 2^62)) a, stream PLACE
+//   MPC-AT, cycle s     the normals of particle id: Box-Muller on threefry((2s, id)) and
+//                       threefry((2s + 1, id)), stream THERMO
 #include <math.h>
 #include <string.h>
 
@@ -480,7 +483,21 @@ int ProgramSim::colloids_collide()
     if ((rc = build_list())) return rc;
     if ((rc = copy_pos_old())) return rc;
     s = sv();
-    if ((rc = rmpcd_simple_mpcd_step(&box, &c, &s, nullptr, seed, (uint64_t)cycles, 0, 0.0, 0, cfg.T, 1))) return rc;
+    if (cycles < cfg.n_thermo) {
+        // thermostat phase: MPC-AT at T, rescale_cells every 10th loop (:277-279)
+        if ((rc = rmpcd_simple_mpcd_step(&box, &c, &s, nullptr, seed, (uint64_t)cycles, 0, 0.0, 1, cfg.T,
+                                          cfg.hydro)))
+            return rc;
+        if ((cycles + 1) % 10 == 0 && (rc = rmpcd_rescale_cells(&box, &c, &s, cfg.T))) return rc;
+    } else {
+        // production loop i (1-based): T ramped from T to T_final over n_loop loops (:334-336)
+        const int64_t i = cycles - cfg.n_thermo + 1;
+        const double T = cfg.n_loop > 1 ? cfg.T + (double)(i - 1) * (cfg.T_final - cfg.T) / (double)(cfg.n_loop - 1)
+                                        : cfg.T;
+        if ((rc = rmpcd_simple_mpcd_step(&box, &c, &s, nullptr, seed, (uint64_t)cycles, 0, 0.0, cfg.thermostat ? 1 : 0,
+                                          T, cfg.hydro)))
+            return rc;
+    }
     if ((rc = rmpcd_compute_temperature(&box, &c, &s, &temperature))) return rc;
     ++cycles;
     return RMPCD_OK;

@@ -10,7 +10,7 @@ module ParseText
   implicit none
   private
   public :: PTo, PTparse, PTkill
-  public :: PTread_d, PTread_i, PTread_l, PTread_s, PTread_dvec, PTread_ivec, PTread_lvec
+  public :: PTread_d, PTread_i, PTread_l, PTread_s, PTread_dvec, PTread_ivec, PTread_lvec, PTread_c_int64
 
   integer, parameter :: LINE_LEN = 1024
 
@@ -101,6 +101,15 @@ contains
     read(value_of(this, name), *, iostat=iostat) x
     if (iostat /= 0) call bad_value(name)
   end function PTread_i
+
+  integer(c_int64_t) function PTread_c_int64(this, name, loc) result(x)
+    type(PTo), intent(in) :: this
+    character(len=*), intent(in) :: name
+    integer(c_int64_t), intent(in), optional :: loc
+    integer :: iostat
+    read(value_of(this, name), *, iostat=iostat) x
+    if (iostat /= 0) call bad_value(name)
+  end function PTread_c_int64
 
   logical function PTread_l(this, name, loc) result(x)
     type(PTo), intent(in) :: this

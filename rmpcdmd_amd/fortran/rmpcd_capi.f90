@@ -151,6 +151,17 @@ module rmpcd_capi
        type(rmpcd_particles_t), intent(in) :: p
        real(c_double) :: te
      end function rmpcd_compute_temperature
+     integer(c_int) function rmpcd_compute_temperature_profile(box, cells, p, te, xmin, dx, n_bins, tz_data, &
+          tz_count) bind(c)
+       import :: c_int, c_int32_t, c_double, c_ptr, rmpcd_box_t, rmpcd_particles_t, rmpcd_cells_t
+       type(rmpcd_box_t), intent(in) :: box
+       type(rmpcd_cells_t), intent(in) :: cells
+       type(rmpcd_particles_t), intent(in) :: p
+       real(c_double) :: te
+       real(c_double), value :: xmin, dx
+       integer(c_int32_t), value :: n_bins
+       type(c_ptr), value :: tz_data, tz_count
+     end function rmpcd_compute_temperature_profile
      integer(c_int) function rmpcd_mpcd_stream_periodic(box, p, dt) bind(c)
        import :: c_int, c_double, rmpcd_box_t, rmpcd_particles_t
        type(rmpcd_box_t), intent(in) :: box

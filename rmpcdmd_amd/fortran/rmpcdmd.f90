@@ -1,6 +1,6 @@
 ! rmpcdmd.f90 -- the RMPCDMD module API (common, hilbert, threefry_module, interaction,
-! cell_system, particle_system, neighbor_list, mpcd, md, rmpcdmd_module) over the HIP
-! library.  Types keep the reference's public members as host arrays, so drivers read and
+! cell_system, particle_system, neighbor_list, mpcd, md; rmpcdmd_module, which also
+! brings particle_system_io, is in rmpcdmd_module.f90) over the HIP library.  Types keep the reference's public members as host arrays, so drivers read and
 ! write them as before; each library routine moves the arrays it needs to the device,
 ! runs the kernels through the C ABI and brings back what changed.  Indices stay 1-based
 ! on the host (ids, cell_start, list entries) and 0-based on the device.
@@ -14,7 +14,8 @@ module common
   use iso_c_binding
   implicit none
   private
-  public :: rel_pos, switch, timer_t, pi
+  public :: rel_pos, switch, timer_t, pi, cross, profile_t, histogram_t
+  public :: timer_list_t, args_t, get_input_filename, get_input_args, numbered_string, max_path_length
   public :: REAC_BIT, MD_BIT, WALL_BIT, PAST_MD_BIT, OUTBOUND_BIT, CATALYZED_BIT, ENZYME_REGION_BIT
   public :: REAC_MASK, MD_MASK, WALL_MASK, CATALYZED_MASK, ENZYME_REGION_MASK
 
@@ -25,6 +26,8 @@ module common
   integer, parameter :: REAC_MASK = 2, MD_MASK = 4, WALL_MASK = 8, CATALYZED_MASK = 64
   integer, parameter :: ENZYME_REGION_MASK = 128
 
+  integer, parameter :: max_path_length = 255
+
   type timer_t
      double precision :: tic_time = 0
      double precision :: total = 0
@@ -34,6 +37,60 @@ module common
      procedure :: tic
      procedure :: tac
   end type timer_t
+
+  !> timers collected for output (src/common.f90:112-126, 365-402)
+  type timer_pointer_t
+     type(timer_t), pointer :: p => null()
+  end type timer_pointer_t
+  type timer_list_t
+     type(timer_pointer_t), allocatable :: timers(:)
+     integer :: current_idx
+   contains
+     generic, public :: init => timer_list_init
+     procedure, private :: timer_list_init
+     generic, public :: append => timer_list_append
+     procedure, private :: timer_list_append
+     generic, public :: write => timer_list_write
+     procedure, private :: timer_list_write
+  end type timer_list_t
+
+  !> the drivers' command line: input output seed (src/common.f90:133-137, 308-333)
+  type args_t
+     character(len=max_path_length) :: input_file
+     character(len=max_path_length) :: output_file
+     integer(c_int64_t) :: seed
+  end type args_t
+
+  !> binned averages along one coordinate (src/common.f90:66-80, 180-225)
+  type profile_t
+     double precision, allocatable :: data(:)
+     integer, allocatable :: count(:)
+     double precision :: xmin
+     double precision :: dx
+     integer :: n
+   contains
+     generic, public :: init => profile_init
+     procedure, private :: profile_init
+     generic, public :: bin => profile_bin
+     procedure, private :: profile_bin
+     generic, public :: reset => profile_reset
+     procedure, private :: profile_reset
+     generic, public :: norm => profile_norm
+     procedure, private :: profile_norm
+  end type profile_t
+
+  !> histogram per species along one coordinate (src/common.f90:82-95, 227-262)
+  type histogram_t
+     double precision, allocatable :: data(:,:)
+     double precision :: xmin
+     double precision :: dx
+     integer :: n, n_species
+   contains
+     generic, public :: init => histogram_init
+     procedure, private :: histogram_init
+     generic, public :: bin => histogram_bin
+     procedure, private :: histogram_bin
+  end type histogram_t
 
   interface switch
      module procedure :: switch_d2
@@ -81,6 +138,152 @@ contains
     p1 => p2
     p2 => p
   end subroutine switch_i2
+
+  !> the cross product x1 x x2 (src/common.f90:404-412)
+  pure function cross(x1, x2) result(r)
+    double precision, intent(in) :: x1(3), x2(3)
+    double precision :: r(3)
+    r(1) = x1(2)*x2(3) - x1(3)*x2(2)
+    r(2) = x1(3)*x2(1) - x1(1)*x2(3)
+    r(3) = x1(1)*x2(2) - x1(2)*x2(1)
+  end function cross
+
+  subroutine profile_init(this, xmin, xmax, n)
+    class(profile_t), intent(out) :: this
+    double precision, intent(in) :: xmin, xmax
+    integer, intent(in) :: n
+    this% n = n
+    this% xmin = xmin
+    this% dx = (xmax - xmin) / n
+    if (this% dx <= 0) error stop 'negative step in profile_init'
+    allocate(this% data(n))
+    this% data = 0
+    allocate(this% count(n))
+    this% count = 0
+  end subroutine profile_init
+
+  !> bin floor((x - xmin)/dx) + 1 gets value and one count when it is in range
+  subroutine profile_bin(this, x, value)
+    class(profile_t), intent(inout) :: this
+    double precision, intent(in) :: x, value
+    integer :: idx
+    idx = floor( (x - this% xmin) / this% dx ) + 1
+    if ( ( idx > 0 ) .and. ( idx <= this% n ) ) then
+       this% data(idx) = this% data(idx) + value
+       this% count(idx) = this% count(idx) + 1
+    end if
+  end subroutine profile_bin
+
+  subroutine profile_reset(this)
+    class(profile_t), intent(inout) :: this
+    this% data = 0
+    this% count = 0
+  end subroutine profile_reset
+
+  subroutine profile_norm(this)
+    class(profile_t), intent(inout) :: this
+    where (this% count > 0)
+       this% data = this% data / this% count
+    end where
+  end subroutine profile_norm
+
+  subroutine histogram_init(this, xmin, xmax, n, n_species)
+    class(histogram_t), intent(out) :: this
+    double precision, intent(in) :: xmin, xmax
+    integer, intent(in) :: n
+    integer, optional, intent(in) :: n_species
+    this% n_species = 1
+    if (present(n_species)) this% n_species = n_species
+    this% n = n
+    this% xmin = xmin
+    this% dx = (xmax - xmin) / n
+    if (this% dx <= 0) error stop 'negative step in histogram_init'
+    allocate(this% data(this% n_species, n))
+    this% data = 0
+  end subroutine histogram_init
+
+  subroutine histogram_bin(this, x, s)
+    class(histogram_t), intent(inout) :: this
+    double precision, intent(in) :: x
+    integer, intent(in), optional :: s
+    integer :: idx, s_var
+    s_var = 1
+    if (present(s)) s_var = s
+    idx = floor( (x - this% xmin) / this% dx ) + 1
+    if ( ( idx > 0 ) .and. ( idx <= this% n ) ) then
+       this% data(s_var, idx) = this% data(s_var, idx) + 1
+    end if
+  end subroutine histogram_bin
+
+  character(len=max_path_length) function get_input_filename() result(r)
+    if (command_argument_count() < 1) error stop 'missing argument for parameter file'
+    call get_command_argument(1, r)
+  end function get_input_filename
+
+  type(args_t) function get_input_args() result(args)
+    integer :: iostat
+    character(max_path_length) :: r
+    if (command_argument_count() /= 3) then
+       write(*,*) 'Usage: program input output seed'
+       write(*,*) '    input: the parameter file; output: the name of the H5MD output file;'
+       write(*,*) '    seed: a signed 64-bit integer'
+       error stop
+    end if
+    call get_command_argument(1, args%input_file)
+    call get_command_argument(2, args%output_file)
+    call get_command_argument(3, r)
+    read(r, *, iostat=iostat) args%seed
+    if (iostat /= 0) then
+       write(*,*) 'Error when reading the seed value (third command-line argument)'
+       error stop
+    end if
+  end function get_input_args
+
+  subroutine timer_list_init(this, n)
+    class(timer_list_t), intent(out) :: this
+    integer, intent(in) :: n
+    allocate(this%timers(n))
+    this%current_idx = 0
+  end subroutine timer_list_init
+
+  subroutine timer_list_append(this, timer_target)
+    class(timer_list_t), intent(inout) :: this
+    type(timer_t), target, intent(in) :: timer_target
+    this%current_idx = this%current_idx + 1
+    if (this%current_idx > size(this%timers)) error stop 'exceeded timer_list_t size'
+    this%timers(this%current_idx)%p => timer_target
+  end subroutine timer_list_append
+
+  !> one dataset per timer (its total) in group, and their sum in total_out
+  subroutine timer_list_write(this, group, total_out)
+    use hdf5, only: HID_T
+    use h5md_module, only: h5md_write_dataset
+    class(timer_list_t), intent(inout) :: this
+    integer(HID_T), intent(inout) :: group
+    double precision, optional, intent(out) :: total_out
+    double precision :: total
+    integer :: i
+    total = 0
+    do i = 1, size(this%timers)
+       if (associated(this%timers(i)%p)) then
+          call h5md_write_dataset(group, trim(this%timers(i)%p%name), this%timers(i)%p%total)
+          total = total + this%timers(i)%p%total
+       end if
+    end do
+    if (present(total_out)) total_out = total
+  end subroutine timer_list_write
+
+  !> base_string followed by index on length digits with leading zeros (src/common.f90:447-462)
+  function numbered_string(base_string, index, length) result(s)
+    character(len=*), intent(in) :: base_string
+    integer, intent(in) :: index
+    integer, intent(in) :: length
+    character(len=:), allocatable :: s
+    character(len=16) :: fmt
+    allocate(character(len=len_trim(base_string) + length) :: s)
+    write(fmt, '(a,i0,a,i0,a)') '(a,i', length, '.', length, ')'
+    write(s, fmt) trim(base_string), index
+  end function numbered_string
 
   subroutine timer_init(this, name, system_name)
     class(timer_t), intent(out) :: this
@@ -594,7 +797,7 @@ module particle_system
   use cell_system
   implicit none
   private
-  public :: particle_system_t, cell_maximum_displacement
+  public :: particle_system_t, cell_maximum_displacement, compute_radial_histogram, correct_radial_histogram
 
   type particle_system_t
      integer :: Nmax
@@ -620,6 +823,7 @@ module particle_system
      type(dbuf_t) :: d_aux, d_id_to_idx
    contains
      procedure :: init
+     procedure :: init_from_file
      procedure :: random_placement
      procedure :: sort
      procedure :: maximum_displacement
@@ -629,6 +833,65 @@ module particle_system
   end type particle_system_t
 
 contains
+
+  !> particle_system%init_from_file (src/particle_system.f90:186-248): positions (and
+  !! species, when stored) of particles/<group_name> in a file written by h5md_module,
+  !! either the fixed element (mode H5MD_FIXED) or frame idx (0-based) of the
+  !! time-dependent one (H5MD_TIME or H5MD_LINEAR)
+  subroutine init_from_file(this, filename, group_name, mode, idx)
+    use h5md_module
+    use hdf5
+    class(particle_system_t), intent(out) :: this
+    character(len=*), intent(in) :: filename, group_name
+    integer, intent(in) :: mode
+    integer, intent(in), optional :: idx
+    type(h5md_file_t) :: f
+    type(h5md_element_t) :: e
+    integer(HID_T) :: g
+    double precision, allocatable :: pos(:,:)
+    integer, allocatable :: species(:)
+    call f% open(filename, H5F_ACC_RDONLY_F)
+    call h5md_check_valid(f% particles, 'particles group not found')
+    call h5md_check_exists(f% particles, group_name, 'group '//group_name//' not found')
+    call h5gopen_f(f% particles, group_name, g, f% error)
+    if (mode == H5MD_FIXED) then
+       call e% read_fixed(g, 'position', pos)
+       call this% init(size(pos, 2), 2)
+       this% pos = pos
+       if (has_fixed(g, 'species')) then
+          call e% read_fixed(g, 'species', species)
+          this% species = species
+       end if
+    else if ( (mode == H5MD_TIME) .or. (mode == H5MD_LINEAR) ) then
+       if (.not. present(idx) ) error stop 'missing idx in init_from_file'
+       call e% open_time(g, 'position')
+       call this% init(e% Nmax)
+       call read_frame_d2(e, idx, this% pos)
+       call e% close()
+    end if
+    call h5gclose_f(g, f% error)
+    call f% close()
+  contains
+    logical function has_fixed(loc, name)
+      integer(HID_T), intent(in) :: loc
+      character(len=*), intent(in) :: name
+      character(len=4096) :: line
+      integer :: u, ios
+      character(len=512) :: path
+      has_fixed = .false.
+      u = h5t_unit(h5t_file(loc))
+      path = trim(h5t_path(loc))//'/'//trim(name)//' '
+      rewind(u)
+      do
+         read(u, '(a)', iostat=ios) line
+         if (ios /= 0) exit
+         if (line(1:6) == 'fixed ' .and. index(line(7:), trim(path)//' ') == 1) then
+            has_fixed = .true.
+            exit
+         end if
+      end do
+    end function has_fixed
+  end subroutine init_from_file
 
   !> particle_system%init (src/particle_system.f90:81-184)
   subroutine init(this, Nmax, n_species, mass, system_name)
@@ -868,6 +1131,51 @@ contains
     call cells%to_device()
     call check(rmpcd_cell_maximum_displacement(cells%box(), cells%view(), v, p%d_aux%p, delta_t, r))
   end function cell_maximum_displacement
+  !> the radial distribution of solvent (species > 0) around x1 into hist, bins per
+  !! species, from the cells of the box [-max_i, max_i]^3 around x1's cell
+  !! (src/particle_system.f90:447-493); reads the host arrays of the last sort
+  subroutine compute_radial_histogram(hist, x1, L, solvent, cells)
+    use hilbert
+    type(histogram_t), intent(inout) :: hist
+    double precision, intent(in), dimension(3) :: x1, L
+    type(particle_system_t), intent(in) :: solvent
+    type(cell_system_t), intent(in) :: cells
+    integer :: max_i, i, j, k, cell_idx, i_particle, s, start, count
+    integer :: p(3)
+    double precision :: delta_x(3), norm, r_max
+    r_max = hist%xmin + hist%dx*hist%n
+    p = cells%cartesian_indices(x1)
+    max_i = floor(r_max/cells%a) + 1
+    do i = -max_i, max_i
+       do j = -max_i, max_i
+          do k = -max_i, max_i
+             cell_idx = compact_p_to_h(modulo(p + [i, j, k], cells%L), cells%M) + 1
+             start = cells%cell_start(cell_idx)
+             count = cells%cell_count(cell_idx)
+             do i_particle = start, start + count - 1
+                s = solvent%species(i_particle)
+                if (s <= 0) cycle
+                delta_x = rel_pos(solvent%pos(:,i_particle), x1, L)
+                norm = norm2(delta_x)
+                if (norm < r_max) call hist%bin(norm, s)
+             end do
+          end do
+       end do
+    end do
+  end subroutine compute_radial_histogram
+
+  !> divide bin i by the volume of its shell, 4 pi r^2 dx at its centre
+  !! (src/particle_system.f90:495-506)
+  subroutine correct_radial_histogram(hist)
+    type(histogram_t), intent(inout) :: hist
+    integer :: i
+    double precision :: r
+    do i = 1, hist%n
+       r = hist%xmin + (i-0.5d0) * hist%dx
+       hist%data(:,i) = hist%data(:,i) / (4*pi*r**2*hist%dx)
+    end do
+  end subroutine correct_radial_histogram
+
 end module particle_system
 
 module neighbor_list
@@ -1006,6 +1314,7 @@ end module neighbor_list
 module mpcd
   use iso_c_binding
   use rmpcd_capi
+  use rmpcd_mirror, only: dbuf_t, put_d, get_d, put_i, get_i
   use common
   use particle_system
   use cell_system
@@ -1014,7 +1323,7 @@ module mpcd
   private
   public :: simple_mpcd_step, compute_temperature, mpcd_stream_periodic, bulk_reaction
   public :: mpcd_stream_nogravity_zwall, rand_sphere, rescale_cells, bulk_reaction_endothermic
-  public :: wall_mpcd_step, mpcd_stream_xforce_yzwall
+  public :: wall_mpcd_step, mpcd_stream_xforce_yzwall, compute_rho, compute_vx
 
 contains
 
@@ -1088,18 +1397,55 @@ contains
   end subroutine simple_mpcd_step
 
   !> compute_temperature (src/mpcd.f90:342-407); the z profile is not provided
-  function compute_temperature(particles, cells) result(te)
+  function compute_temperature(particles, cells, tz) result(te)
     type(particle_system_t), intent(inout) :: particles
     type(cell_system_t), intent(inout) :: cells
+    type(profile_t), intent(inout), optional :: tz
     double precision :: te
     type(rmpcd_particles_t) :: p
+    type(dbuf_t) :: d_data, d_count
     call particles%time_ct%tic()
     call particles%upload('vs')
     p = particles%view()
     call cells%to_device()
-    call check(rmpcd_compute_temperature(cells%box(), cells%view(), p, te))
+    if (present(tz)) then
+       if (.not. (allocated(tz% data) .and. allocated(tz% count))) error stop 'profile_t: data not allocated'
+       ! the profile's bins on the device, added to there, and back
+       call put_d(d_data, tz% data)
+       call put_i(d_count, tz% count)
+       call check(rmpcd_compute_temperature_profile(cells%box(), cells%view(), p, te, tz% xmin, tz% dx, &
+            int(tz% n, c_int32_t), d_data%p, d_count%p))
+       call get_d(d_data, tz% data)
+       call get_i(d_count, tz% count)
+       call check(rmpcd_free(d_data%p))
+       call check(rmpcd_free(d_count%p))
+    else
+       call check(rmpcd_compute_temperature(cells%box(), cells%view(), p, te))
+    end if
     call particles%time_ct%tac()
   end function compute_temperature
+
+  !> density profile along z: every particle's z binned (src/mpcd.f90:410-422)
+  subroutine compute_rho(particles, rhoz)
+    type(particle_system_t), intent(in) :: particles
+    type(histogram_t), intent(inout) :: rhoz
+    integer :: i
+    if (.not. allocated(rhoz% data)) error stop 'histogram_t: data not allocated'
+    do i = 1, particles% Nmax
+       call rhoz% bin(particles% pos(3, i))
+    end do
+  end subroutine compute_rho
+
+  !> x-velocity profile along z (src/mpcd.f90:424-437)
+  subroutine compute_vx(particles, vx)
+    type(particle_system_t), intent(in) :: particles
+    type(profile_t), intent(inout) :: vx
+    integer :: i
+    if (.not. allocated(vx% data)) error stop 'histogram_t: data not allocated'
+    do i = 1, particles% Nmax
+       call vx% bin(particles% pos(3, i), particles% vel(1, i))
+    end do
+  end subroutine compute_vx
 
   !> mpcd_stream_periodic (src/mpcd.f90:440-457)
   subroutine mpcd_stream_periodic(particles, cells, dt)
@@ -1610,15 +1956,3 @@ contains
 end module md
 
 !> the umbrella module the drivers use (src/rmpcdmd_module.f90.in)
-module rmpcdmd_module
-  use common
-  use hilbert
-  use interaction
-  use cell_system
-  use particle_system
-  use neighbor_list
-  use mpcd
-  use md
-  implicit none
-  character(len=*), parameter :: RMPCDMD_REVISION = 'rmpcdmd-amd 0.1 (gfx950)'
-end module rmpcdmd_module

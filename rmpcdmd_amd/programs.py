@@ -24,7 +24,8 @@ DEFAULTS = {
 }
 
 
-def make_config(kind, form=FP64, seed=1, nmax_list=0, **kw):
+def make_config(kind, form=FP64, seed=1, nmax_list=0, n_thermo=0, T_final=None, n_loop=0, thermostat=0, hydro=1,
+                **kw):
     p = dict(DEFAULTS[kind])
     unknown = set(kw) - set(p)
     if unknown:
@@ -38,6 +39,8 @@ def make_config(kind, form=FP64, seed=1, nmax_list=0, **kw):
     for k in ("T", "tau", "sigma", "epsilon", "epsilon_colloids", "prob", "bulk_rate", "skin"):
         setattr(c, k, float(p[k]))
     c.seed = int(seed)
+    c.n_thermo, c.n_loop, c.thermostat, c.hydro = int(n_thermo), int(n_loop), int(thermostat), int(hydro)
+    c.T_final = float(p["T"] if T_final is None else T_final)
     return c
 
 

@@ -112,6 +112,38 @@ def test_colloids_matches_oracle(gpu):
     prog.close()
 
 
+@pytest.mark.parametrize("thermostat", [0, 1])
+def test_colloids_thermostat_phase_matches_oracle(gpu, thermostat):
+    """n_colloids_pbc's thermostat phase (MPC-AT at T every loop, rescale_cells every 10th,
+    n_colloids_pbc.f90:227-280) for 10 cycles, then production loops with the temperature
+    ramp (:334-336), by SRD or (thermostat) MPC-AT, against the oracle's loop on the same
+    Threefry draws (MPC-AT normals: Box-Muller on the device and in numpy, so FP64 to 1e-12)"""
+    from oracle.programs import ColloidsOracle
+    L = (16, 16, 16)
+    kw = dict(L=L, n_md=5, tau=0.1, sigma=1.5, n_colloids=6, T=1.0)
+    ramp = dict(n_thermo=10, T_final=1.6, n_loop=4, thermostat=thermostat, hydro=1)
+    prog = Program(PROG_COLLOIDS, form=capi.FP64, seed=29, **kw, **ramp)
+    st = prog.download()
+    orc = ColloidsOracle(st, L, 0.1, 5, 29, sigma=1.5, T=1.0, n_thermo=10, T_final=1.6, n_loop=4,
+                         thermostat=bool(thermostat), hydro=True)
+    temps = []
+    for cycle in range(13):
+        prog.run(1)
+        orc.cycle()
+        g = prog.download()
+        msg = "after cycle %d" % cycle
+        check_solvent(g, orc, msg)
+        check_colloids(g, orc, msg)
+        assert g["last_sort"] == orc.last_sort, msg
+        assert close(g["temperature"], orc.temperature), msg
+        temps.append(g["temperature"])
+    print("temperatures:", np.round(temps, 4))
+    # the thermostat holds T = 1 during the first phase (the last cycle rescaled the cells to
+    # exactly 3 (n_eff - 1) T / 2)
+    assert abs(temps[9] - 1.0) < 1e-9
+    prog.close()
+
+
 @pytest.mark.parametrize("kind", [PROG_FLUID, PROG_SPHERE, PROG_COLLOIDS])
 def test_programs_fp32_conserve_momentum(gpu, kind):
     """FP32-mixed form: total momentum (solvent + colloids) kept to the FP32 rounding level"""
