@@ -202,6 +202,15 @@ def compute_temperature(c, cnt, st, vel, species):
     return lib().orc_compute_temperature(C.byref(c), P(cnt), P(st), P(vel), P(species))
 
 
+def temperature_profile(c, cnt, st, vel, species, xmin, dx, n_bins):
+    """the tz profile of compute_temperature: (data, count) over n_bins bins from zero"""
+    data = np.zeros(n_bins)
+    count = np.zeros(n_bins, np.int32)
+    lib().orc_temperature_profile(C.byref(c), P(cnt), P(st), P(vel), P(species), D(xmin), D(dx), int(n_bins),
+                                  P(data), P(count))
+    return data, count
+
+
 def mpcd_stream_periodic(c, pos, vel, image, dt):
     lib().orc_mpcd_stream_periodic(C.byref(c), I64(pos.shape[0]), P(pos), P(vel), P(image), D(dt))
 

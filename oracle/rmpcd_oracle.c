@@ -472,6 +472,48 @@ double orc_compute_temperature(const orc_cells_t *c, const int32_t *cell_count,
     return te / count;
 }
 
+void orc_temperature_profile(const orc_cells_t *c, const int32_t *cell_count, const int32_t *cell_start,
+                             const double *vel, const int32_t *species, double xmin, double dx, int n_bins,
+                             double *data, int32_t *count)
+{
+    /* the tz profile of compute_temperature (mpcd.f90:395-398, profile_t%bin at
+       common.f90:196-208): each cell's kinetic term, for cells with n_eff > 1, added at the
+       height of the cell's centre origin(3) + (cell(3) + 1/2) a; one thread, cell order */
+    for (int ci = 0; ci < c->N; ++ci) {
+        if (cell_count[ci] <= 1) continue;
+        int start = cell_start[ci], n = cell_count[ci];
+        int n_eff = 0;
+        double lv0 = 0, lv1 = 0, lv2 = 0, kin = 0;
+        for (int i = start; i < start + n; ++i) {
+            if (species[i] > 0) {
+                lv0 = lv0 + vel[3 * i];
+                lv1 = lv1 + vel[3 * i + 1];
+                lv2 = lv2 + vel[3 * i + 2];
+                n_eff++;
+            }
+        }
+        if (n_eff <= 1) continue;
+        lv0 = lv0 / n;
+        lv1 = lv1 / n;
+        lv2 = lv2 / n;
+        for (int i = start; i < start + n; ++i) {
+            if (species[i] > 0) {
+                double d0 = vel[3 * i] - lv0, d1 = vel[3 * i + 1] - lv1, d2 = vel[3 * i + 2] - lv2;
+                kin = kin + ((d0 * d0 + d1 * d1) + d2 * d2);
+            }
+        }
+        kin = kin / (double)(3 * (n_eff - 1));
+        int p[3];
+        orc_compact_h_to_p(ci, c->M, p);
+        double z = c->origin[2] + (p[2] + 0.5) * c->a;
+        int idx = (int)floor((z - xmin) / dx);
+        if (idx >= 0 && idx < n_bins) {
+            data[idx] = data[idx] + kin;
+            count[idx] = count[idx] + 1;
+        }
+    }
+}
+
 void orc_mpcd_stream_periodic(const orc_cells_t *c, int64_t n, double *pos,
                               const double *vel, int32_t *image, double dt)
 {

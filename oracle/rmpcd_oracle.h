@@ -96,6 +96,9 @@ int64_t orc_srd_axes_from_stream(const orc_cells_t *c, const int32_t *cell_count
 void orc_simple_mpcd_step(const orc_cells_t *c, const int32_t *cell_count,
                           const int32_t *cell_start, double *vel, const int32_t *species,
                           const double *axes, int alpha_given, double alpha); /* [pinned:math] */
+void orc_temperature_profile(const orc_cells_t *c, const int32_t *cell_count, const int32_t *cell_start,
+                             const double *vel, const int32_t *species, double xmin, double dx, int n_bins,
+                             double *data, int32_t *count);
 double orc_compute_temperature(const orc_cells_t *c, const int32_t *cell_count,
                                const int32_t *cell_start, const double *vel,
                                const int32_t *species);              /* [pinned:math] */

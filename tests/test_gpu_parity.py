@@ -219,6 +219,31 @@ def test_compute_temperature(gpu):
     assert abs(te.value - want) <= RTOL * abs(want)
 
 
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+def test_compute_temperature_profile(gpu, form):
+    """compute_temperature with its tz profile: the temperature and the bins of the
+    oracle, binning into the caller's (accumulated, not cleared) arrays"""
+    c, srt, cnt, st = sorted_state((12, 10, 16), 10, 19, origin=(-0.2, -0.7, -0.35), frac2=0.2)
+    srt.species[::9] = 0
+    d = to_device(srt, form=form)
+    if form == capi.FP32:
+        srt.vel[:] = d.host("vel").astype(np.float64)
+    want_te = O.compute_temperature(c, cnt, st, srt.vel, srt.species)
+    xmin, dx, nb = -0.5, 1.5, 12
+    want_d, want_c = O.temperature_profile(c, cnt, st, srt.vel, srt.species, xmin, dx, nb)
+    cs = cells_to_device(c, cnt, st)
+    data = torch.full((nb,), 0.25, dtype=torch.float64, device="cuda")
+    count = torch.full((nb,), 3, dtype=torch.int32, device="cuda")
+    te = C.c_double()
+    capi.call("rmpcd_compute_temperature_profile", cs.boxp(), cs.cellsp(), d.struct(), C.byref(te), xmin, dx,
+              nb, ptr(data), ptr(count))
+    tol = RTOL if form == capi.FP64 else 1e-5
+    assert abs(te.value - want_te) <= tol * abs(want_te)
+    assert (count.cpu().numpy() == want_c + 3).all()
+    assert close(data.cpu().numpy(), want_d + 0.25, rtol=tol)
+    assert want_c.sum() > 0
+
+
 def test_stream_periodic(gpu):
     ps = solvent_state((8, 5, 5), 10, 18)
     c = O.cells_init((8, 5, 5))

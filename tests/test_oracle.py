@@ -275,6 +275,50 @@ def test_temperature_closed_form():
     assert abs(O.compute_temperature(c, cnt, st, srt.vel, sp) - k / ncount) < 1e-12
 
 
+def test_temperature_profile_pins():
+    """the tz profile: its bins hold, per z layer of cells, the cells with n_eff > 1 and
+    the sum of their kinetic terms.  Pinned by a route that does not decode Hilbert keys:
+    each particle's cell from floor(x - origin), cells grouped by their particles'
+    coordinates; the total over bins covering the box equals te * count of
+    compute_temperature; bins outside the range are dropped."""
+    L = (6, 5, 8)
+    c, srt, cnt, st = _cells_state(L, 4, 12)
+    O.set_origin(c, (-0.3, -0.6, -0.45))
+    srt2, i2i, cnt, st = O.sort(c, srt)
+    sp = srt2.species.copy()
+    sp[::7] = 0
+    te = O.compute_temperature(c, cnt, st, srt2.vel, sp)
+    # z bins of width 1 from the bottom of the shifted box, plus one bin beyond its top
+    xmin, dx, nb = c.origin[2], 1.0, L[2] + 1
+    data, count = O.temperature_profile(c, cnt, st, srt2.vel, sp, xmin, dx, nb)
+    # brute force: the cell of each particle from its position, per-cell kinetic terms
+    p = np.floor(srt2.pos - np.array(c.origin[:])).astype(int) % np.array(L)
+    want_d, want_c = np.zeros(nb), np.zeros(nb, int)
+    counted = 0
+    cells = {}
+    for i in range(srt2.n):
+        cells.setdefault(tuple(p[i]), []).append(i)
+    for (x, y, z), idx in cells.items():
+        if len(idx) <= 1:
+            continue
+        counted += 1
+        idx = np.array(idx)
+        keep = idx[sp[idx] > 0]
+        if keep.size <= 1:
+            continue
+        u = srt2.vel[keep].sum(axis=0) / len(idx)
+        kin = ((srt2.vel[keep] - u) ** 2).sum() / (3 * (keep.size - 1))
+        want_d[z] += kin
+        want_c[z] += 1
+    assert (count == want_c).all()
+    assert np.allclose(data, want_d, rtol=1e-12, atol=1e-12)
+    assert count[-1] == 0 and data[-1] == 0
+    assert abs(data.sum() - te * counted) <= 1e-12 * abs(te * counted)
+    # a range that starts at the middle of the box keeps the upper half only
+    d2, c2 = O.temperature_profile(c, cnt, st, srt2.vel, sp, xmin + 4, dx, nb)
+    assert (c2[:4] == want_c[4:8]).all() and c2[4:].sum() == 0
+
+
 def test_stream_periodic_closed_form():
     c = O.cells_init((8, 5, 5))
     pos = np.array([[7.9, 4.9, 0.1], [0.1, 0.2, 0.3]])
