@@ -181,6 +181,16 @@ int rmpcd_bulk_reaction_endothermic(const rmpcd_box_t *box, const rmpcd_cells_t 
 /* compute_temperature (src/mpcd.f90:342-407) */
 int rmpcd_compute_temperature(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
                               const rmpcd_particles_t *p, double *te);
+/* No counterpart in the reference: compute_temperature (src/mpcd.f90:342-407) for callers
+ * that cannot allocate (code under HIP graph capture, tight loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing and does not synchronise: the temperature is written to
+ * te_dev (a device double of the caller) on the library stream, and the per-workgroup
+ * partial sums use a fixed device array of 1 MiB per GPU (held until the process ends), so
+ * boxes of at most 8,388,608 cells (2^23; a larger one is refused with an error).  Calls
+ * that overlap on one GPU share that array and race.  Same kernels and summation order as
+ * rmpcd_compute_temperature, so the same result. */
+int rmpcd_compute_temperature_noalloc(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                                      const rmpcd_particles_t *p, double *te_dev);
 /* compute_temperature with its optional z profile tz (src/mpcd.f90:342-407, the binning at
  * :395-398 through profile_t%bin, src/common.f90:196-208): each cell's kinetic term is
  * added to tz_data[floor((z - xmin) / dx)] and counted in tz_count (device arrays of n_bins,

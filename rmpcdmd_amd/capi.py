@@ -139,6 +139,7 @@ SIGNATURES = {
     "rmpcd_rand_sphere": [_P, _I64, _I64, _P, _P],
     "rmpcd_simple_mpcd_step": [_BOXP, _CELLP, _PARTP, _P, _U64, _U64, _I, _D, _I, _D, _I],
     "rmpcd_compute_temperature": [_BOXP, _CELLP, _PARTP, _D3],
+    "rmpcd_compute_temperature_noalloc": [_BOXP, _CELLP, _PARTP, _P],
     "rmpcd_compute_temperature_profile": [_BOXP, _CELLP, _PARTP, _D3, C.c_double, C.c_double, _I, _P, _P],
     "rmpcd_mpcd_stream_periodic": [_BOXP, _PARTP, _D],
     "rmpcd_bulk_reaction": [_BOXP, _CELLP, _PARTP, _I, _I, _D, _D, _P, _U64, _U64],

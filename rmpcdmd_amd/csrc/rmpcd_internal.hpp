@@ -34,6 +34,10 @@ void *workspace(Slot s, size_t bytes);
 // host-pinned staging for synchronous scalar results
 void *pinned(size_t bytes);
 
+// the current device's address of the fixed scratch array of
+// rmpcd_compute_temperature_noalloc (looked up on the device's first use)
+int noalloc_scratch(double **out);
+
 // Hilbert table of a box (cached by box shape)
 const int32_t *hilbert_lut(const Box &b);
 

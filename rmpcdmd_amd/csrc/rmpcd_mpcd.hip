@@ -10,6 +10,8 @@
 // particles do not fit the tile fall back to direct loads.
 #include <float.h>
 
+#include <mutex>
+
 #include "rmpcd_internal.hpp"
 
 namespace rmpcd {
@@ -989,6 +991,56 @@ int temperature_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
     return RMPCD_OK;
 }
 
+// ---- compute_temperature without allocation ----------------------------------------
+// The workgroups' partial sums live in a fixed device array (one per GPU, held until the
+// process ends) instead of the grow-only workspace, and the result goes to a device
+// double of the caller, on the library stream, with no synchronisation.  The array's
+// device address is looked up once per device (hipGetSymbolAddress) and used wherever
+// host code passes it; calls that overlap on one GPU share it and race.
+constexpr int NOALLOC_MAX_BLOCKS = 65536;  // cells up to NOALLOC_MAX_BLOCKS * CELL_THREADS = 2^23
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_BLOCKS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double *g_scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mutex;
+
+// the current device's address of scratch (set up on the device's first use)
+int noalloc_scratch(double **out)
+{
+    int dev = 0;
+    RMPCD_HIP(hipGetDevice(&dev));
+    RMPCD_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "compute_temperature_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        RMPCD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (double *)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return RMPCD_OK;
+}
+
+template <class F>
+int temperature_noalloc_impl(const rmpcd_box_t *box, const rmpcd_cells_t *cells, const rmpcd_particles_t *pp,
+                             double *te_dev)
+{
+    const int nc = box->ncells;
+    const unsigned nb = blocks_for(nc, CELL_THREADS);
+    RMPCD_CHECK(nb <= (unsigned)NOALLOC_MAX_BLOCKS,
+                "compute_temperature_noalloc: %d cells, more than the %d of its fixed scratch array", nc,
+                NOALLOC_MAX_BLOCKS * CELL_THREADS);
+    double *part = nullptr;
+    int rc = noalloc_scratch(&part);
+    if (rc) return rc;
+    hipStream_t s = stream();
+    TzProfile tz{0, 0, 0, nullptr, nullptr};
+    k_temperature<F><<<nb, CELL_THREADS, 0, s>>>(nc, cells->cell_start, cells->cell_count, view<F>(pp), part,
+                                                 make_box(box), tz);
+    RMPCD_LAUNCHED();
+    k_temperature_final<<<1, CELL_THREADS, 0, s>>>(part, nb, te_dev);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
 template <class F>
 int stream_periodic_impl(const rmpcd_box_t *box, rmpcd_particles_t *pp, double dt)
 {
@@ -1069,6 +1121,17 @@ int rmpcd_compute_temperature(const rmpcd_box_t *box, const rmpcd_cells_t *cells
                 "compute_temperature: null argument");
     TzProfile tz{0, 0, 0, nullptr, nullptr};
     return RMPCD_DISPATCH_FORM(p, temperature_impl, box, cells, p, te, tz);
+}
+
+int rmpcd_compute_temperature_noalloc(const rmpcd_box_t *box, const rmpcd_cells_t *cells,
+                                      const rmpcd_particles_t *p, double *te_dev)
+{
+    int rc = check_box(box, "compute_temperature_noalloc");
+    if (!rc) rc = check_particles(p, "compute_temperature_noalloc", false);
+    if (rc) return rc;
+    RMPCD_CHECK(cells && cells->cell_count && cells->cell_start && te_dev,
+                "compute_temperature_noalloc: null argument");
+    return RMPCD_DISPATCH_FORM(p, temperature_noalloc_impl, box, cells, p, te_dev);
 }
 
 int rmpcd_compute_temperature_profile(const rmpcd_box_t *box, const rmpcd_cells_t *cells,

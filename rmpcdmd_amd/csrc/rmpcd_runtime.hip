@@ -143,7 +143,9 @@ int rmpcd_init(int device)
 {
     RMPCD_HIP(hipSetDevice(device));
     RMPCD_HIP(hipFree(nullptr));
-    return RMPCD_OK;
+    // the device's address of the fixed scratch array of the allocation-free entry point
+    double *scratch = nullptr;
+    return noalloc_scratch(&scratch);
 }
 
 int rmpcd_set_stream(void *s)
