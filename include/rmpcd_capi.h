@@ -453,7 +453,9 @@ typedef struct rmpcd_program_config {
     int32_t n_loop;
     int32_t thermostat;
     int32_t hydro;
-    int32_t pad2_;
+    /* COLLOIDS: 1 runs each collision cycle as one captured hipGraph with every decision on
+     * the device (no host scalar per MD step); production loop with SRD only */
+    int32_t graph;
 } rmpcd_program_config_t;
 typedef struct rmpcd_program_stats {
     int64_t n_solvent, n_colloids, md_steps, cycles, n_extra_sorting;

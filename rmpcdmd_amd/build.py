@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librmpcd_amd.so")
 SOURCES = ["rmpcd_runtime.hip", "rmpcd_sort.hip", "rmpcd_mpcd.hip", "rmpcd_md.hip",
            "rmpcd_nlist.hip", "rmpcd_sim.hip", "rmpcd_fused.hip", "rmpcd_programs.hip",
-           "rmpcd_bodies.hip"]
+           "rmpcd_bodies.hip", "rmpcd_colloids.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

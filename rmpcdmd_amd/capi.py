@@ -103,7 +103,7 @@ class ProgramConfig(C.Structure):
                 ("T", C.c_double), ("tau", C.c_double), ("sigma", C.c_double), ("epsilon", C.c_double),
                 ("epsilon_colloids", C.c_double), ("prob", C.c_double), ("bulk_rate", C.c_double),
                 ("skin", C.c_double), ("seed", C.c_uint64), ("T_final", C.c_double), ("n_loop", C.c_int32),
-                ("thermostat", C.c_int32), ("hydro", C.c_int32), ("pad2_", C.c_int32)]
+                ("thermostat", C.c_int32), ("hydro", C.c_int32), ("graph", C.c_int32)]
 
 
 class ProgramStats(C.Structure):

@@ -1,0 +1,1050 @@
+// rmpcd_colloids.hip -- the production loop of programs/n_colloids_pbc.f90 (:286-346) as a
+// device-resident program: one collision cycle (N_MD velocity-Verlet steps with lazy
+// cell-wise streaming, then the collision) is captured as one hipGraph and replayed,
+// with every decision taken on the GPU, so no host scalar is read inside a run.
+//
+// It continues the state of a ProgramSim (rmpcd_programs.hip: set-up, first sort, list
+// and forces) and reproduces the reference call sequence of ProgramSim::colloids_step /
+// colloids_collide:
+//   * cell_md_pos / cell_md_vel with md_flag true run over the md slot list: the slots of
+//     the cells with is_md set by the last list build, collected at that build;
+//   * the skin test (cell_maximum_displacement over those cells, the colloids'
+//     maximum_displacement) is reduced by one block, which sets need_sort and the catch-up
+//     step k dt;
+//   * the sort chain (catch-up of the other cells, apply_pbc, key and count, scan, place,
+//     stable gather into the other buffer, list build, md slot list, pos_old) is launched
+//     every step and returns at once unless need_sort is set (a graph cannot branch); the
+//     buffer that holds the solvent is chosen on the device (ctl.cur);
+//   * forces: the force parity flips every step as in the reference, so a graph of one
+//     cycle fixes it (two graphs, one per starting parity, when N_MD is odd).  The solvent
+//     force and the MD-bit roll of compute_force are applied to the md slots only: every
+//     particle that carries a force or an MD/PAST_MD bit is within the list radius of a
+//     colloid (the skin test rebuilds the list before one can leave it), hence in an is_md
+//     cell, and for every other particle the reference's full passes change nothing.
+#include "rmpcd_programs.hpp"
+
+namespace rmpcd {
+
+constexpr int CG_T = 256;        // threads per block
+constexpr int CG_GRID = 2048;    // blocks of the grid-stride passes over particles or cells
+constexpr int CG_SCAN_I = 8;     // cells per thread of a scan tile
+constexpr int CG_TILE = CG_T * CG_SCAN_I;
+constexpr int CG_CELL_T = 128;   // cells per block of the collision and temperature passes
+
+// device control of the loop
+struct ColCtl {
+    int32_t cur;        // the solvent buffer holding the state
+    int32_t need_sort;  // the skin test of this MD step fired
+    int32_t error;      // a list longer than its capacity: the reference stops (error stop)
+    int32_t n_md;       // entries of the md slot list
+    int64_t md_steps, last_sort, n_extra, cycles;
+    double catch_dt;    // k dt of the catch-up of the sort in progress
+    double origin[3];   // grid shift of the last collision
+    double e1, e2, temperature;
+};
+
+template <class F>
+struct SolB {
+    using R = typename F::real;
+    using S = typename F::small;
+    int64_t n;
+    R *pos[2], *vel[2], *force[4];  // force of buffer b and parity f: force[2b + f]
+    int32_t *image[2], *id[2];
+    S *species[2], *flags[2];
+};
+
+struct ColB {
+    int n;
+    double *pos, *pos_old, *vel, *force[2], *mass;
+    int32_t *image, *species;
+};
+
+struct ColloidsGraph {
+    ColCtl *ctl = nullptr;
+    int32_t *key = nullptr, *slot = nullptr, *tmp = nullptr, *tsum = nullptr, *mdslot = nullptr;
+    uint8_t *mdflag = nullptr;
+    int32_t *mdc = nullptr, *mdoff = nullptr, *nlcnt = nullptr;
+    int32_t *lut = nullptr;  // the box's Hilbert table, a copy owned by the program
+    double *part = nullptr, *cpart = nullptr, *fpart = nullptr, *epart = nullptr, *fnew = nullptr,
+           *tpart = nullptr;
+    int ntiles = 0, fblocks = 0, cblocks = 0;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // by starting force parity
+    std::vector<void *> allocs;
+};
+
+__device__ inline bool cg_off(const ColCtl *C, int forced) { return C->error || !(forced || C->need_sort); }
+
+__device__ inline int64_t cg_tid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline int64_t cg_stride() { return (int64_t)gridDim.x * blockDim.x; }
+
+// ---- MD step --------------------------------------------------------------------------
+// cell_md_pos with md_flag true (md.f90:79-102) over the md slots, and the squared
+// displacement from pos_old of each moved particle (cell_maximum_displacement,
+// particle_system.f90:380-410), maximum per block
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_stream(SolB<F> S, const ColCtl *C, int fp, double dt,
+                                                    const int32_t *__restrict__ mdslot, double *__restrict__ part)
+{
+    using R = typename F::real;
+    __shared__ double red[16];
+    double m = 0;
+    if (!C->error) {
+        const int b = C->cur;
+        const int64_t n = S.n;
+        R *pos = S.pos[b];
+        const R *vel = S.vel[b], *f = S.force[2 * b + fp], *pold = S.pos[1 - b];
+        const typename F::small *sp = S.species[b];
+        const R rdt = (R)dt, dt_sq = (R)((dt * dt) / 2);
+        const int nmd = C->n_md;
+        for (int64_t k = cg_tid(); k < nmd; k += cg_stride()) {
+            const int64_t j = mdslot[k];
+            R x[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                x[q] = (pos[q * n + j] + rdt * vel[q * n + j]) + dt_sq * f[q * n + j];
+                pos[q * n + j] = x[q];
+            }
+            if ((int)sp[j] >= 0) {
+                const double d0 = (double)(x[0] - pold[j]), d1 = (double)(x[1] - pold[n + j]),
+                             d2 = (double)(x[2] - pold[2 * n + j]);
+                const double r2 = (d0 * d0 + d1 * d1) + d2 * d2;
+                m = r2 > m ? r2 : m;
+            }
+        }
+    }
+    m = block_max(m, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// colloid velocity-Verlet position (n_colloids_pbc.f90:289-291) and its displacement from
+// pos_old (maximum_displacement, particle_system.f90:357-378), maximum per block
+__global__ void __launch_bounds__(CG_T) k_cg_colloid_pos(ColB c, const ColCtl *C, int cfp, double dt,
+                                                         double *__restrict__ cpart)
+{
+    __shared__ double red[16];
+    double m = 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!C->error && i < c.n) {
+        const double *F = c.force[cfp];
+        double r[3];
+        for (int k = 0; k < 3; ++k) {
+            const int64_t q = (int64_t)k * c.n + i;
+            c.pos[q] = (c.pos[q] + dt * c.vel[q]) + ((dt * dt) * F[q]) / (2 * c.mass[i]);
+            r[k] = c.pos[q] - c.pos_old[q];
+        }
+        if (c.species[i] >= 0) m = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+    }
+    m = block_max(m, red);
+    if (threadIdx.x == 0) cpart[blockIdx.x] = m;
+}
+
+// the skin test (n_colloids_pbc.f90:293-296): so_max is the maximum over the md cells,
+// raised to max_v k dt; co_max over the colloids; k = steps since the last sort
+__global__ void __launch_bounds__(CG_T) k_cg_decide(ColCtl *C, const double *__restrict__ part, int np,
+                                                    const double *__restrict__ cpart, int ncp, double dt, double skin,
+                                                    double max_v)
+{
+    __shared__ double red[16];
+    double a = 0, b = 0;
+    for (int q = threadIdx.x; q < np; q += blockDim.x) a = fmax(a, part[q]);
+    for (int q = threadIdx.x; q < ncp; q += blockDim.x) b = fmax(b, cpart[q]);
+    a = block_max(a, red);
+    b = block_max(b, red);
+    if (threadIdx.x != 0 || C->error) return;
+    const double k1 = (double)(C->md_steps + 1 - C->last_sort);
+    double so = sqrt(a);
+    const double bnd = max_v * (dt * k1);
+    so = so > bnd ? so : bnd;
+    const double co = sqrt(b);
+    C->need_sort = (co >= skin * 0.1 || so >= skin * 0.9) ? 1 : 0;
+    C->catch_dt = k1 * dt;
+}
+
+// ---- sort chain (extra sort when need_sort is set, collision sort when forced) -------
+// the grid shift of the collision (cell_system.f90:153-163), threefry((s, 0)) a, b and
+// threefry((s, 1)) a of stream SHIFT; the catch-up length since the last sort
+__global__ void k_cg_shift(ColCtl *C, uint64_t seed, double dt)
+{
+    if (threadIdx.x || blockIdx.x || C->error) return;
+    uint64_t a, b, c, d;
+    const uint64_t s = (uint64_t)C->cycles;
+    threefry2x64(s, 0, seed, STREAM_SHIFT, a, b);
+    threefry2x64(s, 1, seed, STREAM_SHIFT, c, d);
+    C->origin[0] = u01(a) - 1;
+    C->origin[1] = u01(b) - 1;
+    C->origin[2] = u01(c) - 1;
+    C->catch_dt = (double)(C->md_steps - C->last_sort) * dt;
+}
+
+// cell_md_pos then cell_md_vel with md_flag false and k dt (md.f90:79-102, 239-263): the
+// particles of the cells streamed lazily catch up
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_catchup(SolB<F> S, const ColCtl *C, int fp, int forced,
+                                                     const uint8_t *__restrict__ mdflag)
+{
+    using R = typename F::real;
+    if (cg_off(C, forced)) return;
+    const int b = C->cur;
+    const int64_t n = S.n;
+    R *pos = S.pos[b], *vel = S.vel[b];
+    const R *f = S.force[2 * b + fp], *fo = S.force[2 * b + 1 - fp];
+    typename F::small *flags = S.flags[b];
+    const double cdt = C->catch_dt;
+    const R rdt = (R)cdt, dt_sq = (R)((cdt * cdt) / 2);
+    for (int64_t j = cg_tid(); j < n; j += cg_stride()) {
+        if (mdflag[j]) continue;
+        R v[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            v[q] = vel[q * n + j];
+            pos[q * n + j] = (pos[q * n + j] + rdt * v[q]) + dt_sq * f[q * n + j];
+        }
+        const int fl = (int)flags[j];
+        if (!(fl & WALL_MASK)) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) vel[q * n + j] = v[q] + (rdt * (f[q * n + j] + fo[q * n + j])) / (R)2;
+        } else {
+            flags[j] = (typename F::small)(fl & ~WALL_MASK);
+        }
+    }
+}
+
+// apply_pbc of the colloids (md.f90:202-217)
+__global__ void k_cg_colloid_pbc(ColB c, const ColCtl *C, int forced, double e0, double e1, double e2)
+{
+    if (cg_off(C, forced)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const double e[3] = {e0, e1, e2};
+    for (int k = 0; k < 3; ++k) {
+        const int64_t q = (int64_t)k * c.n + i;
+        const double x = c.pos[q];
+        const int jump = (int)floor(x / e[k]);
+        c.image[q] += jump;
+        c.pos[q] = x - (double)jump * e[k];
+    }
+}
+
+__global__ void __launch_bounds__(CG_T) k_cg_zero_i32(int32_t *a, int64_t n, const ColCtl *C, int forced)
+{
+    if (cg_off(C, forced)) return;
+    for (int64_t i = cg_tid(); i < n; i += cg_stride()) a[i] = 0;
+}
+
+// the position in the particle's precision, wrapped as apply_pbc does
+template <class R>
+__device__ inline R cg_wrap(R x, double e, int &jump)
+{
+    const R ed = (R)e;
+    jump = (int)floor((double)(x / ed));
+    return x - (R)jump * ed;
+}
+
+// count_particles with the periodic wrap (cell_system.f90:86-129): cell key from the box's
+// Hilbert table and the arrival slot in the cell (atomic; the gather makes the order stable)
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_key(SolB<F> S, const ColCtl *C, int forced, Box bx,
+                                                 const int32_t *__restrict__ lut, int32_t *__restrict__ count,
+                                                 int32_t *__restrict__ key, int32_t *__restrict__ slot)
+{
+    using R = typename F::real;
+    if (cg_off(C, forced)) return;
+    Box b = bx;
+    for (int k = 0; k < 3; ++k) b.origin[k] = C->origin[k];
+    const int bf = C->cur;
+    const int64_t n = S.n;
+    const R *pos = S.pos[bf];
+    for (int64_t i = cg_tid(); i < n; i += cg_stride()) {
+        int j;
+        const R x = cg_wrap<R>(pos[i], b.edges[0], j), y = cg_wrap<R>(pos[n + i], b.edges[1], j),
+                z = cg_wrap<R>(pos[2 * n + i], b.edges[2], j);
+        int p0, p1, p2, k;
+        cell_of<R>(b, x, y, z, p0, p1, p2);
+        if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] && (unsigned)p2 < (unsigned)b.L[2])
+            k = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+        else
+            k = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+        key[i] = k;
+        slot[i] = atomicAdd(&count[k], 1);
+    }
+}
+
+__device__ inline int cg_block_scan(int v, int *lds, int *total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int inc = wave_inclusive_scan(v);
+    if (lane == 63) lds[wid] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int t = lds[w];
+        if (w < wid) off += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+// exclusive scan of nc int32 in tiles of CG_TILE: tile-local scan and tile totals, one block
+// for the totals, then each tile's offset (the total goes to *total when given)
+__global__ void __launch_bounds__(CG_T) k_cg_scan_tiles(const int32_t *__restrict__ in, int32_t *__restrict__ out,
+                                                        int64_t nc, int32_t *__restrict__ tsum, const ColCtl *C,
+                                                        int forced)
+{
+    __shared__ int lds[CG_T / 64];
+    if (cg_off(C, forced)) return;
+    const int64_t ntiles = (nc + CG_TILE - 1) / CG_TILE;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t base = t * CG_TILE + (int64_t)threadIdx.x * CG_SCAN_I;
+        int v[CG_SCAN_I], s = 0;
+        for (int k = 0; k < CG_SCAN_I; ++k) {
+            v[k] = base + k < nc ? in[base + k] : 0;
+            s += v[k];
+        }
+        int tot;
+        int off = cg_block_scan(s, lds, &tot);
+        for (int k = 0; k < CG_SCAN_I; ++k) {
+            if (base + k < nc) out[base + k] = off;
+            off += v[k];
+        }
+        if (threadIdx.x == 0) tsum[t] = tot;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_cg_scan_top(int32_t *tsum, int64_t ntiles, const ColCtl *C, int forced,
+                                                      int32_t *total)
+{
+    __shared__ int lds[16];
+    if (cg_off(C, forced)) return;
+    int carry = 0;
+    for (int64_t c0 = 0; c0 < ntiles; c0 += blockDim.x) {
+        const int64_t i = c0 + threadIdx.x;
+        const int v = i < ntiles ? tsum[i] : 0;
+        int tot;
+        const int ex = cg_block_scan(v, lds, &tot);
+        if (i < ntiles) tsum[i] = ex + carry;
+        carry += tot;
+    }
+    if (total && threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(CG_T) k_cg_scan_add(int32_t *__restrict__ out, int64_t nc,
+                                                      const int32_t *__restrict__ tsum, const ColCtl *C, int forced)
+{
+    if (cg_off(C, forced)) return;
+    for (int64_t i = cg_tid(); i < nc; i += cg_stride()) out[i] += tsum[i / CG_TILE];
+}
+
+// tmp[cell_start[key] + slot] = storage index (particle_system.f90:320-340, a slot per particle)
+__global__ void __launch_bounds__(CG_T) k_cg_place(int64_t n, const int32_t *__restrict__ key,
+                                                   const int32_t *__restrict__ slot, const int32_t *__restrict__ start,
+                                                   int32_t *__restrict__ tmp, const ColCtl *C, int forced)
+{
+    if (cg_off(C, forced)) return;
+    for (int64_t i = cg_tid(); i < n; i += cg_stride()) tmp[start[key[i]] + slot[i]] = (int32_t)i;
+}
+
+// the stable gather into the other buffer (particle_system.f90:304-355 with apply_pbc):
+// the particle of each slot goes to cell_start + its rank by storage index among the
+// cell's arrivals, which is the reference's single-thread order
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, int forced, double e0, double e1,
+                                                    double e2, const int32_t *__restrict__ key,
+                                                    const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+                                                    const int32_t *__restrict__ tmp, int32_t *__restrict__ id_to_idx)
+{
+    using R = typename F::real;
+    if (cg_off(C, forced)) return;
+    const int b = C->cur, o = 1 - b;
+    const int64_t n = S.n;
+    const double e[3] = {e0, e1, e2};
+    for (int64_t j = cg_tid(); j < n; j += cg_stride()) {
+        const int32_t v = tmp[j];
+        const int c = key[v];
+        const int s = start[c], m = count[c];
+        int rank = 0;
+        for (int q = s; q < s + m; ++q) rank += tmp[q] < v ? 1 : 0;
+        const int64_t d = s + rank;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int jump;
+            const R x = cg_wrap<R>(S.pos[b][k * n + v], e[k], jump);
+            S.pos[o][k * n + d] = x;
+            S.image[o][k * n + d] = S.image[b][k * n + v] + jump;
+            S.vel[o][k * n + d] = S.vel[b][k * n + v];
+            S.force[2 * o][k * n + d] = S.force[2 * b][k * n + v];
+            S.force[2 * o + 1][k * n + d] = S.force[2 * b + 1][k * n + v];
+        }
+        const int32_t id = S.id[b][v];
+        S.id[o][d] = id;
+        S.species[o][d] = S.species[b][v];
+        S.flags[o][d] = S.flags[b][v];
+        if ((uint32_t)id < (uint32_t)n) id_to_idx[id] = (int32_t)d;
+    }
+}
+
+// the sort is done: the other buffer holds the state (the switch() calls of
+// particle_system.f90:346-353); the md step of the last sort
+__global__ void k_cg_commit_sort(ColCtl *C, int forced)
+{
+    if (threadIdx.x || blockIdx.x || cg_off(C, forced)) return;
+    C->cur = 1 - C->cur;
+    if (forced) {
+        C->last_sort = C->md_steps;
+    } else {
+        C->last_sort = C->md_steps + 1;
+        C->n_extra += 1;
+    }
+}
+
+// update_list (neighbor_list.f90:76-157) without lj: is_md cleared and is_reac set, then a
+// count pass per (colloid, stencil cell), a scan per colloid and a fill pass, which keeps
+// the reference's list order (stencil order, then the cell's sorted order)
+__global__ void __launch_bounds__(CG_T) k_cg_list_init(int64_t nc, uint8_t *is_md, uint8_t *is_reac,
+                                                       const ColCtl *C, int forced)
+{
+    if (cg_off(C, forced)) return;
+    for (int64_t c = cg_tid(); c < nc; c += cg_stride()) {
+        is_md[c] = 0;
+        is_reac[c] = 1;
+    }
+}
+
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_nl_pass(SolB<F> S, ColB col, const ColCtl *C, int forced, Box bx,
+                                                     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+                                                     const int32_t *__restrict__ st, int ns, double radius_sq,
+                                                     int32_t *__restrict__ cnt, uint8_t *__restrict__ is_md, int fill,
+                                                     int32_t nmax, int32_t *__restrict__ list)
+{
+    if (cg_off(C, forced)) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= ns) return;
+    Box b = bx;
+    for (int k = 0; k < 3; ++k) b.origin[k] = C->origin[k];
+    const int bf = C->cur;
+    const int64_t n = S.n;
+    const typename F::real *pos = S.pos[bf];
+    const typename F::small *sp = S.species[bf];
+    const double x[3] = {col.pos[i], col.pos[col.n + i], col.pos[2 * (int64_t)col.n + i]};
+    const int key = stencil_key_dev(b, x, st, j);
+    const int s = start[key], m = count[key];
+    const int64_t e = (int64_t)i * ns + j;
+    const int off = fill ? cnt[e] : 0;
+    int k = 0;
+    bool md = false;
+    for (int q = s; q < s + m; ++q) {
+        if ((int)sp[q] <= 0) continue;
+        const double d0 = min_image(x[0] - (double)pos[q], b.edges[0]);
+        const double d1 = min_image(x[1] - (double)pos[n + q], b.edges[1]);
+        const double d2 = min_image(x[2] - (double)pos[2 * n + q], b.edges[2]);
+        if ((d0 * d0 + d1 * d1) + d2 * d2 < radius_sq) {
+            md = true;
+            if (fill && off + k < nmax) list[(int64_t)i * nmax + off + k] = q;
+            ++k;
+        }
+    }
+    if (!fill) {
+        cnt[e] = k;
+        if (md) is_md[key] = 1;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_cg_nl_scan(int32_t *cnt, int ns, int32_t nmax, int32_t *__restrict__ nlist,
+                                                     ColCtl *C, int forced)
+{
+    __shared__ int lds[16];
+    if (cg_off(C, forced)) return;
+    const int i = blockIdx.x;
+    int32_t *c = cnt + (int64_t)i * ns;
+    int carry = 0;
+    for (int base = 0; base < ns; base += blockDim.x) {
+        const int q = base + threadIdx.x;
+        const int v = q < ns ? c[q] : 0;
+        int tot;
+        const int ex = cg_block_scan(v, lds, &tot);
+        if (q < ns) c[q] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        nlist[i] = carry > nmax ? nmax : carry;
+        if (carry > nmax) atomicOr(&C->error, RMPCD_ERR_OVERFLOW);
+    }
+}
+
+// the md slot list: the slots of the is_md cells, in cell order (counts scanned into mdoff),
+// and a flag per slot
+__global__ void __launch_bounds__(CG_T) k_cg_md_count(int64_t nc, const uint8_t *__restrict__ is_md,
+                                                      const int32_t *__restrict__ count, int32_t *__restrict__ mdc,
+                                                      const ColCtl *C, int forced)
+{
+    if (cg_off(C, forced)) return;
+    for (int64_t c = cg_tid(); c < nc; c += cg_stride()) mdc[c] = is_md[c] ? count[c] : 0;
+}
+
+__global__ void __launch_bounds__(CG_T) k_cg_md_fill(int64_t nc, const uint8_t *__restrict__ is_md,
+                                                     const int32_t *__restrict__ start,
+                                                     const int32_t *__restrict__ count,
+                                                     const int32_t *__restrict__ mdoff, int32_t *__restrict__ mdslot,
+                                                     uint8_t *__restrict__ mdflag, const ColCtl *C, int forced)
+{
+    if (cg_off(C, forced)) return;
+    for (int64_t c = cg_tid(); c < nc; c += cg_stride()) {
+        const int s = start[c], m = count[c];
+        const bool md = is_md[c] != 0;
+        const int o = mdoff[c];
+        for (int k = 0; k < m; ++k) {
+            mdflag[s + k] = md ? 1 : 0;
+            if (md) mdslot[o + k] = s + k;
+        }
+    }
+}
+
+// pos_old = pos for the solvent (the other buffer's position array) and the colloids
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_pos_old(SolB<F> S, ColB col, const ColCtl *C, int forced)
+{
+    if (cg_off(C, forced)) return;
+    const int b = C->cur;
+    const int64_t n3 = 3 * S.n;
+    for (int64_t i = cg_tid(); i < n3; i += cg_stride()) S.pos[1 - b][i] = S.pos[b][i];
+    for (int64_t i = cg_tid(); i < 3 * (int64_t)col.n; i += cg_stride()) col.pos_old[i] = col.pos[i];
+}
+
+// ---- forces ---------------------------------------------------------------------------
+// solvent%force = 0 and the MD-bit roll of compute_force (neighbor_list.f90:180-188) over
+// the md slots; colloids%force = 0
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_force_zero(SolB<F> S, ColB col, const ColCtl *C, int fp, int cfp,
+                                                        const int32_t *__restrict__ mdslot)
+{
+    if (C->error) return;
+    const int b = C->cur;
+    const int64_t n = S.n;
+    typename F::real *f = S.force[2 * b + fp];
+    typename F::small *flags = S.flags[b];
+    const int nmd = C->n_md;
+    for (int64_t k = cg_tid(); k < nmd; k += cg_stride()) {
+        const int64_t j = mdslot[k];
+        f[j] = 0;
+        f[n + j] = 0;
+        f[2 * n + j] = 0;
+        int fl = (int)flags[j];
+        fl = (fl & MD_MASK) ? (fl | PAST_MD_MASK) : (fl & ~PAST_MD_MASK);
+        flags[j] = (typename F::small)(fl & ~MD_MASK);
+    }
+    for (int64_t i = cg_tid(); i < 3 * (int64_t)col.n; i += cg_stride()) col.force[cfp][i] = 0;
+}
+
+// compute_force (neighbor_list.f90:191-222): one thread per list entry, solvent forces by
+// atomics, the colloid's force and energy as block sums (the partition of the library's
+// compute_force, so the same sums)
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const ColCtl *C, int fp, int cfp,
+                                                      const int32_t *__restrict__ list, const int32_t *__restrict__ nl,
+                                                      int32_t nmax, double L0, double L1, double L2, LJ lj,
+                                                      double *__restrict__ fpart)
+{
+    using R = typename F::real;
+    __shared__ double red[16];
+    if (C->error) return;
+    const int b = C->cur;
+    const int64_t n = S.n;
+    const int i = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_i = min(nl[i], nmax);
+    double f0 = 0, f1 = 0, f2 = 0, e = 0;
+    const int64_t cn = col.n;
+    const double x0 = col.pos[i], x1 = col.pos[cn + i], x2 = col.pos[2 * cn + i];
+    const int s1 = col.species[i];
+    if (k < n_i) {
+        const int idx = list[(int64_t)i * nmax + k];
+        const int s2 = (int)S.species[b][idx];
+        if (s2 > 0) {
+            const R *pos = S.pos[b];
+            const double d0 = min_image(x0 - (double)pos[idx], L0);
+            const double d1 = min_image(x1 - (double)pos[n + idx], L1);
+            const double d2 = min_image(x2 - (double)pos[2 * n + idx], L2);
+            const double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
+            const int pidx = (s2 - 1) + lj.n1 * (s1 - 1);
+            if (r_sq <= lj.cut_sq[pidx]) {
+                const double fr = lj_force_over_r(r_sq, lj.eps[pidx], lj.sigma[pidx]);
+                f0 = fr * d0;
+                f1 = fr * d1;
+                f2 = fr * d2;
+                e = lj_energy(r_sq, lj.eps[pidx], lj.sigma[pidx], lj.cut_energy[pidx]);
+                R *f = S.force[2 * b + fp];
+                atomicAdd(&f[idx], (R)(-f0));
+                atomicAdd(&f[n + idx], (R)(-f1));
+                atomicAdd(&f[2 * n + idx], (R)(-f2));
+                atomic_or_flag(&S.flags[b][idx], MD_MASK | (s1 == 1 ? CATALYZED_MASK : 0));
+            }
+        }
+    }
+    const double b0 = block_sum(f0, red), b1 = block_sum(f1, red), b2 = block_sum(f2, red), be = block_sum(e, red);
+    if (threadIdx.x == 0) {
+        double *pp = fpart + ((int64_t)i * gridDim.x + blockIdx.x) * 4;
+        pp[0] = b0;
+        pp[1] = b1;
+        pp[2] = b2;
+        pp[3] = be;
+    }
+    (void)cfp;
+}
+
+// each colloid adds its blocks' sums in block order; energy per colloid
+__global__ void k_cg_force_sum(ColB col, const ColCtl *C, int cfp, const double *__restrict__ fpart, int nb,
+                               double *__restrict__ epart)
+{
+    if (C->error) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= col.n) return;
+    double a0 = 0, a1 = 0, a2 = 0, ae = 0;
+    for (int q = 0; q < nb; ++q) {
+        const double *pp = fpart + ((int64_t)i * nb + q) * 4;
+        a0 += pp[0];
+        a1 += pp[1];
+        a2 += pp[2];
+        ae += pp[3];
+    }
+    double *F = col.force[cfp];
+    const int64_t cn = col.n;
+    F[i] = F[i] + a0;
+    F[cn + i] = F[cn + i] + a1;
+    F[2 * cn + i] = F[2 * cn + i] + a2;
+    epart[i] = ae;
+}
+
+// compute_force_n2 (neighbor_list.f90:227-264), owner computes: colloid i receives, in the
+// reference's loop order, -f(j,i) for j < i and then +f(i,j) for j > i
+__global__ void k_cg_force_n2(ColB col, const ColCtl *C, int cfp, double L0, double L1, double L2, LJ lj,
+                              double *__restrict__ fnew, double *__restrict__ epart)
+{
+    if (C->error) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= col.n) return;
+    const int64_t n = col.n;
+    const double *F = col.force[cfp];
+    double F0 = F[i], F1 = F[n + i], F2 = F[2 * n + i], e = 0;
+    for (int64_t j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const int64_t a = j < i ? j : i, bb = j < i ? i : j;
+        const double d0 = min_image(col.pos[a] - col.pos[bb], L0);
+        const double d1 = min_image(col.pos[n + a] - col.pos[n + bb], L1);
+        const double d2 = min_image(col.pos[2 * n + a] - col.pos[2 * n + bb], L2);
+        const double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
+        const int sa = col.species[a], sb = col.species[bb];
+        const int pc = (sa - 1) + lj.n1 * (sb - 1), pe = (sb - 1) + lj.n1 * (sa - 1);
+        if (r_sq <= lj.cut_sq[pc]) {
+            const double fr = lj_force_over_r(r_sq, lj.eps[pe], lj.sigma[pe]);
+            if (j < i) {
+                F0 = F0 - fr * d0;
+                F1 = F1 - fr * d1;
+                F2 = F2 - fr * d2;
+            } else {
+                F0 = F0 + fr * d0;
+                F1 = F1 + fr * d1;
+                F2 = F2 + fr * d2;
+                e = e + lj_energy(r_sq, lj.eps[pe], lj.sigma[pe], lj.cut_energy[pe]);
+            }
+        }
+    }
+    fnew[i] = F0;
+    fnew[n + i] = F1;
+    fnew[2 * n + i] = F2;
+    epart[col.n + i] = e;
+}
+
+// the colloid forces of compute_force_n2, then the energies summed in colloid order
+__global__ void k_cg_force_fin(ColB col, ColCtl *C, int cfp, const double *__restrict__ fnew,
+                               const double *__restrict__ epart)
+{
+    if (C->error) return;
+    const int64_t n3 = 3 * (int64_t)col.n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
+        col.force[cfp][i] = fnew[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s1 = 0, s2 = 0;
+        for (int i = 0; i < col.n; ++i) s1 = s1 + epart[i];
+        for (int i = 0; i < col.n; ++i) s2 = s2 + epart[col.n + i];
+        C->e1 = s1;
+        C->e2 = s2;
+    }
+}
+
+// cell_md_vel with md_flag true over the md slots (md.f90:239-263)
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_vel(SolB<F> S, const ColCtl *C, int fp, double dt,
+                                                 const int32_t *__restrict__ mdslot)
+{
+    using R = typename F::real;
+    if (C->error) return;
+    const int b = C->cur;
+    const int64_t n = S.n;
+    R *vel = S.vel[b];
+    const R *f = S.force[2 * b + fp], *fo = S.force[2 * b + 1 - fp];
+    typename F::small *flags = S.flags[b];
+    const R rdt = (R)dt;
+    const int nmd = C->n_md;
+    for (int64_t k = cg_tid(); k < nmd; k += cg_stride()) {
+        const int64_t j = mdslot[k];
+        const int fl = (int)flags[j];
+        if (!(fl & WALL_MASK)) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) vel[q * n + j] = vel[q * n + j] + (rdt * (f[q * n + j] + fo[q * n + j])) / (R)2;
+        } else {
+            flags[j] = (typename F::small)(fl & ~WALL_MASK);
+        }
+    }
+}
+
+// colloid velocity (n_colloids_pbc.f90:312) and the step counter
+__global__ void k_cg_colloid_vel(ColB c, ColCtl *C, int cfp, double dt)
+{
+    if (C->error) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.n) {
+        const double *F = c.force[cfp], *Fo = c.force[1 - cfp];
+        for (int k = 0; k < 3; ++k) {
+            const int64_t q = (int64_t)k * c.n + i;
+            c.vel[q] = c.vel[q] + (dt * (F[q] + Fo[q])) / (2 * c.mass[i]);
+        }
+    }
+    if (i == 0) C->md_steps += 1;
+}
+
+// ---- collision ------------------------------------------------------------------------
+// simple_mpcd_step with the default angle (mpcd.f90:71-183), axes threefry((s, cell + 2^40 k)) of
+// stream SRD; one thread per cell
+template <class F>
+__global__ void __launch_bounds__(CG_CELL_T) k_cg_srd(SolB<F> S, const ColCtl *C, int ncells,
+                                                      const int32_t *__restrict__ start,
+                                                      const int32_t *__restrict__ count, uint64_t seed)
+{
+    if (C->error) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const int m = count[c];
+    if (m <= 1) return;
+    const int b = C->cur;
+    const int64_t n = S.n, s = start[c];
+    typename F::real *v = S.vel[b];
+    srd_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m, c, ncells, nullptr, seed, (uint64_t)C->cycles,
+             1.0, 0.0);
+}
+
+// compute_temperature (mpcd.f90:342-407): per block, the sum of the cells' kinetic terms
+// and of the cells counted; one block adds the partials in block order
+template <class F>
+__global__ void __launch_bounds__(CG_CELL_T) k_cg_temp(SolB<F> S, const ColCtl *C, int ncells,
+                                                       const int32_t *__restrict__ start,
+                                                       const int32_t *__restrict__ count, double *__restrict__ tpart)
+{
+    __shared__ double red[16];
+    double kin = 0, cnt = 0;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!C->error && c < ncells && count[c] > 1) {
+        const int b = C->cur;
+        const int64_t n = S.n, s = start[c];
+        const typename F::real *v = S.vel[b];
+        cnt = 1;
+        const double k = temperature_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, count[c]);
+        kin = k < 0 ? 0 : k;
+    }
+    const double bk = block_sum(kin, red);
+    const double bc = block_sum(cnt, red);
+    if (threadIdx.x == 0) {
+        tpart[2 * blockIdx.x] = bk;
+        tpart[2 * blockIdx.x + 1] = bc;
+    }
+}
+
+__global__ void __launch_bounds__(CG_CELL_T) k_cg_temp_final(ColCtl *C, const double *__restrict__ tpart, int nb)
+{
+    __shared__ double red[16];
+    double a = 0, b = 0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+        a += tpart[2 * q];
+        b += tpart[2 * q + 1];
+    }
+    a = block_sum(a, red);
+    b = block_sum(b, red);
+    if (threadIdx.x == 0 && !C->error) {
+        C->temperature = a / b;
+        C->cycles += 1;
+    }
+}
+
+// ---- host side ------------------------------------------------------------------------
+template <class F>
+static SolB<F> sol_bufs(const ProgramSim &s)
+{
+    SolB<F> o;
+    o.n = s.n;
+    for (int b = 0; b < 2; ++b) {
+        o.pos[b] = (typename F::real *)s.sol_pos[b];
+        o.vel[b] = (typename F::real *)s.sol_vel[b];
+        o.image[b] = s.sol_image[b];
+        o.id[b] = s.sol_id[b];
+        o.species[b] = (typename F::small *)s.sol_species[b];
+        o.flags[b] = (typename F::small *)s.sol_flags[b];
+    }
+    for (int k = 0; k < 4; ++k) o.force[k] = (typename F::real *)s.sol_force[k];
+    return o;
+}
+
+static ColB col_bufs(const ProgramSim &s)
+{
+    ColB c;
+    c.n = s.nc;
+    c.pos = s.col_pos;
+    c.pos_old = s.col_pos_old;
+    c.vel = s.col_vel;
+    c.force[0] = s.col_force[0];
+    c.force[1] = s.col_force[1];
+    c.mass = s.col_mass;
+    c.image = s.col_image;
+    c.species = s.col_species;
+    return c;
+}
+
+static int cg_alloc(ColloidsGraph &g, void **p, size_t bytes)
+{
+    bytes = ((bytes ? bytes : 1) + 255) & ~(size_t)255;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return set_error(RMPCD_ERR_HIP, "colloids graph: hipMalloc of %zu bytes failed", bytes);
+    }
+    g.allocs.push_back(*p);
+    if (hipMemset(*p, 0, bytes) != hipSuccess) return set_error(RMPCD_ERR_HIP, "colloids graph: hipMemset failed");
+    return RMPCD_OK;
+}
+
+void colloids_graph_free(ColloidsGraph *g)
+{
+    if (!g) return;
+    (void)hipDeviceSynchronize();
+    for (int k = 0; k < 2; ++k)
+        if (g->exec[k]) (void)hipGraphExecDestroy(g->exec[k]);
+    for (void *p : g->allocs) (void)hipFree(p);
+    delete g;
+}
+
+// the sort chain after the catch-up: apply_pbc, count, scan, place, gather, commit, list,
+// md slots, pos_old (ProgramSim::sort, build_list, copy_pos_old)
+template <class F>
+static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int forced, hipStream_t st)
+{
+    const SolB<F> S = sol_bufs<F>(s);
+    const ColB col = col_bufs(s);
+    ColCtl *C = g.ctl;
+    const Box b = make_box(&s.box);
+    const int64_t nc = s.box.ncells, n = s.n;
+    const unsigned cgrid = blocks_for(s.nc, CG_T);
+    const double *E = s.box.edges;
+    k_cg_catchup<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, g.mdflag);
+    k_cg_colloid_pbc<<<cgrid, CG_T, 0, st>>>(col, C, forced, E[0], E[1], E[2]);
+    k_cg_zero_i32<<<CG_GRID, CG_T, 0, st>>>(s.cell_count, nc, C, forced);
+    k_cg_key<F><<<CG_GRID, CG_T, 0, st>>>(S, C, forced, b, g.lut, s.cell_count, g.key, g.slot);
+    k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_count, s.cell_start, nc,
+                                                                                      g.tsum, C, forced);
+    k_cg_scan_top<<<1, 1024, 0, st>>>(g.tsum, g.ntiles, C, forced, nullptr);
+    k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(s.cell_start, nc, g.tsum, C, forced);
+    k_cg_place<<<CG_GRID, CG_T, 0, st>>>(n, g.key, g.slot, s.cell_start, g.tmp, C, forced);
+    k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, forced, E[0], E[1], E[2], g.key, s.cell_start, s.cell_count, g.tmp,
+                                            s.id_to_idx);
+    k_cg_commit_sort<<<1, 1, 0, st>>>(C, forced);
+    k_cg_list_init<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.is_reac, C, forced);
+    const dim3 lgrid(blocks_for(s.n_stencil, CG_T), (unsigned)s.nc);
+    k_cg_nl_pass<F><<<lgrid, CG_T, 0, st>>>(S, col, C, forced, b, s.cell_start, s.cell_count, s.stencil, s.n_stencil,
+                                          s.radius * s.radius, g.nlcnt, s.is_md, 0, s.nmax, s.list);
+    k_cg_nl_scan<<<(unsigned)s.nc, 1024, 0, st>>>(g.nlcnt, s.n_stencil, s.nmax, s.nlist, C, forced);
+    k_cg_nl_pass<F><<<lgrid, CG_T, 0, st>>>(S, col, C, forced, b, s.cell_start, s.cell_count, s.stencil, s.n_stencil,
+                                          s.radius * s.radius, g.nlcnt, s.is_md, 1, s.nmax, s.list);
+    k_cg_md_count<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_count, g.mdc, C, forced);
+    k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(g.mdc, g.mdoff, nc, g.tsum, C,
+                                                                                      forced);
+    k_cg_scan_top<<<1, 1024, 0, st>>>(g.tsum, g.ntiles, C, forced, &C->n_md);
+    k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(g.mdoff, nc, g.tsum, C, forced);
+    k_cg_md_fill<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_start, s.cell_count, g.mdoff, g.mdslot, g.mdflag, C,
+                                           forced);
+    k_cg_pos_old<F><<<CG_GRID, CG_T, 0, st>>>(S, col, C, forced);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+// one MD step (ProgramSim::colloids_step); fp is the force parity before the step
+template <class F>
+static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_t st)
+{
+    const SolB<F> S = sol_bufs<F>(s);
+    const ColB col = col_bufs(s);
+    ColCtl *C = g.ctl;
+    const unsigned cgrid = blocks_for(s.nc, CG_T);
+    k_cg_stream<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
+    k_cg_colloid_pos<<<cgrid, CG_T, 0, st>>>(col, C, fp, s.dt, g.cpart);
+    k_cg_decide<<<1, CG_T, 0, st>>>(C, g.part, CG_GRID, g.cpart, (int)cgrid, s.dt, s.skin, s.box.max_v);
+    int rc = enqueue_sort_chain<F>(s, g, fp, 0, st);
+    if (rc) return rc;
+    // switch(force, force_old) of both systems, zeroed forces, compute_force(_n2), velocities
+    const int np = 1 - fp;
+    const double *E = s.box.edges;
+    k_cg_force_zero<F><<<CG_GRID, CG_T, 0, st>>>(S, col, C, np, np, g.mdslot);
+    const dim3 fgrid((unsigned)g.fblocks, (unsigned)s.nc);
+    k_cg_force_nl<F><<<fgrid, CG_T, 0, st>>>(S, col, C, np, np, s.list, s.nlist, s.nmax, E[0], E[1], E[2],
+                                             make_lj(&s.lj_sc), g.fpart);
+    k_cg_force_sum<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fpart, g.fblocks, g.epart);
+    k_cg_force_n2<<<cgrid, CG_T, 0, st>>>(col, C, np, E[0], E[1], E[2], make_lj(&s.lj_cc), g.fnew, g.epart);
+    k_cg_force_fin<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fnew, g.epart);
+    k_cg_vel<F><<<CG_GRID, CG_T, 0, st>>>(S, C, np, s.dt, g.mdslot);
+    k_cg_colloid_vel<<<cgrid, CG_T, 0, st>>>(col, C, np, s.dt);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+// the collision (ProgramSim::colloids_collide, production loop with SRD)
+template <class F>
+static int enqueue_collision(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_t st)
+{
+    const SolB<F> S = sol_bufs<F>(s);
+    ColCtl *C = g.ctl;
+    k_cg_shift<<<1, 1, 0, st>>>(C, s.seed, s.dt);
+    int rc = enqueue_sort_chain<F>(s, g, fp, 1, st);
+    if (rc) return rc;
+    const int nc = s.box.ncells;
+    const unsigned tb = blocks_for(nc, CG_CELL_T);
+    k_cg_srd<F><<<tb, CG_CELL_T, 0, st>>>(S, C, nc, s.cell_start, s.cell_count, s.seed);
+    k_cg_temp<F><<<tb, CG_CELL_T, 0, st>>>(S, C, nc, s.cell_start, s.cell_count, g.tpart);
+    k_cg_temp_final<<<1, CG_CELL_T, 0, st>>>(C, g.tpart, (int)tb);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+template <class F>
+static int capture_cycle(ProgramSim &s, ColloidsGraph &g, int fp0, hipGraphExec_t *out)
+{
+    hipStream_t st;
+    RMPCD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    RMPCD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = RMPCD_OK, fp = fp0;
+    for (int j = 0; j < s.cfg.n_md && !rc; ++j) {
+        rc = enqueue_step<F>(s, g, fp, st);
+        fp = 1 - fp;
+    }
+    if (!rc) rc = enqueue_collision<F>(s, g, fp, st);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (!rc && e != hipSuccess) rc = set_error(RMPCD_ERR_HIP, "colloids graph: capture failed: %s", hipGetErrorString(e));
+    if (!rc) {
+        e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = set_error(RMPCD_ERR_HIP, "colloids graph: instantiate failed: %s", hipGetErrorString(e));
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+template <class F>
+static int graph_setup(ProgramSim &s)
+{
+    ColloidsGraph *g = new ColloidsGraph();
+    s.graph = g;
+    const int64_t n = s.n, nc = s.box.ncells;
+    int rc;
+    g->ntiles = (int)((nc + CG_TILE - 1) / CG_TILE);
+    g->fblocks = (int)blocks_for(s.nmax, CG_T);
+    g->cblocks = (int)blocks_for(s.nc, CG_T);
+    RMPCD_CHECK(g->ntiles <= 1 << 20, "colloids graph: too many cells");
+    if ((rc = cg_alloc(*g, (void **)&g->ctl, sizeof(ColCtl)))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->key, 4 * n))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->slot, 4 * n))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->tmp, 4 * n))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->mdslot, 4 * n))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->mdflag, n))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->tsum, 4 * (size_t)(g->ntiles + 1)))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->mdc, 4 * nc))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->mdoff, 4 * nc))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->nlcnt, 4 * (size_t)s.nc * s.n_stencil))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->part, 8 * CG_GRID))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->cpart, 8 * (size_t)g->cblocks))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->fpart, 32 * (size_t)g->fblocks * s.nc))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->epart, 16 * (size_t)s.nc))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->fnew, 24 * (size_t)s.nc))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->tpart, 16 * (size_t)blocks_for(nc, CG_CELL_T)))) return rc;
+    const size_t nreal = (size_t)s.box.L[0] * s.box.L[1] * s.box.L[2];
+    if ((rc = cg_alloc(*g, (void **)&g->lut, 4 * nreal))) return rc;
+    const int32_t *lut = hilbert_lut(make_box(&s.box));
+    if (!lut) return set_error(RMPCD_ERR_HIP, "colloids graph: Hilbert table allocation failed");
+    RMPCD_HIP(hipMemcpy(g->lut, lut, 4 * nreal, hipMemcpyDeviceToDevice));
+    // the host state into the device control, and the md slot list of the current is_md
+    ColCtl h{};
+    h.cur = s.cur;
+    h.md_steps = s.md_steps;
+    h.last_sort = s.last_sort;
+    h.n_extra = s.n_extra;
+    h.cycles = s.cycles;
+    for (int k = 0; k < 3; ++k) h.origin[k] = s.box.origin[k];
+    h.e1 = s.e1;
+    h.e2 = s.e2;
+    h.temperature = s.temperature;
+    RMPCD_HIP(hipMemcpy(g->ctl, &h, sizeof(h), hipMemcpyHostToDevice));
+    hipStream_t st = stream();
+    ColCtl *C = g->ctl;
+    k_cg_md_count<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_count, g->mdc, C, 1);
+    k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g->ntiles, CG_GRID), CG_T, 0, st>>>(g->mdc, g->mdoff, nc, g->tsum, C,
+                                                                                       1);
+    k_cg_scan_top<<<1, 1024, 0, st>>>(g->tsum, g->ntiles, C, 1, &C->n_md);
+    k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(g->mdoff, nc, g->tsum, C, 1);
+    k_cg_md_fill<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_start, s.cell_count, g->mdoff, g->mdslot, g->mdflag, C,
+                                           1);
+    RMPCD_LAUNCHED();
+    RMPCD_HIP(hipStreamSynchronize(st));
+    return RMPCD_OK;
+}
+
+template <class F>
+static int graph_run(ProgramSim &s, int n_cycles)
+{
+    if (!s.graph) {
+        int rc = graph_setup<F>(s);
+        if (rc) return rc;
+    }
+    ColloidsGraph &g = *s.graph;
+    hipStream_t st = stream();
+    for (int cy = 0; cy < n_cycles; ++cy) {
+        const int fp = s.fpar;
+        if (!g.exec[fp]) {
+            int rc = capture_cycle<F>(s, g, fp, &g.exec[fp]);
+            if (rc) return rc;
+        }
+        RMPCD_HIP(hipGraphLaunch(g.exec[fp], st));
+        if (s.cfg.n_md % 2) {
+            s.fpar = 1 - s.fpar;
+            s.cfpar = 1 - s.cfpar;
+        }
+    }
+    ColCtl h;
+    RMPCD_HIP(hipMemcpyAsync(&h, g.ctl, sizeof(h), hipMemcpyDeviceToHost, st));
+    RMPCD_HIP(hipStreamSynchronize(st));
+    s.cur = h.cur;
+    s.md_steps = h.md_steps;
+    s.last_sort = h.last_sort;
+    s.n_extra = h.n_extra;
+    s.cycles = h.cycles;
+    for (int k = 0; k < 3; ++k) s.box.origin[k] = h.origin[k];
+    s.e1 = h.e1;
+    s.e2 = h.e2;
+    s.temperature = h.temperature;
+    if (h.error) return set_error(RMPCD_ERR_OVERFLOW, "maximum of neighbor list reached");
+    return RMPCD_OK;
+}
+
+int colloids_graph_run(ProgramSim &s, int n_cycles)
+{
+    RMPCD_CHECK(s.fpar == s.cfpar, "colloids graph: solvent and colloid force parities differ");
+    return s.cfg.form == RMPCD_FP64 ? graph_run<F64>(s, n_cycles) : graph_run<F32>(s, n_cycles);
+}
+
+}  // namespace rmpcd

@@ -83,44 +83,6 @@ __device__ inline void stage_tile(const PView<F> &p, const int32_t *start, const
     __syncthreads();
 }
 
-// SRD rotation (mpcd.f90:120-180, default or given alpha); axes injected per cell or
-// drawn from Threefry keyed by (seed, step, cell)
-// one cell of the SRD collision on arrays vx, vy, vz, sp (the LDS tile or global memory)
-template <class R, class S>
-__device__ inline void srd_cell(R *vx, R *vy, R *vz, const S *sp, int n, int c, int ncells,
-                                const double *axes, uint64_t seed, uint64_t step, double sin_a, double cos_a)
-{
-    Acc3<R> acc;
-    int n_eff = 0;
-    for (int i = 0; i < n; ++i) {
-        if ((int)sp[i] > 0) {
-            acc.add(0, vx[i]);
-            acc.add(1, vy[i]);
-            acc.add(2, vz[i]);
-            ++n_eff;
-        }
-    }
-    if (n_eff <= 1) return;
-    const R u0 = acc.sum(0) / (R)n_eff, u1 = acc.sum(1) / (R)n_eff, u2 = acc.sum(2) / (R)n_eff;
-    double v0, v1, v2;
-    if (axes) {
-        v0 = axes[c];
-        v1 = axes[ncells + c];
-        v2 = axes[2 * ncells + c];
-    } else {
-        Sphere sph = rand_sphere_threefry(seed, STREAM_SRD, step, (uint64_t)c);
-        v0 = sph.n[0];
-        v1 = sph.n[1];
-        v2 = sph.n[2];
-    }
-    const double omc = 1 - cos_a;
-    // omega(r,k) as filled column-major by mpcd.f90:163-171
-    const R o[9] = {(R)(cos_a + (v0 * v0) * omc), (R)((v1 * v0) * omc + v2 * sin_a), (R)((v2 * v0) * omc - v1 * sin_a),
-                    (R)((v0 * v1) * omc - v2 * sin_a), (R)(cos_a + (v1 * v1) * omc), (R)((v2 * v1) * omc + v0 * sin_a),
-                    (R)((v0 * v2) * omc + v1 * sin_a), (R)((v1 * v2) * omc - v0 * sin_a), (R)(cos_a + (v2 * v2) * omc)};
-    srd_rotate(vx, vy, vz, n, o, u0, u1, u2, acc, n_eff == n);
-}
-
 template <class F>
 __global__ void __launch_bounds__(CELL_THREADS)
     k_srd(int ncells, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
@@ -504,35 +466,6 @@ __global__ void __launch_bounds__(CELL_THREADS)
         p.vel[p.n + t.base + q] = ly[q];
         p.vel[2 * p.n + t.base + q] = lz[q];
     }
-}
-
-// compute_temperature (mpcd.f90:342-407): per block partial sums of kin and of the
-// number of counted cells; the last block adds the partials in block order
-// one cell of compute_temperature on arrays vx, vy, vz, sp (the LDS tile or global
-// memory): its kinetic term, 0 when n_eff <= 1
-template <class R, class S>
-__device__ inline double temperature_cell(const R *vx, const R *vy, const R *vz, const S *sp, int n)
-{
-    double u0 = 0, u1 = 0, u2 = 0;
-    int n_eff = 0;
-    for (int i = 0; i < n; ++i)
-        if ((int)sp[i] > 0) {
-            u0 = u0 + vx[i];
-            u1 = u1 + vy[i];
-            u2 = u2 + vz[i];
-            ++n_eff;
-        }
-    if (n_eff <= 1) return -1;  // the cell counts, but adds nothing (mpcd.f90:385)
-    u0 = u0 / n;  // divided by n (mpcd.f90:388)
-    u1 = u1 / n;
-    u2 = u2 / n;
-    double k = 0;
-    for (int i = 0; i < n; ++i)
-        if ((int)sp[i] > 0) {
-            const double d0 = vx[i] - u0, d1 = vy[i] - u1, d2 = vz[i] - u2;
-            k = k + ((d0 * d0 + d1 * d1) + d2 * d2);
-        }
-    return k / (double)(3 * (n_eff - 1));
 }
 
 // the optional z profile of compute_temperature (mpcd.f90:354-358, 395-398): a cell's

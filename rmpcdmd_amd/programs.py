@@ -25,7 +25,7 @@ DEFAULTS = {
 
 
 def make_config(kind, form=FP64, seed=1, nmax_list=0, n_thermo=0, T_final=None, n_loop=0, thermostat=0, hydro=1,
-                **kw):
+                graph=0, **kw):
     p = dict(DEFAULTS[kind])
     unknown = set(kw) - set(p)
     if unknown:
@@ -40,6 +40,7 @@ def make_config(kind, form=FP64, seed=1, nmax_list=0, n_thermo=0, T_final=None, 
         setattr(c, k, float(p[k]))
     c.seed = int(seed)
     c.n_thermo, c.n_loop, c.thermostat, c.hydro = int(n_thermo), int(n_loop), int(thermostat), int(hydro)
+    c.graph = int(graph)
     c.T_final = float(p["T"] if T_final is None else T_final)
     return c
 

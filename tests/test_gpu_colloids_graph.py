@@ -1,0 +1,104 @@
+"""The graph-captured n_colloids_pbc production loop (programs/n_colloids_pbc.f90:286-346;
+rmpcd_colloids.hip): every decision on the device, one hipGraph per collision cycle.  It
+must reproduce the reference call sequence of the same program (integer state bit for
+bit: sort order, ids, species, flags, images, cell counts, lists, the number of extra
+sorts; FP64 to 1e-12, the solvent forces of particles in several lists being summed by
+atomics in either mode), for an even and an odd N_MD (the graph of each starting force
+parity), with many extra sorts (small skin), and the oracle's loop; a list overflow stops
+it with the reference's message."""
+import numpy as np
+import pytest
+
+from rmpcdmd_amd import capi
+from rmpcdmd_amd.programs import PROG_COLLOIDS, Program
+
+pytestmark = pytest.mark.gpu
+
+INT_KEYS = ("id", "species", "flags", "image", "cell_count", "cell_start", "is_md", "nlist", "colloid_image")
+REAL_KEYS = ("pos", "vel", "force", "force_old", "pos_old", "colloid_pos", "colloid_vel", "colloid_force",
+             "colloid_force_old", "colloid_pos_old")
+
+
+def close(a, b, rtol):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return a.shape == b.shape and bool((np.abs(a - b) <= rtol * np.maximum(1.0, np.abs(b))).all())
+
+
+def compare(a, b, msg, rtol):
+    for k in INT_KEYS:
+        assert a[k].tobytes() == b[k].tobytes(), "%s %s" % (k, msg)
+    for j in range(a["nlist"].size):
+        n = a["nlist"][j]
+        assert (a["list"][j, :n] == b["list"][j, :n]).all(), "list %d %s" % (j, msg)
+    for k in REAL_KEYS:
+        assert close(a[k], b[k], rtol), "%s %s" % (k, msg)
+    for k in ("md_steps", "cycles", "n_extra_sorting", "last_sort"):
+        assert a[k] == b[k], "%s %s: %s %s" % (k, msg, a[k], b[k])
+    assert close(a["origin"], b["origin"], 0.0), msg
+    for k in ("e1", "e2", "temperature"):
+        assert close(a[k], b[k], rtol), "%s %s" % (k, msg)
+
+
+@pytest.mark.parametrize("form,n_md,skin", [(capi.FP64, 10, 0.0), (capi.FP64, 5, 0.0), (capi.FP32, 10, 0.0),
+                                            (capi.FP64, 10, 0.3), (capi.FP32, 7, 0.3)],
+                         ids=["fp64", "fp64-odd-nmd", "fp32", "fp64-small-skin", "fp32-odd-small-skin"])
+def test_graph_equals_reference_sequence(gpu, form, n_md, skin):
+    kw = dict(L=(20, 20, 20), n_md=n_md, tau=0.2, sigma=1.5, n_colloids=12, T=1.0, skin=skin)
+    ref = Program(PROG_COLLOIDS, form=form, seed=31, **kw)
+    gra = Program(PROG_COLLOIDS, form=form, seed=31, graph=1, **kw)
+    rtol = 1e-12 if form == capi.FP64 else 1e-5
+    for cycle in range(3):
+        ref.run(1)
+        gra.run(1)
+        compare(gra.download(), ref.download(), "after cycle %d" % cycle, rtol)
+    # two cycles in one run: the graph is replayed (and, for an odd N_MD, both parities)
+    ref.run(2)
+    gra.run(2)
+    a, b = gra.download(), ref.download()
+    compare(a, b, "after a run of two cycles", rtol)
+    print("extra sorts:", a["n_extra_sorting"], "md slots cells:", int(a["is_md"].sum()))
+    if skin:
+        assert a["n_extra_sorting"] > 5
+    ref.close()
+    gra.close()
+
+
+def test_graph_matches_oracle(gpu):
+    from oracle.programs import ColloidsOracle
+    L = (20, 20, 20)
+    kw = dict(L=L, n_md=10, tau=0.2, sigma=1.5, n_colloids=12, T=1.0)
+    prog = Program(PROG_COLLOIDS, form=capi.FP64, seed=13, graph=1, **kw)
+    st = prog.download()
+    orc = ColloidsOracle(st, L, 0.2, 10, 13, sigma=1.5)
+    for cycle in range(3):
+        prog.run(1)
+        orc.cycle()
+        g = prog.download()
+        msg = "after cycle %d" % cycle
+        assert (g["id"] == orc.s.id).all(), msg
+        assert (g["flags"] == orc.s.flags).all(), msg
+        assert (g["cell_count"] == orc.cnt).all() and (g["cell_start"] == orc.st).all(), msg
+        for k in ("pos", "vel", "force", "force_old"):
+            assert close(g[k], getattr(orc.s, k), 1e-12), "%s %s" % (k, msg)
+            assert close(g["colloid_" + k], getattr(orc.col, k), 1e-12), "colloid %s %s" % (k, msg)
+        assert (g["nlist"] == orc.nl).all(), msg
+        assert g["n_extra_sorting"] == orc.n_extra and g["last_sort"] == orc.last_sort, msg
+        assert close(g["temperature"], orc.temperature, 1e-12), msg
+    prog.close()
+
+
+def test_graph_list_overflow_stops(gpu):
+    kw = dict(L=(20, 20, 20), n_md=10, tau=0.2, sigma=1.5, n_colloids=12, T=1.0)
+    probe = Program(PROG_COLLOIDS, form=capi.FP64, seed=37, **kw)
+    n0 = int(probe.download()["nlist"].max())
+    probe.close()
+    prog = Program(PROG_COLLOIDS, form=capi.FP64, seed=37, graph=1, nmax_list=n0, **kw)
+    with pytest.raises(capi.RmpcdError) as e:
+        prog.run(6)
+    assert e.value.code == capi.RMPCD_ERR_OVERFLOW
+    assert "maximum of neighbor list reached" in e.value.msg
+    g = prog.download()
+    assert np.isfinite(g["pos"]).all() and (g["nlist"] <= n0).all()
+    assert g["md_steps"] < 60
+    prog.close()
