@@ -54,6 +54,11 @@ def parse():
                     help="skip the wall, forced-flow, thermostat and energy routines")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the 128-replica run with L drawn from {24, 32, 40, 48}")
+    ap.add_argument("--no-colloids", action="store_true",
+                    help="skip the n_colloids_pbc workload (L=128, 1000 colloids, lazy streaming)")
+    ap.add_argument("--colloids-only", action="store_true",
+                    help="run only the n_colloids_pbc workload (for profiling it)")
+    ap.add_argument("--colloids-cycles", type=int, default=4, help="timed cycles of the n_colloids_pbc workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device to device once (the reference dispatch for "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -304,6 +309,75 @@ def linked_bodies(with_cpu):
     return out
 
 
+def n_colloids(cycles, warmup=1, forms=("fp32", "fp64")):
+    """programs/n_colloids_pbc.f90's production loop at L=128, rho=10, 1000 colloids
+    sigma=2.5 (20,316,937 solvent), tau=0.5, N_MD=50, as the graph-captured program (one
+    hipGraph per collision cycle).  Lazy streaming: the solvent of the cells near the
+    colloids (fraction f_md) moves every step, the rest at the sorts, so the algorithmic
+    bytes per particle-step are f_md 9w + (n_sort / n_dt)(2S + 16/rho + 9w).  The FP32-mixed
+    form and the FP64 form start from the same state and draws; their difference after the
+    timed cycles is the FP32 error at this box size."""
+    import numpy as np
+    import torch
+
+    from rmpcdmd_amd import capi
+    from rmpcdmd_amd.programs import PROG_COLLOIDS, Program
+    out = {"workload": "n_colloids_pbc production loop: L=128^3, rho=10, 1000 colloids sigma=2.5 (colloid-colloid "
+                       "LJ cut 6 sigma), skin 0.8, tau=0.5, N_MD=50, T=1, lazy cell-wise streaming; a step is one "
+                       "collision cycle"}
+    runs = {}
+    for form in forms:
+        f = capi.FP32 if form == "fp32" else capi.FP64
+        t0 = time.perf_counter()
+        prog = Program(PROG_COLLOIDS, form=f, seed=20261202, graph=1, L=(128, 128, 128), n_md=50, tau=0.5,
+                       sigma=2.5, n_colloids=1000, T=1.0)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        prog.run(warmup)
+        torch.cuda.synchronize()
+        e0 = int(prog.state()[4].n_extra_sorting)
+        temps = []
+        fmd = []
+        t_run = 0.0
+        for c in range(cycles):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            prog.run(1)
+            torch.cuda.synchronize()
+            t_run += time.perf_counter() - t1
+            st = prog.state()[4]
+            temps.append(float(st.temperature))
+            d = prog.download_cells()
+            fmd.append(float(d["cell_count"][d["is_md"] != 0].sum()) / float(st.n_solvent))
+        st = prog.state()[4]
+        n = int(st.n_solvent)
+        extra = int(st.n_extra_sorting) - e0
+        w = 4 if form == "fp32" else 8
+        S = 38 if form == "fp32" else 68
+        f_md = float(np.mean(fmd))
+        n_sort = 1.0 + extra / cycles
+        bps = f_md * 9 * w + (n_sort / 50.0) * (2 * S + 16.0 / 10 + 9 * w)
+        value = n * 50 * cycles / t_run
+        col = prog.download_colloids()
+        runs[form] = {"value": value, "ms_per_step": 1e3 * t_run / cycles, "n_solvent": n, "setup_s": setup,
+                      "f_md": f_md, "sorts_per_cycle": n_sort, "extra_sorts": extra,
+                      "bytes_per_particle_step": bps, "effective_GBs": bps * value / 1e9,
+                      "frac_of_hbm_peak": bps * value / 1e9 / HBM_PEAK_GBS, "temperature": temps,
+                      "_colloid_pos": col}
+        prog.close()
+    if "fp32" in runs and "fp64" in runs:
+        a, b = runs["fp32"], runs["fp64"]
+        ta, tb = np.array(a["temperature"]), np.array(b["temperature"])
+        out["fp32_vs_fp64"] = {
+            "temperature_rel_max": float(np.abs(ta - tb).max() / np.abs(tb).max()),
+            "colloid_pos_abs_max": float(np.abs(a["_colloid_pos"] - b["_colloid_pos"]).max()),
+            "cycles": cycles + warmup}
+    for form in runs:
+        runs[form].pop("_colloid_pos")
+        out[form] = runs[form]
+    return out
+
+
 def mpcd_variants(with_cpu, L=64, reps=20):
     """the wall, forced-flow, thermostat and energy routines (mpcd.f90:144-160, 193-337,
     459-511, 646-700, 748-801) on one L^3 box at rho = 10 (walls: z walls, periodic x, y),
@@ -419,6 +493,9 @@ def main():
         capi.call("rmpcd_memcpy", dst, src, nbytes, 2)
         capi.call("rmpcd_free", src)
         capi.call("rmpcd_free", dst)
+    if a.colloids_only:
+        print(json.dumps({"n_colloids_L128": n_colloids(a.colloids_cycles)}))
+        return
     main_run = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, a.replicas, profile=True)
     compare = None
     if not a.no_compare and a.form == "fp32":
@@ -488,6 +565,8 @@ def main():
     line["step_bytes_per_particle_step"] = 9 * w + n_sorts * (2 * S + 16 / 10)
     line["step_effective_GBs"] = line["step_bytes_per_particle_step"] * main_run["ps_per_s"] / 1e9
     line["step_frac_of_hbm_peak"] = line["step_effective_GBs"] / HBM_PEAK_GBS
+    if not a.no_colloids:
+        line["n_colloids_L128"] = n_colloids(a.colloids_cycles)
     if not a.no_bodies:
         line["linked_bodies"] = linked_bodies(not a.no_cpu)
     if not a.no_variants:

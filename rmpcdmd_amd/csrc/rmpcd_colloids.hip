@@ -21,6 +21,7 @@
 //     particle that carries a force or an MD/PAST_MD bit is within the list radius of a
 //     colloid (the skin test rebuilds the list before one can leave it), hence in an is_md
 //     cell, and for every other particle the reference's full passes change nothing.
+#include "rmpcd_pairforce.hpp"
 #include "rmpcd_programs.hpp"
 
 namespace rmpcd {
@@ -617,58 +618,24 @@ __global__ void k_cg_force_sum(ColB col, const ColCtl *C, int cfp, const double 
     epart[i] = ae;
 }
 
-// compute_force_n2 (neighbor_list.f90:227-264), owner computes: colloid i receives, in the
-// reference's loop order, -f(j,i) for j < i and then +f(i,j) for j > i
-__global__ void k_cg_force_n2(ColB col, const ColCtl *C, int cfp, double L0, double L1, double L2, LJ lj,
-                              double *__restrict__ fnew, double *__restrict__ epart)
+// the colloid forces of compute_force_n2 (rmpcd_pairforce.hpp), then the energies (tree sums)
+__global__ void __launch_bounds__(CG_T) k_cg_force_fin(ColB col, ColCtl *C, int cfp, const double *__restrict__ fnew,
+                                                       const double *__restrict__ epart)
 {
-    if (C->error) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= col.n) return;
-    const int64_t n = col.n;
-    const double *F = col.force[cfp];
-    double F0 = F[i], F1 = F[n + i], F2 = F[2 * n + i], e = 0;
-    for (int64_t j = 0; j < n; ++j) {
-        if (j == i) continue;
-        const int64_t a = j < i ? j : i, bb = j < i ? i : j;
-        const double d0 = min_image(col.pos[a] - col.pos[bb], L0);
-        const double d1 = min_image(col.pos[n + a] - col.pos[n + bb], L1);
-        const double d2 = min_image(col.pos[2 * n + a] - col.pos[2 * n + bb], L2);
-        const double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
-        const int sa = col.species[a], sb = col.species[bb];
-        const int pc = (sa - 1) + lj.n1 * (sb - 1), pe = (sb - 1) + lj.n1 * (sa - 1);
-        if (r_sq <= lj.cut_sq[pc]) {
-            const double fr = lj_force_over_r(r_sq, lj.eps[pe], lj.sigma[pe]);
-            if (j < i) {
-                F0 = F0 - fr * d0;
-                F1 = F1 - fr * d1;
-                F2 = F2 - fr * d2;
-            } else {
-                F0 = F0 + fr * d0;
-                F1 = F1 + fr * d1;
-                F2 = F2 + fr * d2;
-                e = e + lj_energy(r_sq, lj.eps[pe], lj.sigma[pe], lj.cut_energy[pe]);
-            }
-        }
-    }
-    fnew[i] = F0;
-    fnew[n + i] = F1;
-    fnew[2 * n + i] = F2;
-    epart[col.n + i] = e;
-}
-
-// the colloid forces of compute_force_n2, then the energies summed in colloid order
-__global__ void k_cg_force_fin(ColB col, ColCtl *C, int cfp, const double *__restrict__ fnew,
-                               const double *__restrict__ epart)
-{
+    __shared__ double red[16];
     if (C->error) return;
     const int64_t n3 = 3 * (int64_t)col.n;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
         col.force[cfp][i] = fnew[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        double s1 = 0, s2 = 0;
-        for (int i = 0; i < col.n; ++i) s1 = s1 + epart[i];
-        for (int i = 0; i < col.n; ++i) s2 = s2 + epart[col.n + i];
+    if (blockIdx.x != 0) return;
+    double s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < col.n; i += blockDim.x) {
+        s1 += epart[i];
+        s2 += epart[col.n + i];
+    }
+    s1 = block_sum(s1, red);
+    s2 = block_sum(s2, red);
+    if (threadIdx.x == 0) {
         C->e1 = s1;
         C->e2 = s2;
     }
@@ -716,23 +683,75 @@ __global__ void k_cg_colloid_vel(ColB c, ColCtl *C, int cfp, double dt)
 }
 
 // ---- collision ------------------------------------------------------------------------
-// simple_mpcd_step with the default angle (mpcd.f90:71-183), axes threefry((s, cell + 2^40 k)) of
-// stream SRD; one thread per cell
+// A block owns CG_CELL_T consecutive cells: their particles (a contiguous slot range) are
+// staged in LDS with coalesced loads when they fit CG_STAGE, and each thread works on one
+// cell there; a block whose cells hold more works from global memory.
+constexpr int CG_STAGE = 2048;
+
+template <class F>
+struct CgStage {
+    typename F::real x[CG_STAGE], y[CG_STAGE], z[CG_STAGE];
+    typename F::small sp[CG_STAGE];
+    int64_t base, end;
+};
+
+// stage the velocities and species of cells [c0, c1) of buffer b; returns the staged count
+// (0 when the range does not fit)
+template <class F>
+__device__ inline int cg_stage(CgStage<F> &t, const SolB<F> &S, int b, const int32_t *start, const int32_t *count,
+                               int c0, int c1)
+{
+    if (threadIdx.x == 0) {
+        t.base = start[c0];
+        t.end = (int64_t)start[c1 - 1] + count[c1 - 1];
+    }
+    __syncthreads();
+    const int64_t base = t.base, n = S.n;
+    const int tc = t.end - base <= CG_STAGE ? (int)(t.end - base) : 0;
+    const typename F::real *v = S.vel[b];
+    const typename F::small *sp = S.species[b];
+    for (int q = threadIdx.x; q < tc; q += blockDim.x) {
+        t.x[q] = v[base + q];
+        t.y[q] = v[n + base + q];
+        t.z[q] = v[2 * n + base + q];
+        t.sp[q] = sp[base + q];
+    }
+    __syncthreads();
+    return tc;
+}
+
+// simple_mpcd_step with the default angle (mpcd.f90:71-183), axes threefry((s, cell + 2^40 k))
+// of stream SRD
 template <class F>
 __global__ void __launch_bounds__(CG_CELL_T) k_cg_srd(SolB<F> S, const ColCtl *C, int ncells,
                                                       const int32_t *__restrict__ start,
                                                       const int32_t *__restrict__ count, uint64_t seed)
 {
+    __shared__ CgStage<F> t;
     if (C->error) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const int m = count[c];
-    if (m <= 1) return;
+    const int c0 = blockIdx.x * CG_CELL_T, c1 = min(c0 + CG_CELL_T, ncells);
     const int b = C->cur;
-    const int64_t n = S.n, s = start[c];
+    const int64_t n = S.n;
+    const uint64_t cyc = (uint64_t)C->cycles;
+    const int tc = cg_stage<F>(t, S, b, start, count, c0, c1);
     typename F::real *v = S.vel[b];
-    srd_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m, c, ncells, nullptr, seed, (uint64_t)C->cycles,
-             1.0, 0.0);
+    const int c = c0 + threadIdx.x;
+    if (c < c1 && count[c] > 1) {
+        const int m = count[c];
+        const int64_t s = start[c];
+        if (tc > 0) {
+            const int o = (int)(s - t.base);
+            srd_cell(t.x + o, t.y + o, t.z + o, t.sp + o, m, c, ncells, nullptr, seed, cyc, 1.0, 0.0);
+        } else {
+            srd_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m, c, ncells, nullptr, seed, cyc, 1.0, 0.0);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < tc; q += blockDim.x) {
+        v[t.base + q] = t.x[q];
+        v[n + t.base + q] = t.y[q];
+        v[2 * n + t.base + q] = t.z[q];
+    }
 }
 
 // compute_temperature (mpcd.f90:342-407): per block, the sum of the cells' kinetic terms
@@ -742,15 +761,23 @@ __global__ void __launch_bounds__(CG_CELL_T) k_cg_temp(SolB<F> S, const ColCtl *
                                                        const int32_t *__restrict__ start,
                                                        const int32_t *__restrict__ count, double *__restrict__ tpart)
 {
+    __shared__ CgStage<F> t;
     __shared__ double red[16];
+    if (C->error) return;
+    const int c0 = blockIdx.x * CG_CELL_T, c1 = min(c0 + CG_CELL_T, ncells);
+    const int b = C->cur;
+    const int64_t n = S.n;
+    const int tc = cg_stage<F>(t, S, b, start, count, c0, c1);
     double kin = 0, cnt = 0;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!C->error && c < ncells && count[c] > 1) {
-        const int b = C->cur;
-        const int64_t n = S.n, s = start[c];
+    const int c = c0 + threadIdx.x;
+    if (c < c1 && count[c] > 1) {
+        const int m = count[c];
+        const int64_t s = start[c];
         const typename F::real *v = S.vel[b];
         cnt = 1;
-        const double k = temperature_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, count[c]);
+        const double k = tc > 0 ? temperature_cell(t.x + (s - t.base), t.y + (s - t.base), t.z + (s - t.base),
+                                                   t.sp + (s - t.base), m)
+                                : temperature_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m);
         kin = k < 0 ? 0 : k;
     }
     const double bk = block_sum(kin, red);
@@ -896,7 +923,8 @@ static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_t st)
     k_cg_force_nl<F><<<fgrid, CG_T, 0, st>>>(S, col, C, np, np, s.list, s.nlist, s.nmax, E[0], E[1], E[2],
                                              make_lj(&s.lj_sc), g.fpart);
     k_cg_force_sum<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fpart, g.fblocks, g.epart);
-    k_cg_force_n2<<<cgrid, CG_T, 0, st>>>(col, C, np, E[0], E[1], E[2], make_lj(&s.lj_cc), g.fnew, g.epart);
+    k_force_n2_blocks<CG_T><<<(unsigned)s.nc, CG_T, 0, st>>>(col.pos, col.species, col.n, col.force[np], E[0], E[1], E[2],
+                                                             make_lj(&s.lj_cc), g.fnew, g.epart + s.nc);
     k_cg_force_fin<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fnew, g.epart);
     k_cg_vel<F><<<CG_GRID, CG_T, 0, st>>>(S, C, np, s.dt, g.mdslot);
     k_cg_colloid_vel<<<cgrid, CG_T, 0, st>>>(col, C, np, s.dt);

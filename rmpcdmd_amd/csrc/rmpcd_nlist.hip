@@ -6,6 +6,7 @@
 // order) with two passes over (point, stencil cell): a count pass, a per-point scan of
 // the counts, and a fill pass that writes every accepted particle at its final offset.
 #include "rmpcd_internal.hpp"
+#include "rmpcd_pairforce.hpp"
 
 namespace rmpcd {
 
@@ -192,51 +193,6 @@ __global__ void __launch_bounds__(256)
         e_out[i] = ae;
         done[i] = 0;
     }
-}
-
-// compute_force_n2 (neighbor_list.f90:227-264), owner computes: colloid i receives, in
-// the reference's loop order, -f(j,i) for j < i and then +f(i,j) for j > i
-__global__ void k_force_n2(PView<F64> c, double L0, double L1, double L2, LJ lj,
-                           double *__restrict__ fnew, double *__restrict__ e_part)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    int64_t n = c.n;
-    double F0 = c.force[i], F1 = c.force[n + i], F2 = c.force[2 * n + i], e = 0;
-    double xi0 = c.pos[i], xi1 = c.pos[n + i], xi2 = c.pos[2 * n + i];
-    int si = c.species[i];
-    for (int64_t j = 0; j < n; ++j) {
-        if (j == i) continue;
-        // pair (a, b) with a < b as the reference visits it
-        int64_t a = j < i ? j : i, bb = j < i ? i : j;
-        double d0 = min_image(c.pos[a] - c.pos[bb], L0);
-        double d1 = min_image(c.pos[n + a] - c.pos[n + bb], L1);
-        double d2 = min_image(c.pos[2 * n + a] - c.pos[2 * n + bb], L2);
-        double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
-        int sa = c.species[a], sb = c.species[bb];
-        int pc = (sa - 1) + lj.n1 * (sb - 1), pe = (sb - 1) + lj.n1 * (sa - 1);
-        if (r_sq <= lj.cut_sq[pc]) {
-            double fr = lj_force_over_r(r_sq, lj.eps[pe], lj.sigma[pe]);
-            if (j < i) {
-                F0 = F0 - fr * d0;
-                F1 = F1 - fr * d1;
-                F2 = F2 - fr * d2;
-            } else {
-                F0 = F0 + fr * d0;
-                F1 = F1 + fr * d1;
-                F2 = F2 + fr * d2;
-                e = e + lj_energy(r_sq, lj.eps[pe], lj.sigma[pe], lj.cut_energy[pe]);
-            }
-        }
-    }
-    (void)xi0;
-    (void)xi1;
-    (void)xi2;
-    (void)si;
-    fnew[i] = F0;
-    fnew[n + i] = F1;
-    fnew[2 * n + i] = F2;
-    e_part[i] = e;
 }
 
 __global__ void k_copy_f64(const double *__restrict__ a, double *__restrict__ b, int64_t n)
@@ -477,11 +433,13 @@ int rmpcd_compute_force_n2(rmpcd_particles_t *ps, const double L[3], const rmpcd
     double *host = (double *)pinned(64);
     if (!fnew || !ep || !host) return set_error(RMPCD_ERR_HIP, "compute_force_n2: workspace allocation failed");
     hipStream_t s = stream();
-    k_force_n2<<<blocks_for(n, 64), 64, 0, s>>>(view<F64>(ps), L[0], L[1], L[2], make_lj(lj), fnew, ep);
+    const PView<F64> c = view<F64>(ps);
+    k_force_n2_blocks<256><<<(unsigned)n, 256, 0, s>>>(c.pos, c.species, n, c.force, L[0], L[1], L[2], make_lj(lj), fnew,
+                                                       ep);
     RMPCD_LAUNCHED();
     k_copy_f64<<<blocks_for(3 * n, 256), 256, 0, s>>>(fnew, (double *)ps->force, 3 * n);
     RMPCD_LAUNCHED();
-    k_sum_serial<<<1, 64, 0, s>>>(ep, n, ep + n);
+    k_sum_block<256><<<1, 256, 0, s>>>(ep, n, ep + n);
     RMPCD_LAUNCHED();
     RMPCD_HIP(hipMemcpyAsync(host, ep + n, sizeof(double), hipMemcpyDeviceToHost, s));
     RMPCD_HIP(hipStreamSynchronize(s));

@@ -99,6 +99,24 @@ class Program:
             out[k] = getattr(st, k)
         return out
 
+    def download_cells(self):
+        """host copies of the cell arrays (count, start, is_md) only"""
+        sol, col, box, cells, st = self.state()
+        out = {}
+        for name, ptr, dt in (("cell_count", cells.cell_count, np.int32), ("cell_start", cells.cell_start, np.int32),
+                              ("is_md", cells.is_md, np.uint8)):
+            a = np.empty(box.ncells, dtype=dt)
+            capi.call("rmpcd_memcpy", a.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), a.nbytes, 1)
+            out[name] = a
+        return out
+
+    def download_colloids(self):
+        """host copy of the colloid positions, (n, 3)"""
+        sol, col, box, cells, st = self.state()
+        a = np.empty(3 * col.n, dtype=np.float64)
+        capi.call("rmpcd_memcpy", a.ctypes.data_as(C.c_void_p), C.c_void_p(col.pos), a.nbytes, 1)
+        return a.reshape(3, col.n).T.copy()
+
     def close(self):
         if self._h:
             capi.lib().rmpcd_program_destroy(self._h)

@@ -102,3 +102,37 @@ def test_graph_list_overflow_stops(gpu):
     assert np.isfinite(g["pos"]).all() and (g["nlist"] <= n0).all()
     assert g["md_steps"] < 60
     prog.close()
+
+
+def test_fp32_error_at_L128(gpu):
+    """the n_colloids_pbc workload at its size (L=128, 1000 colloids sigma=2.5, 20.3M
+    solvent): the FP32-mixed form against the FP64 form from the same state and draws,
+    temperature within 0.1 % every cycle, and the FP32 form's total momentum kept to its
+    rounding level (FP32 positions at L=128 carry an ulp of 2^-17)"""
+    kw = dict(L=(128, 128, 128), n_md=50, tau=0.5, sigma=2.5, n_colloids=1000, T=1.0, graph=1)
+    s64 = Program(PROG_COLLOIDS, form=capi.FP64, seed=20261202, **kw)
+    s32 = Program(PROG_COLLOIDS, form=capi.FP32, seed=20261202, **kw)
+    mass = float(s64.state()[4].mass)
+
+    def momentum(p):
+        sol, col, box, cells, st = p.state()
+        d = p.download()
+        return d["vel"].astype(np.float64).sum(axis=0) + mass * d["colloid_vel"].sum(axis=0), int(st.n_solvent)
+
+    p0, n = momentum(s32)
+    worst = {"temperature": 0.0, "colloid_pos": 0.0}
+    for cycle in range(2):
+        s64.run(1)
+        s32.run(1)
+        t64, t32 = s64.state()[4].temperature, s32.state()[4].temperature
+        worst["temperature"] = max(worst["temperature"], abs(t32 - t64) / t64)
+        d = np.abs(s32.download_colloids() - s64.download_colloids()).max()
+        worst["colloid_pos"] = max(worst["colloid_pos"], float(d))
+        print("cycle %d: T64 %.9f T32 %.9f, colloid positions differ by %.3e" % (cycle, t64, t32, d), flush=True)
+    p1, _ = momentum(s32)
+    drift = float(np.abs(p1 - p0).max()) / (n + 1000 * mass)
+    print("FP32 vs FP64 at L=128:", worst, "momentum drift per particle %.2e" % drift)
+    assert worst["temperature"] < 1e-3
+    assert drift < 1e-6
+    s64.close()
+    s32.close()
