@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--colloids-only", action="store_true",
                     help="run only the n_colloids_pbc workload (for profiling it)")
     ap.add_argument("--colloids-cycles", type=int, default=4, help="timed cycles of the n_colloids_pbc workload")
+    ap.add_argument("--colloids-forms", default="fp32,fp64", help="forms of the n_colloids_pbc workload")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="run only the 128-replica mixed-L batch (for profiling it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device to device once (the reference dispatch for "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -200,14 +203,13 @@ def cpu_baseline_child(L, n_md, threads, min_seconds, max_cycles=40):
 def measured_traffic(form, L, replicas):
     """HBM bytes per launch of the streaming kernel, from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), when one matches"""
-    path = os.path.join(REPO, "profiles", "stream_kernel_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    key = "%s L=%d replicas=%d" % (form, L, replicas)
-    for line in open(path):
-        m = re.match(r"\s*traffic_bytes_per_launch\[(.*)\]\s*=\s*([0-9.eE+]+)", line)
-        if m and m.group(1) == key:
-            return float(m.group(2))
+    import glob
+    key = "%s L=%d replicas=%d" % (form, L, replicas) if L else "%s mixed-L replicas=%d" % (form, replicas)
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "*.txt"))):
+        for line in open(path):
+            m = re.match(r"\s*traffic_bytes_per_launch\[(.*)\]\s*=\s*([0-9.eE+]+)", line)
+            if m and m.group(1) == key:
+                return float(m.group(2))
     return None
 
 
@@ -306,6 +308,31 @@ def linked_bodies(with_cpu):
         cpu_vel = (time.perf_counter() - t) * 1e3
         out["cpu_baseline"] = {"kind": "port", "cores": 1, "rattle_pos_ms": cpu_pos, "rattle_vel_ms": cpu_vel,
                                "sample": "the oracle (C, one thread) on the same shell and step"}
+    return out
+
+
+def mixed_batch(a):
+    """128 single-dimer replicas with L drawn from {24, 32, 40, 48} (67,496,960 solvent, a
+    working set far above the 256 MB Infinity Cache), with the roofline of its streaming
+    kernel (event nodes in the captured cycle, as for the headline) and the whole step
+    against B_ps"""
+    sizes = mixed_sizes()
+    mx = run_gpu(a.form, a.mode, 0, a.n_md, max(2, a.steps // 2), 1, 0, profile=a.mode == "fused", sizes=sizes)
+    out = {"value": mx["ps_per_s"], "ms_per_step": mx["ms_per_cycle"], "n_solvent": mx["n"],
+           "replicas": len(sizes), "L": "128 replicas, L uniform in {24,32,40,48}, seeded"}
+    w = 4 if a.form == "fp32" else 8
+    S = 38 if a.form == "fp32" else 68
+    n_sorts = (mx["extra_sorts"] / max(1, mx["cycles"] * len(sizes)) + 1) / a.n_md
+    bps = 9 * w + n_sorts * (2 * S + 16 / 10)
+    out["step_bytes_per_particle_step"] = bps
+    out["step_frac_of_hbm_peak"] = bps * mx["ps_per_s"] / 1e9 / HBM_PEAK_GBS
+    if mx.get("stream_ms", 0) > 0:
+        alg = 9 * w * mx["n"]
+        ach = alg / (mx["stream_ms"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "kernel": "k_fz_step", "algorithmic_bytes_per_launch": alg,
+                           "avg_launch_ms": mx["stream_ms"], "launches_timed": mx["stream_samples"],
+                           "traffic": measured_traffic(a.form, 0, len(sizes))}
     return out
 
 
@@ -494,7 +521,10 @@ def main():
         capi.call("rmpcd_free", src)
         capi.call("rmpcd_free", dst)
     if a.colloids_only:
-        print(json.dumps({"n_colloids_L128": n_colloids(a.colloids_cycles)}))
+        print(json.dumps({"n_colloids_L128": n_colloids(a.colloids_cycles, forms=tuple(a.colloids_forms.split(",")))}))
+        return
+    if a.mixed_only:
+        print(json.dumps({"batched_mixed_L": mixed_batch(a)}))
         return
     main_run = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, a.replicas, profile=True)
     compare = None
@@ -551,10 +581,7 @@ def main():
         line["lazy_streaming"] = {"value": lz["ps_per_s"], "ms_per_step": lz["ms_per_cycle"],
                                   "over_eager": lz["ps_per_s"] / main_run["ps_per_s"]}
     if not a.no_mixed:
-        sizes = mixed_sizes()
-        mx = run_gpu(a.form, a.mode, 0, a.n_md, max(2, a.steps // 2), 1, 0, profile=False, sizes=sizes)
-        line["batched_mixed_L"] = {"value": mx["ps_per_s"], "ms_per_step": mx["ms_per_cycle"], "n_solvent": mx["n"],
-                                   "replicas": len(sizes), "L": "128 replicas, L uniform in {24,32,40,48}, seeded"}
+        line["batched_mixed_L"] = mixed_batch(a)
     if not a.no_single and a.replicas > 1:
         single = run_gpu(a.form, a.mode, a.L, a.n_md, a.steps, a.warmup, 1, profile=False)
         line["single_replica"] = {"value": single["ps_per_s"], "ms_per_step": single["ms_per_cycle"],
