@@ -66,6 +66,7 @@ struct ColloidsGraph {
     uint8_t *mdflag = nullptr;
     int32_t *mdc = nullptr, *mdoff = nullptr, *nlcnt = nullptr;
     int32_t *lut = nullptr;  // the box's Hilbert table, a copy owned by the program
+    void *rec = nullptr, *frec = nullptr;  // staging records of a sort, forces of the members
     double *part = nullptr, *cpart = nullptr, *fpart = nullptr, *epart = nullptr, *fnew = nullptr,
            *tpart = nullptr;
     int ntiles = 0, fblocks = 0, cblocks = 0;
@@ -77,6 +78,23 @@ __device__ inline bool cg_off(const ColCtl *C, int forced) { return C->error || 
 
 __device__ inline int64_t cg_tid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline int64_t cg_stride() { return (int64_t)gridDim.x * blockDim.x; }
+
+__device__ inline int cg_block_scan(int v, int *lds, int *total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int inc = wave_inclusive_scan(v);
+    if (lane == 63) lds[wid] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int t = lds[w];
+        if (w < wid) off += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
 
 // ---- MD step --------------------------------------------------------------------------
 // cell_md_pos with md_flag true (md.f90:79-102) over the md slots, and the squared
@@ -177,40 +195,7 @@ __global__ void k_cg_shift(ColCtl *C, uint64_t seed, double dt)
     C->catch_dt = (double)(C->md_steps - C->last_sort) * dt;
 }
 
-// cell_md_pos then cell_md_vel with md_flag false and k dt (md.f90:79-102, 239-263): the
-// particles of the cells streamed lazily catch up
-template <class F>
-__global__ void __launch_bounds__(CG_T) k_cg_catchup(SolB<F> S, const ColCtl *C, int fp, int forced,
-                                                     const uint8_t *__restrict__ mdflag)
-{
-    using R = typename F::real;
-    if (cg_off(C, forced)) return;
-    const int b = C->cur;
-    const int64_t n = S.n;
-    R *pos = S.pos[b], *vel = S.vel[b];
-    const R *f = S.force[2 * b + fp], *fo = S.force[2 * b + 1 - fp];
-    typename F::small *flags = S.flags[b];
-    const double cdt = C->catch_dt;
-    const R rdt = (R)cdt, dt_sq = (R)((cdt * cdt) / 2);
-    for (int64_t j = cg_tid(); j < n; j += cg_stride()) {
-        if (mdflag[j]) continue;
-        R v[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            v[q] = vel[q * n + j];
-            pos[q * n + j] = (pos[q * n + j] + rdt * v[q]) + dt_sq * f[q * n + j];
-        }
-        const int fl = (int)flags[j];
-        if (!(fl & WALL_MASK)) {
-#pragma unroll
-            for (int q = 0; q < 3; ++q) vel[q * n + j] = v[q] + (rdt * (f[q * n + j] + fo[q * n + j])) / (R)2;
-        } else {
-            flags[j] = (typename F::small)(fl & ~WALL_MASK);
-        }
-    }
-}
-
-// apply_pbc of the colloids (md.f90:202-217)
+// apply_pbc of the colloids (md.f90:202-217) and colloids%pos_old = colloids%pos
 __global__ void k_cg_colloid_pbc(ColB c, const ColCtl *C, int forced, double e0, double e1, double e2)
 {
     if (cg_off(C, forced)) return;
@@ -223,6 +208,148 @@ __global__ void k_cg_colloid_pbc(ColB c, const ColCtl *C, int forced, double e0,
         const int jump = (int)floor(x / e[k]);
         c.image[q] += jump;
         c.pos[q] = x - (double)jump * e[k];
+        c.pos_old[q] = c.pos[q];
+    }
+}
+
+// The staging record of a sort: a particle's caught-up, wrapped state in one piece, written
+// by the key pass in storage order and read whole by the gather (one or two cache lines
+// where the SoA arrays would cost one line each).  Forces travel apart (only particles
+// with the MD or PAST_MD bit carry one; the others' are zero).
+template <class R>
+struct alignas(16) CgRec {
+    R p[3], v[3];
+    int32_t im[3], id, key;
+    int32_t sf;  // species | flags << 16
+};
+
+// The key pass of a sort, a unit being a chunk of CG_KCH particles in storage order:
+// cell_md_pos then cell_md_vel with md_flag false and the catch-up k dt for the particles
+// outside the md cells (md.f90:79-102, 239-263), apply_pbc (md.f90:202-217), the cell key
+// (cell_system.f90:86-129) and the staging record; the chunk's keys are counted in an LDS
+// hash, then one global atomic per distinct cell reserves the chunk's slots in that cell
+// (the gather makes the order within a cell stable).
+constexpr int CG_KPT = 4, CG_KCH = CG_T * CG_KPT;
+constexpr int CG_KH_BITS = 11, CG_KH = 1 << CG_KH_BITS;  // hash entries, twice a chunk
+
+template <class F>
+__global__ void __launch_bounds__(CG_T) k_cg_key(SolB<F> S, const ColCtl *C, int fp, int forced, Box bx,
+                                                 const int32_t *__restrict__ lut, const uint8_t *__restrict__ mdflag,
+                                                 int32_t *__restrict__ count, int32_t *__restrict__ key,
+                                                 int32_t *__restrict__ slot, CgRec<typename F::real> *__restrict__ rec,
+                                                 typename F::real *__restrict__ frec)
+{
+    using R = typename F::real;
+    __shared__ int hkey[CG_KH], hcnt[CG_KH], occ[CG_KH];
+    __shared__ int lds[CG_T / 64];
+    if (cg_off(C, forced)) return;
+    Box b = bx;
+    for (int k = 0; k < 3; ++k) b.origin[k] = C->origin[k];
+    const int bf = C->cur;
+    const int64_t n = S.n;
+    R *pos = S.pos[bf], *vel = S.vel[bf];
+    const R *f = S.force[2 * bf + fp], *fo = S.force[2 * bf + 1 - fp];
+    const int32_t *img = S.image[bf], *ids = S.id[bf];
+    const typename F::small *spc = S.species[bf], *flg = S.flags[bf];
+    const double cdt = C->catch_dt;
+    const R rdt = (R)cdt, dt_sq = (R)((cdt * cdt) / 2);
+    const int64_t nch = (n + CG_KCH - 1) / CG_KCH;
+    for (int64_t u = blockIdx.x; u < nch; u += gridDim.x) {
+        const int64_t c0 = u * CG_KCH;
+        int keys[CG_KPT], ent[CG_KPT], loc[CG_KPT];
+        for (int v = 0; v < CG_KPT; ++v) {
+            const int64_t i = c0 + v * CG_T + threadIdx.x;
+            keys[v] = -1;
+            ent[v] = 0;
+            loc[v] = 0;
+            if (i >= n) continue;
+            R x[3], vv[3], ff[3] = {0, 0, 0}, fof[3] = {0, 0, 0};
+            int fl = (int)flg[i];
+            const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
+            for (int q = 0; q < 3; ++q) {
+                x[q] = pos[q * n + i];
+                vv[q] = vel[q * n + i];
+                if (has_f) {
+                    ff[q] = f[q * n + i];
+                    fof[q] = fo[q * n + i];
+                }
+            }
+            if (!mdflag[i]) {
+                for (int q = 0; q < 3; ++q) x[q] = (x[q] + rdt * vv[q]) + dt_sq * ff[q];
+                if (!(fl & WALL_MASK)) {
+                    for (int q = 0; q < 3; ++q) vv[q] = vv[q] + (rdt * (ff[q] + fof[q])) / (R)2;
+                } else {
+                    fl &= ~WALL_MASK;
+                }
+            }
+            CgRec<R> r;
+            for (int q = 0; q < 3; ++q) {
+                const R ed = (R)b.edges[q];
+                const int jump = (int)floor((double)(x[q] / ed));
+                x[q] = x[q] - (R)jump * ed;
+                r.p[q] = x[q];
+                r.v[q] = vv[q];
+                r.im[q] = img[q * n + i] + jump;
+            }
+            int p0, p1, p2, k;
+            cell_of<R>(b, x[0], x[1], x[2], p0, p1, p2);
+            if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] && (unsigned)p2 < (unsigned)b.L[2])
+                k = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+            else
+                k = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+            r.id = ids[i];
+            r.key = k;
+            r.sf = (int)spc[i] | (fl << 16);
+            rec[i] = r;
+            if (has_f)
+                for (int q = 0; q < 3; ++q) {
+                    frec[6 * i + q] = ff[q];
+                    frec[6 * i + 3 + q] = fof[q];
+                }
+            keys[v] = k;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < CG_KH; q += CG_T) {
+            hkey[q] = -1;
+            hcnt[q] = 0;
+        }
+        __syncthreads();
+        for (int v = 0; v < CG_KPT; ++v) {
+            const int k = keys[v];
+            if (k < 0) continue;
+            unsigned h = ((unsigned)k * 2654435761u) >> (32 - CG_KH_BITS);
+            for (int probe = 0; probe < CG_KH; ++probe) {
+                const int prev = atomicCAS(&hkey[h], -1, k);
+                if (prev == -1 || prev == k) break;
+                h = (h + 1) & (CG_KH - 1);
+            }
+            ent[v] = (int)h;
+            loc[v] = atomicAdd(&hcnt[h], 1);
+        }
+        __syncthreads();
+        {
+            // the occupied entries compacted, so that a thread waits for one global atomic a round
+            constexpr int PER = CG_KH / CG_T;
+            int c = 0;
+            for (int j = 0; j < PER; ++j) c += hkey[threadIdx.x * PER + j] >= 0 ? 1 : 0;
+            int tot;
+            int o = cg_block_scan(c, lds, &tot);
+            for (int j = 0; j < PER; ++j)
+                if (hkey[threadIdx.x * PER + j] >= 0) occ[o++] = threadIdx.x * PER + j;
+            __syncthreads();
+            for (int e = threadIdx.x; e < tot; e += CG_T) {
+                const int q = occ[e];
+                hcnt[q] = atomicAdd(&count[hkey[q]], hcnt[q]);
+            }
+        }
+        __syncthreads();
+        for (int v = 0; v < CG_KPT; ++v) {
+            const int64_t i = c0 + v * CG_T + threadIdx.x;
+            if (keys[v] < 0) continue;
+            key[i] = keys[v];
+            slot[i] = hcnt[ent[v]] + loc[v];
+        }
+        __syncthreads();
     }
 }
 
@@ -241,51 +368,6 @@ __device__ inline R cg_wrap(R x, double e, int &jump)
     return x - (R)jump * ed;
 }
 
-// count_particles with the periodic wrap (cell_system.f90:86-129): cell key from the box's
-// Hilbert table and the arrival slot in the cell (atomic; the gather makes the order stable)
-template <class F>
-__global__ void __launch_bounds__(CG_T) k_cg_key(SolB<F> S, const ColCtl *C, int forced, Box bx,
-                                                 const int32_t *__restrict__ lut, int32_t *__restrict__ count,
-                                                 int32_t *__restrict__ key, int32_t *__restrict__ slot)
-{
-    using R = typename F::real;
-    if (cg_off(C, forced)) return;
-    Box b = bx;
-    for (int k = 0; k < 3; ++k) b.origin[k] = C->origin[k];
-    const int bf = C->cur;
-    const int64_t n = S.n;
-    const R *pos = S.pos[bf];
-    for (int64_t i = cg_tid(); i < n; i += cg_stride()) {
-        int j;
-        const R x = cg_wrap<R>(pos[i], b.edges[0], j), y = cg_wrap<R>(pos[n + i], b.edges[1], j),
-                z = cg_wrap<R>(pos[2 * n + i], b.edges[2], j);
-        int p0, p1, p2, k;
-        cell_of<R>(b, x, y, z, p0, p1, p2);
-        if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] && (unsigned)p2 < (unsigned)b.L[2])
-            k = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
-        else
-            k = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
-        key[i] = k;
-        slot[i] = atomicAdd(&count[k], 1);
-    }
-}
-
-__device__ inline int cg_block_scan(int v, int *lds, int *total)
-{
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int inc = wave_inclusive_scan(v);
-    if (lane == 63) lds[wid] = inc;
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int w = 0; w < nw; ++w) {
-        const int t = lds[w];
-        if (w < wid) off += t;
-        tot += t;
-    }
-    __syncthreads();
-    *total = tot;
-    return off + inc - v;
-}
 
 // exclusive scan of nc int32 in tiles of CG_TILE: tile-local scan and tile totals, one block
 // for the totals, then each tile's offset (the total goes to *total when given)
@@ -346,42 +428,49 @@ __global__ void __launch_bounds__(CG_T) k_cg_place(int64_t n, const int32_t *__r
     for (int64_t i = cg_tid(); i < n; i += cg_stride()) tmp[start[key[i]] + slot[i]] = (int32_t)i;
 }
 
-// the stable gather into the other buffer (particle_system.f90:304-355 with apply_pbc):
-// the particle of each slot goes to cell_start + its rank by storage index among the
-// cell's arrivals, which is the reference's single-thread order
+// The stable gather into the other buffer (particle_system.f90:304-355): the particle of
+// each slot goes to cell_start + its rank by storage index among the cell's arrivals,
+// which is the reference's single-thread order.  Reads only the records (never the
+// source arrays), so it also writes pos_old = pos into the source buffer's positions
+// (after the sort they are the other buffer's).  Forces come from the side records of
+// the particles with the MD or PAST_MD bit and are zero for the others.
 template <class F>
-__global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, int forced, double e0, double e1,
-                                                    double e2, const int32_t *__restrict__ key,
+__global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, int fp, int forced,
                                                     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
-                                                    const int32_t *__restrict__ tmp, int32_t *__restrict__ id_to_idx)
+                                                    const int32_t *__restrict__ tmp,
+                                                    const CgRec<typename F::real> *__restrict__ rec,
+                                                    const typename F::real *__restrict__ frec,
+                                                    int32_t *__restrict__ id_to_idx)
 {
     using R = typename F::real;
+    using Sm = typename F::small;
     if (cg_off(C, forced)) return;
     const int b = C->cur, o = 1 - b;
     const int64_t n = S.n;
-    const double e[3] = {e0, e1, e2};
+    R *dpos = S.pos[o], *dvel = S.vel[o], *dold = S.pos[b];
+    R *df = S.force[2 * o + fp], *dfo = S.force[2 * o + 1 - fp];
     for (int64_t j = cg_tid(); j < n; j += cg_stride()) {
         const int32_t v = tmp[j];
-        const int c = key[v];
-        const int s = start[c], m = count[c];
+        const CgRec<R> r = rec[v];
+        const int s = start[r.key], m = count[r.key];
         int rank = 0;
         for (int q = s; q < s + m; ++q) rank += tmp[q] < v ? 1 : 0;
         const int64_t d = s + rank;
+        const int fl = r.sf >> 16;
+        const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            int jump;
-            const R x = cg_wrap<R>(S.pos[b][k * n + v], e[k], jump);
-            S.pos[o][k * n + d] = x;
-            S.image[o][k * n + d] = S.image[b][k * n + v] + jump;
-            S.vel[o][k * n + d] = S.vel[b][k * n + v];
-            S.force[2 * o][k * n + d] = S.force[2 * b][k * n + v];
-            S.force[2 * o + 1][k * n + d] = S.force[2 * b + 1][k * n + v];
+            dpos[k * n + d] = r.p[k];
+            dold[k * n + d] = r.p[k];
+            dvel[k * n + d] = r.v[k];
+            S.image[o][k * n + d] = r.im[k];
+            df[k * n + d] = has_f ? frec[6 * (int64_t)v + k] : (R)0;
+            dfo[k * n + d] = has_f ? frec[6 * (int64_t)v + 3 + k] : (R)0;
         }
-        const int32_t id = S.id[b][v];
-        S.id[o][d] = id;
-        S.species[o][d] = S.species[b][v];
-        S.flags[o][d] = S.flags[b][v];
-        if ((uint32_t)id < (uint32_t)n) id_to_idx[id] = (int32_t)d;
+        S.id[o][d] = r.id;
+        S.species[o][d] = (Sm)(r.sf & 0xffff);
+        S.flags[o][d] = (Sm)fl;
+        if ((uint32_t)r.id < (uint32_t)n) id_to_idx[r.id] = (int32_t)d;
     }
 }
 
@@ -501,17 +590,6 @@ __global__ void __launch_bounds__(CG_T) k_cg_md_fill(int64_t nc, const uint8_t *
             if (md) mdslot[o + k] = s + k;
         }
     }
-}
-
-// pos_old = pos for the solvent (the other buffer's position array) and the colloids
-template <class F>
-__global__ void __launch_bounds__(CG_T) k_cg_pos_old(SolB<F> S, ColB col, const ColCtl *C, int forced)
-{
-    if (cg_off(C, forced)) return;
-    const int b = C->cur;
-    const int64_t n3 = 3 * S.n;
-    for (int64_t i = cg_tid(); i < n3; i += cg_stride()) S.pos[1 - b][i] = S.pos[b][i];
-    for (int64_t i = cg_tid(); i < 3 * (int64_t)col.n; i += cg_stride()) col.pos_old[i] = col.pos[i];
 }
 
 // ---- forces ---------------------------------------------------------------------------
@@ -871,17 +949,18 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
     const int64_t nc = s.box.ncells, n = s.n;
     const unsigned cgrid = blocks_for(s.nc, CG_T);
     const double *E = s.box.edges;
-    k_cg_catchup<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, g.mdflag);
+    using R = typename F::real;
     k_cg_colloid_pbc<<<cgrid, CG_T, 0, st>>>(col, C, forced, E[0], E[1], E[2]);
     k_cg_zero_i32<<<CG_GRID, CG_T, 0, st>>>(s.cell_count, nc, C, forced);
-    k_cg_key<F><<<CG_GRID, CG_T, 0, st>>>(S, C, forced, b, g.lut, s.cell_count, g.key, g.slot);
+    k_cg_key<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, b, g.lut, g.mdflag, s.cell_count, g.key, g.slot,
+                                          (CgRec<R> *)g.rec, (R *)g.frec);
     k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_count, s.cell_start, nc,
                                                                                       g.tsum, C, forced);
     k_cg_scan_top<<<1, 1024, 0, st>>>(g.tsum, g.ntiles, C, forced, nullptr);
     k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(s.cell_start, nc, g.tsum, C, forced);
     k_cg_place<<<CG_GRID, CG_T, 0, st>>>(n, g.key, g.slot, s.cell_start, g.tmp, C, forced);
-    k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, forced, E[0], E[1], E[2], g.key, s.cell_start, s.cell_count, g.tmp,
-                                            s.id_to_idx);
+    k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, s.cell_start, s.cell_count, g.tmp,
+                                             (const CgRec<R> *)g.rec, (const R *)g.frec, s.id_to_idx);
     k_cg_commit_sort<<<1, 1, 0, st>>>(C, forced);
     k_cg_list_init<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.is_reac, C, forced);
     const dim3 lgrid(blocks_for(s.n_stencil, CG_T), (unsigned)s.nc);
@@ -897,7 +976,6 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
     k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(g.mdoff, nc, g.tsum, C, forced);
     k_cg_md_fill<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_start, s.cell_count, g.mdoff, g.mdslot, g.mdflag, C,
                                            forced);
-    k_cg_pos_old<F><<<CG_GRID, CG_T, 0, st>>>(S, col, C, forced);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
@@ -1001,6 +1079,11 @@ static int graph_setup(ProgramSim &s)
     if ((rc = cg_alloc(*g, (void **)&g->epart, 16 * (size_t)s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->fnew, 24 * (size_t)s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->tpart, 16 * (size_t)blocks_for(nc, CG_CELL_T)))) return rc;
+    {
+        using R = typename F::real;
+        if ((rc = cg_alloc(*g, &g->rec, sizeof(CgRec<R>) * n))) return rc;
+        if ((rc = cg_alloc(*g, &g->frec, 6 * sizeof(R) * n))) return rc;
+    }
     const size_t nreal = (size_t)s.box.L[0] * s.box.L[1] * s.box.L[2];
     if ((rc = cg_alloc(*g, (void **)&g->lut, 4 * nreal))) return rc;
     const int32_t *lut = hilbert_lut(make_box(&s.box));
