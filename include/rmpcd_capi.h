@@ -383,6 +383,12 @@ typedef struct rmpcd_dimer_config {
     int32_t extra_sort_blocks; /* fused mode launch shape, 0: sort_blocks: workgroups of the
                                   key, place and rank passes of the sorts that may follow
                                   each MD step (they return at once when none is due) */
+    int32_t sort_order;        /* fused mode launch shape: how the sort passes' workgroups
+                                  take their units.  0: default (2), 1: a contiguous range
+                                  per workgroup, 2: XCD-grouped (the workgroups of one XCD,
+                                  blockIdx mod 8, share a contiguous eighth of the units,
+                                  interleaved, so that the units in flight on an XCD are
+                                  neighbours and share its L2) */
 } rmpcd_dimer_config_t;
 typedef struct rmpcd_dimer_stats {
     int64_t n_solvent, md_steps, cycles, n_extra_sorting;  /* of the replica asked for */

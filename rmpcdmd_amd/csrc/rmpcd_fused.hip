@@ -115,6 +115,7 @@ struct FB {
     int64_t *plan;             // the replicas a sort chain works on (k_fz_plan)
     double *gpart;             // |v|^2 maximum per gather unit
     int32_t nmax;
+    int32_t xcd_order;  // sort passes: units grouped by XCD (1) or a contiguous range per workgroup (0)
     const RepDev *rep;
     CtlDev *ctl;
 };
@@ -685,20 +686,34 @@ __device__ inline int find_unit(const int64_t *pre, int nact, int64_t u)
     return lo;
 }
 
-// the unit range [u0, u1) of this block for pass `kind`
+// the units u0, u0 + step, ... below u1 of this block for pass `kind`: a contiguous range
+// per workgroup, or, XCD-grouped, the workgroups that run on one XCD (consecutive
+// workgroups go to the XCDs in turn) share a contiguous eighth of the units, interleaved:
+// the units in flight on an XCD are then neighbours, whose particles come from, and go
+// to, the same stretch of memory, cached once in that XCD's L2
+constexpr int XCDS = 8;
 struct UnitRange {
-    int64_t u0, u1;
+    int64_t u0, u1, step;
     int nact;
     const int64_t *pre, *act;
 };
-__device__ inline UnitRange unit_range(const int64_t *plan, int R, int kind)
+__device__ inline UnitRange unit_range(const int64_t *plan, int R, int kind, int xcd_order)
 {
     UnitRange w;
     w.nact = (int)plan[0];
     const int64_t T = plan[1 + kind];
-    const int64_t per = (T + gridDim.x - 1) / gridDim.x;
-    w.u0 = (int64_t)blockIdx.x * per;
-    w.u1 = w.u0 + per < T ? w.u0 + per : T;
+    const int G = (int)gridDim.x;
+    if (xcd_order && G % XCDS == 0) {
+        const int64_t per = (T + XCDS - 1) / XCDS, lo = (int64_t)(blockIdx.x % XCDS) * per;
+        w.u0 = lo + blockIdx.x / XCDS;
+        w.u1 = lo + per < T ? lo + per : T;
+        w.step = G / XCDS;
+    } else {
+        const int64_t per = (T + G - 1) / G;
+        w.u0 = (int64_t)blockIdx.x * per;
+        w.u1 = w.u0 + per < T ? w.u0 + per : T;
+        w.step = 1;
+    }
     w.pre = plan_pre(plan, R, kind);
     w.act = plan_act(plan);
     return w;
@@ -718,9 +733,9 @@ __device__ inline void key_pass(const FB<F> &B, const FP &P, int R)
     __shared__ int hkey[KHASH], hcnt[KHASH], occ[KHASH];
     __shared__ int lds[FT / 64];
     __shared__ double red[16];
-    const UnitRange w = unit_range(B.plan, R, U_KEY);
+    const UnitRange w = unit_range(B.plan, R, U_KEY, B.xcd_order);
     int a = -1;
-    for (int64_t u = w.u0; u < w.u1; ++u) {
+    for (int64_t u = w.u0; u < w.u1; u += w.step) {
         if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
         const int r = (int)w.act[a];
         const int64_t c0 = (u - w.pre[a]) * KCHUNK;
@@ -969,9 +984,9 @@ template <class F>
 __global__ void __launch_bounds__(SCAN_T) k_fz_scan_tiles(FB<F> B, int R)
 {
     __shared__ int lds[SCAN_T / 64];
-    const UnitRange w = unit_range(B.plan, R, U_SCAN);
+    const UnitRange w = unit_range(B.plan, R, U_SCAN, B.xcd_order);
     int a = -1;
-    for (int64_t u = w.u0; u < w.u1; ++u) {
+    for (int64_t u = w.u0; u < w.u1; u += w.step) {
         if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
         const int r = (int)w.act[a];
         const int64_t tile = u - w.pre[a];
@@ -1011,9 +1026,9 @@ __global__ void __launch_bounds__(FT) k_fz_place(FB<F> B, int forced, int R)
     using Rl = typename F::real;
     __shared__ int pre[MAX_TILES];
     __shared__ int lds[FT / 64];
-    const UnitRange w = unit_range(B.plan, R, U_PLACE);
+    const UnitRange w = unit_range(B.plan, R, U_PLACE, B.xcd_order);
     int a = -1, r_pre = -1;
-    for (int64_t u = w.u0; u < w.u1; ++u) {
+    for (int64_t u = w.u0; u < w.u1; u += w.step) {
         if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
         const int r = (int)w.act[a];
         const int64_t local = u - w.pre[a];
@@ -1156,9 +1171,9 @@ __device__ inline void gather_pass(const FB<F> &B, const FP &P, int R)
     using S = typename F::small;
     __shared__ double red[16];
     __shared__ int32_t ltmp[GT + 2 * GHALO], lkey[GT + 2 * GHALO];
-    const UnitRange w = unit_range(B.plan, R, U_GATHER);
+    const UnitRange w = unit_range(B.plan, R, U_GATHER, B.xcd_order);
     int a = -1;
-    for (int64_t u = w.u0; u < w.u1; ++u) {
+    for (int64_t u = w.u0; u < w.u1; u += w.step) {
         if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
         const int r = (int)w.act[a];
         const int64_t t0 = (u - w.pre[a]) * GT;
@@ -1259,9 +1274,9 @@ template <class F>
 __global__ void __launch_bounds__(FT) k_fz_rank(FB<F> B, int R)
 {
     __shared__ int32_t lref[GT + 2 * GHALO], lkey[GT + 2 * GHALO];
-    const UnitRange w = unit_range(B.plan, R, U_GATHER);
+    const UnitRange w = unit_range(B.plan, R, U_GATHER, B.xcd_order);
     int a = -1;
-    for (int64_t u = w.u0; u < w.u1; ++u) {
+    for (int64_t u = w.u0; u < w.u1; u += w.step) {
         if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
         const int r = (int)w.act[a];
         const int64_t t0 = (u - w.pre[a]) * GT;
@@ -1838,6 +1853,7 @@ static FB<F> make_fb(DimerSim &s)
     b.gpart = s.f_gpart;
     b.mcount = s.f_mcount;
     b.nmax = s.nmax;
+    b.xcd_order = s.sort_order == 2 ? 1 : 0;
     b.slocal = s.f_slocal;
     b.scan_stride = (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE + 1;
     b.rep = s.rep_dev;
@@ -2076,6 +2092,8 @@ int fused_prepare(DimerSim &s)
     s.stream_blocks = s.cfg.stream_blocks > 0 ? s.cfg.stream_blocks : STREAM_BLOCKS;
     s.sort_blocks = s.cfg.sort_blocks > 0 ? s.cfg.sort_blocks : SORT_BLOCKS;
     s.extra_blocks = s.cfg.extra_sort_blocks > 0 ? s.cfg.extra_sort_blocks : s.sort_blocks;
+    RMPCD_CHECK(s.cfg.sort_order >= 0 && s.cfg.sort_order <= 2, "dimer: sort_order must be 0, 1 or 2");
+    s.sort_order = s.cfg.sort_order > 0 ? s.cfg.sort_order : 2;
     RMPCD_CHECK(s.stream_blocks <= STREAM_BLOCKS && s.sort_blocks <= 65536 && s.extra_blocks <= 65536,
                 "dimer: stream_blocks must be at most 16384 and sort_blocks, extra_sort_blocks at most 65536");
     RMPCD_CHECK((s.ncmax_rep + CT - 1) / CT <= PS && (int64_t)member_blocks(s) * MPV + STREAM_BLOCKS <= PS &&

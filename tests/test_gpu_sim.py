@@ -97,22 +97,27 @@ def test_fused_dense_cells(gpu, form):
     """rho = 40: cells hold more particles than the gather and rank passes stage on either
     side of their tiles, so the ranks of those cells read the rest from memory, and the
     collision blocks overflow their LDS tiles; the fused program must still equal the
-    reference call sequence, and its odd-grid run the tuned one bit for bit"""
+    reference call sequence, and its odd-grid run and its run with a contiguous unit range
+    per workgroup (instead of XCD-grouped units) the tuned one bit for bit"""
     kw = dict(L=(12, 12, 12), rho=40, n_md=10, tau=0.1, form=form, seed=123)
     ref = SingleDimer(mode=capi.SIM_REFERENCE, **kw)
     fus = SingleDimer(mode=capi.SIM_FUSED, **kw)
     odd = SingleDimer(mode=capi.SIM_FUSED, sort_blocks=96, extra_sort_blocks=7, **kw)
+    flat = SingleDimer(mode=capi.SIM_FUSED, sort_order=1, **kw)
     for cycle in range(2):
         ref.run(1)
         fus.run(1)
         odd.run(1)
-        a, b, d = ref.download(), fus.download(), odd.download()
+        flat.run(1)
+        a, b, d, e = ref.download(), fus.download(), odd.download(), flat.download()
         assert (a["cell_count"] > 32).sum() > 100  # cells larger than the 32-slot halo
         compare(a, b, "after cycle %d" % cycle, 1e-12 if form == capi.FP64 else 1e-5)
         compare(b, d, "odd grids after cycle %d" % cycle, rtol=0.0)
+        compare(b, e, "contiguous unit ranges after cycle %d" % cycle, rtol=0.0)
     ref.close()
     fus.close()
     odd.close()
+    flat.close()
 
 
 def test_fused_replicas_of_uneven_size(gpu):
@@ -255,19 +260,24 @@ def test_many_extra_sorts_per_cycle(gpu, form):
     # odd grids for the sort passes (units are split over whatever grid there is): the
     # same trajectory bit for bit
     odd = SingleDimer(mode=capi.SIM_FUSED, sort_blocks=96, extra_sort_blocks=7, **kw)
+    # a contiguous unit range per workgroup instead of XCD-grouped units: the same again
+    flat = SingleDimer(mode=capi.SIM_FUSED, sort_order=1, sort_blocks=64, extra_sort_blocks=16, **kw)
     for cycle in range(2):
         ref.run(1)
         fus.run(1)
         lazy.run(1)
         odd.run(1)
+        flat.run(1)
         for r in range(len(Ls)):
             a, b, c, d = ref.download(r), fus.download(r), lazy.download(r), odd.download(r)
             compare(a, b, "replica %d cycle %d" % (r, cycle), 1e-12 if form == capi.FP64 else 1e-5)
             compare(b, c, "lazy replica %d cycle %d" % (r, cycle), rtol=0.0)
             compare(b, d, "odd grids replica %d cycle %d" % (r, cycle), rtol=0.0)
+            compare(b, flat.download(r), "contiguous unit ranges replica %d cycle %d" % (r, cycle), rtol=0.0)
     for r in range(len(Ls)):
         assert fus.download(r)["n_extra_sorting"] >= 6, "replica %d" % r
     ref.close()
     fus.close()
     lazy.close()
     odd.close()
+    flat.close()
