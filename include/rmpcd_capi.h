@@ -389,6 +389,12 @@ typedef struct rmpcd_dimer_config {
                                   blockIdx mod 8, share a contiguous eighth of the units,
                                   interleaved, so that the units in flight on an XCD are
                                   neighbours and share its L2) */
+    int32_t groups;            /* fused mode launch shape: the replicas are split into this
+                                  many groups of about equal particle count, each group's
+                                  collision cycle one branch of the captured graph, so that
+                                  one group's latency-bound work (colloids, sorts after MD
+                                  steps) runs beside another's streaming.  0: default (2
+                                  with two or more replicas), at most 8 and n_replicas */
 } rmpcd_dimer_config_t;
 typedef struct rmpcd_dimer_stats {
     int64_t n_solvent, md_steps, cycles, n_extra_sorting;  /* of the replica asked for */

@@ -64,7 +64,8 @@ class DimerConfig(C.Structure):
                 ("prob", C.c_double), ("bulk_rate", C.c_double), ("skin", C.c_double),
                 ("nmax_list", C.c_int32), ("n_replicas", C.c_int32), ("seed", C.c_uint64),
                 ("replica_L", C.c_void_p), ("stream_blocks", C.c_int32), ("sort_blocks", C.c_int32),
-                ("lazy", C.c_int32), ("extra_sort_blocks", C.c_int32), ("sort_order", C.c_int32)]
+                ("lazy", C.c_int32), ("extra_sort_blocks", C.c_int32), ("sort_order", C.c_int32),
+                ("groups", C.c_int32)]
 
 
 class DimerStats(C.Structure):

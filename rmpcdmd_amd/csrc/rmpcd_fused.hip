@@ -1,6 +1,7 @@
 // rmpcd_fused.hip -- the single-dimer MD step and collision cycle as fused device-side
-// kernels over every replica at once (blockIdx.y = replica), captured as one hipGraph
-// per collision cycle.
+// kernels over a group of replicas at once (blockIdx.y = replica within the group),
+// captured as one hipGraph per collision cycle with one independent branch per group (a
+// group's latency-bound kernels run beside another group's streaming).
 //
 // Per MD step (programs/single_dimer_pbc.f90:306-383), one kernel k_fz_step: the
 // neighbour-list members finish the previous step (forces, md_vel, catalytic flagging
@@ -116,6 +117,9 @@ struct FB {
     double *gpart;             // |v|^2 maximum per gather unit
     int32_t nmax;
     int32_t xcd_order;  // sort passes: units grouped by XCD (1) or a contiguous range per workgroup (0)
+    // the replica group a launch works on: replicas r0 + blockIdx.y (R of them); plan and
+    // gpart are the group's own
+    int32_t r0;
     const RepDev *rep;
     CtlDev *ctl;
 };
@@ -206,7 +210,7 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
     using R = typename F::real;
     using S = typename F::small;
     __shared__ double red[16];
-    const int r = blockIdx.y;
+    const int r = B.r0 + blockIdx.y;
     const RepDev &rp = B.rep[r];
     CtlDev &C = B.ctl[r];
     if (C.error) return;  // the replica stopped at a failed list build
@@ -436,7 +440,7 @@ __global__ void __launch_bounds__(1024) k_fz_colloid(FB<F> B, FP P, int mode, in
     __shared__ double red[16];
     __shared__ double acc[MPV];
     __shared__ double lp[MAX_MEMBER_BLOCKS * MPV];
-    const int r = blockIdx.y;
+    const int r = B.r0 + blockIdx.y;
     const RepDev &rp = B.rep[r];
     CtlDev &C = B.ctl[r];
     if (C.error) return;
@@ -615,8 +619,9 @@ __device__ inline int64_t units_of(const RepDev &rp, int kind)
     }
 }
 
-// plan layout (int64): [0] active count, [1 + kind] total units, then act[R] (replica of
-// active index a), amap[R] (active index of replica r, or -1), pre[kind][R + 1]
+// plan layout (int64), for a group of R replicas from r0 on: [0] active count, [1 + kind]
+// total units, then act[R] (replica of active index a), amap[R] (active index of replica
+// r0 + j at j, or -1), pre[kind][R + 1]
 __device__ inline const int64_t *plan_act(const int64_t *plan) { return plan + 1 + U_KINDS; }
 __device__ inline const int64_t *plan_amap(const int64_t *plan, int R) { return plan + 1 + U_KINDS + R; }
 __device__ inline const int64_t *plan_pre(const int64_t *plan, int R, int kind)
@@ -632,7 +637,8 @@ __global__ void __launch_bounds__(FT) k_fz_plan(FB<F> B, int forced, int R)
     int64_t *act = plan + 1 + U_KINDS, *amap = act + R;
     // the common case first: no replica sorts
     bool any = forced != 0;
-    for (int r = threadIdx.x; r < R && !any; r += blockDim.x) any = B.ctl[r].need_sort != 0 && !B.ctl[r].error;
+    for (int r = B.r0 + threadIdx.x; r < B.r0 + R && !any; r += blockDim.x)
+        any = B.ctl[r].need_sort != 0 && !B.ctl[r].error;
     if (!__syncthreads_or(any)) {
         if (threadIdx.x <= U_KINDS) plan[threadIdx.x] = 0;
         return;
@@ -640,9 +646,9 @@ __global__ void __launch_bounds__(FT) k_fz_plan(FB<F> B, int forced, int R)
     int carry = 0;
     int64_t csum[U_KINDS] = {0, 0, 0, 0};
     for (int base = 0; base < R; base += blockDim.x) {
-        const int r = base + threadIdx.x;
+        const int rl = base + threadIdx.x, r = B.r0 + rl;
         // a replica whose list build failed has stopped (the reference's error stop)
-        const bool on = r < R && !B.ctl[r].error && sort_active(B.ctl[r], forced);
+        const bool on = rl < R && !B.ctl[r].error && sort_active(B.ctl[r], forced);
         // an extra sort is deferred when the previous one was not and the box allows it:
         // only the key pass and the list build run for it
         const bool dfr = on && !forced && B.ctl[r].def == 0 && B.rep[r].defer != 0;
@@ -652,7 +658,7 @@ __global__ void __launch_bounds__(FT) k_fz_plan(FB<F> B, int forced, int R)
         }
         int tot;
         const int off = carry + scan_block_exclusive(on ? 1 : 0, lds, &tot);
-        if (r < R) amap[r] = on ? off : -1;
+        if (rl < R) amap[rl] = on ? off : -1;
         if (on) act[off] = r;
         for (int kind = 0; kind < U_KINDS; ++kind) {
             const int u = on && !(dfr && kind != U_KEY) ? (int)units_of(B.rep[r], kind) : 0;
@@ -1453,7 +1459,7 @@ __device__ void deferred_list(FB<F> &B, const FP &P, int R, int r, int col, int 
     }
     if (col != 0) return;
     // |v|max: the previous bound raised to the list members' speeds (the key pass's units)
-    const int a = (int)plan_amap(B.plan, R)[r];
+    const int a = (int)plan_amap(B.plan, R)[r - B.r0];
     const int64_t *kpre = plan_pre(B.plan, R, U_KEY);
     double all = !(C.vmax2 < INFINITY) ? 0.0 : C.vmax2;
     for (int64_t q = kpre[a] + threadIdx.x; q < kpre[a + 1]; q += blockDim.x) all = fmax(all, B.gpart[q]);
@@ -1476,13 +1482,13 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
 {
     __shared__ int lds[16];
     __shared__ double red[16];
-    const int r = blockIdx.y;
+    const int r = B.r0 + blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (!sort_active(C, forced)) return;
     // replicas left out of this sort by the plan (stopped at an earlier failed list
     // build; with no replica sorting the plan holds only its zero count); both blocks
     // read the same plan, so they agree even while one of them sets the error flag
-    if (B.plan[0] == 0 || plan_amap(B.plan, R)[r] < 0) return;
+    if (B.plan[0] == 0 || plan_amap(B.plan, R)[r - B.r0] < 0) return;
     __shared__ int skey[MAX_STENCIL], sstart[MAX_STENCIL], soff[MAX_STENCIL + 1], sflag[MAX_STENCIL];
     if (!forced && C.kind == 1) {
         __shared__ short dlut[DEFER_BOX];
@@ -1582,7 +1588,7 @@ __global__ void __launch_bounds__(1024) k_fz_commit_list(FB<F> B, FP P, int forc
     // |v|max over this replica's gather units; after an extra sort, the previous bound
     // raised to the list members' speeds (the key pass's units): only list members have
     // changed their velocity since the bound was taken
-    const int a = (int)plan_amap(B.plan, R)[r];
+    const int a = (int)plan_amap(B.plan, R)[r - B.r0];
     const int64_t *gpre = plan_pre(B.plan, R, forced ? U_GATHER : U_KEY);
     double all = forced || !(C.vmax2 < INFINITY) ? 0.0 : C.vmax2;
     for (int64_t q = gpre[a] + threadIdx.x; q < gpre[a + 1]; q += blockDim.x) all = fmax(all, B.gpart[q]);
@@ -1651,7 +1657,7 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
     __shared__ S lsp[CTILE];
     __shared__ double red[16];
     __shared__ int64_t s_range[2];
-    const int r = blockIdx.y;
+    const int r = B.r0 + blockIdx.y;
     const RepDev &rp = B.rep[r];
     CtlDev &C = B.ctl[r];
     if (C.error) return;
@@ -1771,7 +1777,7 @@ template <class F>
 __global__ void __launch_bounds__(1024) k_fz_collide_commit(FB<F> B, FP P, int ncollide)
 {
     __shared__ double red[16];
-    const int r = blockIdx.y;
+    const int r = B.r0 + blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (C.error) return;
     double all = 0;
@@ -1787,7 +1793,7 @@ __global__ void __launch_bounds__(1024) k_fz_collide_commit(FB<F> B, FP P, int n
 template <class F>
 __global__ void k_fz_memb_init(FB<F> B, FP P)
 {
-    const int r = blockIdx.y;
+    const int r = B.r0 + blockIdx.y;
     const int e = blockIdx.x * FT + threadIdx.x;
     const int col = e / P.nmax, k = e - col * P.nmax;
     if (col >= 2) return;
@@ -1801,8 +1807,14 @@ __global__ void k_fz_memb_init(FB<F> B, FP P)
 }
 
 // ---------------------------------------------------------------------------------
+constexpr int MAX_GROUPS = 8;
+static int64_t plan_len(int R) { return 1 + U_KINDS + 2 * (int64_t)R + (int64_t)U_KINDS * (R + 1); }
+static int64_t gather_units(const Replica &rp) { return (std::max<int64_t>(rp.n, rp.box.ncells) + GT - 1) / GT; }
+
+// the kernels' view of the buffers, for replica group g (replicas from r0 on; its plan,
+// and its partials from gpart_off on)
 template <class F>
-static FB<F> make_fb(DimerSim &s)
+static FB<F> make_fb(DimerSim &s, int r0 = 0, int g = 0, int64_t gpart_off = 0)
 {
     using R = typename F::real;
     using S = typename F::small;
@@ -1849,8 +1861,9 @@ static FB<F> make_fb(DimerSim &s)
     b.part = s.part;
     b.scan_sums = s.scan_sums;
     b.mslots = s.f_mslots;
-    b.plan = s.f_plan;
-    b.gpart = s.f_gpart;
+    b.plan = s.f_plan + g * plan_len(s.R);
+    b.gpart = s.f_gpart + gpart_off;
+    b.r0 = r0;
     b.mcount = s.f_mcount;
     b.nmax = s.nmax;
     b.xcd_order = s.sort_order == 2 ? 1 : 0;
@@ -1901,44 +1914,51 @@ static int stream_blocks(const DimerSim &s)
     return (int)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, cap);
 }
 
+// (each enqueue_* works on the R replicas of b's group)
 template <class F>
-static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int forced)
+static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int R, int forced)
 {
-    const unsigned Rr = (unsigned)s.R;
+    const unsigned Rr = (unsigned)R;
     // the sorts that may follow each MD step get their own grid: when none is due their
     // blocks only read the plan and return, and fewer blocks return sooner
     const int nb = forced ? s.sort_blocks : s.extra_blocks;
-    k_fz_plan<F><<<1, FT, 0, st>>>(b, forced, s.R);
+    k_fz_plan<F><<<1, FT, 0, st>>>(b, forced, R);
     if (forced)
-        k_fz_key_collision<F><<<nb, FT, 0, st>>>(b, p, s.R);
+        k_fz_key_collision<F><<<nb, FT, 0, st>>>(b, p, R);
     else
-        k_fz_key<F><<<nb, FT, 0, st>>>(b, p, s.R);
-    k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, s.R);
-    k_fz_place<F><<<nb, FT, 0, st>>>(b, forced, s.R);
+        k_fz_key<F><<<nb, FT, 0, st>>>(b, p, R);
+    k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, R);
+    k_fz_place<F><<<nb, FT, 0, st>>>(b, forced, R);
     if (forced)
-        k_fz_gather<F><<<nb, FT, 0, st>>>(b, p, s.R);
+        k_fz_gather<F><<<nb, FT, 0, st>>>(b, p, R);
     else
-        k_fz_rank<F><<<nb, FT, 0, st>>>(b, s.R);
-    k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, s.R);
+        k_fz_rank<F><<<nb, FT, 0, st>>>(b, R);
+    k_fz_commit_list<F><<<dim3(2, Rr), 1024, 0, st>>>(b, p, forced, R);
 }
 
+// an MD step in two parts: the step kernel (the streaming), then the colloids and the
+// sort that may follow
 template <class F>
-static void enqueue_step(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int mode)
+static void enqueue_step_kernel(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int R, int mode)
 {
-    const unsigned Rr = (unsigned)s.R;
     const int nbm = member_blocks(s);
     const unsigned grid = (unsigned)(nbm + (mode <= 1 ? stream_blocks(s) : 0));
-    k_fz_step<F><<<dim3(grid, Rr), FT, 0, st>>>(b, p, mode, nbm);
-    k_fz_colloid<F><<<dim3(1, Rr), 1024, 0, st>>>(b, p, mode, nbm, mode <= 1 ? stream_blocks(s) : 0);
-    if (mode <= 1) enqueue_sort<F>(s, b, p, st, 0);
+    k_fz_step<F><<<dim3(grid, (unsigned)R), FT, 0, st>>>(b, p, mode, nbm);
+}
+template <class F>
+static void enqueue_step_rest(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int R, int mode)
+{
+    k_fz_colloid<F><<<dim3(1, (unsigned)R), 1024, 0, st>>>(b, p, mode, member_blocks(s),
+                                                            mode <= 1 ? stream_blocks(s) : 0);
+    if (mode <= 1) enqueue_sort<F>(s, b, p, st, R, 0);
 }
 
 template <class F>
-static void enqueue_collision(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st)
+static void enqueue_collision(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int R)
 {
-    const unsigned Rr = (unsigned)s.R;
+    const unsigned Rr = (unsigned)R;
     const unsigned gc = (unsigned)((s.ncmax_rep + CT - 1) / CT);
-    enqueue_sort<F>(s, b, p, st, 1);
+    enqueue_sort<F>(s, b, p, st, R, 1);
     k_fz_collide<F><<<dim3(gc, Rr), CT, 0, st>>>(b, p);
     k_fz_collide_commit<F><<<dim3(1, Rr), 1024, 0, st>>>(b, p, (int)gc);
 }
@@ -2017,12 +2037,42 @@ static int add_stream_timers(DimerSim &s, hipGraph_t g)
     return RMPCD_OK;
 }
 
+// the first replica of each of G groups (1 <= G <= R) of about equal particle count, and R
+// at the end: G + 1 increasing entries
+static std::vector<int> group_ranges(const DimerSim &s, int G)
+{
+    int64_t tot = 0, acc = 0;
+    for (const Replica &rp : s.rep) tot += rp.n;
+    std::vector<int> r0(1, 0);
+    for (int r = 0; r < s.R; ++r) {
+        acc += s.rep[r].n;
+        const int g = (int)r0.size();  // group g - 1 holds replicas r0[g - 1] .. r
+        const int left = s.R - (r + 1);  // replicas for the G - g groups after it
+        // close the group once it holds its share, or when each later group needs one of
+        // the replicas left; never while a later group would get none
+        if (g < G && left >= G - g && (acc * G >= tot * g || left == G - g)) r0.push_back(r + 1);
+    }
+    r0.push_back(s.R);
+    return r0;
+}
+
+// One collision cycle per replica group, each group's kernels captured from its own
+// stream (forked from and joined to fstream).  Group q starts after group q - 1's first
+// step kernel and then runs free, so that the groups stay out of step: the latency-bound
+// work after one group's step (its colloids, the sort that may follow) runs while another
+// group streams.  (A dependency between streams costs about 10 us, too much to make the
+// groups take turns at every step.)  A profiled graph keeps one group, so that every
+// timed streaming launch has the device to itself.
 template <class F>
 static int build_graph(DimerSim &s)
 {
-    FB<F> b = make_fb<F>(s);
     FP p = make_fp(s);
     hipStream_t st = s.fstream;
+    const int G = s.prof_stride > 0 ? 1 : s.groups;
+    const std::vector<int> r0 = group_ranges(s, G);
+    bool ok = (int)r0.size() == G + 1;
+    for (int q = 0; ok && q < G; ++q) ok = r0[q] < r0[q + 1];
+    RMPCD_CHECK(ok, "dimer: internal error in the replica groups");
     if (s.graph_exec) {
         RMPCD_HIP(hipGraphExecDestroy(s.graph_exec));
         s.graph_exec = nullptr;
@@ -2033,18 +2083,51 @@ static int build_graph(DimerSim &s)
     }
     RMPCD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = RMPCD_OK;
+    hipError_t e = hipSuccess;
+    if (G > 1) {
+        e = hipEventRecord(s.gevents[0], st);
+        for (int q = 1; q < G && e == hipSuccess; ++q) e = hipStreamWaitEvent(s.gstreams[q - 1], s.gevents[0], 0);
+    }
+    std::vector<FB<F>> bq;
+    std::vector<hipStream_t> sq(G);
+    int64_t goff = 0;
+    for (int q = 0; q < G; ++q) {
+        sq[q] = q == 0 ? st : s.gstreams[q - 1];
+        bq.push_back(make_fb<F>(s, r0[q], q, goff));
+        for (int r = r0[q]; r < r0[q + 1]; ++r) goff += gather_units(s.rep[r]);
+    }
     // the first step streams with the last force; each later step first finishes its
-    // predecessor; the last step is finished right before the collision
-    for (int j = 0; j < s.cfg.n_md; ++j) enqueue_step<F>(s, b, p, st, j == 0 ? 0 : 1);
-    enqueue_step<F>(s, b, p, st, 2);
-    enqueue_collision<F>(s, b, p, st);
+    // predecessor; the last step (mode 2) is finished right before the collision
+    for (int j = 0; j <= s.cfg.n_md && e == hipSuccess; ++j) {
+        const int mode = j == 0 ? 0 : j < s.cfg.n_md ? 1 : 2;
+        for (int q = 0; q < G && e == hipSuccess; ++q) {
+            const int Rq = r0[q + 1] - r0[q];
+            // group q starts after group q - 1's first step kernel
+            if (G > 1 && j == 0 && q > 0) e = hipStreamWaitEvent(sq[q], s.gevents[G + q - 1], 0);
+            if (e != hipSuccess) break;
+            enqueue_step_kernel<F>(s, bq[q], p, sq[q], Rq, mode);
+            if (G > 1 && j == 0) e = hipEventRecord(s.gevents[G + q], sq[q]);
+            enqueue_step_rest<F>(s, bq[q], p, sq[q], Rq, mode);
+        }
+    }
+    for (int q = 0; q < G && e == hipSuccess; ++q) {
+        enqueue_collision<F>(s, bq[q], p, sq[q], r0[q + 1] - r0[q]);
+        if (q > 0) {
+            e = hipEventRecord(s.gevents[q], sq[q]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, s.gevents[q], 0);
+        }
+    }
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(st, &g);
+    const hipError_t e_end = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess) e = e_end;
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
     }
-    if (e != hipSuccess) return set_error(RMPCD_ERR_HIP, "dimer: graph capture failed: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return set_error(RMPCD_ERR_HIP, "dimer: graph capture failed: %s", hipGetErrorString(e));
+    }
     s.graph = g;
     s.prof_pairs = 0;
     if (s.prof_stride > 0 && (rc = add_stream_timers<F>(s, g))) return rc;
@@ -2093,6 +2176,7 @@ int fused_prepare(DimerSim &s)
     s.sort_blocks = s.cfg.sort_blocks > 0 ? s.cfg.sort_blocks : SORT_BLOCKS;
     s.extra_blocks = s.cfg.extra_sort_blocks > 0 ? s.cfg.extra_sort_blocks : s.sort_blocks;
     RMPCD_CHECK(s.cfg.sort_order >= 0 && s.cfg.sort_order <= 2, "dimer: sort_order must be 0, 1 or 2");
+    RMPCD_CHECK(s.cfg.groups >= 0 && s.cfg.groups <= MAX_GROUPS, "dimer: groups must be 0 to 8");
     s.sort_order = s.cfg.sort_order > 0 ? s.cfg.sort_order : 2;
     RMPCD_CHECK(s.stream_blocks <= STREAM_BLOCKS && s.sort_blocks <= 65536 && s.extra_blocks <= 65536,
                 "dimer: stream_blocks must be at most 16384 and sort_blocks, extra_sort_blocks at most 65536");
@@ -2128,15 +2212,29 @@ int fused_prepare(DimerSim &s)
     if ((rc = s.alloc((void **)&s.scan_sums, (size_t)s.R * sstride * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_slocal, s.nctot * 4))) return rc;
     if ((rc = s.alloc((void **)&s.f_mslots, (size_t)s.R * 4 * s.nmax * 4))) return rc;
-    if ((rc = s.alloc((void **)&s.f_plan, (1 + 4 + 2 * (size_t)s.R + 4 * ((size_t)s.R + 1)) * 8))) return rc;
+    if ((rc = s.alloc((void **)&s.f_plan, MAX_GROUPS * plan_len(s.R) * 8))) return rc;
     int64_t gunits = 0;
-    for (const Replica &rp : s.rep) gunits += (std::max<int64_t>(rp.n, rp.box.ncells) + GT - 1) / GT;
+    for (const Replica &rp : s.rep) gunits += gather_units(rp);
     if ((rc = s.alloc((void **)&s.f_gpart, (size_t)gunits * 8))) return rc;
     if ((rc = s.alloc((void **)&s.f_mcount, (size_t)s.R * 4 * 4))) return rc;
     if ((rc = s.alloc((void **)&s.part, (size_t)s.R * PS * 8))) return rc;
     if ((rc = s.alloc((void **)&s.rep_dev, sizeof(RepDev) * s.R))) return rc;
     if ((rc = s.alloc((void **)&s.ctl_dev, sizeof(CtlDev) * s.R))) return rc;
     RMPCD_HIP(hipStreamCreate(&s.fstream));
+    // replica groups: streams to capture the groups after the first from, events to fork
+    // and join them
+    // (events 0 .. groups - 1: the fork, then the joins; groups .. 2 groups - 1: each
+    // group's turn at the step kernel)
+    s.groups = std::min(s.cfg.groups > 0 ? s.cfg.groups : 2, s.R);
+    for (int q = 0; q < 2 * s.groups; ++q) {
+        hipEvent_t ev;
+        RMPCD_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        s.gevents.push_back(ev);
+        if (q == 0 || q >= s.groups) continue;
+        hipStream_t sq;
+        RMPCD_HIP(hipStreamCreate(&sq));
+        s.gstreams.push_back(sq);
+    }
     std::vector<RepDev> rd(s.R);
     std::vector<CtlDev> cd(s.R);
     for (int r = 0; r < s.R; ++r) {

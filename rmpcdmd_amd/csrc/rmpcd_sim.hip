@@ -58,6 +58,8 @@ DimerSim::~DimerSim()
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (fstream) (void)hipStreamDestroy(fstream);
+    for (hipStream_t st : gstreams) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : gevents) (void)hipEventDestroy(e);
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     for (void *p : allocs) (void)hipFree(p);
 }

@@ -89,6 +89,11 @@ struct DimerSim {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     hipStream_t fstream = nullptr;
+    // replica groups, each a branch of the captured cycle; groups after the first are
+    // captured from their own streams, forked from and joined to fstream
+    int groups = 1;
+    std::vector<hipStream_t> gstreams;
+    std::vector<hipEvent_t> gevents;
     bool fused_ready = false;
     int prof_stride = 0, prof_pairs = 0;  // event nodes around the streaming kernel
     std::vector<hipEvent_t> prof_ev;

@@ -215,15 +215,25 @@ def test_batched_replicas_of_mixed_sizes(gpu):
     ref = SingleDimer(mode=capi.SIM_REFERENCE, replica_L=Ls, **kw)
     fus = SingleDimer(mode=capi.SIM_FUSED, replica_L=Ls, **kw)
     one = SingleDimer(mode=capi.SIM_FUSED, L=Ls[0], **kw)
+    # the batch as one graph branch, and as three branches of unequal replica counts (the
+    # groups split the particle count evenly): the same trajectories bit for bit
+    g1 = SingleDimer(mode=capi.SIM_FUSED, replica_L=Ls, groups=1, **kw)
+    g3 = SingleDimer(mode=capi.SIM_FUSED, replica_L=Ls, groups=3, **kw)
     ref.run(1)
     fus.run(1)
     one.run(1)
+    g1.run(1)
+    g3.run(1)
     for r in range(len(Ls)):
         compare(ref.download(r), fus.download(r), "replica %d" % r)
+        compare(fus.download(r), g1.download(r), "one group, replica %d" % r, rtol=0.0)
+        compare(fus.download(r), g3.download(r), "three groups, replica %d" % r, rtol=0.0)
     compare(fus.download(0), one.download(0), "replica 0 alone")
     ref.close()
     fus.close()
     one.close()
+    g1.close()
+    g3.close()
 
 
 @pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
@@ -260,8 +270,9 @@ def test_many_extra_sorts_per_cycle(gpu, form):
     # odd grids for the sort passes (units are split over whatever grid there is): the
     # same trajectory bit for bit
     odd = SingleDimer(mode=capi.SIM_FUSED, sort_blocks=96, extra_sort_blocks=7, **kw)
-    # a contiguous unit range per workgroup instead of XCD-grouped units: the same again
-    flat = SingleDimer(mode=capi.SIM_FUSED, sort_order=1, sort_blocks=64, extra_sort_blocks=16, **kw)
+    # a contiguous unit range per workgroup instead of XCD-grouped units, in one graph
+    # branch instead of one per replica: the same again
+    flat = SingleDimer(mode=capi.SIM_FUSED, sort_order=1, sort_blocks=64, extra_sort_blocks=16, groups=1, **kw)
     for cycle in range(2):
         ref.run(1)
         fus.run(1)
