@@ -395,6 +395,9 @@ typedef struct rmpcd_dimer_config {
                                   one group's latency-bound work (colloids, sorts after MD
                                   steps) runs beside another's streaming.  0: default (2
                                   with two or more replicas), at most 8 and n_replicas */
+    int32_t collide_cells;     /* fused mode launch shape: cells (one thread each) per
+                                  workgroup of the collision kernel, 64, 128 or 256; 0:
+                                  default (128) */
 } rmpcd_dimer_config_t;
 typedef struct rmpcd_dimer_stats {
     int64_t n_solvent, md_steps, cycles, n_extra_sorting;  /* of the replica asked for */

@@ -6,6 +6,10 @@ at a time (coordinate descent, two passes):
     stream_blocks  stream-role workgroups per replica of the step kernel
     sort_blocks    workgroups of the key, place and gather passes of the collision sort
     extra_sort_blocks  the same for the sorts that may follow each MD step
+    sort_order     the sort passes' units: a contiguous range per workgroup (1) or
+                   grouped by XCD (2)
+    groups         replica groups run as concurrent branches of the captured cycle
+    collide_cells  cells (threads) per workgroup of the collision kernel (its LDS tile)
 The result is stored in a JSON table keyed by the workload; bench.py applies the entry
 of its workload when the table exists.
 
@@ -22,7 +26,8 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 CANDIDATES = {"stream_blocks": [2048, 4096, 8192, 16384], "sort_blocks": [2048, 4096, 8192, 16384, 32768],
-              "extra_sort_blocks": [256, 512, 1024, 2048, 4096]}
+              "extra_sort_blocks": [256, 512, 1024, 2048, 4096], "sort_order": [1, 2], "groups": [1, 2, 3, 4],
+              "collide_cells": [64, 128, 256]}
 
 
 def workload_key(form, L, replicas, mixed=False):
@@ -42,7 +47,7 @@ def lookup(form, L, replicas, mixed=False, path=TABLE):
     return {k: int(e[k]) for k in CANDIDATES if e and k in e}
 
 
-def time_shape(make, shape, cycles=3, warmup=1):
+def time_shape(make, shape, cycles=4, warmup=2):
     import torch
     sim = make(**shape)
     sim.run(warmup)

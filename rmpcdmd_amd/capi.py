@@ -65,7 +65,7 @@ class DimerConfig(C.Structure):
                 ("nmax_list", C.c_int32), ("n_replicas", C.c_int32), ("seed", C.c_uint64),
                 ("replica_L", C.c_void_p), ("stream_blocks", C.c_int32), ("sort_blocks", C.c_int32),
                 ("lazy", C.c_int32), ("extra_sort_blocks", C.c_int32), ("sort_order", C.c_int32),
-                ("groups", C.c_int32)]
+                ("groups", C.c_int32), ("collide_cells", C.c_int32)]
 
 
 class DimerStats(C.Structure):

@@ -42,8 +42,8 @@ constexpr int STREAM_BLOCKS = 16384;  // stream-role blocks per replica, default
 constexpr int LAZY_STREAM_BLOCKS = 1024;  // stream-role blocks per replica in lazy mode
 constexpr int SORT_BLOCKS = 4096;     // blocks of the particle passes of a sort (all replicas), default
 constexpr int SCAN_BLOCKS = 1024;     // blocks of the cell-count tile scan (all replicas)
-constexpr int CT = 128;            // cells per block in the collision kernel
-constexpr int CTILE = 1536;  // staged particles per collision block (12 per cell; a block with more works from memory)
+constexpr int CT_DEFAULT = 128;  // cells per block in the collision kernel (64, 128 or 256; 12 particles per cell
+                                 // staged in LDS, a block with more works from memory)
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 constexpr int MAX_TILES = 4096;  // cells per replica up to MAX_TILES * SCAN_TILE
 constexpr int MAX_STENCIL = 2048;  // stencil cells of the fused list build
@@ -1648,11 +1648,13 @@ __device__ inline void fz_srd_cell(R *vx, R *vy, R *vz, const S *sp, int m, uint
     srd_rotate(vx, vy, vz, m, o, u0, u1, u2, acc, n_eff == m);
 }
 
-template <class F>
+// CT cells per workgroup, one thread per cell, their particles staged in LDS (CTILE = 12 CT)
+template <class F, int CT>
 __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
 {
     using R = typename F::real;
     using S = typename F::small;
+    constexpr int CTILE = 12 * CT;
     __shared__ R lx[CTILE], ly[CTILE], lz[CTILE];
     __shared__ S lsp[CTILE];
     __shared__ double red[16];
@@ -1957,9 +1959,15 @@ template <class F>
 static void enqueue_collision(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int R)
 {
     const unsigned Rr = (unsigned)R;
-    const unsigned gc = (unsigned)((s.ncmax_rep + CT - 1) / CT);
+    const int ct = s.collide_cells;
+    const unsigned gc = (unsigned)((s.ncmax_rep + ct - 1) / ct);
     enqueue_sort<F>(s, b, p, st, R, 1);
-    k_fz_collide<F><<<dim3(gc, Rr), CT, 0, st>>>(b, p);
+    if (ct == 64)
+        k_fz_collide<F, 64><<<dim3(gc, Rr), 64, 0, st>>>(b, p);
+    else if (ct == 256)
+        k_fz_collide<F, 256><<<dim3(gc, Rr), 256, 0, st>>>(b, p);
+    else
+        k_fz_collide<F, 128><<<dim3(gc, Rr), 128, 0, st>>>(b, p);
     k_fz_collide_commit<F><<<dim3(1, Rr), 1024, 0, st>>>(b, p, (int)gc);
 }
 
@@ -2180,7 +2188,11 @@ int fused_prepare(DimerSim &s)
     s.sort_order = s.cfg.sort_order > 0 ? s.cfg.sort_order : 2;
     RMPCD_CHECK(s.stream_blocks <= STREAM_BLOCKS && s.sort_blocks <= 65536 && s.extra_blocks <= 65536,
                 "dimer: stream_blocks must be at most 16384 and sort_blocks, extra_sort_blocks at most 65536");
-    RMPCD_CHECK((s.ncmax_rep + CT - 1) / CT <= PS && (int64_t)member_blocks(s) * MPV + STREAM_BLOCKS <= PS &&
+    RMPCD_CHECK(s.cfg.collide_cells == 0 || s.cfg.collide_cells == 64 || s.cfg.collide_cells == 128 ||
+                    s.cfg.collide_cells == 256,
+                "dimer: collide_cells must be 0, 64, 128 or 256");
+    s.collide_cells = s.cfg.collide_cells > 0 ? s.cfg.collide_cells : CT_DEFAULT;
+    RMPCD_CHECK((s.ncmax_rep + s.collide_cells - 1) / s.collide_cells <= PS && (int64_t)member_blocks(s) * MPV + STREAM_BLOCKS <= PS &&
                     member_blocks(s) <= MAX_MEMBER_BLOCKS && s.n_stencil <= MAX_STENCIL &&
                     (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE <= MAX_TILES,
                 "dimer: box or list too large for the fused kernels");

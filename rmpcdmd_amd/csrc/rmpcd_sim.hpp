@@ -53,7 +53,7 @@ struct DimerSim {
     double mass[2] = {0, 0};
     double dt = 0, skin = 2, max_cut = 0, radius = 0;
     int nmax = 0, n_stencil = 0;
-    int stream_blocks = 0, sort_blocks = 0, extra_blocks = 0, sort_order = 2;  // fused-mode launch shape
+    int stream_blocks = 0, sort_blocks = 0, extra_blocks = 0, sort_order = 2, collide_cells = 128;  // fused-mode launch shape
     // solvent double buffers (sort) and the four force buffers: for sort buffer b and
     // force parity f, force is sol_force[2*b + f] and force_old sol_force[2*b + 1 - f]
     void *sol_pos[2] = {}, *sol_vel[2] = {}, *sol_species[2] = {}, *sol_flags[2] = {};

@@ -40,7 +40,7 @@ module rmpcd_capi
      integer(c_int64_t) :: seed = 0
      type(c_ptr) :: replica_L = c_null_ptr
      integer(c_int32_t) :: stream_blocks = 0, sort_blocks = 0, lazy = 0, extra_sort_blocks = 0
-     integer(c_int32_t) :: sort_order = 0, groups = 0
+     integer(c_int32_t) :: sort_order = 0, groups = 0, collide_cells = 0
   end type rmpcd_dimer_config_t
 
   type, bind(c) :: rmpcd_dimer_stats_t

@@ -13,7 +13,7 @@ DIMER_DEFAULTS = dict(L=(48, 48, 48), rho=10, n_md=50, T=1.0, tau=0.5, sigma_C=2
                       epsilon_NC=1.0, epsilon_NN=1.0, prob=1.0, bulk_rmpcd=0, bulk_rate=0.01,
                       skin=2.0, nmax_list=0, seed=20261202, form=FP32, mode=SIM_FUSED, n_replicas=1,
                       replica_L=None, stream_blocks=0, sort_blocks=0, lazy=0, extra_sort_blocks=0,
-                      sort_order=0, groups=0)
+                      sort_order=0, groups=0, collide_cells=0)
 
 
 def mixed_L_sizes(n=128, seed=20261202):
@@ -32,7 +32,7 @@ def make_config(**kw):
     c = capi.DimerConfig()
     c.L = (C.c_int32 * 3)(*p["L"])
     for k in ("rho", "n_md", "form", "mode", "bulk_rmpcd", "nmax_list", "n_replicas", "stream_blocks",
-              "sort_blocks", "lazy", "extra_sort_blocks", "sort_order", "groups"):
+              "sort_blocks", "lazy", "extra_sort_blocks", "sort_order", "groups", "collide_cells"):
         setattr(c, k, int(p[k]))
     for k in ("T", "tau", "sigma_C", "sigma_N", "d", "epsilon_CC", "epsilon_NC", "epsilon_NN", "prob",
               "bulk_rate", "skin"):

@@ -97,27 +97,35 @@ def test_fused_dense_cells(gpu, form):
     """rho = 40: cells hold more particles than the gather and rank passes stage on either
     side of their tiles, so the ranks of those cells read the rest from memory, and the
     collision blocks overflow their LDS tiles; the fused program must still equal the
-    reference call sequence, and its odd-grid run and its run with a contiguous unit range
-    per workgroup (instead of XCD-grouped units) the tuned one bit for bit"""
+    reference call sequence, and its odd-grid run, its run with a contiguous unit range per
+    workgroup (instead of XCD-grouped units) and its runs with 64 and 256 cells per
+    collision workgroup the tuned one bit for bit"""
     kw = dict(L=(12, 12, 12), rho=40, n_md=10, tau=0.1, form=form, seed=123)
     ref = SingleDimer(mode=capi.SIM_REFERENCE, **kw)
     fus = SingleDimer(mode=capi.SIM_FUSED, **kw)
     odd = SingleDimer(mode=capi.SIM_FUSED, sort_blocks=96, extra_sort_blocks=7, **kw)
     flat = SingleDimer(mode=capi.SIM_FUSED, sort_order=1, **kw)
+    tiles = [SingleDimer(mode=capi.SIM_FUSED, collide_cells=ct, **kw) for ct in (64, 256)]
     for cycle in range(2):
         ref.run(1)
         fus.run(1)
         odd.run(1)
         flat.run(1)
+        for t in tiles:
+            t.run(1)
         a, b, d, e = ref.download(), fus.download(), odd.download(), flat.download()
         assert (a["cell_count"] > 32).sum() > 100  # cells larger than the 32-slot halo
         compare(a, b, "after cycle %d" % cycle, 1e-12 if form == capi.FP64 else 1e-5)
         compare(b, d, "odd grids after cycle %d" % cycle, rtol=0.0)
         compare(b, e, "contiguous unit ranges after cycle %d" % cycle, rtol=0.0)
+        for t, ct in zip(tiles, (64, 256)):
+            compare(b, t.download(), "%d cells per collision workgroup after cycle %d" % (ct, cycle), rtol=0.0)
     ref.close()
     fus.close()
     odd.close()
     flat.close()
+    for t in tiles:
+        t.close()
 
 
 def test_fused_replicas_of_uneven_size(gpu):
