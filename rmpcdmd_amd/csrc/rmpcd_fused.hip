@@ -736,7 +736,11 @@ __device__ inline void key_pass(const FB<F> &B, const FP &P, int R)
 {
     constexpr int forced = kForced;
     using Rl = typename F::real;
-    __shared__ int hkey[KHASH], hcnt[KHASH], occ[KHASH];
+    // the LDS hash (hkey, hcnt, occ); a collision sort stages each slab of FT records in
+    // the same memory before the hash is built
+    __shared__ __align__(16) int hbuf[3 * KHASH];
+    int *hkey = hbuf, *hcnt = hbuf + KHASH, *occ = hbuf + 2 * KHASH;
+    static_assert(sizeof(typename FB<F>::Rec) * FT <= sizeof(int) * 3 * KHASH, "record slab in the hash LDS");
     __shared__ int lds[FT / 64];
     __shared__ double red[16];
     const UnitRange w = unit_range(B.plan, R, U_KEY, B.xcd_order);
@@ -766,6 +770,7 @@ __device__ inline void key_pass(const FB<F> &B, const FP &P, int R)
         const typename F::small *spc = B.species[C.cur] + rp.p_off, *flg = B.flags[C.cur] + rp.p_off;
         uint8_t *memb = B.memb + rp.p_off;
         typename FB<F>::Rec *rec = B.rec + rp.p_off;
+        typename FB<F>::Rec *stage = reinterpret_cast<typename FB<F>::Rec *>(hbuf);
         int32_t *cnt = B.count[C.cnt ^ 1] + rp.c_off;
         const int lazy_k = P.lazy ? C.k_since : 0;
         // before the first sort the bound is not set: take every particle's speed
@@ -786,105 +791,119 @@ __device__ inline void key_pass(const FB<F> &B, const FP &P, int R)
             keys[v] = -1;
             ent[v] = 0;
             loc[v] = 0;
-            if (i >= n) continue;
-            Rl x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
-            int jmp[3];
-            if (forced) {
-                const Rl vl[3] = {vel[i], vel[n + i], vel[2 * n + i]};
-                if (lazy_k > 0 && !memb[i]) {
-                    // replay the k_since additions x = x + dt v of the lazily moved particle
-                    for (int k = 0; k < 3; ++k) {
-                        const Rl t = rdt * vl[k];
-                        for (int s2 = 0; s2 < lazy_k; ++s2) x[k] = x[k] + t;
+            if (i < n) {
+                Rl x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+                int jmp[3];
+                if (forced) {
+                    const Rl vl[3] = {vel[i], vel[n + i], vel[2 * n + i]};
+                    if (lazy_k > 0 && !memb[i]) {
+                        // replay the k_since additions x = x + dt v of the lazily moved particle
+                        for (int k = 0; k < 3; ++k) {
+                            const Rl t = rdt * vl[k];
+                            for (int s2 = 0; s2 < lazy_k; ++s2) x[k] = x[k] + t;
+                        }
                     }
-                }
-                for (int k = 0; k < 3; ++k) {
-                    Rl e = (Rl)b.edges[k];
-                    int jump = (int)floor((double)(x[k] / e));
-                    x[k] = x[k] - (Rl)jump * e;
-                    jmp[k] = jump;
-                }
-                // the staging record: wrapped position (apply_pbc, md.f90:202-217), image,
-                // and the rest of the particle's state
-                typename FB<F>::Rec q;
-                for (int k = 0; k < 3; ++k) {
-                    q.p[k] = x[k];
-                    q.v[k] = vl[k];
-                    q.im[k] = img[k * n + i] + jmp[k];
-                }
-                q.id = ids[i];
-                q.set_sf((int)spc[i], (int)flg[i]);
-                int key;
-                {
-                    int p0, p1, p2;
+                    for (int k = 0; k < 3; ++k) {
+                        Rl e = (Rl)b.edges[k];
+                        int jump = (int)floor((double)(x[k] / e));
+                        x[k] = x[k] - (Rl)jump * e;
+                        jmp[k] = jump;
+                    }
+                    // the staging record: wrapped position (apply_pbc, md.f90:202-217), image,
+                    // and the rest of the particle's state
+                    typename FB<F>::Rec q;
+                    for (int k = 0; k < 3; ++k) {
+                        q.p[k] = x[k];
+                        q.v[k] = vl[k];
+                        q.im[k] = img[k * n + i] + jmp[k];
+                    }
+                    q.id = ids[i];
+                    q.set_sf((int)spc[i], (int)flg[i]);
+                    int key;
+                    {
+                        int p0, p1, p2;
+                        cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
+                        if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
+                            (unsigned)p2 < (unsigned)b.L[2])
+                            key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+                        else
+                            key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+                    }
+                    q.key = key;
+                    keys[v] = key;
+                    stage[threadIdx.x] = q;  // (written out below)
+                } else {
+                    // extra sort: the particle stays in place; apply_pbc and pos_old = pos
+                    // (single_dimer_pbc.f90:325-335) are done here
+                    const int mb = memb[i];
+                    bool write = false;
+                    if (lazy_k > 0 && !mb) {
+                        for (int k = 0; k < 3; ++k) {
+                            const Rl t = rdt * vel[k * n + i];
+                            for (int s2 = 0; s2 < lazy_k; ++s2) x[k] = x[k] + t;
+                        }
+                        write = true;
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        Rl e = (Rl)b.edges[k];
+                        int jump = (int)floor((double)(x[k] / e));
+                        x[k] = x[k] - (Rl)jump * e;
+                        if (jump != 0) {
+                            img[k * n + i] += jump;
+                            write = true;
+                        }
+                    }
+                    for (int k = 0; k < 3; ++k) {
+                        if (write) pos[k * n + i] = x[k];
+                        pold[k * n + i] = x[k];
+                    }
+                    if (mb || all_v) {
+                        double v2 = 0;
+                        for (int k = 0; k < 3; ++k) {
+                            const double vv = (double)vel[k * n + i];
+                            v2 += vv * vv;
+                        }
+                        vm = v2 > vm ? v2 : vm;
+                        if (mb) memb[i] = 0;
+                    }
+                    int key, p0, p1, p2;
                     cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
                     if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
                         (unsigned)p2 < (unsigned)b.L[2])
                         key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
                     else
                         key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
-                }
-                q.key = key;
-                keys[v] = key;
-                rec[i] = q;
-            } else {
-                // extra sort: the particle stays in place; apply_pbc and pos_old = pos
-                // (single_dimer_pbc.f90:325-335) are done here
-                const int mb = memb[i];
-                bool write = false;
-                if (lazy_k > 0 && !mb) {
-                    for (int k = 0; k < 3; ++k) {
-                        const Rl t = rdt * vel[k * n + i];
-                        for (int s2 = 0; s2 < lazy_k; ++s2) x[k] = x[k] + t;
-                    }
-                    write = true;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    Rl e = (Rl)b.edges[k];
-                    int jump = (int)floor((double)(x[k] / e));
-                    x[k] = x[k] - (Rl)jump * e;
-                    if (jump != 0) {
-                        img[k * n + i] += jump;
-                        write = true;
+                    keys[v] = key;
+                    if (dfr) {
+                        // a deferred sort keeps the cell; the particle is a candidate of the
+                        // list build when it lies within the list radius of either colloid
+                        // (with a margin: the colloids may be read before or after their wrap)
+                        for (int cc = 0; cc < 2; ++cc) {
+                            const double d0 = min_image(ccol[cc][0] - (double)x[0], b.edges[0]);
+                            const double d1 = min_image(ccol[cc][1] - (double)x[1], b.edges[1]);
+                            const double d2 = min_image(ccol[cc][2] - (double)x[2], b.edges[2]);
+                            candf[v] = candf[v] || (d0 * d0 + d1 * d1) + d2 * d2 < cand_r2;
+                        }
                     }
                 }
-                for (int k = 0; k < 3; ++k) {
-                    if (write) pos[k * n + i] = x[k];
-                    pold[k * n + i] = x[k];
-                }
-                if (mb || all_v) {
-                    double v2 = 0;
-                    for (int k = 0; k < 3; ++k) {
-                        const double vv = (double)vel[k * n + i];
-                        v2 += vv * vv;
-                    }
-                    vm = v2 > vm ? v2 : vm;
-                    if (mb) memb[i] = 0;
-                }
-                int key, p0, p1, p2;
-                cell_of<Rl>(b, x[0], x[1], x[2], p0, p1, p2);
-                if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] &&
-                    (unsigned)p2 < (unsigned)b.L[2])
-                    key = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+                if (dfr)
+                    B.dkey[rp.p_off + i] = keys[v];
                 else
-                    key = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
-                keys[v] = key;
-                if (dfr) {
-                    // a deferred sort keeps the cell; the particle is a candidate of the
-                    // list build when it lies within the list radius of either colloid
-                    // (with a margin: the colloids may be read before or after their wrap)
-                    for (int cc = 0; cc < 2; ++cc) {
-                        const double d0 = min_image(ccol[cc][0] - (double)x[0], b.edges[0]);
-                        const double d1 = min_image(ccol[cc][1] - (double)x[1], b.edges[1]);
-                        const double d2 = min_image(ccol[cc][2] - (double)x[2], b.edges[2]);
-                        candf[v] = candf[v] || (d0 * d0 + d1 * d1) + d2 * d2 < cand_r2;
-                    }
-                }
+                    B.key[rp.p_off + i] = keys[v];
             }
-            if (dfr)
-                B.dkey[rp.p_off + i] = keys[v];
-            else
-                B.key[rp.p_off + i] = keys[v];
+            if (forced) {
+                // the records leave through LDS: each store writes one contiguous 16-byte
+                // piece per thread, where each thread's own record would take pieces a
+                // record apart
+                __syncthreads();
+                constexpr int RW = (int)(sizeof(typename FB<F>::Rec) / 16);
+                const int64_t s0 = c0 + v * FT, nv = n - s0 < FT ? n - s0 : FT;
+                uint4 *dst = reinterpret_cast<uint4 *>(rec + s0);
+                const uint4 *src = reinterpret_cast<const uint4 *>(hbuf);
+                for (int e = threadIdx.x; e < RW * FT; e += FT)
+                    if (e < RW * nv) dst[e] = src[e];
+                __syncthreads();
+            }
         }
         if (!forced) {
             const double m = block_max(vm, red);
