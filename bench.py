@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--colloids-forms", default="fp32,fp64", help="forms of the n_colloids_pbc workload")
     ap.add_argument("--mixed-only", action="store_true",
                     help="run only the 128-replica mixed-L batch (for profiling it)")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="replica groups of the fused graph instead of the tuned number (1 for traffic "
+                         "counters of whole-batch launches)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="first copy MB megabytes device to device once (the reference dispatch for "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -72,6 +75,9 @@ def mixed_sizes():
     """128 replicas with L drawn uniformly from {24, 32, 40, 48} (seeded)"""
     from rmpcdmd_amd.sim import mixed_L_sizes
     return mixed_L_sizes()
+
+
+GROUPS = 0  # --groups: replica groups instead of the tuned number (0: tuned)
 
 
 def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile, sizes=None, lazy=0):
@@ -89,6 +95,8 @@ def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile, sizes=None, l
         sim = SingleDimer(n_md=n_md, form=f, mode=m, replica_L=[(q, q, q) for q in sizes], **shape)
     else:
         shape = lookup(form, L, replicas) if mode == "fused" else {}
+        if GROUPS and mode == "fused":
+            shape = dict(shape, groups=GROUPS)
         sim = SingleDimer(L=(L, L, L), n_md=n_md, form=f, mode=m, n_replicas=replicas, lazy=lazy, **shape)
     st0 = sim.state()[4]
     n_total = int(st0.n_solvent_total)
@@ -500,7 +508,9 @@ def mpcd_variants(with_cpu, L=64, reps=20):
 
 
 def main():
+    global GROUPS
     a = parse()
+    GROUPS = a.groups
     if a.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, so --gpus %d "
               "is not supported (replicas would shard one subset per GPU with no communication)" % a.gpus,

@@ -37,6 +37,7 @@
 namespace rmpcd {
 
 constexpr int FT = 256;            // threads per block
+constexpr int COLLOID_T = 256;     // threads per block of the colloid kernel
 constexpr int PS = 32768;          // partial-sum slots per replica
 constexpr int STREAM_BLOCKS = 16384;  // stream-role blocks per replica, default and maximum
 constexpr int LAZY_STREAM_BLOCKS = 1024;  // stream-role blocks per replica in lazy mode
@@ -430,12 +431,14 @@ __global__ void __launch_bounds__(FT) k_fz_step(FB<F> B, FP P, int mode, int nbm
     }
 }
 
-// The colloids' share of the step, one block per replica after k_fz_step: the member
+// The colloids' share of the step, one block per replica after k_fz_step (COLLOID_T
+// threads: most of the work is one thread's, and a small block starts sooner beside
+// another replica group's streaming): the member
 // partial sums in block order (deterministic), the stream-role maxima, then the
 // colloid force, compute_force_n2, Verlet velocity and RATTLE of the previous step and
 // the Verlet position step, RATTLE and skin decision of this one.
 template <class F>
-__global__ void __launch_bounds__(1024) k_fz_colloid(FB<F> B, FP P, int mode, int nbm, int nstream)
+__global__ void __launch_bounds__(COLLOID_T) k_fz_colloid(FB<F> B, FP P, int mode, int nbm, int nstream)
 {
     __shared__ double red[16];
     __shared__ double acc[MPV];
@@ -1969,7 +1972,7 @@ static void enqueue_step_kernel(DimerSim &s, const FB<F> &b, const FP &p, hipStr
 template <class F>
 static void enqueue_step_rest(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t st, int R, int mode)
 {
-    k_fz_colloid<F><<<dim3(1, (unsigned)R), 1024, 0, st>>>(b, p, mode, member_blocks(s),
+    k_fz_colloid<F><<<dim3(1, (unsigned)R), COLLOID_T, 0, st>>>(b, p, mode, member_blocks(s),
                                                             mode <= 1 ? stream_blocks(s) : 0);
     if (mode <= 1) enqueue_sort<F>(s, b, p, st, R, 0);
 }
