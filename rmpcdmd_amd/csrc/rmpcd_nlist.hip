@@ -123,76 +123,85 @@ __global__ void k_roll_md(int64_t n, typename F::small *flags)
     flags[i] = (typename F::small)(f & ~MD_MASK);
 }
 
-// compute_force part 2 (neighbor_list.f90:191-222): one thread per list entry; solvent
-// forces by atomics, colloid force and energy through per-block partials that the last
-// block of each colloid adds in block order
+// compute_force part 2 (neighbor_list.f90:191-222): one thread per list entry of the
+// colloids c0 .. c0 + gridDim.y - 1.  With one colloid per launch (ordered launches) each
+// solvent particle is in the list at most once, so its force is updated without atomics,
+// in the reference's colloid order, bit for bit; a launch over several colloids adds
+// with atomics (the order of two colloids' contributions to one particle is then not
+// fixed).  The colloid force and energy go to per-block partials, summed in block order by
+// k_force_nl_sum.
 template <class F>
 __global__ void __launch_bounds__(256)
     k_force_nl(PView<F64> c, PView<F> sol, const int32_t *__restrict__ list,
                const int32_t *__restrict__ nl, int32_t nmax, double L0, double L1, double L2, LJ lj,
-               double *__restrict__ part, unsigned *__restrict__ done, double *__restrict__ e_out)
+               double *__restrict__ part, int c0, int atomic_add)
 {
     using R = typename F::real;
     __shared__ double red[16];
-    __shared__ bool s_last;
-    int i = blockIdx.y;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int n_i = min(nl[i], nmax);
+    const int i = c0 + (int)blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n_i = min(nl[i], nmax);
     double f0 = 0, f1 = 0, f2 = 0, e = 0;
-    double x0 = c.pos[i], x1 = c.pos[c.n + i], x2 = c.pos[2 * c.n + i];
-    int s1 = c.species[i];
+    const double x0 = c.pos[i], x1 = c.pos[c.n + i], x2 = c.pos[2 * c.n + i];
+    const int s1 = c.species[i];
     if (k < n_i) {
-        int idx = list[(int64_t)i * nmax + k];
-        int s2 = (int)sol.species[idx];
+        const int idx = list[(int64_t)i * nmax + k];
+        const int s2 = (int)sol.species[idx];
         if (s2 > 0) {
-            double d0 = min_image(x0 - (double)sol.pos[idx], L0);
-            double d1 = min_image(x1 - (double)sol.pos[sol.n + idx], L1);
-            double d2 = min_image(x2 - (double)sol.pos[2 * sol.n + idx], L2);
-            double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
-            int pidx = (s2 - 1) + lj.n1 * (s1 - 1);
+            const double d0 = min_image(x0 - (double)sol.pos[idx], L0);
+            const double d1 = min_image(x1 - (double)sol.pos[sol.n + idx], L1);
+            const double d2 = min_image(x2 - (double)sol.pos[2 * sol.n + idx], L2);
+            const double r_sq = (d0 * d0 + d1 * d1) + d2 * d2;
+            const int pidx = (s2 - 1) + lj.n1 * (s1 - 1);
             if (r_sq <= lj.cut_sq[pidx]) {
-                double fr = lj_force_over_r(r_sq, lj.eps[pidx], lj.sigma[pidx]);
+                const double fr = lj_force_over_r(r_sq, lj.eps[pidx], lj.sigma[pidx]);
                 f0 = fr * d0;
                 f1 = fr * d1;
                 f2 = fr * d2;
                 e = lj_energy(r_sq, lj.eps[pidx], lj.sigma[pidx], lj.cut_energy[pidx]);
-                atomicAdd(&sol.force[idx], (R)(-f0));
-                atomicAdd(&sol.force[sol.n + idx], (R)(-f1));
-                atomicAdd(&sol.force[2 * sol.n + idx], (R)(-f2));
-                int mask = MD_MASK | (s1 == 1 ? CATALYZED_MASK : 0);
+                if (atomic_add) {
+                    atomicAdd(&sol.force[idx], (R)(-f0));
+                    atomicAdd(&sol.force[sol.n + idx], (R)(-f1));
+                    atomicAdd(&sol.force[2 * sol.n + idx], (R)(-f2));
+                } else {
+                    sol.force[idx] = sol.force[idx] - (R)f0;
+                    sol.force[sol.n + idx] = sol.force[sol.n + idx] - (R)f1;
+                    sol.force[2 * sol.n + idx] = sol.force[2 * sol.n + idx] - (R)f2;
+                }
+                const int mask = MD_MASK | (s1 == 1 ? CATALYZED_MASK : 0);
                 atomic_or_flag(&sol.flags[idx], mask);
             }
         }
     }
-    double b0 = block_sum(f0, red), b1 = block_sum(f1, red), b2 = block_sum(f2, red),
-           be = block_sum(e, red);
-    int nb = gridDim.x;
+    const double b0 = block_sum(f0, red), b1 = block_sum(f1, red), b2 = block_sum(f2, red),
+                 be = block_sum(e, red);
     if (threadIdx.x == 0) {
-        double *pp = part + ((int64_t)i * nb + blockIdx.x) * 4;
+        double *pp = part + ((int64_t)i * gridDim.x + blockIdx.x) * 4;
         pp[0] = b0;
         pp[1] = b1;
         pp[2] = b2;
         pp[3] = be;
-        __threadfence();
-        s_last = atomicAdd(&done[i], 1u) == (unsigned)nb - 1;
     }
-    __syncthreads();
-    if (s_last && threadIdx.x == 0) {
-        __threadfence();
-        double a0 = 0, a1 = 0, a2 = 0, ae = 0;
-        for (int q = 0; q < nb; ++q) {
-            const double *pp = part + ((int64_t)i * nb + q) * 4;
-            a0 += pp[0];
-            a1 += pp[1];
-            a2 += pp[2];
-            ae += pp[3];
-        }
-        c.force[i] = c.force[i] + a0;
-        c.force[c.n + i] = c.force[c.n + i] + a1;
-        c.force[2 * c.n + i] = c.force[2 * c.n + i] + a2;
-        e_out[i] = ae;
-        done[i] = 0;
+}
+
+// the colloid force and energy of compute_force: colloid blockIdx.x's block partials in
+// block order, added to the force it holds (one thread per colloid)
+__global__ void k_force_nl_sum(PView<F64> c, const double *__restrict__ part, int nb, double *__restrict__ e_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    double a0 = 0, a1 = 0, a2 = 0, ae = 0;
+    for (int q = 0; q < nb; ++q) {
+        const double *pp = part + ((int64_t)i * nb + q) * 4;
+        a0 += pp[0];
+        a1 += pp[1];
+        a2 += pp[2];
+        ae += pp[3];
     }
+    c.force[i] = c.force[i] + a0;
+    c.force[c.n + i] = c.force[c.n + i] + a1;
+    c.force[2 * c.n + i] = c.force[2 * c.n + i] + a2;
+    e_out[i] = ae;
 }
 
 __global__ void k_copy_f64(const double *__restrict__ a, double *__restrict__ b, int64_t n)
@@ -358,6 +367,8 @@ int update_list_impl(const rmpcd_box_t *box, rmpcd_cells_t *cells, const rmpcd_p
     return RMPCD_OK;
 }
 
+constexpr int FORCE_ORDERED = 64;  // colloids of compute_force taken one launch each
+
 template <class F>
 int force_impl(rmpcd_particles_t *ps1, rmpcd_particles_t *ps2, const int32_t *list, const int32_t *nl,
                int32_t nmax, const double *L, const rmpcd_lj_t *lj, double *e)
@@ -365,18 +376,26 @@ int force_impl(rmpcd_particles_t *ps1, rmpcd_particles_t *ps2, const int32_t *li
     int np = (int)ps1->n;
     unsigned nb = blocks_for(nmax, 256);
     double *part = (double *)workspace(WS_PART, sizeof(double) * 4 * (size_t)nb * np);
-    unsigned *done = (unsigned *)workspace(WS_FLAG, sizeof(unsigned) * (np + 1));
     double *eb = (double *)workspace(WS_RED, sizeof(double) * (np + 1));
     double *host = (double *)pinned(64);
-    if (!part || !done || !eb || !host) return set_error(RMPCD_ERR_HIP, "compute_force: workspace allocation failed");
+    if (!part || !eb || !host) return set_error(RMPCD_ERR_HIP, "compute_force: workspace allocation failed");
     hipStream_t s = stream();
     PView<F> sol = view<F>(ps2);
     k_roll_md<F><<<blocks_for(sol.n, 256), 256, 0, s>>>(sol.n, sol.flags);
     RMPCD_LAUNCHED();
-    RMPCD_HIP(hipMemsetAsync(done, 0, sizeof(unsigned) * np, s));
-    dim3 grid(nb, np);
-    k_force_nl<F><<<grid, 256, 0, s>>>(view<F64>(ps1), sol, list, nl, nmax, L[0], L[1], L[2], make_lj(lj),
-                                       part, done, eb);
+    // up to FORCE_ORDERED colloids, one launch per colloid: the solvent forces are then
+    // updated in the reference's colloid order, without atomics
+    const LJ l = make_lj(lj);
+    if (np <= FORCE_ORDERED) {
+        for (int i = 0; i < np; ++i)
+            k_force_nl<F><<<dim3(nb, 1), 256, 0, s>>>(view<F64>(ps1), sol, list, nl, nmax, L[0], L[1], L[2], l, part,
+                                                      i, 0);
+    } else {
+        k_force_nl<F><<<dim3(nb, np), 256, 0, s>>>(view<F64>(ps1), sol, list, nl, nmax, L[0], L[1], L[2], l, part, 0,
+                                                   1);
+    }
+    RMPCD_LAUNCHED();
+    k_force_nl_sum<<<blocks_for(np, 64), 64, 0, s>>>(view<F64>(ps1), part, (int)nb, eb);
     RMPCD_LAUNCHED();
     k_sum_serial<<<1, 64, 0, s>>>(eb, np, eb + np);
     RMPCD_LAUNCHED();

@@ -459,7 +459,11 @@ def test_compute_force_and_helpers(gpu):
               C.byref(lj), C.byref(e))
     assert abs(e.value - e_o) <= RTOL * abs(e_o)
     assert same_bits(d.host("flags"), want_s.flags)
-    assert close(d.host("force"), want_s.force)
+    # the solvent forces in the reference's colloid order (one launch per colloid), also for
+    # the particles in both lists
+    in_both = np.intersect1d(lst[0, :nl[0]], lst[1, :nl[1]])
+    assert in_both.size > 0
+    assert same_bits(d.host("force"), want_s.force)
     scale = np.abs(want_c.force - cp.force).sum() + 1.0
     assert close(cd.host("force"), want_c.force, scale=scale)
     # colloid-colloid (n2) with overlapping pair
