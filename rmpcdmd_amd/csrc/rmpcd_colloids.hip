@@ -79,6 +79,31 @@ __device__ inline bool cg_off(const ColCtl *C, int forced) { return C->error || 
 __device__ inline int64_t cg_tid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline int64_t cg_stride() { return (int64_t)gridDim.x * blockDim.x; }
 
+// An XCD-grouped walk over tiles of CG_T items: the workgroups of one XCD (consecutive
+// workgroups go to the XCDs in turn, so blockIdx mod 8) take one contiguous eighth of the
+// tiles, interleaved, so that the tiles in flight on an XCD are neighbours and share its
+// L2 (the sort's place and gather passes, whose scattered accesses stay near the tile)
+struct CgWalk {
+    int64_t t, t1, step;
+};
+__device__ inline CgWalk cg_walk(int64_t n)
+{
+    const int64_t tiles = (n + CG_T - 1) / CG_T;
+    const int G = (int)gridDim.x;
+    CgWalk w;
+    if (G % 8 == 0) {
+        const int64_t per = (tiles + 7) / 8, lo = (int64_t)(blockIdx.x % 8) * per;
+        w.t = lo + blockIdx.x / 8;
+        w.t1 = lo + per < tiles ? lo + per : tiles;
+        w.step = G / 8;
+    } else {
+        w.t = blockIdx.x;
+        w.t1 = tiles;
+        w.step = G;
+    }
+    return w;
+}
+
 __device__ inline int cg_block_scan(int v, int *lds, int *total)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -240,7 +265,12 @@ __global__ void __launch_bounds__(CG_T) k_cg_key(SolB<F> S, const ColCtl *C, int
                                                  typename F::real *__restrict__ frec)
 {
     using R = typename F::real;
-    __shared__ int hkey[CG_KH], hcnt[CG_KH], occ[CG_KH];
+    // the LDS hash (hkey, hcnt, occ); each slab of CG_T records is staged in the same
+    // memory before the hash is built, so that the record stores are contiguous
+    __shared__ __align__(16) int hbuf[3 * CG_KH];
+    int *hkey = hbuf, *hcnt = hbuf + CG_KH, *occ = hbuf + 2 * CG_KH;
+    static_assert(sizeof(CgRec<R>) * CG_T <= sizeof(int) * 3 * CG_KH, "record slab in the hash LDS");
+    CgRec<R> *stage = reinterpret_cast<CgRec<R> *>(hbuf);
     __shared__ int lds[CG_T / 64];
     if (cg_off(C, forced)) return;
     Box b = bx;
@@ -262,51 +292,64 @@ __global__ void __launch_bounds__(CG_T) k_cg_key(SolB<F> S, const ColCtl *C, int
             keys[v] = -1;
             ent[v] = 0;
             loc[v] = 0;
-            if (i >= n) continue;
-            R x[3], vv[3], ff[3] = {0, 0, 0}, fof[3] = {0, 0, 0};
-            int fl = (int)flg[i];
-            const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
-            for (int q = 0; q < 3; ++q) {
-                x[q] = pos[q * n + i];
-                vv[q] = vel[q * n + i];
-                if (has_f) {
-                    ff[q] = f[q * n + i];
-                    fof[q] = fo[q * n + i];
-                }
-            }
-            if (!mdflag[i]) {
-                for (int q = 0; q < 3; ++q) x[q] = (x[q] + rdt * vv[q]) + dt_sq * ff[q];
-                if (!(fl & WALL_MASK)) {
-                    for (int q = 0; q < 3; ++q) vv[q] = vv[q] + (rdt * (ff[q] + fof[q])) / (R)2;
-                } else {
-                    fl &= ~WALL_MASK;
-                }
-            }
-            CgRec<R> r;
-            for (int q = 0; q < 3; ++q) {
-                const R ed = (R)b.edges[q];
-                const int jump = (int)floor((double)(x[q] / ed));
-                x[q] = x[q] - (R)jump * ed;
-                r.p[q] = x[q];
-                r.v[q] = vv[q];
-                r.im[q] = img[q * n + i] + jump;
-            }
-            int p0, p1, p2, k;
-            cell_of<R>(b, x[0], x[1], x[2], p0, p1, p2);
-            if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] && (unsigned)p2 < (unsigned)b.L[2])
-                k = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
-            else
-                k = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
-            r.id = ids[i];
-            r.key = k;
-            r.sf = (int)spc[i] | (fl << 16);
-            rec[i] = r;
-            if (has_f)
+            if (i < n) {
+                R x[3], vv[3], ff[3] = {0, 0, 0}, fof[3] = {0, 0, 0};
+                int fl = (int)flg[i];
+                const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
                 for (int q = 0; q < 3; ++q) {
-                    frec[6 * i + q] = ff[q];
-                    frec[6 * i + 3 + q] = fof[q];
+                    x[q] = pos[q * n + i];
+                    vv[q] = vel[q * n + i];
+                    if (has_f) {
+                        ff[q] = f[q * n + i];
+                        fof[q] = fo[q * n + i];
+                    }
                 }
-            keys[v] = k;
+                if (!mdflag[i]) {
+                    for (int q = 0; q < 3; ++q) x[q] = (x[q] + rdt * vv[q]) + dt_sq * ff[q];
+                    if (!(fl & WALL_MASK)) {
+                        for (int q = 0; q < 3; ++q) vv[q] = vv[q] + (rdt * (ff[q] + fof[q])) / (R)2;
+                    } else {
+                        fl &= ~WALL_MASK;
+                    }
+                }
+                CgRec<R> r;
+                for (int q = 0; q < 3; ++q) {
+                    const R ed = (R)b.edges[q];
+                    const int jump = (int)floor((double)(x[q] / ed));
+                    x[q] = x[q] - (R)jump * ed;
+                    r.p[q] = x[q];
+                    r.v[q] = vv[q];
+                    r.im[q] = img[q * n + i] + jump;
+                }
+                int p0, p1, p2, k;
+                cell_of<R>(b, x[0], x[1], x[2], p0, p1, p2);
+                if ((unsigned)p0 < (unsigned)b.L[0] && (unsigned)p1 < (unsigned)b.L[1] && (unsigned)p2 < (unsigned)b.L[2])
+                    k = lut[p0 + b.L[0] * (p1 + b.L[1] * p2)];
+                else
+                    k = compact_key(p0, p1, p2, b.M[0], b.M[1], b.M[2]);
+                r.id = ids[i];
+                r.key = k;
+                r.sf = (int)spc[i] | (fl << 16);
+                stage[threadIdx.x] = r;  // (written out below)
+                if (has_f)
+                    for (int q = 0; q < 3; ++q) {
+                        frec[6 * i + q] = ff[q];
+                        frec[6 * i + 3 + q] = fof[q];
+                    }
+                keys[v] = k;
+            }
+            // the slab's records leave through LDS: each store writes one contiguous 16-byte
+            // piece per thread, where each thread's own record would take pieces a record apart
+            __syncthreads();
+            {
+                constexpr int RW = (int)(sizeof(CgRec<R>) / 16);
+                const int64_t s0 = c0 + v * CG_T, nv = n - s0 < CG_T ? n - s0 : CG_T;
+                uint4 *dst = reinterpret_cast<uint4 *>(rec + s0);
+                const uint4 *src = reinterpret_cast<const uint4 *>(hbuf);
+                for (int e = threadIdx.x; e < RW * CG_T; e += CG_T)
+                    if (e < RW * nv) dst[e] = src[e];
+            }
+            __syncthreads();
         }
         __syncthreads();
         for (int q = threadIdx.x; q < CG_KH; q += CG_T) {
@@ -425,7 +468,10 @@ __global__ void __launch_bounds__(CG_T) k_cg_place(int64_t n, const int32_t *__r
                                                    int32_t *__restrict__ tmp, const ColCtl *C, int forced)
 {
     if (cg_off(C, forced)) return;
-    for (int64_t i = cg_tid(); i < n; i += cg_stride()) tmp[start[key[i]] + slot[i]] = (int32_t)i;
+    for (CgWalk w = cg_walk(n); w.t < w.t1; w.t += w.step) {
+        const int64_t i = w.t * CG_T + threadIdx.x;
+        if (i < n) tmp[start[key[i]] + slot[i]] = (int32_t)i;
+    }
 }
 
 // The stable gather into the other buffer (particle_system.f90:304-355): the particle of
@@ -449,7 +495,9 @@ __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, 
     const int64_t n = S.n;
     R *dpos = S.pos[o], *dvel = S.vel[o], *dold = S.pos[b];
     R *df = S.force[2 * o + fp], *dfo = S.force[2 * o + 1 - fp];
-    for (int64_t j = cg_tid(); j < n; j += cg_stride()) {
+    for (CgWalk w = cg_walk(n); w.t < w.t1; w.t += w.step) {
+        const int64_t j = w.t * CG_T + threadIdx.x;
+        if (j >= n) continue;
         const int32_t v = tmp[j];
         const CgRec<R> r = rec[v];
         const int s = start[r.key], m = count[r.key];
