@@ -92,6 +92,8 @@ def run_gpu(form, mode, L, n_md, steps, warmup, replicas, profile, sizes=None, l
     if sizes is not None:
         replicas = len(sizes)
         shape = lookup(form, 0, replicas, mixed=True) if mode == "fused" else {}
+        if GROUPS and mode == "fused":
+            shape = dict(shape, groups=GROUPS)
         sim = SingleDimer(n_md=n_md, form=f, mode=m, replica_L=[(q, q, q) for q in sizes], **shape)
     else:
         shape = lookup(form, L, replicas) if mode == "fused" else {}
