@@ -118,6 +118,7 @@ struct FB {
     double *gpart;             // |v|^2 maximum per gather unit
     int32_t nmax;
     int32_t xcd_order;  // sort passes: units grouped by XCD (1) or a contiguous range per workgroup (0)
+    int32_t fuse_srd;   // the collision sort's gather also makes the collision (no bulk reaction)
     // the replica group a launch works on: replicas r0 + blockIdx.y (R of them); plan and
     // gpart are the group's own
     int32_t r0;
@@ -607,9 +608,11 @@ __device__ inline int scan_block_exclusive(int v, int *lds, int *total)
 // S2: exclusive scan of the new cell counts inside tiles of SCAN_TILE cells (tile-local
 // offsets into slocal, tile totals into scan_sums)
 
-__device__ inline int64_t units_of(const RepDev &rp, int kind)
+constexpr int GC = 128, GC_TILE = 12 * GC;  // cells and staged particles per unit of the fused gather
+__device__ inline int64_t units_of(const RepDev &rp, int kind, bool fused_gather)
 {
     const int64_t n = rp.n, nc = rp.box.ncells;
+    if (kind == U_GATHER && fused_gather) return (nc + GC - 1) / GC;
     switch (kind) {
     case U_KEY:
         return (n + KCHUNK - 1) / KCHUNK;
@@ -664,7 +667,7 @@ __global__ void __launch_bounds__(FT) k_fz_plan(FB<F> B, int forced, int R)
         if (rl < R) amap[rl] = on ? off : -1;
         if (on) act[off] = r;
         for (int kind = 0; kind < U_KINDS; ++kind) {
-            const int u = on && !(dfr && kind != U_KEY) ? (int)units_of(B.rep[r], kind) : 0;
+            const int u = on && !(dfr && kind != U_KEY) ? (int)units_of(B.rep[r], kind, forced && B.fuse_srd) : 0;
             int ktot;
             const int kx = scan_block_exclusive(u, lds, &ktot);
             if (on) plan[1 + U_KINDS + 2 * (int64_t)R + (int64_t)kind * (R + 1) + off] = csum[kind] + kx;
@@ -1670,6 +1673,171 @@ __device__ inline void fz_srd_cell(R *vx, R *vy, R *vz, const S *sp, int m, uint
     srd_rotate(vx, vy, vz, m, o, u0, u1, u2, acc, n_eff == m);
 }
 
+// S4 of a collision sort with the collision fused in (no bulk reaction): a unit is a tile
+// of GC cells, so that all arrivals of its cells are in the unit.  The slots' reference
+// positions (and deferred cells) go to LDS; each slot's particle is ranked among its
+// cell's arrivals there and written to its destination, its velocity and species into an
+// LDS image of the tile in destination order; then SRD per cell on that image (the
+// reference's order within a cell, as k_fz_collide from the sorted arrays), then refuel
+// (single_dimer_pbc.f90:544-565), |v|max and the write-back.  A tile of more than
+// GC_TILE particles ranks and collides from memory.  Also clears the previous cell counts
+// and the is_md/is_reac marks of its cells for the list build.
+template <class F>
+__global__ void __launch_bounds__(FT) k_fz_gather_srd(FB<F> B, FP P, int R)
+{
+    using Rl = typename F::real;
+    using S = typename F::small;
+    __shared__ Rl lx[GC_TILE], ly[GC_TILE], lz[GC_TILE];
+    __shared__ S lsp[GC_TILE];
+    __shared__ int32_t lref[GC_TILE], lkey[GC_TILE];
+    __shared__ int32_t lstart[GC], lcnt[GC];
+    __shared__ double red[16];
+    const UnitRange w = unit_range(B.plan, R, U_GATHER, B.xcd_order);
+    int a = -1;
+    double vm = 0;
+    for (int64_t u = w.u0; u < w.u1; u += w.step) {
+        if (a < 0 || u >= w.pre[a + 1]) a = find_unit(w.pre, w.nact, u);
+        const int r = (int)w.act[a];
+        const int64_t c0 = (u - w.pre[a]) * GC;
+        const CtlDev &C = B.ctl[r];
+        const RepDev &rp = B.rep[r];
+        const int64_t n = rp.n, nc = rp.box.ncells;
+        const int ncl = (int)(c0 + GC < nc ? GC : nc - c0);
+        const int cur = C.cur, dst = 1 - cur, fp = C.fpar;
+        const int64_t po3 = 3 * rp.p_off;
+        const typename FB<F>::Rec *rec = B.rec + rp.p_off;
+        const Rl *sf = B.force[2 * cur + fp] + po3, *sfo = B.force[2 * cur + 1 - fp] + po3;
+        Rl *dpos = B.pos[dst] + po3, *dvel = B.vel[dst] + po3, *dold = B.pos_old + po3;
+        Rl *df = B.force[2 * dst + fp] + po3, *dfo = B.force[2 * dst + 1 - fp] + po3;
+        int32_t *dim = B.image[dst] + po3, *did = B.id[dst] + rp.p_off;
+        S *dsp = B.species[dst] + rp.p_off, *dfl = B.flags[dst] + rp.p_off;
+        const int2 *pair = B.pair + rp.p_off;
+        const int32_t *start = B.start + rp.c_off, *cnt = B.count[C.cnt ^ 1] + rp.c_off;
+        const int32_t *dkey = C.def ? B.dkey + rp.p_off : nullptr;
+        __syncthreads();  // the previous unit's LDS reads are done
+        for (int t = threadIdx.x; t < ncl; t += FT) {
+            lstart[t] = start[c0 + t];
+            lcnt[t] = cnt[c0 + t];
+        }
+        __syncthreads();
+        const int64_t sa = lstart[0], sb = (int64_t)lstart[ncl - 1] + lcnt[ncl - 1];
+        const int tcount = (int)(sb - sa);
+        const bool fits = tcount <= GC_TILE;
+        if (fits) {
+            for (int q = threadIdx.x; q < tcount; q += FT) {
+                const int2 pr = pair[sa + q];
+                lref[q] = pr.x;
+                if (dkey) lkey[q] = dkey[pr.y];
+            }
+            __syncthreads();
+        }
+        for (int q = threadIdx.x; q < tcount; q += FT) {
+            const int2 pr = pair[sa + q];
+            const int32_t ref = pr.x;
+            const typename FB<F>::Rec qr = rec[pr.y];
+            const int cl = qr.key - (int)c0;
+            const int s0 = lstart[cl], m0 = lcnt[cl];
+            int rank = 0;
+            if (fits) {
+                const int e0 = (int)(s0 - sa);
+                if (dkey) {
+                    const int32_t dk = lkey[q];
+                    for (int e = e0; e < e0 + m0; ++e) {
+                        const int32_t k = lkey[e];
+                        rank += (k < dk || (k == dk && lref[e] < ref)) ? 1 : 0;
+                    }
+                } else {
+                    for (int e = e0; e < e0 + m0; ++e) rank += lref[e] < ref ? 1 : 0;
+                }
+            } else if (dkey) {
+                const int32_t dk = dkey[pr.y];
+                for (int e = s0; e < s0 + m0; ++e) {
+                    const int2 pe = pair[e];
+                    const int32_t k = dkey[pe.y];
+                    rank += (k < dk || (k == dk && pe.x < ref)) ? 1 : 0;
+                }
+            } else {
+                for (int e = s0; e < s0 + m0; ++e) rank += pair[e].x < ref ? 1 : 0;
+            }
+            const int64_t d = s0 + rank;
+            const int fl = qr.flags();
+            const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
+            for (int k = 0; k < 3; ++k) {
+                dpos[k * n + d] = qr.p[k];
+                dold[k * n + d] = qr.p[k];
+                dim[k * n + d] = qr.im[k];
+                if (has_f) {
+                    df[k * n + d] = sf[k * n + pr.y];
+                    dfo[k * n + d] = sfo[k * n + pr.y];
+                }
+            }
+            did[d] = qr.id;
+            dfl[d] = (S)fl;
+            B.memb[rp.p_off + d] = 0;
+            if (fits) {
+                lx[d - sa] = qr.v[0];
+                ly[d - sa] = qr.v[1];
+                lz[d - sa] = qr.v[2];
+                lsp[d - sa] = (S)qr.species();
+            } else {
+                for (int k = 0; k < 3; ++k) dvel[k * n + d] = qr.v[k];
+                dsp[d] = (S)qr.species();
+            }
+        }
+        // the tile's positions and (overflow) velocities in memory are read back below
+        __syncthreads();
+        for (int t = threadIdx.x; t < ncl; t += FT) {
+            const int m = lcnt[t];
+            const int64_t sc = lstart[t];
+            if (m > 1) {
+                if (fits)
+                    fz_srd_cell(lx + (sc - sa), ly + (sc - sa), lz + (sc - sa), lsp + (sc - sa), m, rp.seed,
+                                (uint64_t)C.cycle, (int)(c0 + t));
+                else
+                    fz_srd_cell(dvel + sc, dvel + n + sc, dvel + 2 * n + sc, dsp + sc, m, rp.seed, (uint64_t)C.cycle,
+                                (int)(c0 + t));
+            }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < tcount; q += FT) {
+            const int64_t j = sa + q;
+            Rl v0, v1, v2;
+            int spc;
+            if (fits) {
+                v0 = lx[q];
+                v1 = ly[q];
+                v2 = lz[q];
+                spc = (int)lsp[q];
+            } else {
+                v0 = dvel[j];
+                v1 = dvel[n + j];
+                v2 = dvel[2 * n + j];
+                spc = (int)dsp[j];
+            }
+            const double q2 = ((double)v0 * (double)v0 + (double)v1 * (double)v1) + (double)v2 * (double)v2;
+            vm = q2 > vm ? q2 : vm;
+            const bool refuel = spc == 2 && refuel_far(dpos, n, j, B.cpos + 6 * r, rp.box);
+            if (fits) {
+                dvel[j] = v0;
+                dvel[n + j] = v1;
+                dvel[2 * n + j] = v2;
+                dsp[j] = (S)(refuel ? 1 : spc);
+            } else if (refuel) {
+                dsp[j] = (S)1;
+            }
+        }
+        int32_t *old_cnt = B.count[C.cnt] + rp.c_off;
+        for (int t = threadIdx.x; t < ncl; t += FT) {
+            old_cnt[c0 + t] = 0;
+            B.is_md[rp.c_off + c0 + t] = 0;
+            B.is_reac[rp.c_off + c0 + t] = 1;
+        }
+        const double m = block_max(vm, red);
+        if (threadIdx.x == 0) B.gpart[u] = m;
+        vm = 0;
+    }
+}
+
 // CT cells per workgroup, one thread per cell, their particles staged in LDS (CTILE = 12 CT)
 template <class F, int CT>
 __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
@@ -1797,15 +1965,24 @@ __global__ void __launch_bounds__(CT) k_fz_collide(FB<F> B, FP P)
 }
 
 // |v|max after the collision and the cycle counter, one block per replica
+// (ncollide < 0: the collision was made by the collision sort's fused gather, whose unit
+// maxima the sort's plan locates; R is then the group's replica count)
 template <class F>
-__global__ void __launch_bounds__(1024) k_fz_collide_commit(FB<F> B, FP P, int ncollide)
+__global__ void __launch_bounds__(1024) k_fz_collide_commit(FB<F> B, FP P, int ncollide, int R)
 {
     __shared__ double red[16];
     const int r = B.r0 + blockIdx.y;
     CtlDev &C = B.ctl[r];
     if (C.error) return;
     double all = 0;
-    for (int q = threadIdx.x; q < ncollide; q += blockDim.x) all = fmax(all, B.part[(int64_t)r * PS + q]);
+    if (ncollide >= 0) {
+        for (int q = threadIdx.x; q < ncollide; q += blockDim.x) all = fmax(all, B.part[(int64_t)r * PS + q]);
+    } else {
+        const int a = (int)plan_amap(B.plan, R)[r - B.r0];
+        const int64_t *gpre = plan_pre(B.plan, R, U_GATHER);
+        if (a >= 0)
+            for (int64_t q = gpre[a] + threadIdx.x; q < gpre[a + 1]; q += blockDim.x) all = fmax(all, B.gpart[q]);
+    }
     all = block_max(all, red);
     if (threadIdx.x != 0) return;
     C.vmax2 = all;
@@ -1833,7 +2010,11 @@ __global__ void k_fz_memb_init(FB<F> B, FP P)
 // ---------------------------------------------------------------------------------
 constexpr int MAX_GROUPS = 8;
 static int64_t plan_len(int R) { return 1 + U_KINDS + 2 * (int64_t)R + (int64_t)U_KINDS * (R + 1); }
-static int64_t gather_units(const Replica &rp) { return (std::max<int64_t>(rp.n, rp.box.ncells) + GT - 1) / GT; }
+static int64_t gather_units(const Replica &rp)
+{
+    // the units of a gather or rank pass (slot tiles) and of the fused gather (cell tiles)
+    return std::max<int64_t>((std::max<int64_t>(rp.n, rp.box.ncells) + GT - 1) / GT, (rp.box.ncells + GC - 1) / GC);
+}
 
 // the kernels' view of the buffers, for replica group g (replicas from r0 on; its plan,
 // and its partials from gpart_off on)
@@ -1891,6 +2072,7 @@ static FB<F> make_fb(DimerSim &s, int r0 = 0, int g = 0, int64_t gpart_off = 0)
     b.mcount = s.f_mcount;
     b.nmax = s.nmax;
     b.xcd_order = s.sort_order == 2 ? 1 : 0;
+    b.fuse_srd = s.cfg.bulk_rmpcd ? 0 : 1;
     b.slocal = s.f_slocal;
     b.scan_stride = (s.ncmax_rep + SCAN_TILE - 1) / SCAN_TILE + 1;
     b.rep = s.rep_dev;
@@ -1953,7 +2135,9 @@ static void enqueue_sort(DimerSim &s, const FB<F> &b, const FP &p, hipStream_t s
         k_fz_key<F><<<nb, FT, 0, st>>>(b, p, R);
     k_fz_scan_tiles<F><<<SCAN_BLOCKS, SCAN_T, 0, st>>>(b, R);
     k_fz_place<F><<<nb, FT, 0, st>>>(b, forced, R);
-    if (forced)
+    if (forced && b.fuse_srd)
+        k_fz_gather_srd<F><<<nb, FT, 0, st>>>(b, p, R);
+    else if (forced)
         k_fz_gather<F><<<nb, FT, 0, st>>>(b, p, R);
     else
         k_fz_rank<F><<<nb, FT, 0, st>>>(b, R);
@@ -1984,13 +2168,18 @@ static void enqueue_collision(DimerSim &s, const FB<F> &b, const FP &p, hipStrea
     const int ct = s.collide_cells;
     const unsigned gc = (unsigned)((s.ncmax_rep + ct - 1) / ct);
     enqueue_sort<F>(s, b, p, st, R, 1);
+    if (b.fuse_srd) {
+        // the collision was made by the sort's gather
+        k_fz_collide_commit<F><<<dim3(1, Rr), 1024, 0, st>>>(b, p, -1, R);
+        return;
+    }
     if (ct == 64)
         k_fz_collide<F, 64><<<dim3(gc, Rr), 64, 0, st>>>(b, p);
     else if (ct == 256)
         k_fz_collide<F, 256><<<dim3(gc, Rr), 256, 0, st>>>(b, p);
     else
         k_fz_collide<F, 128><<<dim3(gc, Rr), 128, 0, st>>>(b, p);
-    k_fz_collide_commit<F><<<dim3(1, Rr), 1024, 0, st>>>(b, p, (int)gc);
+    k_fz_collide_commit<F><<<dim3(1, Rr), 1024, 0, st>>>(b, p, (int)gc, R);
 }
 
 // Event-record nodes around every prof_stride-th launch of the streaming kernel in the
