@@ -608,7 +608,7 @@ __device__ inline int scan_block_exclusive(int v, int *lds, int *total)
 // S2: exclusive scan of the new cell counts inside tiles of SCAN_TILE cells (tile-local
 // offsets into slocal, tile totals into scan_sums)
 
-constexpr int GC = 128, GC_TILE = 12 * GC;  // cells and staged particles per unit of the fused gather
+constexpr int GC = 96, GC_TILE = 12 * GC;  // cells and staged particles per unit of the fused gather
 __device__ inline int64_t units_of(const RepDev &rp, int kind, bool fused_gather)
 {
     const int64_t n = rp.n, nc = rp.box.ncells;
