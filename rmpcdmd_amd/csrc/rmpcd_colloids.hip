@@ -251,7 +251,8 @@ struct alignas(16) CgRec {
 // The key pass of a sort, a unit being a chunk of CG_KCH particles in storage order:
 // cell_md_pos then cell_md_vel with md_flag false and the catch-up k dt for the particles
 // outside the md cells (md.f90:79-102, 239-263), apply_pbc (md.f90:202-217), the cell key
-// (cell_system.f90:86-129) and the staging record; the chunk's keys are counted in an LDS
+// (cell_system.f90:86-129) and the staging record (the forces of the particles with the
+// MD or PAST_MD bit in side records, zeroed in place); the chunk's keys are counted in an LDS
 // hash, then one global atomic per distinct cell reserves the chunk's slots in that cell
 // (the gather makes the order within a cell stable).
 constexpr int CG_KPT = 4, CG_KCH = CG_T * CG_KPT;
@@ -278,7 +279,7 @@ __global__ void __launch_bounds__(CG_T) k_cg_key(SolB<F> S, const ColCtl *C, int
     const int bf = C->cur;
     const int64_t n = S.n;
     R *pos = S.pos[bf], *vel = S.vel[bf];
-    const R *f = S.force[2 * bf + fp], *fo = S.force[2 * bf + 1 - fp];
+    R *f = S.force[2 * bf + fp], *fo = S.force[2 * bf + 1 - fp];
     const int32_t *img = S.image[bf], *ids = S.id[bf];
     const typename F::small *spc = S.species[bf], *flg = S.flags[bf];
     const double cdt = C->catch_dt;
@@ -335,6 +336,11 @@ __global__ void __launch_bounds__(CG_T) k_cg_key(SolB<F> S, const ColCtl *C, int
                     for (int q = 0; q < 3; ++q) {
                         frec[6 * i + q] = ff[q];
                         frec[6 * i + 3 + q] = fof[q];
+                        // the source buffer's forces are zero again once staged, so that
+                        // the buffer the next sort writes needs forces for flagged
+                        // particles only
+                        f[q * n + i] = 0;
+                        fo[q * n + i] = 0;
                     }
                 keys[v] = k;
             }
@@ -478,15 +484,19 @@ __global__ void __launch_bounds__(CG_T) k_cg_place(int64_t n, const int32_t *__r
 // each slot goes to cell_start + its rank by storage index among the cell's arrivals,
 // which is the reference's single-thread order.  Reads only the records (never the
 // source arrays), so it also writes pos_old = pos into the source buffer's positions
-// (after the sort they are the other buffer's).  Forces come from the side records of
-// the particles with the MD or PAST_MD bit and are zero for the others.
+// (after the sort they are the other buffer's).  Only the particles with the MD or
+// PAST_MD bit carry forces (compute_force sets MD on every particle it gives a force,
+// the roll keeps PAST_MD for one more step, and force_zero clears both over the md
+// slots); their forces come from the side records.  The destination's force planes are
+// all zero (the key pass of the previous sort zeroed what it staged, the set-up cleared
+// them before the first), so the others' forces are not written.  id_to_idx is not kept
+// (nothing in the production loop reads it).
 template <class F>
 __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, int fp, int forced,
                                                     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
                                                     const int32_t *__restrict__ tmp,
                                                     const CgRec<typename F::real> *__restrict__ rec,
-                                                    const typename F::real *__restrict__ frec,
-                                                    int32_t *__restrict__ id_to_idx)
+                                                    const typename F::real *__restrict__ frec)
 {
     using R = typename F::real;
     using Sm = typename F::small;
@@ -512,13 +522,16 @@ __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, 
             dold[k * n + d] = r.p[k];
             dvel[k * n + d] = r.v[k];
             S.image[o][k * n + d] = r.im[k];
-            df[k * n + d] = has_f ? frec[6 * (int64_t)v + k] : (R)0;
-            dfo[k * n + d] = has_f ? frec[6 * (int64_t)v + 3 + k] : (R)0;
         }
+        if (has_f)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                df[k * n + d] = frec[6 * (int64_t)v + k];
+                dfo[k * n + d] = frec[6 * (int64_t)v + 3 + k];
+            }
         S.id[o][d] = r.id;
         S.species[o][d] = (Sm)(r.sf & 0xffff);
         S.flags[o][d] = (Sm)fl;
-        if ((uint32_t)r.id < (uint32_t)n) id_to_idx[r.id] = (int32_t)d;
     }
 }
 
@@ -1008,7 +1021,7 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
     k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(s.cell_start, nc, g.tsum, C, forced);
     k_cg_place<<<CG_GRID, CG_T, 0, st>>>(n, g.key, g.slot, s.cell_start, g.tmp, C, forced);
     k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, s.cell_start, s.cell_count, g.tmp,
-                                             (const CgRec<R> *)g.rec, (const R *)g.frec, s.id_to_idx);
+                                             (const CgRec<R> *)g.rec, (const R *)g.frec);
     k_cg_commit_sort<<<1, 1, 0, st>>>(C, forced);
     k_cg_list_init<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.is_reac, C, forced);
     const dim3 lgrid(blocks_for(s.n_stencil, CG_T), (unsigned)s.nc);
@@ -1150,6 +1163,10 @@ static int graph_setup(ProgramSim &s)
     h.temperature = s.temperature;
     RMPCD_HIP(hipMemcpy(g->ctl, &h, sizeof(h), hipMemcpyHostToDevice));
     hipStream_t st = stream();
+    // the buffer the first sort writes starts with zero forces (the gather writes the
+    // forces of flagged particles only)
+    for (int f = 0; f < 2; ++f)
+        RMPCD_HIP(hipMemsetAsync(s.sol_force[2 * (1 - s.cur) + f], 0, 3 * (size_t)n * sizeof(typename F::real), st));
     ColCtl *C = g->ctl;
     k_cg_md_count<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_count, g->mdc, C, 1);
     k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g->ntiles, CG_GRID), CG_T, 0, st>>>(g->mdc, g->mdoff, nc, g->tsum, C,
