@@ -122,32 +122,61 @@ __device__ inline int cg_block_scan(int v, int *lds, int *total)
 }
 
 // ---- MD step --------------------------------------------------------------------------
-// cell_md_pos with md_flag true (md.f90:79-102) over the md slots, and the squared
-// displacement from pos_old of each moved particle (cell_maximum_displacement,
-// particle_system.f90:380-410), maximum per block
-template <class F>
-__global__ void __launch_bounds__(CG_T) k_cg_stream(SolB<F> S, const ColCtl *C, int fp, double dt,
-                                                    const int32_t *__restrict__ mdslot, double *__restrict__ part)
+// One pass over the md slots per MD step.  With PENDING, first cell_md_vel with md_flag
+// true of the step before (md.f90:239-263), which in the reference ends that step; then
+// cell_md_pos with md_flag true (md.f90:79-102) and the squared displacement from pos_old
+// of each moved particle (cell_maximum_displacement, particle_system.f90:380-410), maximum
+// per block; then the solvent part of this step's force reset (switch, zero, MD-bit roll;
+// neighbor_list.f90:180-188), which the reference makes after a sort of the same step.
+// Taking it before the sort gives the same state: only particles with the MD or PAST_MD
+// bit carry a force or change under the roll, each of them sits in an md slot (it is within
+// the list radius of a colloid), and the sort carries force planes and flags along.
+template <class F, bool PENDING>
+__global__ void __launch_bounds__(CG_T) k_cg_md(SolB<F> S, const ColCtl *C, int fp, double dt,
+                                                const int32_t *__restrict__ mdslot, double *__restrict__ part)
 {
     using R = typename F::real;
+    using Sm = typename F::small;
     __shared__ double red[16];
     double m = 0;
     if (!C->error) {
         const int b = C->cur;
         const int64_t n = S.n;
-        R *pos = S.pos[b];
-        const R *vel = S.vel[b], *f = S.force[2 * b + fp], *pold = S.pos[1 - b];
-        const typename F::small *sp = S.species[b];
+        R *pos = S.pos[b], *vel = S.vel[b];
+        const R *f = S.force[2 * b + fp], *pold = S.pos[1 - b];
+        R *fz = S.force[2 * b + 1 - fp];  // force_old of the step before, the new force of this one
+        const Sm *sp = S.species[b];
+        Sm *flags = S.flags[b];
         const R rdt = (R)dt, dt_sq = (R)((dt * dt) / 2);
         const int nmd = C->n_md;
         for (int64_t k = cg_tid(); k < nmd; k += cg_stride()) {
             const int64_t j = mdslot[k];
-            R x[3];
+            int fl = (int)flags[j];
+            R v[3], fc[3], x[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                x[q] = (pos[q * n + j] + rdt * vel[q * n + j]) + dt_sq * f[q * n + j];
-                pos[q * n + j] = x[q];
+                v[q] = vel[q * n + j];
+                fc[q] = f[q * n + j];
             }
+            if (PENDING) {
+                if (!(fl & WALL_MASK)) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        v[q] = v[q] + (rdt * (fc[q] + fz[q * n + j])) / (R)2;
+                        vel[q * n + j] = v[q];
+                    }
+                } else {
+                    fl &= ~WALL_MASK;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                x[q] = (pos[q * n + j] + rdt * v[q]) + dt_sq * fc[q];
+                pos[q * n + j] = x[q];
+                fz[q * n + j] = 0;
+            }
+            fl = (fl & MD_MASK) ? (fl | PAST_MD_MASK) : (fl & ~PAST_MD_MASK);
+            flags[j] = (Sm)(fl & ~MD_MASK);
             if ((int)sp[j] >= 0) {
                 const double d0 = (double)(x[0] - pold[j]), d1 = (double)(x[1] - pold[n + j]),
                              d2 = (double)(x[2] - pold[2 * n + j]);
@@ -160,24 +189,34 @@ __global__ void __launch_bounds__(CG_T) k_cg_stream(SolB<F> S, const ColCtl *C, 
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
-// colloid velocity-Verlet position (n_colloids_pbc.f90:289-291) and its displacement from
-// pos_old (maximum_displacement, particle_system.f90:357-378), maximum per block
-__global__ void __launch_bounds__(CG_T) k_cg_colloid_pos(ColB c, const ColCtl *C, int cfp, double dt,
-                                                         double *__restrict__ cpart)
+// The colloids' part of one MD step.  With PENDING, the velocity of the step before
+// (n_colloids_pbc.f90:312) and the step counter; the velocity-Verlet position
+// (n_colloids_pbc.f90:289-291) and its displacement from pos_old (maximum_displacement,
+// particle_system.f90:357-378), maximum per block; then colloids%force = 0 of this step's
+// compute_force (the plane the switch makes the new force; neither the skin test nor a sort
+// reads it).
+template <bool PENDING>
+__global__ void __launch_bounds__(CG_T) k_cg_colloid_step(ColB c, ColCtl *C, int fp, double dt,
+                                                          double *__restrict__ cpart)
 {
     __shared__ double red[16];
     double m = 0;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!C->error && i < c.n) {
-        const double *F = c.force[cfp];
+    const bool run = !C->error;
+    if (run && i < c.n) {
+        const double *F = c.force[fp];
+        double *Fz = c.force[1 - fp];
         double r[3];
         for (int k = 0; k < 3; ++k) {
             const int64_t q = (int64_t)k * c.n + i;
+            if (PENDING) c.vel[q] = c.vel[q] + (dt * (F[q] + Fz[q])) / (2 * c.mass[i]);
             c.pos[q] = (c.pos[q] + dt * c.vel[q]) + ((dt * dt) * F[q]) / (2 * c.mass[i]);
+            Fz[q] = 0;
             r[k] = c.pos[q] - c.pos_old[q];
         }
         if (c.species[i] >= 0) m = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
     }
+    if (PENDING && run && i == 0) C->md_steps += 1;
     m = block_max(m, red);
     if (threadIdx.x == 0) cpart[blockIdx.x] = m;
 }
@@ -654,33 +693,11 @@ __global__ void __launch_bounds__(CG_T) k_cg_md_fill(int64_t nc, const uint8_t *
 }
 
 // ---- forces ---------------------------------------------------------------------------
-// solvent%force = 0 and the MD-bit roll of compute_force (neighbor_list.f90:180-188) over
-// the md slots; colloids%force = 0
-template <class F>
-__global__ void __launch_bounds__(CG_T) k_cg_force_zero(SolB<F> S, ColB col, const ColCtl *C, int fp, int cfp,
-                                                        const int32_t *__restrict__ mdslot)
-{
-    if (C->error) return;
-    const int b = C->cur;
-    const int64_t n = S.n;
-    typename F::real *f = S.force[2 * b + fp];
-    typename F::small *flags = S.flags[b];
-    const int nmd = C->n_md;
-    for (int64_t k = cg_tid(); k < nmd; k += cg_stride()) {
-        const int64_t j = mdslot[k];
-        f[j] = 0;
-        f[n + j] = 0;
-        f[2 * n + j] = 0;
-        int fl = (int)flags[j];
-        fl = (fl & MD_MASK) ? (fl | PAST_MD_MASK) : (fl & ~PAST_MD_MASK);
-        flags[j] = (typename F::small)(fl & ~MD_MASK);
-    }
-    for (int64_t i = cg_tid(); i < 3 * (int64_t)col.n; i += cg_stride()) col.force[cfp][i] = 0;
-}
-
 // compute_force (neighbor_list.f90:191-222): one thread per list entry, solvent forces by
 // atomics, the colloid's force and energy as block sums (the partition of the library's
-// compute_force, so the same sums)
+// compute_force, so the same sums).  The grid spans the list capacity; the blocks past a
+// colloid's list return at once and k_cg_force_sum stops before them (their sums would
+// be +0, which leaves a sum that starts at +0 unchanged).
 template <class F>
 __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const ColCtl *C, int fp, int cfp,
                                                       const int32_t *__restrict__ list, const int32_t *__restrict__ nl,
@@ -695,6 +712,7 @@ __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const
     const int i = blockIdx.y;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int n_i = min(nl[i], nmax);
+    if ((int)(blockIdx.x * blockDim.x) >= n_i) return;
     double f0 = 0, f1 = 0, f2 = 0, e = 0;
     const int64_t cn = col.n;
     const double x0 = col.pos[i], x1 = col.pos[cn + i], x2 = col.pos[2 * cn + i];
@@ -734,15 +752,17 @@ __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const
     (void)cfp;
 }
 
-// each colloid adds its blocks' sums in block order; energy per colloid
+// each colloid adds its blocks' sums in block order (the blocks that cover its list);
+// energy per colloid
 __global__ void k_cg_force_sum(ColB col, const ColCtl *C, int cfp, const double *__restrict__ fpart, int nb,
-                               double *__restrict__ epart)
+                               const int32_t *__restrict__ nl, int32_t nmax, double *__restrict__ epart)
 {
     if (C->error) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= col.n) return;
+    const int nbi = min(nb, (min(nl[i], nmax) + CG_T - 1) / CG_T);
     double a0 = 0, a1 = 0, a2 = 0, ae = 0;
-    for (int q = 0; q < nb; ++q) {
+    for (int q = 0; q < nbi; ++q) {
         const double *pp = fpart + ((int64_t)i * nb + q) * 4;
         a0 += pp[0];
         a1 += pp[1];
@@ -780,7 +800,8 @@ __global__ void __launch_bounds__(CG_T) k_cg_force_fin(ColB col, ColCtl *C, int 
     }
 }
 
-// cell_md_vel with md_flag true over the md slots (md.f90:239-263)
+// cell_md_vel with md_flag true over the md slots (md.f90:239-263), after the last MD step
+// of a cycle (the other steps' velocities are updated in k_cg_md)
 template <class F>
 __global__ void __launch_bounds__(CG_T) k_cg_vel(SolB<F> S, const ColCtl *C, int fp, double dt,
                                                  const int32_t *__restrict__ mdslot)
@@ -806,7 +827,8 @@ __global__ void __launch_bounds__(CG_T) k_cg_vel(SolB<F> S, const ColCtl *C, int
     }
 }
 
-// colloid velocity (n_colloids_pbc.f90:312) and the step counter
+// colloid velocity (n_colloids_pbc.f90:312) and the step counter, after the last MD step of
+// a cycle (the other steps': k_cg_colloid_step)
 __global__ void k_cg_colloid_vel(ColB c, ColCtl *C, int cfp, double dt)
 {
     if (C->error) return;
@@ -1041,32 +1063,48 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
     return RMPCD_OK;
 }
 
-// one MD step (ProgramSim::colloids_step); fp is the force parity before the step
+// one MD step (ProgramSim::colloids_step); fp is the force parity before the step.  The
+// velocities of a step are updated at the start of the next one (`pending`, the same pass
+// over the md slots as the positions) or, after the last step of a cycle, by finish_steps.
 template <class F>
-static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_t st)
+static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, bool pending, hipStream_t st)
 {
     const SolB<F> S = sol_bufs<F>(s);
     const ColB col = col_bufs(s);
     ColCtl *C = g.ctl;
     const unsigned cgrid = blocks_for(s.nc, CG_T);
-    k_cg_stream<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-    k_cg_colloid_pos<<<cgrid, CG_T, 0, st>>>(col, C, fp, s.dt, g.cpart);
+    if (pending) {
+        k_cg_md<F, true><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
+        k_cg_colloid_step<true><<<cgrid, CG_T, 0, st>>>(col, C, fp, s.dt, g.cpart);
+    } else {
+        k_cg_md<F, false><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
+        k_cg_colloid_step<false><<<cgrid, CG_T, 0, st>>>(col, C, fp, s.dt, g.cpart);
+    }
     k_cg_decide<<<1, CG_T, 0, st>>>(C, g.part, CG_GRID, g.cpart, (int)cgrid, s.dt, s.skin, s.box.max_v);
     int rc = enqueue_sort_chain<F>(s, g, fp, 0, st);
     if (rc) return rc;
-    // switch(force, force_old) of both systems, zeroed forces, compute_force(_n2), velocities
+    // compute_force(_n2) into the planes zeroed above
     const int np = 1 - fp;
     const double *E = s.box.edges;
-    k_cg_force_zero<F><<<CG_GRID, CG_T, 0, st>>>(S, col, C, np, np, g.mdslot);
     const dim3 fgrid((unsigned)g.fblocks, (unsigned)s.nc);
     k_cg_force_nl<F><<<fgrid, CG_T, 0, st>>>(S, col, C, np, np, s.list, s.nlist, s.nmax, E[0], E[1], E[2],
                                              make_lj(&s.lj_sc), g.fpart);
-    k_cg_force_sum<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fpart, g.fblocks, g.epart);
+    k_cg_force_sum<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fpart, g.fblocks, s.nlist, s.nmax, g.epart);
     k_force_n2_blocks<CG_T><<<(unsigned)s.nc, CG_T, 0, st>>>(col.pos, col.species, col.n, col.force[np], E[0], E[1], E[2],
                                                              make_lj(&s.lj_cc), g.fnew, g.epart + s.nc);
     k_cg_force_fin<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fnew, g.epart);
-    k_cg_vel<F><<<CG_GRID, CG_T, 0, st>>>(S, C, np, s.dt, g.mdslot);
-    k_cg_colloid_vel<<<cgrid, CG_T, 0, st>>>(col, C, np, s.dt);
+    RMPCD_LAUNCHED();
+    return RMPCD_OK;
+}
+
+// the velocities of the last MD step of a cycle; fp is the force parity after it
+template <class F>
+static int finish_steps(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_t st)
+{
+    const SolB<F> S = sol_bufs<F>(s);
+    const ColB col = col_bufs(s);
+    k_cg_vel<F><<<CG_GRID, CG_T, 0, st>>>(S, g.ctl, fp, s.dt, g.mdslot);
+    k_cg_colloid_vel<<<blocks_for(s.nc, CG_T), CG_T, 0, st>>>(col, g.ctl, fp, s.dt);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
@@ -1097,9 +1135,10 @@ static int capture_cycle(ProgramSim &s, ColloidsGraph &g, int fp0, hipGraphExec_
     RMPCD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = RMPCD_OK, fp = fp0;
     for (int j = 0; j < s.cfg.n_md && !rc; ++j) {
-        rc = enqueue_step<F>(s, g, fp, st);
+        rc = enqueue_step<F>(s, g, fp, j > 0, st);
         fp = 1 - fp;
     }
+    if (!rc && s.cfg.n_md > 0) rc = finish_steps<F>(s, g, fp, st);
     if (!rc) rc = enqueue_collision<F>(s, g, fp, st);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(st, &graph);
