@@ -31,6 +31,7 @@ constexpr int CG_GRID = 2048;    // blocks of the grid-stride passes over partic
 constexpr int CG_SCAN_I = 8;     // cells per thread of a scan tile
 constexpr int CG_TILE = CG_T * CG_SCAN_I;
 constexpr int CG_CELL_T = 128;   // cells per block of the collision and temperature passes
+constexpr int CG_NL_T = 1024;    // threads (and stencil cells at a time) of a colloid's list build
 
 // device control of the loop
 struct ColCtl {
@@ -64,7 +65,7 @@ struct ColloidsGraph {
     ColCtl *ctl = nullptr;
     int32_t *key = nullptr, *slot = nullptr, *tmp = nullptr, *tsum = nullptr, *mdslot = nullptr;
     uint8_t *mdflag = nullptr;
-    int32_t *mdc = nullptr, *mdoff = nullptr, *nlcnt = nullptr;
+    int32_t *mdoff = nullptr;
     int32_t *lut = nullptr;  // the box's Hilbert table, a copy owned by the program
     void *rec = nullptr, *frec = nullptr;  // staging records of a sort, forces of the members
     double *part = nullptr, *cpart = nullptr, *fpart = nullptr, *epart = nullptr, *fnew = nullptr,
@@ -259,21 +260,23 @@ __global__ void k_cg_shift(ColCtl *C, uint64_t seed, double dt)
     C->catch_dt = (double)(C->md_steps - C->last_sort) * dt;
 }
 
-// apply_pbc of the colloids (md.f90:202-217) and colloids%pos_old = colloids%pos
-__global__ void k_cg_colloid_pbc(ColB c, const ColCtl *C, int forced, double e0, double e1, double e2)
+// the start of a sort: cell_count = 0, and apply_pbc of the colloids (md.f90:202-217) with
+// colloids%pos_old = colloids%pos
+__global__ void __launch_bounds__(CG_T) k_cg_sort_begin(ColB c, const ColCtl *C, int forced, double e0, double e1,
+                                                        double e2, int32_t *__restrict__ count, int64_t nc)
 {
     if (cg_off(C, forced)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
+    for (int64_t q = cg_tid(); q < nc; q += cg_stride()) count[q] = 0;
     const double e[3] = {e0, e1, e2};
-    for (int k = 0; k < 3; ++k) {
-        const int64_t q = (int64_t)k * c.n + i;
-        const double x = c.pos[q];
-        const int jump = (int)floor(x / e[k]);
-        c.image[q] += jump;
-        c.pos[q] = x - (double)jump * e[k];
-        c.pos_old[q] = c.pos[q];
-    }
+    for (int64_t i = cg_tid(); i < c.n; i += cg_stride())
+        for (int k = 0; k < 3; ++k) {
+            const int64_t q = (int64_t)k * c.n + i;
+            const double x = c.pos[q];
+            const int jump = (int)floor(x / e[k]);
+            c.image[q] += jump;
+            c.pos[q] = x - (double)jump * e[k];
+            c.pos_old[q] = c.pos[q];
+        }
 }
 
 // The staging record of a sort: a particle's caught-up, wrapped state in one piece, written
@@ -441,12 +444,6 @@ __global__ void __launch_bounds__(CG_T) k_cg_key(SolB<F> S, const ColCtl *C, int
     }
 }
 
-__global__ void __launch_bounds__(CG_T) k_cg_zero_i32(int32_t *a, int64_t n, const ColCtl *C, int forced)
-{
-    if (cg_off(C, forced)) return;
-    for (int64_t i = cg_tid(); i < n; i += cg_stride()) a[i] = 0;
-}
-
 // the position in the particle's precision, wrapped as apply_pbc does
 template <class R>
 __device__ inline R cg_wrap(R x, double e, int &jump)
@@ -458,10 +455,11 @@ __device__ inline R cg_wrap(R x, double e, int &jump)
 
 
 // exclusive scan of nc int32 in tiles of CG_TILE: tile-local scan and tile totals, one block
-// for the totals, then each tile's offset (the total goes to *total when given)
+// for the totals, then each tile's offset (the total goes to *total when given); with a
+// mask, the entries whose mask byte is 0 count as 0
 __global__ void __launch_bounds__(CG_T) k_cg_scan_tiles(const int32_t *__restrict__ in, int32_t *__restrict__ out,
                                                         int64_t nc, int32_t *__restrict__ tsum, const ColCtl *C,
-                                                        int forced)
+                                                        int forced, const uint8_t *__restrict__ mask)
 {
     __shared__ int lds[CG_T / 64];
     if (cg_off(C, forced)) return;
@@ -470,7 +468,7 @@ __global__ void __launch_bounds__(CG_T) k_cg_scan_tiles(const int32_t *__restric
         const int64_t base = t * CG_TILE + (int64_t)threadIdx.x * CG_SCAN_I;
         int v[CG_SCAN_I], s = 0;
         for (int k = 0; k < CG_SCAN_I; ++k) {
-            v[k] = base + k < nc ? in[base + k] : 0;
+            v[k] = base + k < nc && (!mask || mask[base + k]) ? in[base + k] : 0;
             s += v[k];
         }
         int tot;
@@ -574,44 +572,49 @@ __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, 
     }
 }
 
-// the sort is done: the other buffer holds the state (the switch() calls of
-// particle_system.f90:346-353); the md step of the last sort
-__global__ void k_cg_commit_sort(ColCtl *C, int forced)
-{
-    if (threadIdx.x || blockIdx.x || cg_off(C, forced)) return;
-    C->cur = 1 - C->cur;
-    if (forced) {
-        C->last_sort = C->md_steps;
-    } else {
-        C->last_sort = C->md_steps + 1;
-        C->n_extra += 1;
-    }
-}
-
-// update_list (neighbor_list.f90:76-157) without lj: is_md cleared and is_reac set, then a
-// count pass per (colloid, stencil cell), a scan per colloid and a fill pass, which keeps
-// the reference's list order (stencil order, then the cell's sorted order)
-__global__ void __launch_bounds__(CG_T) k_cg_list_init(int64_t nc, uint8_t *is_md, uint8_t *is_reac,
-                                                       const ColCtl *C, int forced)
+// The sort is done: the other buffer holds the state (the switch() calls of
+// particle_system.f90:346-353), with the md step of the last sort.  Then update_list
+// (neighbor_list.f90:76-157) without lj begins: is_md cleared and is_reac set; the lists
+// follow (k_cg_nl_build).
+__global__ void __launch_bounds__(CG_T) k_cg_list_init(int64_t nc, uint8_t *is_md, uint8_t *is_reac, ColCtl *C,
+                                                       int forced)
 {
     if (cg_off(C, forced)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        C->cur = 1 - C->cur;
+        if (forced) {
+            C->last_sort = C->md_steps;
+        } else {
+            C->last_sort = C->md_steps + 1;
+            C->n_extra += 1;
+        }
+    }
     for (int64_t c = cg_tid(); c < nc; c += cg_stride()) {
         is_md[c] = 0;
         is_reac[c] = 1;
     }
 }
 
+// The count and fill of update_list in one pass, a workgroup per colloid: the particle
+// ranges of the stencil cells (CG_NL_T cells at a time) are laid end to end by a scan of
+// their counts in LDS, the threads walk that sequence (neighbouring threads read
+// neighbouring particles of a cell), and an in-order compaction of the accepted particles
+// (wave ballots, then the waves' totals) writes the list in the reference's order: stencil
+// order, then the cell's sorted order.  A list longer than nmax keeps its first nmax
+// entries and raises the overflow (the reference's error stop).
 template <class F>
-__global__ void __launch_bounds__(CG_T) k_cg_nl_pass(SolB<F> S, ColB col, const ColCtl *C, int forced, Box bx,
-                                                     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
-                                                     const int32_t *__restrict__ st, int ns, double radius_sq,
-                                                     int32_t *__restrict__ cnt, uint8_t *__restrict__ is_md, int fill,
-                                                     int32_t nmax, int32_t *__restrict__ list)
+__global__ void __launch_bounds__(CG_NL_T) k_cg_nl_build(SolB<F> S, ColB col, ColCtl *C, int forced, Box bx,
+                                                         const int32_t *__restrict__ start,
+                                                         const int32_t *__restrict__ count,
+                                                         const int32_t *__restrict__ st, int ns, double radius_sq,
+                                                         uint8_t *__restrict__ is_md, int32_t nmax,
+                                                         int32_t *__restrict__ list, int32_t *__restrict__ nlist)
 {
+    constexpr int NW = CG_NL_T / 64;
+    __shared__ int s_start[CG_NL_T], s_off[CG_NL_T], s_key[CG_NL_T];
+    __shared__ int lds[NW], wsum[NW];
     if (cg_off(C, forced)) return;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    if (j >= ns) return;
+    const int i = blockIdx.x;
     Box b = bx;
     for (int k = 0; k < 3; ++k) b.origin[k] = C->origin[k];
     const int bf = C->cur;
@@ -619,61 +622,72 @@ __global__ void __launch_bounds__(CG_T) k_cg_nl_pass(SolB<F> S, ColB col, const 
     const typename F::real *pos = S.pos[bf];
     const typename F::small *sp = S.species[bf];
     const double x[3] = {col.pos[i], col.pos[col.n + i], col.pos[2 * (int64_t)col.n + i]};
-    const int key = stencil_key_dev(b, x, st, j);
-    const int s = start[key], m = count[key];
-    const int64_t e = (int64_t)i * ns + j;
-    const int off = fill ? cnt[e] : 0;
-    int k = 0;
-    bool md = false;
-    for (int q = s; q < s + m; ++q) {
-        if ((int)sp[q] <= 0) continue;
-        const double d0 = min_image(x[0] - (double)pos[q], b.edges[0]);
-        const double d1 = min_image(x[1] - (double)pos[n + q], b.edges[1]);
-        const double d2 = min_image(x[2] - (double)pos[2 * n + q], b.edges[2]);
-        if ((d0 * d0 + d1 * d1) + d2 * d2 < radius_sq) {
-            md = true;
-            if (fill && off + k < nmax) list[(int64_t)i * nmax + off + k] = q;
-            ++k;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int total = 0;
+    for (int j0 = 0; j0 < ns; j0 += CG_NL_T) {
+        const int j = j0 + threadIdx.x;
+        int key = 0, s0 = 0, m = 0;
+        if (j < ns) {
+            key = stencil_key_dev(b, x, st, j);
+            s0 = start[key];
+            m = count[key];
+        }
+        int tot;
+        const int off = cg_block_scan(m, lds, &tot);
+        s_start[threadIdx.x] = s0;
+        s_off[threadIdx.x] = off;
+        s_key[threadIdx.x] = key;
+        __syncthreads();
+        for (int p0 = 0; p0 < tot; p0 += CG_NL_T) {
+            const int p = p0 + threadIdx.x;
+            bool acc = false;
+            int q = 0, c = 0;
+            if (p < tot) {
+                // the cell holding p: the last one whose offset is at most p (the empty cells
+                // before it share its offset; the cells past ns have offset tot)
+                int lo = 0, hi = CG_NL_T - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= p)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                c = lo;
+                q = s_start[c] + (p - s_off[c]);
+                if ((int)sp[q] > 0) {
+                    const double d0 = min_image(x[0] - (double)pos[q], b.edges[0]);
+                    const double d1 = min_image(x[1] - (double)pos[n + q], b.edges[1]);
+                    const double d2 = min_image(x[2] - (double)pos[2 * n + q], b.edges[2]);
+                    acc = (d0 * d0 + d1 * d1) + d2 * d2 < radius_sq;
+                }
+            }
+            const unsigned long long bal = __ballot(acc);
+            if (lane == 0) wsum[w] = __popcll(bal);
+            __syncthreads();
+            int woff = 0, btot = 0;
+            for (int u = 0; u < NW; ++u) {
+                const int t = wsum[u];
+                woff += u < w ? t : 0;
+                btot += t;
+            }
+            if (acc) {
+                const int e = total + woff + __popcll(bal & ((1ull << lane) - 1ull));
+                if (e < nmax) list[(int64_t)i * nmax + e] = q;
+                is_md[s_key[c]] = 1;
+            }
+            total += btot;
+            __syncthreads();
         }
     }
-    if (!fill) {
-        cnt[e] = k;
-        if (md) is_md[key] = 1;
-    }
-}
-
-__global__ void __launch_bounds__(1024) k_cg_nl_scan(int32_t *cnt, int ns, int32_t nmax, int32_t *__restrict__ nlist,
-                                                     ColCtl *C, int forced)
-{
-    __shared__ int lds[16];
-    if (cg_off(C, forced)) return;
-    const int i = blockIdx.x;
-    int32_t *c = cnt + (int64_t)i * ns;
-    int carry = 0;
-    for (int base = 0; base < ns; base += blockDim.x) {
-        const int q = base + threadIdx.x;
-        const int v = q < ns ? c[q] : 0;
-        int tot;
-        const int ex = cg_block_scan(v, lds, &tot);
-        if (q < ns) c[q] = carry + ex;
-        carry += tot;
-    }
     if (threadIdx.x == 0) {
-        nlist[i] = carry > nmax ? nmax : carry;
-        if (carry > nmax) atomicOr(&C->error, RMPCD_ERR_OVERFLOW);
+        nlist[i] = total > nmax ? nmax : total;
+        if (total > nmax) atomicOr(&C->error, RMPCD_ERR_OVERFLOW);
     }
 }
 
-// the md slot list: the slots of the is_md cells, in cell order (counts scanned into mdoff),
-// and a flag per slot
-__global__ void __launch_bounds__(CG_T) k_cg_md_count(int64_t nc, const uint8_t *__restrict__ is_md,
-                                                      const int32_t *__restrict__ count, int32_t *__restrict__ mdc,
-                                                      const ColCtl *C, int forced)
-{
-    if (cg_off(C, forced)) return;
-    for (int64_t c = cg_tid(); c < nc; c += cg_stride()) mdc[c] = is_md[c] ? count[c] : 0;
-}
-
+// the md slot list: the slots of the is_md cells, in cell order (their counts scanned
+// into mdoff), and a flag per slot
 __global__ void __launch_bounds__(CG_T) k_cg_md_fill(int64_t nc, const uint8_t *__restrict__ is_md,
                                                      const int32_t *__restrict__ start,
                                                      const int32_t *__restrict__ count,
@@ -1030,31 +1044,24 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
     ColCtl *C = g.ctl;
     const Box b = make_box(&s.box);
     const int64_t nc = s.box.ncells, n = s.n;
-    const unsigned cgrid = blocks_for(s.nc, CG_T);
     const double *E = s.box.edges;
     using R = typename F::real;
-    k_cg_colloid_pbc<<<cgrid, CG_T, 0, st>>>(col, C, forced, E[0], E[1], E[2]);
-    k_cg_zero_i32<<<CG_GRID, CG_T, 0, st>>>(s.cell_count, nc, C, forced);
+    k_cg_sort_begin<<<CG_GRID, CG_T, 0, st>>>(col, C, forced, E[0], E[1], E[2], s.cell_count, nc);
     k_cg_key<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, b, g.lut, g.mdflag, s.cell_count, g.key, g.slot,
                                           (CgRec<R> *)g.rec, (R *)g.frec);
     k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_count, s.cell_start, nc,
-                                                                                      g.tsum, C, forced);
+                                                                                      g.tsum, C, forced, nullptr);
     k_cg_scan_top<<<1, 1024, 0, st>>>(g.tsum, g.ntiles, C, forced, nullptr);
     k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(s.cell_start, nc, g.tsum, C, forced);
     k_cg_place<<<CG_GRID, CG_T, 0, st>>>(n, g.key, g.slot, s.cell_start, g.tmp, C, forced);
     k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, s.cell_start, s.cell_count, g.tmp,
                                              (const CgRec<R> *)g.rec, (const R *)g.frec);
-    k_cg_commit_sort<<<1, 1, 0, st>>>(C, forced);
     k_cg_list_init<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.is_reac, C, forced);
-    const dim3 lgrid(blocks_for(s.n_stencil, CG_T), (unsigned)s.nc);
-    k_cg_nl_pass<F><<<lgrid, CG_T, 0, st>>>(S, col, C, forced, b, s.cell_start, s.cell_count, s.stencil, s.n_stencil,
-                                          s.radius * s.radius, g.nlcnt, s.is_md, 0, s.nmax, s.list);
-    k_cg_nl_scan<<<(unsigned)s.nc, 1024, 0, st>>>(g.nlcnt, s.n_stencil, s.nmax, s.nlist, C, forced);
-    k_cg_nl_pass<F><<<lgrid, CG_T, 0, st>>>(S, col, C, forced, b, s.cell_start, s.cell_count, s.stencil, s.n_stencil,
-                                          s.radius * s.radius, g.nlcnt, s.is_md, 1, s.nmax, s.list);
-    k_cg_md_count<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_count, g.mdc, C, forced);
-    k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(g.mdc, g.mdoff, nc, g.tsum, C,
-                                                                                      forced);
+    k_cg_nl_build<F><<<(unsigned)s.nc, CG_NL_T, 0, st>>>(S, col, C, forced, b, s.cell_start, s.cell_count, s.stencil,
+                                                         s.n_stencil, s.radius * s.radius, s.is_md, s.nmax, s.list,
+                                                         s.nlist);
+    k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_count, g.mdoff, nc, g.tsum,
+                                                                                      C, forced, s.is_md);
     k_cg_scan_top<<<1, 1024, 0, st>>>(g.tsum, g.ntiles, C, forced, &C->n_md);
     k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(g.mdoff, nc, g.tsum, C, forced);
     k_cg_md_fill<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_start, s.cell_count, g.mdoff, g.mdslot, g.mdflag, C,
@@ -1170,9 +1177,7 @@ static int graph_setup(ProgramSim &s)
     if ((rc = cg_alloc(*g, (void **)&g->mdslot, 4 * n))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->mdflag, n))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->tsum, 4 * (size_t)(g->ntiles + 1)))) return rc;
-    if ((rc = cg_alloc(*g, (void **)&g->mdc, 4 * nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->mdoff, 4 * nc))) return rc;
-    if ((rc = cg_alloc(*g, (void **)&g->nlcnt, 4 * (size_t)s.nc * s.n_stencil))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->part, 8 * CG_GRID))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->cpart, 8 * (size_t)g->cblocks))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->fpart, 32 * (size_t)g->fblocks * s.nc))) return rc;
@@ -1207,9 +1212,8 @@ static int graph_setup(ProgramSim &s)
     for (int f = 0; f < 2; ++f)
         RMPCD_HIP(hipMemsetAsync(s.sol_force[2 * (1 - s.cur) + f], 0, 3 * (size_t)n * sizeof(typename F::real), st));
     ColCtl *C = g->ctl;
-    k_cg_md_count<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_count, g->mdc, C, 1);
-    k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g->ntiles, CG_GRID), CG_T, 0, st>>>(g->mdc, g->mdoff, nc, g->tsum, C,
-                                                                                       1);
+    k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g->ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_count, g->mdoff, nc,
+                                                                                       g->tsum, C, 1, s.is_md);
     k_cg_scan_top<<<1, 1024, 0, st>>>(g->tsum, g->ntiles, C, 1, &C->n_md);
     k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(g->mdoff, nc, g->tsum, C, 1);
     k_cg_md_fill<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_start, s.cell_count, g->mdoff, g->mdslot, g->mdflag, C,
