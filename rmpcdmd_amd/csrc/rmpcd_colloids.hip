@@ -68,9 +68,8 @@ struct ColloidsGraph {
     int32_t *mdoff = nullptr;
     int32_t *lut = nullptr;  // the box's Hilbert table, a copy owned by the program
     void *rec = nullptr, *frec = nullptr;  // staging records of a sort, forces of the members
-    double *part = nullptr, *cpart = nullptr, *fpart = nullptr, *epart = nullptr, *fnew = nullptr,
-           *tpart = nullptr;
-    int ntiles = 0, fblocks = 0, cblocks = 0;
+    double *part = nullptr, *fpart = nullptr, *epart = nullptr, *tpart = nullptr;
+    int ntiles = 0, fblocks = 0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};  // by starting force parity
     std::vector<void *> allocs;
 };
@@ -190,23 +189,25 @@ __global__ void __launch_bounds__(CG_T) k_cg_md(SolB<F> S, const ColCtl *C, int 
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
-// The colloids' part of one MD step.  With PENDING, the velocity of the step before
-// (n_colloids_pbc.f90:312) and the step counter; the velocity-Verlet position
-// (n_colloids_pbc.f90:289-291) and its displacement from pos_old (maximum_displacement,
-// particle_system.f90:357-378), maximum per block; then colloids%force = 0 of this step's
-// compute_force (the plane the switch makes the new force; neither the skin test nor a sort
-// reads it).
+// The colloids' part of one MD step and the skin test, one workgroup.  With PENDING, the
+// velocity of the step before (n_colloids_pbc.f90:312) and the step counter; the
+// velocity-Verlet position (n_colloids_pbc.f90:289-291) and its displacement from pos_old
+// (maximum_displacement, particle_system.f90:357-378); colloids%force = 0 of this step's
+// compute_force (the plane the switch makes the new force; neither the skin test nor a
+// sort reads it).  Then the skin test (n_colloids_pbc.f90:293-296): so_max is the maximum
+// over the md cells (part, from k_cg_md), raised to max_v k dt; co_max over the colloids;
+// k = steps since the last sort.
 template <bool PENDING>
-__global__ void __launch_bounds__(CG_T) k_cg_colloid_step(ColB c, ColCtl *C, int fp, double dt,
-                                                          double *__restrict__ cpart)
+__global__ void __launch_bounds__(CG_T) k_cg_colloids_decide(ColB c, ColCtl *C, int fp, double dt,
+                                                             const double *__restrict__ part, int np, double skin,
+                                                             double max_v)
 {
     __shared__ double red[16];
-    double m = 0;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool run = !C->error;
-    if (run && i < c.n) {
-        const double *F = c.force[fp];
-        double *Fz = c.force[1 - fp];
+    if (C->error) return;
+    const double *F = c.force[fp];
+    double *Fz = c.force[1 - fp];
+    double a = 0, bm = 0;
+    for (int i = threadIdx.x; i < c.n; i += blockDim.x) {
         double r[3];
         for (int k = 0; k < 3; ++k) {
             const int64_t q = (int64_t)k * c.n + i;
@@ -215,31 +216,18 @@ __global__ void __launch_bounds__(CG_T) k_cg_colloid_step(ColB c, ColCtl *C, int
             Fz[q] = 0;
             r[k] = c.pos[q] - c.pos_old[q];
         }
-        if (c.species[i] >= 0) m = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+        if (c.species[i] >= 0) bm = fmax(bm, (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
     }
-    if (PENDING && run && i == 0) C->md_steps += 1;
-    m = block_max(m, red);
-    if (threadIdx.x == 0) cpart[blockIdx.x] = m;
-}
-
-// the skin test (n_colloids_pbc.f90:293-296): so_max is the maximum over the md cells,
-// raised to max_v k dt; co_max over the colloids; k = steps since the last sort
-__global__ void __launch_bounds__(CG_T) k_cg_decide(ColCtl *C, const double *__restrict__ part, int np,
-                                                    const double *__restrict__ cpart, int ncp, double dt, double skin,
-                                                    double max_v)
-{
-    __shared__ double red[16];
-    double a = 0, b = 0;
     for (int q = threadIdx.x; q < np; q += blockDim.x) a = fmax(a, part[q]);
-    for (int q = threadIdx.x; q < ncp; q += blockDim.x) b = fmax(b, cpart[q]);
     a = block_max(a, red);
-    b = block_max(b, red);
-    if (threadIdx.x != 0 || C->error) return;
+    bm = block_max(bm, red);
+    if (threadIdx.x != 0) return;
+    if (PENDING) C->md_steps += 1;
     const double k1 = (double)(C->md_steps + 1 - C->last_sort);
     double so = sqrt(a);
     const double bnd = max_v * (dt * k1);
     so = so > bnd ? so : bnd;
-    const double co = sqrt(b);
+    const double co = sqrt(bm);
     C->need_sort = (co >= skin * 0.1 || so >= skin * 0.9) ? 1 : 0;
     C->catch_dt = k1 * dt;
 }
@@ -481,28 +469,32 @@ __global__ void __launch_bounds__(CG_T) k_cg_scan_tiles(const int32_t *__restric
     }
 }
 
-__global__ void __launch_bounds__(1024) k_cg_scan_top(int32_t *tsum, int64_t ntiles, const ColCtl *C, int forced,
-                                                      int32_t *total)
+// the sum of the tile totals before tile t (every thread of the workgroup gets it)
+__device__ inline int cg_tile_prefix(const int32_t *__restrict__ tsum, int64_t t, int *lds)
 {
-    __shared__ int lds[16];
-    if (cg_off(C, forced)) return;
-    int carry = 0;
-    for (int64_t c0 = 0; c0 < ntiles; c0 += blockDim.x) {
-        const int64_t i = c0 + threadIdx.x;
-        const int v = i < ntiles ? tsum[i] : 0;
-        int tot;
-        const int ex = cg_block_scan(v, lds, &tot);
-        if (i < ntiles) tsum[i] = ex + carry;
-        carry += tot;
-    }
-    if (total && threadIdx.x == 0) *total = carry;
+    int v = 0;
+    for (int64_t u = threadIdx.x; u < t; u += blockDim.x) v += tsum[u];
+    int tot;
+    (void)cg_block_scan(v, lds, &tot);
+    return tot;
 }
 
-__global__ void __launch_bounds__(CG_T) k_cg_scan_add(int32_t *__restrict__ out, int64_t nc,
-                                                      const int32_t *__restrict__ tsum, const ColCtl *C, int forced)
+// the end of a scan: each tile's offsets raised by the totals of the tiles before it (a
+// workgroup sums those itself, so there is no pass over the totals alone); the grand
+// total goes to *total when given
+__global__ void __launch_bounds__(CG_T) k_cg_scan_finish(int32_t *__restrict__ out, int64_t nc,
+                                                         const int32_t *__restrict__ tsum, const ColCtl *C, int forced,
+                                                         int32_t *total)
 {
+    __shared__ int lds[CG_T / 64];
     if (cg_off(C, forced)) return;
-    for (int64_t i = cg_tid(); i < nc; i += cg_stride()) out[i] += tsum[i / CG_TILE];
+    const int64_t ntiles = (nc + CG_TILE - 1) / CG_TILE;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int pre = cg_tile_prefix(tsum, t, lds);
+        const int64_t c1 = (t + 1) * CG_TILE < nc ? (t + 1) * CG_TILE : nc;
+        for (int64_t c = t * CG_TILE + threadIdx.x; c < c1; c += CG_T) out[c] += pre;
+        if (total && t == ntiles - 1 && threadIdx.x == 0) *total = pre + tsum[t];
+    }
 }
 
 // tmp[cell_start[key] + slot] = storage index (particle_system.f90:320-340, a slot per particle)
@@ -686,23 +678,33 @@ __global__ void __launch_bounds__(CG_NL_T) k_cg_nl_build(SolB<F> S, ColB col, Co
     }
 }
 
-// the md slot list: the slots of the is_md cells, in cell order (their counts scanned
-// into mdoff), and a flag per slot
-__global__ void __launch_bounds__(CG_T) k_cg_md_fill(int64_t nc, const uint8_t *__restrict__ is_md,
-                                                     const int32_t *__restrict__ start,
-                                                     const int32_t *__restrict__ count,
-                                                     const int32_t *__restrict__ mdoff, int32_t *__restrict__ mdslot,
-                                                     uint8_t *__restrict__ mdflag, const ColCtl *C, int forced)
+// The md slot list: the slots of the is_md cells, in cell order, and a flag per slot.  The
+// end of the scan of the md cells' counts (mdoff holds the tile-local offsets from
+// k_cg_scan_tiles), as in k_cg_scan_finish, with the list written from the offsets; the
+// number of md slots goes to n_md.
+__global__ void __launch_bounds__(CG_T) k_cg_md_finish(int64_t nc, const uint8_t *__restrict__ is_md,
+                                                       const int32_t *__restrict__ start,
+                                                       const int32_t *__restrict__ count,
+                                                       const int32_t *__restrict__ mdoff,
+                                                       const int32_t *__restrict__ tsum, int32_t *__restrict__ mdslot,
+                                                       uint8_t *__restrict__ mdflag, ColCtl *C, int forced)
 {
+    __shared__ int lds[CG_T / 64];
     if (cg_off(C, forced)) return;
-    for (int64_t c = cg_tid(); c < nc; c += cg_stride()) {
-        const int s = start[c], m = count[c];
-        const bool md = is_md[c] != 0;
-        const int o = mdoff[c];
-        for (int k = 0; k < m; ++k) {
-            mdflag[s + k] = md ? 1 : 0;
-            if (md) mdslot[o + k] = s + k;
+    const int64_t ntiles = (nc + CG_TILE - 1) / CG_TILE;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int pre = cg_tile_prefix(tsum, t, lds);
+        const int64_t c1 = (t + 1) * CG_TILE < nc ? (t + 1) * CG_TILE : nc;
+        for (int64_t c = t * CG_TILE + threadIdx.x; c < c1; c += CG_T) {
+            const int s0 = start[c], m = count[c];
+            const bool md = is_md[c] != 0;
+            const int o = mdoff[c] + pre;
+            for (int k = 0; k < m; ++k) {
+                mdflag[s0 + k] = md ? 1 : 0;
+                if (md) mdslot[o + k] = s0 + k;
+            }
         }
+        if (t == ntiles - 1 && threadIdx.x == 0) C->n_md = pre + tsum[t];
     }
 }
 
@@ -710,7 +712,7 @@ __global__ void __launch_bounds__(CG_T) k_cg_md_fill(int64_t nc, const uint8_t *
 // compute_force (neighbor_list.f90:191-222): one thread per list entry, solvent forces by
 // atomics, the colloid's force and energy as block sums (the partition of the library's
 // compute_force, so the same sums).  The grid spans the list capacity; the blocks past a
-// colloid's list return at once and k_cg_force_sum stops before them (their sums would
+// colloid's list return at once and k_cg_colloid_forces stops before them (their sums would
 // be +0, which leaves a sum that starts at +0 unchanged).
 template <class F>
 __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const ColCtl *C, int fp, int cfp,
@@ -766,41 +768,48 @@ __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const
     (void)cfp;
 }
 
-// each colloid adds its blocks' sums in block order (the blocks that cover its list);
-// energy per colloid
-__global__ void k_cg_force_sum(ColB col, const ColCtl *C, int cfp, const double *__restrict__ fpart, int nb,
-                               const int32_t *__restrict__ nl, int32_t nmax, double *__restrict__ epart)
-{
-    if (C->error) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= col.n) return;
-    const int nbi = min(nb, (min(nl[i], nmax) + CG_T - 1) / CG_T);
-    double a0 = 0, a1 = 0, a2 = 0, ae = 0;
-    for (int q = 0; q < nbi; ++q) {
-        const double *pp = fpart + ((int64_t)i * nb + q) * 4;
-        a0 += pp[0];
-        a1 += pp[1];
-        a2 += pp[2];
-        ae += pp[3];
-    }
-    double *F = col.force[cfp];
-    const int64_t cn = col.n;
-    F[i] = F[i] + a0;
-    F[cn + i] = F[cn + i] + a1;
-    F[2 * cn + i] = F[2 * cn + i] + a2;
-    epart[i] = ae;
-}
-
-// the colloid forces of compute_force_n2 (rmpcd_pairforce.hpp), then the energies (tree sums)
-__global__ void __launch_bounds__(CG_T) k_cg_force_fin(ColB col, ColCtl *C, int cfp, const double *__restrict__ fnew,
-                                                       const double *__restrict__ epart)
+// The colloid forces of one MD step, a workgroup per colloid: the solvent part, the sum of
+// its k_cg_force_nl block sums in block order (over the blocks that cover its list), then
+// compute_force_n2 (rmpcd_pairforce.hpp) added to it; each workgroup reads and writes its
+// own colloid's force only, so the result goes straight to the force plane.  The energies
+// per colloid (solvent, then colloid pairs) go to epart for k_cg_energy.
+__global__ void __launch_bounds__(CG_T) k_cg_colloid_forces(ColB col, const ColCtl *C, int cfp,
+                                                            const double *__restrict__ fpart, int nb,
+                                                            const int32_t *__restrict__ nl, int32_t nmax, double L0,
+                                                            double L1, double L2, LJ lj, double *__restrict__ epart)
 {
     __shared__ double red[16];
     if (C->error) return;
-    const int64_t n3 = 3 * (int64_t)col.n;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (int64_t)gridDim.x * blockDim.x)
-        col.force[cfp][i] = fnew[i];
-    if (blockIdx.x != 0) return;
+    const int i = blockIdx.x;
+    double f0, f1, f2, e;
+    n2_pair_sums<CG_T>(col.pos, col.species, col.n, i, L0, L1, L2, lj, red, f0, f1, f2, e);
+    if (threadIdx.x == 0) {
+        const int nbi = min(nb, (min(nl[i], nmax) + CG_T - 1) / CG_T);
+        double a0 = 0, a1 = 0, a2 = 0, ae = 0;
+        for (int q = 0; q < nbi; ++q) {
+            const double *pp = fpart + ((int64_t)i * nb + q) * 4;
+            a0 += pp[0];
+            a1 += pp[1];
+            a2 += pp[2];
+            ae += pp[3];
+        }
+        // the force plane holds 0 here; 0 + a is a (a sum that starts at +0 is never -0)
+        double *F = col.force[cfp];
+        const int64_t cn = col.n;
+        F[i] = a0 + f0;
+        F[cn + i] = a1 + f1;
+        F[2 * cn + i] = a2 + f2;
+        epart[i] = ae;
+        epart[cn + i] = e;
+    }
+}
+
+// the energies of the last computed step (tree sums over the colloids): solvent-colloid and
+// colloid-colloid.  Run once per cycle, after its last step; it runs after an error stop too,
+// when epart still holds the last step that completed.
+__global__ void __launch_bounds__(CG_T) k_cg_energy(const ColB col, ColCtl *C, const double *__restrict__ epart)
+{
+    __shared__ double red[16];
     double s1 = 0, s2 = 0;
     for (int i = threadIdx.x; i < col.n; i += blockDim.x) {
         s1 += epart[i];
@@ -842,7 +851,7 @@ __global__ void __launch_bounds__(CG_T) k_cg_vel(SolB<F> S, const ColCtl *C, int
 }
 
 // colloid velocity (n_colloids_pbc.f90:312) and the step counter, after the last MD step of
-// a cycle (the other steps': k_cg_colloid_step)
+// a cycle (the other steps': k_cg_colloids_decide)
 __global__ void k_cg_colloid_vel(ColB c, ColCtl *C, int cfp, double dt)
 {
     if (C->error) return;
@@ -1034,8 +1043,9 @@ void colloids_graph_free(ColloidsGraph *g)
     delete g;
 }
 
-// the sort chain after the catch-up: apply_pbc, count, scan, place, gather, commit, list,
-// md slots, pos_old (ProgramSim::sort, build_list, copy_pos_old)
+// the sort chain (ProgramSim::sort, build_list, copy_pos_old): colloid apply_pbc and the
+// count reset; the key pass (catch-up, apply_pbc, keys, counts, records); the scan of the
+// counts; place; gather (with pos_old); commit and list reset; the lists; the md slots
 template <class F>
 static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int forced, hipStream_t st)
 {
@@ -1051,8 +1061,8 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
                                           (CgRec<R> *)g.rec, (R *)g.frec);
     k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_count, s.cell_start, nc,
                                                                                       g.tsum, C, forced, nullptr);
-    k_cg_scan_top<<<1, 1024, 0, st>>>(g.tsum, g.ntiles, C, forced, nullptr);
-    k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(s.cell_start, nc, g.tsum, C, forced);
+    k_cg_scan_finish<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_start, nc, g.tsum, C,
+                                                                                       forced, nullptr);
     k_cg_place<<<CG_GRID, CG_T, 0, st>>>(n, g.key, g.slot, s.cell_start, g.tmp, C, forced);
     k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, s.cell_start, s.cell_count, g.tmp,
                                              (const CgRec<R> *)g.rec, (const R *)g.frec);
@@ -1062,10 +1072,8 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
                                                          s.nlist);
     k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_count, g.mdoff, nc, g.tsum,
                                                                                       C, forced, s.is_md);
-    k_cg_scan_top<<<1, 1024, 0, st>>>(g.tsum, g.ntiles, C, forced, &C->n_md);
-    k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(g.mdoff, nc, g.tsum, C, forced);
-    k_cg_md_fill<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_start, s.cell_count, g.mdoff, g.mdslot, g.mdflag, C,
-                                           forced);
+    k_cg_md_finish<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(
+        nc, s.is_md, s.cell_start, s.cell_count, g.mdoff, g.tsum, g.mdslot, g.mdflag, C, forced);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
@@ -1079,15 +1087,13 @@ static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, bool pending, h
     const SolB<F> S = sol_bufs<F>(s);
     const ColB col = col_bufs(s);
     ColCtl *C = g.ctl;
-    const unsigned cgrid = blocks_for(s.nc, CG_T);
     if (pending) {
         k_cg_md<F, true><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-        k_cg_colloid_step<true><<<cgrid, CG_T, 0, st>>>(col, C, fp, s.dt, g.cpart);
+        k_cg_colloids_decide<true><<<1, CG_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_GRID, s.skin, s.box.max_v);
     } else {
         k_cg_md<F, false><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-        k_cg_colloid_step<false><<<cgrid, CG_T, 0, st>>>(col, C, fp, s.dt, g.cpart);
+        k_cg_colloids_decide<false><<<1, CG_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_GRID, s.skin, s.box.max_v);
     }
-    k_cg_decide<<<1, CG_T, 0, st>>>(C, g.part, CG_GRID, g.cpart, (int)cgrid, s.dt, s.skin, s.box.max_v);
     int rc = enqueue_sort_chain<F>(s, g, fp, 0, st);
     if (rc) return rc;
     // compute_force(_n2) into the planes zeroed above
@@ -1096,10 +1102,8 @@ static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, bool pending, h
     const dim3 fgrid((unsigned)g.fblocks, (unsigned)s.nc);
     k_cg_force_nl<F><<<fgrid, CG_T, 0, st>>>(S, col, C, np, np, s.list, s.nlist, s.nmax, E[0], E[1], E[2],
                                              make_lj(&s.lj_sc), g.fpart);
-    k_cg_force_sum<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fpart, g.fblocks, s.nlist, s.nmax, g.epart);
-    k_force_n2_blocks<CG_T><<<(unsigned)s.nc, CG_T, 0, st>>>(col.pos, col.species, col.n, col.force[np], E[0], E[1], E[2],
-                                                             make_lj(&s.lj_cc), g.fnew, g.epart + s.nc);
-    k_cg_force_fin<<<cgrid, CG_T, 0, st>>>(col, C, np, g.fnew, g.epart);
+    k_cg_colloid_forces<<<(unsigned)s.nc, CG_T, 0, st>>>(col, C, np, g.fpart, g.fblocks, s.nlist, s.nmax, E[0], E[1],
+                                                         E[2], make_lj(&s.lj_cc), g.epart);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
@@ -1112,6 +1116,7 @@ static int finish_steps(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_t st)
     const ColB col = col_bufs(s);
     k_cg_vel<F><<<CG_GRID, CG_T, 0, st>>>(S, g.ctl, fp, s.dt, g.mdslot);
     k_cg_colloid_vel<<<blocks_for(s.nc, CG_T), CG_T, 0, st>>>(col, g.ctl, fp, s.dt);
+    k_cg_energy<<<1, CG_T, 0, st>>>(col, g.ctl, g.epart);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
@@ -1168,7 +1173,6 @@ static int graph_setup(ProgramSim &s)
     int rc;
     g->ntiles = (int)((nc + CG_TILE - 1) / CG_TILE);
     g->fblocks = (int)blocks_for(s.nmax, CG_T);
-    g->cblocks = (int)blocks_for(s.nc, CG_T);
     RMPCD_CHECK(g->ntiles <= 1 << 20, "colloids graph: too many cells");
     if ((rc = cg_alloc(*g, (void **)&g->ctl, sizeof(ColCtl)))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->key, 4 * n))) return rc;
@@ -1179,10 +1183,8 @@ static int graph_setup(ProgramSim &s)
     if ((rc = cg_alloc(*g, (void **)&g->tsum, 4 * (size_t)(g->ntiles + 1)))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->mdoff, 4 * nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->part, 8 * CG_GRID))) return rc;
-    if ((rc = cg_alloc(*g, (void **)&g->cpart, 8 * (size_t)g->cblocks))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->fpart, 32 * (size_t)g->fblocks * s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->epart, 16 * (size_t)s.nc))) return rc;
-    if ((rc = cg_alloc(*g, (void **)&g->fnew, 24 * (size_t)s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->tpart, 16 * (size_t)blocks_for(nc, CG_CELL_T)))) return rc;
     {
         using R = typename F::real;
@@ -1214,10 +1216,8 @@ static int graph_setup(ProgramSim &s)
     ColCtl *C = g->ctl;
     k_cg_scan_tiles<<<(unsigned)std::min<int64_t>(g->ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_count, g->mdoff, nc,
                                                                                        g->tsum, C, 1, s.is_md);
-    k_cg_scan_top<<<1, 1024, 0, st>>>(g->tsum, g->ntiles, C, 1, &C->n_md);
-    k_cg_scan_add<<<CG_GRID, CG_T, 0, st>>>(g->mdoff, nc, g->tsum, C, 1);
-    k_cg_md_fill<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.cell_start, s.cell_count, g->mdoff, g->mdslot, g->mdflag, C,
-                                           1);
+    k_cg_md_finish<<<(unsigned)std::min<int64_t>(g->ntiles, CG_GRID), CG_T, 0, st>>>(
+        nc, s.is_md, s.cell_start, s.cell_count, g->mdoff, g->tsum, g->mdslot, g->mdflag, C, 1);
     RMPCD_LAUNCHED();
     RMPCD_HIP(hipStreamSynchronize(st));
     return RMPCD_OK;

@@ -9,15 +9,17 @@
 
 namespace rmpcd {
 
+// the pair sums of colloid i over a workgroup of T threads (every thread returns the tree
+// sums of the force and the energy)
 template <int T>
-__global__ void __launch_bounds__(T)
-    k_force_n2_blocks(const double *__restrict__ pos, const int32_t *__restrict__ species, int64_t n,
-                      const double *__restrict__ force_in, double L0, double L1, double L2, LJ lj,
-                      double *__restrict__ fnew, double *__restrict__ e_out)
+__device__ inline void n2_pair_sums(const double *__restrict__ pos, const int32_t *__restrict__ species, int64_t n,
+                                    int64_t i, double L0, double L1, double L2, const LJ &lj, double *red, double &f0,
+                                    double &f1, double &f2, double &e)
 {
-    __shared__ double red[16];
-    const int64_t i = blockIdx.x;
-    double f0 = 0, f1 = 0, f2 = 0, e = 0;
+    f0 = 0;
+    f1 = 0;
+    f2 = 0;
+    e = 0;
     for (int64_t j = threadIdx.x; j < n; j += T) {
         if (j == i) continue;
         const int64_t a = j < i ? j : i, bb = j < i ? i : j;
@@ -45,6 +47,18 @@ __global__ void __launch_bounds__(T)
     f1 = block_sum(f1, red);
     f2 = block_sum(f2, red);
     e = block_sum(e, red);
+}
+
+template <int T>
+__global__ void __launch_bounds__(T)
+    k_force_n2_blocks(const double *__restrict__ pos, const int32_t *__restrict__ species, int64_t n,
+                      const double *__restrict__ force_in, double L0, double L1, double L2, LJ lj,
+                      double *__restrict__ fnew, double *__restrict__ e_out)
+{
+    __shared__ double red[16];
+    const int64_t i = blockIdx.x;
+    double f0, f1, f2, e;
+    n2_pair_sums<T>(pos, species, n, i, L0, L1, L2, lj, red, f0, f1, f2, e);
     if (threadIdx.x == 0) {
         fnew[i] = force_in[i] + f0;
         fnew[n + i] = force_in[n + i] + f1;
