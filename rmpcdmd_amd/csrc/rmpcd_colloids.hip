@@ -905,13 +905,17 @@ __device__ inline int cg_stage(CgStage<F> &t, const SolB<F> &S, int b, const int
 }
 
 // simple_mpcd_step with the default angle (mpcd.f90:71-183), axes threefry((s, cell + 2^40 k))
-// of stream SRD
+// of stream SRD, then compute_temperature (mpcd.f90:342-407) of the collided cells: per
+// block, the sum of the cells' kinetic terms and of the cells counted (k_cg_temp_final adds
+// the partials in block order)
 template <class F>
 __global__ void __launch_bounds__(CG_CELL_T) k_cg_srd(SolB<F> S, const ColCtl *C, int ncells,
                                                       const int32_t *__restrict__ start,
-                                                      const int32_t *__restrict__ count, uint64_t seed)
+                                                      const int32_t *__restrict__ count, uint64_t seed,
+                                                      double *__restrict__ tpart)
 {
     __shared__ CgStage<F> t;
+    __shared__ double red[16];
     if (C->error) return;
     const int c0 = blockIdx.x * CG_CELL_T, c1 = min(c0 + CG_CELL_T, ncells);
     const int b = C->cur;
@@ -920,49 +924,27 @@ __global__ void __launch_bounds__(CG_CELL_T) k_cg_srd(SolB<F> S, const ColCtl *C
     const int tc = cg_stage<F>(t, S, b, start, count, c0, c1);
     typename F::real *v = S.vel[b];
     const int c = c0 + threadIdx.x;
+    double kin = 0, cnt = 0;
     if (c < c1 && count[c] > 1) {
         const int m = count[c];
         const int64_t s = start[c];
+        double k;
         if (tc > 0) {
             const int o = (int)(s - t.base);
             srd_cell(t.x + o, t.y + o, t.z + o, t.sp + o, m, c, ncells, nullptr, seed, cyc, 1.0, 0.0);
+            k = temperature_cell(t.x + o, t.y + o, t.z + o, t.sp + o, m);
         } else {
             srd_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m, c, ncells, nullptr, seed, cyc, 1.0, 0.0);
+            k = temperature_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m);
         }
+        cnt = 1;
+        kin = k < 0 ? 0 : k;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < tc; q += blockDim.x) {
         v[t.base + q] = t.x[q];
         v[n + t.base + q] = t.y[q];
         v[2 * n + t.base + q] = t.z[q];
-    }
-}
-
-// compute_temperature (mpcd.f90:342-407): per block, the sum of the cells' kinetic terms
-// and of the cells counted; one block adds the partials in block order
-template <class F>
-__global__ void __launch_bounds__(CG_CELL_T) k_cg_temp(SolB<F> S, const ColCtl *C, int ncells,
-                                                       const int32_t *__restrict__ start,
-                                                       const int32_t *__restrict__ count, double *__restrict__ tpart)
-{
-    __shared__ CgStage<F> t;
-    __shared__ double red[16];
-    if (C->error) return;
-    const int c0 = blockIdx.x * CG_CELL_T, c1 = min(c0 + CG_CELL_T, ncells);
-    const int b = C->cur;
-    const int64_t n = S.n;
-    const int tc = cg_stage<F>(t, S, b, start, count, c0, c1);
-    double kin = 0, cnt = 0;
-    const int c = c0 + threadIdx.x;
-    if (c < c1 && count[c] > 1) {
-        const int m = count[c];
-        const int64_t s = start[c];
-        const typename F::real *v = S.vel[b];
-        cnt = 1;
-        const double k = tc > 0 ? temperature_cell(t.x + (s - t.base), t.y + (s - t.base), t.z + (s - t.base),
-                                                   t.sp + (s - t.base), m)
-                                : temperature_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m);
-        kin = k < 0 ? 0 : k;
     }
     const double bk = block_sum(kin, red);
     const double bc = block_sum(cnt, red);
@@ -1132,8 +1114,7 @@ static int enqueue_collision(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_
     if (rc) return rc;
     const int nc = s.box.ncells;
     const unsigned tb = blocks_for(nc, CG_CELL_T);
-    k_cg_srd<F><<<tb, CG_CELL_T, 0, st>>>(S, C, nc, s.cell_start, s.cell_count, s.seed);
-    k_cg_temp<F><<<tb, CG_CELL_T, 0, st>>>(S, C, nc, s.cell_start, s.cell_count, g.tpart);
+    k_cg_srd<F><<<tb, CG_CELL_T, 0, st>>>(S, C, nc, s.cell_start, s.cell_count, s.seed, g.tpart);
     k_cg_temp_final<<<1, CG_CELL_T, 0, st>>>(C, g.tpart, (int)tb);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
