@@ -32,6 +32,7 @@ constexpr int CG_SCAN_I = 8;     // cells per thread of a scan tile
 constexpr int CG_TILE = CG_T * CG_SCAN_I;
 constexpr int CG_CELL_T = 128;   // cells per block of the collision and temperature passes
 constexpr int CG_NL_T = 1024;    // threads (and stencil cells at a time) of a colloid's list build
+constexpr int CG_DEC_T = 1024;   // threads of the colloid step and skin test (one workgroup)
 
 // device control of the loop
 struct ColCtl {
@@ -198,7 +199,7 @@ __global__ void __launch_bounds__(CG_T) k_cg_md(SolB<F> S, const ColCtl *C, int 
 // over the md cells (part, from k_cg_md), raised to max_v k dt; co_max over the colloids;
 // k = steps since the last sort.
 template <bool PENDING>
-__global__ void __launch_bounds__(CG_T) k_cg_colloids_decide(ColB c, ColCtl *C, int fp, double dt,
+__global__ void __launch_bounds__(CG_DEC_T) k_cg_colloids_decide(ColB c, ColCtl *C, int fp, double dt,
                                                              const double *__restrict__ part, int np, double skin,
                                                              double max_v)
 {
@@ -1071,10 +1072,10 @@ static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, bool pending, h
     ColCtl *C = g.ctl;
     if (pending) {
         k_cg_md<F, true><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-        k_cg_colloids_decide<true><<<1, CG_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_GRID, s.skin, s.box.max_v);
+        k_cg_colloids_decide<true><<<1, CG_DEC_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_GRID, s.skin, s.box.max_v);
     } else {
         k_cg_md<F, false><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-        k_cg_colloids_decide<false><<<1, CG_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_GRID, s.skin, s.box.max_v);
+        k_cg_colloids_decide<false><<<1, CG_DEC_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_GRID, s.skin, s.box.max_v);
     }
     int rc = enqueue_sort_chain<F>(s, g, fp, 0, st);
     if (rc) return rc;
