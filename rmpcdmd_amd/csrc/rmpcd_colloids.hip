@@ -758,13 +758,17 @@ __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const
             }
         }
     }
-    const double b0 = block_sum(f0, red), b1 = block_sum(f1, red), b2 = block_sum(f2, red), be = block_sum(e, red);
-    if (threadIdx.x == 0) {
-        double *pp = fpart + ((int64_t)i * gridDim.x + blockIdx.x) * 4;
-        pp[0] = b0;
-        pp[1] = b1;
-        pp[2] = b2;
-        pp[3] = be;
+    // the four block sums of block_sum (wave sums, then the waves in order) with one barrier
+    static_assert(4 * (CG_T / 64) <= 16, "four sums per wave in red");
+    const double w4[4] = {wave_sum(f0), wave_sum(f1), wave_sum(f2), wave_sum(e)};
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) red[4 * wid + q] = w4[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0;
+        for (int w = 0; w < CG_T / 64; ++w) t += red[4 * w + threadIdx.x];
+        fpart[((int64_t)i * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = t;
     }
     (void)cfp;
 }
