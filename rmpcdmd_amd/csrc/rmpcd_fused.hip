@@ -792,13 +792,36 @@ __device__ inline void key_pass(const FB<F> &B, const FP &P, int R)
         for (int cc = 0; cc < 2; ++cc)
             for (int k = 0; k < 3; ++k) ccol[cc][k] = dfr ? B.cpos[6 * r + 2 * k + cc] : 0.0;
         const double cand_r2 = (sqrt(P.radius_sq) + 1e-3) * (sqrt(P.radius_sq) + 1e-3);
+        // an extra sort loads the positions and membership bytes of the thread's KPT
+        // particles before it stores anything, so that their loads are in flight together
+        Rl xs[KPT][3];
+        int mbs[KPT];
+        if (!forced)
+            for (int v = 0; v < KPT; ++v) {
+                const int64_t i = c0 + v * FT + threadIdx.x;
+                if (i < n) {
+                    xs[v][0] = pos[i];
+                    xs[v][1] = pos[n + i];
+                    xs[v][2] = pos[2 * n + i];
+                    mbs[v] = memb[i];
+                }
+            }
         for (int v = 0; v < KPT; ++v) {
             const int64_t i = c0 + v * FT + threadIdx.x;
             keys[v] = -1;
             ent[v] = 0;
             loc[v] = 0;
             if (i < n) {
-                Rl x[3] = {pos[i], pos[n + i], pos[2 * n + i]};
+                Rl x[3];
+                if (forced) {
+                    x[0] = pos[i];
+                    x[1] = pos[n + i];
+                    x[2] = pos[2 * n + i];
+                } else {
+                    x[0] = xs[v][0];
+                    x[1] = xs[v][1];
+                    x[2] = xs[v][2];
+                }
                 int jmp[3];
                 if (forced) {
                     const Rl vl[3] = {vel[i], vel[n + i], vel[2 * n + i]};
@@ -841,7 +864,7 @@ __device__ inline void key_pass(const FB<F> &B, const FP &P, int R)
                 } else {
                     // extra sort: the particle stays in place; apply_pbc and pos_old = pos
                     // (single_dimer_pbc.f90:325-335) are done here
-                    const int mb = memb[i];
+                    const int mb = mbs[v];
                     bool write = false;
                     if (lazy_k > 0 && !mb) {
                         for (int k = 0; k < 3; ++k) {
