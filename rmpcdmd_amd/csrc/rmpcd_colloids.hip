@@ -33,6 +33,7 @@ constexpr int CG_TILE = CG_T * CG_SCAN_I;
 constexpr int CG_CELL_T = 128;   // cells per block of the collision and temperature passes
 constexpr int CG_NL_T = 1024;    // threads (and stencil cells at a time) of a colloid's list build
 constexpr int CG_DEC_T = 1024;   // threads of the colloid step and skin test (one workgroup)
+constexpr int CG_GC = 96, CG_GC_TILE = 12 * CG_GC;  // cells per gather unit, its LDS slots
 
 // device control of the loop
 struct ColCtl {
@@ -87,9 +88,8 @@ __device__ inline int64_t cg_stride() { return (int64_t)gridDim.x * blockDim.x; 
 struct CgWalk {
     int64_t t, t1, step;
 };
-__device__ inline CgWalk cg_walk(int64_t n)
+__device__ inline CgWalk cg_walk_units(int64_t tiles)
 {
-    const int64_t tiles = (n + CG_T - 1) / CG_T;
     const int G = (int)gridDim.x;
     CgWalk w;
     if (G % 8 == 0) {
@@ -104,6 +104,7 @@ __device__ inline CgWalk cg_walk(int64_t n)
     }
     return w;
 }
+__device__ inline CgWalk cg_walk(int64_t n) { return cg_walk_units((n + CG_T - 1) / CG_T); }
 
 __device__ inline int cg_block_scan(int v, int *lds, int *total)
 {
@@ -512,17 +513,20 @@ __global__ void __launch_bounds__(CG_T) k_cg_place(int64_t n, const int32_t *__r
 
 // The stable gather into the other buffer (particle_system.f90:304-355): the particle of
 // each slot goes to cell_start + its rank by storage index among the cell's arrivals,
-// which is the reference's single-thread order.  Reads only the records (never the
-// source arrays), so it also writes pos_old = pos into the source buffer's positions
-// (after the sort they are the other buffer's).  Only the particles with the MD or
-// PAST_MD bit carry forces (compute_force sets MD on every particle it gives a force,
-// the roll keeps PAST_MD for one more step, and force_zero clears both over the md
+// which is the reference's single-thread order.  A unit is a tile of CG_GC cells: their
+// starts and counts and the storage indices of their slots are loaded into LDS with
+// coalesced reads, so that each slot makes one random access (its record) and ranks in
+// LDS (tiles of more than CG_GC_TILE particles rank from memory).  Reads only the records
+// (never the source arrays), so it also writes pos_old = pos into the source buffer's
+// positions (after the sort they are the other buffer's).  Only the particles with the MD
+// or PAST_MD bit carry forces (compute_force sets MD on every particle it gives a force,
+// the roll keeps PAST_MD for one more step, and the force reset clears both over the md
 // slots); their forces come from the side records.  The destination's force planes are
 // all zero (the key pass of the previous sort zeroed what it staged, the set-up cleared
 // them before the first), so the others' forces are not written.  id_to_idx is not kept
 // (nothing in the production loop reads it).
 template <class F>
-__global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, int fp, int forced,
+__global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, int fp, int forced, int64_t nc,
                                                     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
                                                     const int32_t *__restrict__ tmp,
                                                     const CgRec<typename F::real> *__restrict__ rec,
@@ -530,38 +534,63 @@ __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, 
 {
     using R = typename F::real;
     using Sm = typename F::small;
+    __shared__ int32_t lref[CG_GC_TILE];
+    __shared__ int32_t lstart[CG_GC], lcnt[CG_GC];
     if (cg_off(C, forced)) return;
     const int b = C->cur, o = 1 - b;
     const int64_t n = S.n;
     R *dpos = S.pos[o], *dvel = S.vel[o], *dold = S.pos[b];
     R *df = S.force[2 * o + fp], *dfo = S.force[2 * o + 1 - fp];
-    for (CgWalk w = cg_walk(n); w.t < w.t1; w.t += w.step) {
-        const int64_t j = w.t * CG_T + threadIdx.x;
-        if (j >= n) continue;
-        const int32_t v = tmp[j];
-        const CgRec<R> r = rec[v];
-        const int s = start[r.key], m = count[r.key];
-        int rank = 0;
-        for (int q = s; q < s + m; ++q) rank += tmp[q] < v ? 1 : 0;
-        const int64_t d = s + rank;
-        const int fl = r.sf >> 16;
-        const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            dpos[k * n + d] = r.p[k];
-            dold[k * n + d] = r.p[k];
-            dvel[k * n + d] = r.v[k];
-            S.image[o][k * n + d] = r.im[k];
+    int32_t *dim = S.image[o], *did = S.id[o];
+    Sm *dsp = S.species[o], *dfl = S.flags[o];
+    for (CgWalk w = cg_walk_units((nc + CG_GC - 1) / CG_GC); w.t < w.t1; w.t += w.step) {
+        const int64_t c0 = w.t * CG_GC;
+        const int ncl = (int)(c0 + CG_GC < nc ? CG_GC : nc - c0);
+        __syncthreads();  // the previous unit's LDS reads are done
+        for (int t = threadIdx.x; t < ncl; t += CG_T) {
+            lstart[t] = start[c0 + t];
+            lcnt[t] = count[c0 + t];
         }
-        if (has_f)
+        __syncthreads();
+        const int64_t sa = lstart[0], sb = (int64_t)lstart[ncl - 1] + lcnt[ncl - 1];
+        const int tcount = (int)(sb - sa);
+        const bool fits = tcount <= CG_GC_TILE;
+        if (fits) {
+            for (int q = threadIdx.x; q < tcount; q += CG_T) lref[q] = tmp[sa + q];
+            __syncthreads();
+        }
+        for (int q = threadIdx.x; q < tcount; q += CG_T) {
+            const int32_t v = fits ? lref[q] : tmp[sa + q];
+            const CgRec<R> r = rec[v];
+            const int cl = r.key - (int)c0;
+            const int s0 = lstart[cl], m0 = lcnt[cl];
+            int rank = 0;
+            if (fits) {
+                const int e0 = s0 - (int)sa;
+                for (int e = e0; e < e0 + m0; ++e) rank += lref[e] < v ? 1 : 0;
+            } else {
+                for (int e = s0; e < s0 + m0; ++e) rank += tmp[e] < v ? 1 : 0;
+            }
+            const int64_t d = s0 + rank;
+            const int fl = r.sf >> 16;
+            const bool has_f = (fl & (MD_MASK | PAST_MD_MASK)) != 0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                df[k * n + d] = frec[6 * (int64_t)v + k];
-                dfo[k * n + d] = frec[6 * (int64_t)v + 3 + k];
+                dpos[k * n + d] = r.p[k];
+                dold[k * n + d] = r.p[k];
+                dvel[k * n + d] = r.v[k];
+                dim[k * n + d] = r.im[k];
             }
-        S.id[o][d] = r.id;
-        S.species[o][d] = (Sm)(r.sf & 0xffff);
-        S.flags[o][d] = (Sm)fl;
+            if (has_f)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    df[k * n + d] = frec[6 * (int64_t)v + k];
+                    dfo[k * n + d] = frec[6 * (int64_t)v + 3 + k];
+                }
+            did[d] = r.id;
+            dsp[d] = (Sm)(r.sf & 0xffff);
+            dfl[d] = (Sm)fl;
+        }
     }
 }
 
@@ -1051,7 +1080,7 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
     k_cg_scan_finish<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_start, nc, g.tsum, C,
                                                                                        forced, nullptr);
     k_cg_place<<<CG_GRID, CG_T, 0, st>>>(n, g.key, g.slot, s.cell_start, g.tmp, C, forced);
-    k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, s.cell_start, s.cell_count, g.tmp,
+    k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, nc, s.cell_start, s.cell_count, g.tmp,
                                              (const CgRec<R> *)g.rec, (const R *)g.frec);
     k_cg_list_init<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.is_reac, C, forced);
     k_cg_nl_build<F><<<(unsigned)s.nc, CG_NL_T, 0, st>>>(S, col, C, forced, b, s.cell_start, s.cell_count, s.stencil,
