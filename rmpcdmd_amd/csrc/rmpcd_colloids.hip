@@ -511,6 +511,22 @@ __global__ void __launch_bounds__(CG_T) k_cg_place(int64_t n, const int32_t *__r
     }
 }
 
+// LDS of a gather unit: the storage indices of its slots and its cells' starts and counts;
+// a collision sort's gather adds the image of the velocities and species
+template <class F, bool COLL>
+struct CgGatherLds {
+    int32_t lref[CG_GC_TILE];
+    int32_t lstart[CG_GC], lcnt[CG_GC];
+};
+template <class F>
+struct CgGatherLds<F, true> {
+    int32_t lref[CG_GC_TILE];
+    int32_t lstart[CG_GC], lcnt[CG_GC];
+    typename F::real lx[CG_GC_TILE], ly[CG_GC_TILE], lz[CG_GC_TILE];
+    typename F::small lsp[CG_GC_TILE];
+    double red[16];
+};
+
 // The stable gather into the other buffer (particle_system.f90:304-355): the particle of
 // each slot goes to cell_start + its rank by storage index among the cell's arrivals,
 // which is the reference's single-thread order.  A unit is a tile of CG_GC cells: their
@@ -525,20 +541,27 @@ __global__ void __launch_bounds__(CG_T) k_cg_place(int64_t n, const int32_t *__r
 // all zero (the key pass of the previous sort zeroed what it staged, the set-up cleared
 // them before the first), so the others' forces are not written.  id_to_idx is not kept
 // (nothing in the production loop reads it).
-template <class F>
+// With COLL (the collision sort) the velocities and species go to an LDS image of the tile,
+// and the collision follows on it: simple_mpcd_step with the default angle
+// (mpcd.f90:71-183; axes threefry((s, cell + 2^40 k)) of stream SRD) and
+// compute_temperature (mpcd.f90:342-407) of the collided cells, per unit the sum of the
+// cells' kinetic terms and of the cells counted (k_cg_temp_final adds the units' partials
+// in unit order).  The list build that follows in the sort chain reads positions only.
+template <class F, bool COLL>
 __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, int fp, int forced, int64_t nc,
                                                     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
                                                     const int32_t *__restrict__ tmp,
                                                     const CgRec<typename F::real> *__restrict__ rec,
-                                                    const typename F::real *__restrict__ frec)
+                                                    const typename F::real *__restrict__ frec, uint64_t seed,
+                                                    double *__restrict__ tpart)
 {
     using R = typename F::real;
     using Sm = typename F::small;
-    __shared__ int32_t lref[CG_GC_TILE];
-    __shared__ int32_t lstart[CG_GC], lcnt[CG_GC];
+    __shared__ CgGatherLds<F, COLL> L;
     if (cg_off(C, forced)) return;
     const int b = C->cur, o = 1 - b;
     const int64_t n = S.n;
+    const uint64_t cyc = (uint64_t)C->cycles;
     R *dpos = S.pos[o], *dvel = S.vel[o], *dold = S.pos[b];
     R *df = S.force[2 * o + fp], *dfo = S.force[2 * o + 1 - fp];
     int32_t *dim = S.image[o], *did = S.id[o];
@@ -548,26 +571,26 @@ __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, 
         const int ncl = (int)(c0 + CG_GC < nc ? CG_GC : nc - c0);
         __syncthreads();  // the previous unit's LDS reads are done
         for (int t = threadIdx.x; t < ncl; t += CG_T) {
-            lstart[t] = start[c0 + t];
-            lcnt[t] = count[c0 + t];
+            L.lstart[t] = start[c0 + t];
+            L.lcnt[t] = count[c0 + t];
         }
         __syncthreads();
-        const int64_t sa = lstart[0], sb = (int64_t)lstart[ncl - 1] + lcnt[ncl - 1];
+        const int64_t sa = L.lstart[0], sb = (int64_t)L.lstart[ncl - 1] + L.lcnt[ncl - 1];
         const int tcount = (int)(sb - sa);
         const bool fits = tcount <= CG_GC_TILE;
         if (fits) {
-            for (int q = threadIdx.x; q < tcount; q += CG_T) lref[q] = tmp[sa + q];
+            for (int q = threadIdx.x; q < tcount; q += CG_T) L.lref[q] = tmp[sa + q];
             __syncthreads();
         }
         for (int q = threadIdx.x; q < tcount; q += CG_T) {
-            const int32_t v = fits ? lref[q] : tmp[sa + q];
+            const int32_t v = fits ? L.lref[q] : tmp[sa + q];
             const CgRec<R> r = rec[v];
             const int cl = r.key - (int)c0;
-            const int s0 = lstart[cl], m0 = lcnt[cl];
+            const int s0 = L.lstart[cl], m0 = L.lcnt[cl];
             int rank = 0;
             if (fits) {
                 const int e0 = s0 - (int)sa;
-                for (int e = e0; e < e0 + m0; ++e) rank += lref[e] < v ? 1 : 0;
+                for (int e = e0; e < e0 + m0; ++e) rank += L.lref[e] < v ? 1 : 0;
             } else {
                 for (int e = s0; e < s0 + m0; ++e) rank += tmp[e] < v ? 1 : 0;
             }
@@ -578,7 +601,6 @@ __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, 
             for (int k = 0; k < 3; ++k) {
                 dpos[k * n + d] = r.p[k];
                 dold[k * n + d] = r.p[k];
-                dvel[k * n + d] = r.v[k];
                 dim[k * n + d] = r.im[k];
             }
             if (has_f)
@@ -588,8 +610,58 @@ __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, 
                     dfo[k * n + d] = frec[6 * (int64_t)v + 3 + k];
                 }
             did[d] = r.id;
-            dsp[d] = (Sm)(r.sf & 0xffff);
             dfl[d] = (Sm)fl;
+            if constexpr (COLL) {
+                if (fits) {
+                    L.lx[d - sa] = r.v[0];
+                    L.ly[d - sa] = r.v[1];
+                    L.lz[d - sa] = r.v[2];
+                    L.lsp[d - sa] = (Sm)(r.sf & 0xffff);
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dvel[k * n + d] = r.v[k];
+            dsp[d] = (Sm)(r.sf & 0xffff);
+        }
+        if constexpr (COLL) {
+            // the collision of the tile's cells, one thread per cell (the tile's image, or
+            // memory for a tile that does not fit), then the image written back
+            __syncthreads();
+            double kin = 0, cnt = 0;
+            for (int t = threadIdx.x; t < ncl; t += CG_T) {
+                const int m = L.lcnt[t];
+                const int64_t sc = L.lstart[t];
+                if (m > 1) {
+                    const int c = (int)(c0 + t);
+                    double k;
+                    if (fits) {
+                        const int e = (int)(sc - sa);
+                        srd_cell(L.lx + e, L.ly + e, L.lz + e, L.lsp + e, m, c, (int)nc, nullptr, seed, cyc, 1.0, 0.0);
+                        k = temperature_cell(L.lx + e, L.ly + e, L.lz + e, L.lsp + e, m);
+                    } else {
+                        srd_cell(dvel + sc, dvel + n + sc, dvel + 2 * n + sc, dsp + sc, m, c, (int)nc, nullptr, seed,
+                                 cyc, 1.0, 0.0);
+                        k = temperature_cell(dvel + sc, dvel + n + sc, dvel + 2 * n + sc, dsp + sc, m);
+                    }
+                    cnt += 1;
+                    kin += k < 0 ? 0 : k;
+                }
+            }
+            __syncthreads();
+            if (fits)
+                for (int q = threadIdx.x; q < tcount; q += CG_T) {
+                    dvel[sa + q] = L.lx[q];
+                    dvel[n + sa + q] = L.ly[q];
+                    dvel[2 * n + sa + q] = L.lz[q];
+                    dsp[sa + q] = L.lsp[q];
+                }
+            const double bk = block_sum(kin, L.red);
+            const double bc = block_sum(cnt, L.red);
+            if (threadIdx.x == 0) {
+                tpart[2 * w.t] = bk;
+                tpart[2 * w.t + 1] = bc;
+            }
         }
     }
 }
@@ -900,94 +972,7 @@ __global__ void k_cg_colloid_vel(ColB c, ColCtl *C, int cfp, double dt)
     if (i == 0) C->md_steps += 1;
 }
 
-// ---- collision ------------------------------------------------------------------------
-// A block owns CG_CELL_T consecutive cells: their particles (a contiguous slot range) are
-// staged in LDS with coalesced loads when they fit CG_STAGE, and each thread works on one
-// cell there; a block whose cells hold more works from global memory.
-constexpr int CG_STAGE = 2048;
-
-template <class F>
-struct CgStage {
-    typename F::real x[CG_STAGE], y[CG_STAGE], z[CG_STAGE];
-    typename F::small sp[CG_STAGE];
-    int64_t base, end;
-};
-
-// stage the velocities and species of cells [c0, c1) of buffer b; returns the staged count
-// (0 when the range does not fit)
-template <class F>
-__device__ inline int cg_stage(CgStage<F> &t, const SolB<F> &S, int b, const int32_t *start, const int32_t *count,
-                               int c0, int c1)
-{
-    if (threadIdx.x == 0) {
-        t.base = start[c0];
-        t.end = (int64_t)start[c1 - 1] + count[c1 - 1];
-    }
-    __syncthreads();
-    const int64_t base = t.base, n = S.n;
-    const int tc = t.end - base <= CG_STAGE ? (int)(t.end - base) : 0;
-    const typename F::real *v = S.vel[b];
-    const typename F::small *sp = S.species[b];
-    for (int q = threadIdx.x; q < tc; q += blockDim.x) {
-        t.x[q] = v[base + q];
-        t.y[q] = v[n + base + q];
-        t.z[q] = v[2 * n + base + q];
-        t.sp[q] = sp[base + q];
-    }
-    __syncthreads();
-    return tc;
-}
-
-// simple_mpcd_step with the default angle (mpcd.f90:71-183), axes threefry((s, cell + 2^40 k))
-// of stream SRD, then compute_temperature (mpcd.f90:342-407) of the collided cells: per
-// block, the sum of the cells' kinetic terms and of the cells counted (k_cg_temp_final adds
-// the partials in block order)
-template <class F>
-__global__ void __launch_bounds__(CG_CELL_T) k_cg_srd(SolB<F> S, const ColCtl *C, int ncells,
-                                                      const int32_t *__restrict__ start,
-                                                      const int32_t *__restrict__ count, uint64_t seed,
-                                                      double *__restrict__ tpart)
-{
-    __shared__ CgStage<F> t;
-    __shared__ double red[16];
-    if (C->error) return;
-    const int c0 = blockIdx.x * CG_CELL_T, c1 = min(c0 + CG_CELL_T, ncells);
-    const int b = C->cur;
-    const int64_t n = S.n;
-    const uint64_t cyc = (uint64_t)C->cycles;
-    const int tc = cg_stage<F>(t, S, b, start, count, c0, c1);
-    typename F::real *v = S.vel[b];
-    const int c = c0 + threadIdx.x;
-    double kin = 0, cnt = 0;
-    if (c < c1 && count[c] > 1) {
-        const int m = count[c];
-        const int64_t s = start[c];
-        double k;
-        if (tc > 0) {
-            const int o = (int)(s - t.base);
-            srd_cell(t.x + o, t.y + o, t.z + o, t.sp + o, m, c, ncells, nullptr, seed, cyc, 1.0, 0.0);
-            k = temperature_cell(t.x + o, t.y + o, t.z + o, t.sp + o, m);
-        } else {
-            srd_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m, c, ncells, nullptr, seed, cyc, 1.0, 0.0);
-            k = temperature_cell(v + s, v + n + s, v + 2 * n + s, S.species[b] + s, m);
-        }
-        cnt = 1;
-        kin = k < 0 ? 0 : k;
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < tc; q += blockDim.x) {
-        v[t.base + q] = t.x[q];
-        v[n + t.base + q] = t.y[q];
-        v[2 * n + t.base + q] = t.z[q];
-    }
-    const double bk = block_sum(kin, red);
-    const double bc = block_sum(cnt, red);
-    if (threadIdx.x == 0) {
-        tpart[2 * blockIdx.x] = bk;
-        tpart[2 * blockIdx.x + 1] = bc;
-    }
-}
-
+// ---- collision (in the collision sort's gather) --------------------------------------
 __global__ void __launch_bounds__(CG_CELL_T) k_cg_temp_final(ColCtl *C, const double *__restrict__ tpart, int nb)
 {
     __shared__ double red[16];
@@ -1080,8 +1065,12 @@ static int enqueue_sort_chain(ProgramSim &s, ColloidsGraph &g, int fp, int force
     k_cg_scan_finish<<<(unsigned)std::min<int64_t>(g.ntiles, CG_GRID), CG_T, 0, st>>>(s.cell_start, nc, g.tsum, C,
                                                                                        forced, nullptr);
     k_cg_place<<<CG_GRID, CG_T, 0, st>>>(n, g.key, g.slot, s.cell_start, g.tmp, C, forced);
-    k_cg_gather<F><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, nc, s.cell_start, s.cell_count, g.tmp,
-                                             (const CgRec<R> *)g.rec, (const R *)g.frec);
+    if (forced)
+        k_cg_gather<F, true><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, nc, s.cell_start, s.cell_count, g.tmp,
+                                                       (const CgRec<R> *)g.rec, (const R *)g.frec, s.seed, g.tpart);
+    else
+        k_cg_gather<F, false><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, forced, nc, s.cell_start, s.cell_count, g.tmp,
+                                                        (const CgRec<R> *)g.rec, (const R *)g.frec, s.seed, nullptr);
     k_cg_list_init<<<CG_GRID, CG_T, 0, st>>>(nc, s.is_md, s.is_reac, C, forced);
     k_cg_nl_build<F><<<(unsigned)s.nc, CG_NL_T, 0, st>>>(S, col, C, forced, b, s.cell_start, s.cell_count, s.stencil,
                                                          s.n_stencil, s.radius * s.radius, s.is_md, s.nmax, s.list,
@@ -1141,15 +1130,13 @@ static int finish_steps(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_t st)
 template <class F>
 static int enqueue_collision(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_t st)
 {
-    const SolB<F> S = sol_bufs<F>(s);
     ColCtl *C = g.ctl;
     k_cg_shift<<<1, 1, 0, st>>>(C, s.seed, s.dt);
     int rc = enqueue_sort_chain<F>(s, g, fp, 1, st);
     if (rc) return rc;
-    const int nc = s.box.ncells;
-    const unsigned tb = blocks_for(nc, CG_CELL_T);
-    k_cg_srd<F><<<tb, CG_CELL_T, 0, st>>>(S, C, nc, s.cell_start, s.cell_count, s.seed, g.tpart);
-    k_cg_temp_final<<<1, CG_CELL_T, 0, st>>>(C, g.tpart, (int)tb);
+    // the collision ran in the sort's gather; the temperature of its units' partials
+    const int64_t units = (s.box.ncells + CG_GC - 1) / CG_GC;
+    k_cg_temp_final<<<1, CG_CELL_T, 0, st>>>(C, g.tpart, (int)units);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
@@ -1200,7 +1187,7 @@ static int graph_setup(ProgramSim &s)
     if ((rc = cg_alloc(*g, (void **)&g->part, 8 * CG_GRID))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->fpart, 32 * (size_t)g->fblocks * s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->epart, 16 * (size_t)s.nc))) return rc;
-    if ((rc = cg_alloc(*g, (void **)&g->tpart, 16 * (size_t)blocks_for(nc, CG_CELL_T)))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->tpart, 16 * (size_t)((nc + CG_GC - 1) / CG_GC)))) return rc;
     {
         using R = typename F::real;
         if ((rc = cg_alloc(*g, &g->rec, sizeof(CgRec<R>) * n))) return rc;
