@@ -66,8 +66,9 @@ def test_graph_equals_reference_sequence(gpu, form, n_md, skin):
 
 @pytest.mark.parametrize("rho", [12, 16])
 def test_graph_dense_cells(gpu, rho):
-    """denser cells: the gather works on tiles of 96 cells with 1152 slots in LDS; at
-    rho = 12 about half of the tiles hold more and rank from memory, at rho = 16 all do"""
+    """denser cells: the gather works on tiles of 128 cells with 1536 slots in LDS; at
+    rho = 12 about half of the tiles hold more and rank (and collide) from memory, at
+    rho = 16 all do"""
     kw = dict(L=(20, 20, 20), rho=rho, n_md=10, tau=0.2, sigma=1.5, n_colloids=12, T=1.0, skin=0.3)
     ref = Program(PROG_COLLOIDS, form=capi.FP64, seed=41, **kw)
     gra = Program(PROG_COLLOIDS, form=capi.FP64, seed=41, graph=1, **kw)
