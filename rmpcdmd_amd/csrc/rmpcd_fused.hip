@@ -2140,7 +2140,13 @@ static int stream_blocks(const DimerSim &s)
 {
     // lazy mode: the stream role only measures displacements, and only for an exact test
     const int cap = s.cfg.lazy ? std::min(s.stream_blocks, LAZY_STREAM_BLOCKS) : s.stream_blocks;
-    return (int)std::min<int64_t>((s.nmax_rep + FT - 1) / FT, cap);
+    // with 16-byte accesses in every replica a thread takes 16 / sizeof(real) particles of
+    // each plane, and the grid needs no more blocks than that covers (blocks beyond it
+    // would only be dispatched and return: 0.08 ms per cycle of the 16 x L=64 batch)
+    bool vec = !s.cfg.lazy;
+    for (const Replica &rp : s.rep) vec = vec && rp.n % 4 == 0 && rp.p_off % 4 == 0;
+    const int64_t per = (int64_t)FT * (vec ? 16 / (s.cfg.form == RMPCD_FP64 ? 8 : 4) : 1);
+    return (int)std::min<int64_t>((s.nmax_rep + per - 1) / per, cap);
 }
 
 // (each enqueue_* works on the R replicas of b's group)
