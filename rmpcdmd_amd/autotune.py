@@ -3,7 +3,8 @@
 For a workload (box size, replica count or the mixed-size batch, form) it times a few
 collision cycles for each candidate launch shape and keeps the fastest, one parameter
 at a time (coordinate descent, two passes):
-    stream_blocks  stream-role workgroups per replica of the step kernel
+    stream_blocks  stream-role workgroups per replica of the step kernel (a cap: the grid
+                   takes no more than its 16-byte accesses need)
     sort_blocks    workgroups of the key, place and gather passes of the collision sort
     extra_sort_blocks  the same for the sorts that may follow each MD step
     sort_order     the sort passes' units: a contiguous range per workgroup (1) or
@@ -25,7 +26,7 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
-CANDIDATES = {"stream_blocks": [2048, 4096, 8192, 16384], "sort_blocks": [2048, 4096, 8192, 16384, 32768],
+CANDIDATES = {"stream_blocks": [1024, 2048, 2560, 4096, 5120, 8192, 16384], "sort_blocks": [2048, 4096, 8192, 16384, 32768],
               "extra_sort_blocks": [256, 512, 1024, 2048, 4096], "sort_order": [1, 2], "groups": [1, 2, 3, 4],
               "collide_cells": [64, 128, 256]}
 
