@@ -144,6 +144,24 @@ def test_fused_replicas_of_uneven_size(gpu):
     fus.close()
 
 
+@pytest.mark.parametrize("form", [capi.FP64, capi.FP32])
+def test_fused_streaming_without_vectors(gpu, form):
+    """a replica whose particle count is not a multiple of 4 (13^3 cells at rho = 10) streams
+    one particle per thread and sizes the streaming grid for that; the replica before it
+    keeps its 16-byte accesses"""
+    Ls = [(12, 12, 12), (13, 13, 13)]
+    kw = dict(n_md=10, tau=0.1, form=form, seed=9, replica_L=Ls)
+    ref = SingleDimer(mode=capi.SIM_REFERENCE, **kw)
+    fus = SingleDimer(mode=capi.SIM_FUSED, **kw)
+    ref.run(2)
+    fus.run(2)
+    for r in range(len(Ls)):
+        compare(ref.download(r), fus.download(r), "replica %d" % r, 1e-12 if form == capi.FP64 else 1e-5)
+    assert fus.download(1)["pos"].shape[0] % 4 != 0
+    ref.close()
+    fus.close()
+
+
 def test_dimer_invariants(gpu):
     sim = SingleDimer(L=(20, 20, 20), form=capi.FP64, mode=capi.SIM_FUSED, n_md=25, tau=0.25, seed=3)
     st = sim.state()[4]
