@@ -31,7 +31,7 @@ constexpr int CG_GRID = 2048;    // blocks of the grid-stride passes over partic
 constexpr int CG_SCAN_I = 8;     // cells per thread of a scan tile
 constexpr int CG_TILE = CG_T * CG_SCAN_I;
 constexpr int CG_CELL_T = 128;   // cells per block of the collision and temperature passes
-constexpr int CG_NL_T = 1024;    // threads (and stencil cells at a time) of a colloid's list build
+constexpr int CG_NL_T = 256;     // threads (and stencil cells at a time) of a colloid's list build
 constexpr int CG_DEC_T = 1024;   // threads of the colloid step and skin test (one workgroup)
 constexpr int CG_GC = 128, CG_GC_TILE = 12 * CG_GC;  // cells per gather unit, its LDS slots
 
