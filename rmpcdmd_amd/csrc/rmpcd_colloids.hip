@@ -28,6 +28,7 @@ namespace rmpcd {
 
 constexpr int CG_T = 256;        // threads per block
 constexpr int CG_GRID = 2048;    // blocks of the grid-stride passes over particles or cells
+constexpr int CG_MD_GRID = 4096; // blocks of the md-slot pass (measured against 1024 to 8192)
 constexpr int CG_SCAN_I = 8;     // cells per thread of a scan tile
 constexpr int CG_TILE = CG_T * CG_SCAN_I;
 constexpr int CG_CELL_T = 128;   // cells per block of the collision and temperature passes
@@ -1093,11 +1094,11 @@ static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, bool pending, h
     const ColB col = col_bufs(s);
     ColCtl *C = g.ctl;
     if (pending) {
-        k_cg_md<F, true><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-        k_cg_colloids_decide<true><<<1, CG_DEC_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_GRID, s.skin, s.box.max_v);
+        k_cg_md<F, true><<<CG_MD_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
+        k_cg_colloids_decide<true><<<1, CG_DEC_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_MD_GRID, s.skin, s.box.max_v);
     } else {
-        k_cg_md<F, false><<<CG_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-        k_cg_colloids_decide<false><<<1, CG_DEC_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_GRID, s.skin, s.box.max_v);
+        k_cg_md<F, false><<<CG_MD_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
+        k_cg_colloids_decide<false><<<1, CG_DEC_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_MD_GRID, s.skin, s.box.max_v);
     }
     int rc = enqueue_sort_chain<F>(s, g, fp, 0, st);
     if (rc) return rc;
@@ -1184,7 +1185,7 @@ static int graph_setup(ProgramSim &s)
     if ((rc = cg_alloc(*g, (void **)&g->mdflag, n))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->tsum, 4 * (size_t)(g->ntiles + 1)))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->mdoff, 4 * nc))) return rc;
-    if ((rc = cg_alloc(*g, (void **)&g->part, 8 * CG_GRID))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->part, 8 * CG_MD_GRID))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->fpart, 32 * (size_t)g->fblocks * s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->epart, 16 * (size_t)s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->tpart, 16 * (size_t)((nc + CG_GC - 1) / CG_GC)))) return rc;
