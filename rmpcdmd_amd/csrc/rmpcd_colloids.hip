@@ -71,7 +71,7 @@ struct ColloidsGraph {
     int32_t *mdoff = nullptr;
     int32_t *lut = nullptr;  // the box's Hilbert table, a copy owned by the program
     void *rec = nullptr, *frec = nullptr;  // staging records of a sort, forces of the members
-    double *part = nullptr, *fpart = nullptr, *epart = nullptr, *tpart = nullptr;
+    double *part = nullptr, *cpart = nullptr, *fpart = nullptr, *epart = nullptr, *tpart = nullptr;
     int ntiles = 0, fblocks = 0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};  // by starting force parity
     std::vector<void *> allocs;
@@ -134,14 +134,42 @@ __device__ inline int cg_block_scan(int v, int *lds, int *total)
 // Taking it before the sort gives the same state: only particles with the MD or PAST_MD
 // bit carry a force or change under the roll, each of them sits in an md slot (it is within
 // the list radius of a colloid), and the sort carries force planes and flags along.
+// The first workgroups also make the colloids' part of the step, one thread per colloid,
+// ahead of their share of the md slots: with PENDING the velocity of the step before
+// (n_colloids_pbc.f90:312) and the step counter; the velocity-Verlet position
+// (n_colloids_pbc.f90:289-291) and its displacement from pos_old (maximum_displacement,
+// particle_system.f90:357-378), maximum per workgroup; colloids%force = 0 of this step's
+// compute_force (the plane the switch makes the new force; neither the skin test nor a
+// sort reads it).
 template <class F, bool PENDING>
-__global__ void __launch_bounds__(CG_T) k_cg_md(SolB<F> S, const ColCtl *C, int fp, double dt,
-                                                const int32_t *__restrict__ mdslot, double *__restrict__ part)
+__global__ void __launch_bounds__(CG_T) k_cg_md(SolB<F> S, ColB c, ColCtl *C, int fp, double dt,
+                                                const int32_t *__restrict__ mdslot, double *__restrict__ part,
+                                                double *__restrict__ cpart)
 {
     using R = typename F::real;
     using Sm = typename F::small;
     __shared__ double red[16];
     double m = 0;
+    if ((int64_t)blockIdx.x * CG_T < c.n) {
+        double cm = 0;
+        const int i = blockIdx.x * CG_T + threadIdx.x;
+        if (!C->error && i < c.n) {
+            const double *Fc = c.force[fp];
+            double *Fz = c.force[1 - fp];
+            double r[3];
+            for (int k = 0; k < 3; ++k) {
+                const int64_t q = (int64_t)k * c.n + i;
+                if (PENDING) c.vel[q] = c.vel[q] + (dt * (Fc[q] + Fz[q])) / (2 * c.mass[i]);
+                c.pos[q] = (c.pos[q] + dt * c.vel[q]) + ((dt * dt) * Fc[q]) / (2 * c.mass[i]);
+                Fz[q] = 0;
+                r[k] = c.pos[q] - c.pos_old[q];
+            }
+            if (c.species[i] >= 0) cm = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+        }
+        if (PENDING && i == 0 && !C->error) C->md_steps += 1;
+        cm = block_max(cm, red);
+        if (threadIdx.x == 0) cpart[blockIdx.x] = cm;
+    }
     if (!C->error) {
         const int b = C->cur;
         const int64_t n = S.n;
@@ -192,40 +220,20 @@ __global__ void __launch_bounds__(CG_T) k_cg_md(SolB<F> S, const ColCtl *C, int 
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
-// The colloids' part of one MD step and the skin test, one workgroup.  With PENDING, the
-// velocity of the step before (n_colloids_pbc.f90:312) and the step counter; the
-// velocity-Verlet position (n_colloids_pbc.f90:289-291) and its displacement from pos_old
-// (maximum_displacement, particle_system.f90:357-378); colloids%force = 0 of this step's
-// compute_force (the plane the switch makes the new force; neither the skin test nor a
-// sort reads it).  Then the skin test (n_colloids_pbc.f90:293-296): so_max is the maximum
-// over the md cells (part, from k_cg_md), raised to max_v k dt; co_max over the colloids;
-// k = steps since the last sort.
-template <bool PENDING>
-__global__ void __launch_bounds__(CG_DEC_T) k_cg_colloids_decide(ColB c, ColCtl *C, int fp, double dt,
-                                                             const double *__restrict__ part, int np, double skin,
-                                                             double max_v)
+// the skin test (n_colloids_pbc.f90:293-296), one workgroup: so_max is the maximum over the
+// md cells (part, from k_cg_md), raised to max_v k dt; co_max over the colloids (cpart);
+// k = steps since the last sort
+__global__ void __launch_bounds__(CG_DEC_T) k_cg_decide(ColCtl *C, const double *__restrict__ part, int np,
+                                                        const double *__restrict__ cpart, int ncp, double dt,
+                                                        double skin, double max_v)
 {
     __shared__ double red[16];
-    if (C->error) return;
-    const double *F = c.force[fp];
-    double *Fz = c.force[1 - fp];
     double a = 0, bm = 0;
-    for (int i = threadIdx.x; i < c.n; i += blockDim.x) {
-        double r[3];
-        for (int k = 0; k < 3; ++k) {
-            const int64_t q = (int64_t)k * c.n + i;
-            if (PENDING) c.vel[q] = c.vel[q] + (dt * (F[q] + Fz[q])) / (2 * c.mass[i]);
-            c.pos[q] = (c.pos[q] + dt * c.vel[q]) + ((dt * dt) * F[q]) / (2 * c.mass[i]);
-            Fz[q] = 0;
-            r[k] = c.pos[q] - c.pos_old[q];
-        }
-        if (c.species[i] >= 0) bm = fmax(bm, (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
-    }
     for (int q = threadIdx.x; q < np; q += blockDim.x) a = fmax(a, part[q]);
+    for (int q = threadIdx.x; q < ncp; q += blockDim.x) bm = fmax(bm, cpart[q]);
     a = block_max(a, red);
     bm = block_max(bm, red);
-    if (threadIdx.x != 0) return;
-    if (PENDING) C->md_steps += 1;
+    if (threadIdx.x != 0 || C->error) return;
     const double k1 = (double)(C->md_steps + 1 - C->last_sort);
     double so = sqrt(a);
     const double bnd = max_v * (dt * k1);
@@ -958,7 +966,7 @@ __global__ void __launch_bounds__(CG_T) k_cg_vel(SolB<F> S, const ColCtl *C, int
 }
 
 // colloid velocity (n_colloids_pbc.f90:312) and the step counter, after the last MD step of
-// a cycle (the other steps': k_cg_colloids_decide)
+// a cycle (the other steps': k_cg_md)
 __global__ void k_cg_colloid_vel(ColB c, ColCtl *C, int cfp, double dt)
 {
     if (C->error) return;
@@ -1093,13 +1101,12 @@ static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, bool pending, h
     const SolB<F> S = sol_bufs<F>(s);
     const ColB col = col_bufs(s);
     ColCtl *C = g.ctl;
-    if (pending) {
-        k_cg_md<F, true><<<CG_MD_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-        k_cg_colloids_decide<true><<<1, CG_DEC_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_MD_GRID, s.skin, s.box.max_v);
-    } else {
-        k_cg_md<F, false><<<CG_MD_GRID, CG_T, 0, st>>>(S, C, fp, s.dt, g.mdslot, g.part);
-        k_cg_colloids_decide<false><<<1, CG_DEC_T, 0, st>>>(col, C, fp, s.dt, g.part, CG_MD_GRID, s.skin, s.box.max_v);
-    }
+    const int ncb = (int)blocks_for(s.nc, CG_T);  // the md pass's workgroups that move the colloids
+    if (pending)
+        k_cg_md<F, true><<<CG_MD_GRID, CG_T, 0, st>>>(S, col, C, fp, s.dt, g.mdslot, g.part, g.cpart);
+    else
+        k_cg_md<F, false><<<CG_MD_GRID, CG_T, 0, st>>>(S, col, C, fp, s.dt, g.mdslot, g.part, g.cpart);
+    k_cg_decide<<<1, CG_DEC_T, 0, st>>>(C, g.part, CG_MD_GRID, g.cpart, ncb, s.dt, s.skin, s.box.max_v);
     int rc = enqueue_sort_chain<F>(s, g, fp, 0, st);
     if (rc) return rc;
     // compute_force(_n2) into the planes zeroed above
@@ -1186,6 +1193,8 @@ static int graph_setup(ProgramSim &s)
     if ((rc = cg_alloc(*g, (void **)&g->tsum, 4 * (size_t)(g->ntiles + 1)))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->mdoff, 4 * nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->part, 8 * CG_MD_GRID))) return rc;
+    RMPCD_CHECK(blocks_for(s.nc, CG_T) <= CG_MD_GRID, "colloids graph: too many colloids for the md pass");
+    if ((rc = cg_alloc(*g, (void **)&g->cpart, 8 * (size_t)blocks_for(s.nc, CG_T)))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->fpart, 32 * (size_t)g->fblocks * s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->epart, 16 * (size_t)s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->tpart, 16 * (size_t)((nc + CG_GC - 1) / CG_GC)))) return rc;
