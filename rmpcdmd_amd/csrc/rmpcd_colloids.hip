@@ -71,7 +71,7 @@ struct ColloidsGraph {
     int32_t *mdoff = nullptr;
     int32_t *lut = nullptr;  // the box's Hilbert table, a copy owned by the program
     void *rec = nullptr, *frec = nullptr;  // staging records of a sort, forces of the members
-    double *part = nullptr, *cpart = nullptr, *fpart = nullptr, *epart = nullptr, *tpart = nullptr;
+    double *part = nullptr, *cpart = nullptr, *fpart = nullptr, *npart = nullptr, *epart = nullptr, *tpart = nullptr;
     int ntiles = 0, fblocks = 0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};  // by starting force parity
     std::vector<void *> allocs;
@@ -824,12 +824,13 @@ __global__ void __launch_bounds__(CG_T) k_cg_md_finish(int64_t nc, const uint8_t
 // atomics, the colloid's force and energy as block sums (the partition of the library's
 // compute_force, so the same sums).  The grid spans the list capacity; the blocks past a
 // colloid's list return at once and k_cg_colloid_forces stops before them (their sums would
-// be +0, which leaves a sum that starts at +0 unchanged).
+// be +0, which leaves a sum that starts at +0 unchanged).  One more workgroup per colloid
+// makes its compute_force_n2 sums at the same time.
 template <class F>
 __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const ColCtl *C, int fp, int cfp,
                                                       const int32_t *__restrict__ list, const int32_t *__restrict__ nl,
                                                       int32_t nmax, double L0, double L1, double L2, LJ lj,
-                                                      double *__restrict__ fpart)
+                                                      double *__restrict__ fpart, LJ lj_cc, double *__restrict__ npart)
 {
     using R = typename F::real;
     __shared__ double red[16];
@@ -837,6 +838,19 @@ __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const
     const int b = C->cur;
     const int64_t n = S.n;
     const int i = blockIdx.y;
+    if (blockIdx.x == gridDim.x - 1) {
+        // the last workgroup of a colloid's row: compute_force_n2's sums for it
+        // (rmpcd_pairforce.hpp), beside the list blocks
+        double g0, g1, g2, ge;
+        n2_pair_sums<CG_T>(col.pos, col.species, col.n, i, L0, L1, L2, lj_cc, red, g0, g1, g2, ge);
+        if (threadIdx.x == 0) {
+            npart[4 * i] = g0;
+            npart[4 * i + 1] = g1;
+            npart[4 * i + 2] = g2;
+            npart[4 * i + 3] = ge;
+        }
+        return;
+    }
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int n_i = min(nl[i], nmax);
     if ((int)(blockIdx.x * blockDim.x) >= n_i) return;
@@ -878,45 +892,41 @@ __global__ void __launch_bounds__(CG_T) k_cg_force_nl(SolB<F> S, ColB col, const
     if (threadIdx.x < 4) {
         double t = 0;
         for (int w = 0; w < CG_T / 64; ++w) t += red[4 * w + threadIdx.x];
-        fpart[((int64_t)i * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = t;
+        fpart[((int64_t)i * (gridDim.x - 1) + blockIdx.x) * 4 + threadIdx.x] = t;
     }
     (void)cfp;
 }
 
-// The colloid forces of one MD step, a workgroup per colloid: the solvent part, the sum of
-// its k_cg_force_nl block sums in block order (over the blocks that cover its list), then
-// compute_force_n2 (rmpcd_pairforce.hpp) added to it; each workgroup reads and writes its
-// own colloid's force only, so the result goes straight to the force plane.  The energies
-// per colloid (solvent, then colloid pairs) go to epart for k_cg_energy.
+// The colloid forces of one MD step, a thread per colloid: the solvent part, the sum of its
+// k_cg_force_nl block sums in block order (over the blocks that cover its list), then the
+// compute_force_n2 sums (the last workgroup of its k_cg_force_nl row) added to it.  The
+// energies per colloid (solvent, then colloid pairs) go to epart for k_cg_energy.
 __global__ void __launch_bounds__(CG_T) k_cg_colloid_forces(ColB col, const ColCtl *C, int cfp,
                                                             const double *__restrict__ fpart, int nb,
-                                                            const int32_t *__restrict__ nl, int32_t nmax, double L0,
-                                                            double L1, double L2, LJ lj, double *__restrict__ epart)
+                                                            const int32_t *__restrict__ nl, int32_t nmax,
+                                                            const double *__restrict__ npart,
+                                                            double *__restrict__ epart)
 {
-    __shared__ double red[16];
     if (C->error) return;
-    const int i = blockIdx.x;
-    double f0, f1, f2, e;
-    n2_pair_sums<CG_T>(col.pos, col.species, col.n, i, L0, L1, L2, lj, red, f0, f1, f2, e);
-    if (threadIdx.x == 0) {
-        const int nbi = min(nb, (min(nl[i], nmax) + CG_T - 1) / CG_T);
-        double a0 = 0, a1 = 0, a2 = 0, ae = 0;
-        for (int q = 0; q < nbi; ++q) {
-            const double *pp = fpart + ((int64_t)i * nb + q) * 4;
-            a0 += pp[0];
-            a1 += pp[1];
-            a2 += pp[2];
-            ae += pp[3];
-        }
-        // the force plane holds 0 here; 0 + a is a (a sum that starts at +0 is never -0)
-        double *F = col.force[cfp];
-        const int64_t cn = col.n;
-        F[i] = a0 + f0;
-        F[cn + i] = a1 + f1;
-        F[2 * cn + i] = a2 + f2;
-        epart[i] = ae;
-        epart[cn + i] = e;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= col.n) return;
+    const int nbi = min(nb, (min(nl[i], nmax) + CG_T - 1) / CG_T);
+    double a0 = 0, a1 = 0, a2 = 0, ae = 0;
+    for (int q = 0; q < nbi; ++q) {
+        const double *pp = fpart + ((int64_t)i * nb + q) * 4;
+        a0 += pp[0];
+        a1 += pp[1];
+        a2 += pp[2];
+        ae += pp[3];
     }
+    // the force plane holds 0 here; 0 + a is a (a sum that starts at +0 is never -0)
+    double *F = col.force[cfp];
+    const int64_t cn = col.n;
+    F[i] = a0 + npart[4 * i];
+    F[cn + i] = a1 + npart[4 * i + 1];
+    F[2 * cn + i] = a2 + npart[4 * i + 2];
+    epart[i] = ae;
+    epart[cn + i] = npart[4 * i + 3];
 }
 
 // the energies of the last computed step (tree sums over the colloids): solvent-colloid and
@@ -1112,11 +1122,11 @@ static int enqueue_step(ProgramSim &s, ColloidsGraph &g, int fp, bool pending, h
     // compute_force(_n2) into the planes zeroed above
     const int np = 1 - fp;
     const double *E = s.box.edges;
-    const dim3 fgrid((unsigned)g.fblocks, (unsigned)s.nc);
+    const dim3 fgrid((unsigned)g.fblocks + 1, (unsigned)s.nc);  // list blocks, then the n^2 sums
     k_cg_force_nl<F><<<fgrid, CG_T, 0, st>>>(S, col, C, np, np, s.list, s.nlist, s.nmax, E[0], E[1], E[2],
-                                             make_lj(&s.lj_sc), g.fpart);
-    k_cg_colloid_forces<<<(unsigned)s.nc, CG_T, 0, st>>>(col, C, np, g.fpart, g.fblocks, s.nlist, s.nmax, E[0], E[1],
-                                                         E[2], make_lj(&s.lj_cc), g.epart);
+                                             make_lj(&s.lj_sc), g.fpart, make_lj(&s.lj_cc), g.npart);
+    k_cg_colloid_forces<<<blocks_for(s.nc, CG_T), CG_T, 0, st>>>(col, C, np, g.fpart, g.fblocks, s.nlist, s.nmax,
+                                                                 g.npart, g.epart);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
@@ -1197,6 +1207,7 @@ static int graph_setup(ProgramSim &s)
     if ((rc = cg_alloc(*g, (void **)&g->cpart, 8 * (size_t)blocks_for(s.nc, CG_T)))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->fpart, 32 * (size_t)g->fblocks * s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->epart, 16 * (size_t)s.nc))) return rc;
+    if ((rc = cg_alloc(*g, (void **)&g->npart, 32 * (size_t)s.nc))) return rc;
     if ((rc = cg_alloc(*g, (void **)&g->tpart, 16 * (size_t)((nc + CG_GC - 1) / CG_GC)))) return rc;
     {
         using R = typename F::real;
