@@ -554,8 +554,8 @@ struct CgGatherLds<F, true> {
 // and the collision follows on it: simple_mpcd_step with the default angle
 // (mpcd.f90:71-183; axes threefry((s, cell + 2^40 k)) of stream SRD) and
 // compute_temperature (mpcd.f90:342-407) of the collided cells, per unit the sum of the
-// cells' kinetic terms and of the cells counted (k_cg_temp_final adds the units' partials
-// in unit order).  The list build that follows in the sort chain reads positions only.
+// cells' kinetic terms and of the cells counted (k_cg_temp_final sums the units' partials,
+// one workgroup).  The list build that follows in the sort chain reads positions only.
 template <class F, bool COLL>
 __global__ void __launch_bounds__(CG_T) k_cg_gather(SolB<F> S, const ColCtl *C, int fp, int forced, int64_t nc,
                                                     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
@@ -992,7 +992,7 @@ __global__ void k_cg_colloid_vel(ColB c, ColCtl *C, int cfp, double dt)
 }
 
 // ---- collision (in the collision sort's gather) --------------------------------------
-__global__ void __launch_bounds__(CG_CELL_T) k_cg_temp_final(ColCtl *C, const double *__restrict__ tpart, int nb)
+__global__ void __launch_bounds__(1024) k_cg_temp_final(ColCtl *C, const double *__restrict__ tpart, int nb)
 {
     __shared__ double red[16];
     double a = 0, b = 0;
@@ -1154,7 +1154,7 @@ static int enqueue_collision(ProgramSim &s, ColloidsGraph &g, int fp, hipStream_
     if (rc) return rc;
     // the collision ran in the sort's gather; the temperature of its units' partials
     const int64_t units = (s.box.ncells + CG_GC - 1) / CG_GC;
-    k_cg_temp_final<<<1, CG_CELL_T, 0, st>>>(C, g.tpart, (int)units);
+    k_cg_temp_final<<<1, 1024, 0, st>>>(C, g.tpart, (int)units);
     RMPCD_LAUNCHED();
     return RMPCD_OK;
 }
