@@ -1712,7 +1712,7 @@ __global__ void __launch_bounds__(FT) k_fz_gather_srd(FB<F> B, FP P, int R)
     using S = typename F::small;
     __shared__ Rl lx[GC_TILE], ly[GC_TILE], lz[GC_TILE];
     __shared__ S lsp[GC_TILE];
-    __shared__ int32_t lref[GC_TILE], lkey[GC_TILE];
+    __shared__ int32_t lref[GC_TILE], lkey[GC_TILE], lsrc[GC_TILE];
     __shared__ int32_t lstart[GC], lcnt[GC];
     __shared__ double red[16];
     const UnitRange w = unit_range(B.plan, R, U_GATHER, B.xcd_order);
@@ -1750,12 +1750,13 @@ __global__ void __launch_bounds__(FT) k_fz_gather_srd(FB<F> B, FP P, int R)
             for (int q = threadIdx.x; q < tcount; q += FT) {
                 const int2 pr = pair[sa + q];
                 lref[q] = pr.x;
+                lsrc[q] = pr.y;
                 if (dkey) lkey[q] = dkey[pr.y];
             }
             __syncthreads();
         }
         for (int q = threadIdx.x; q < tcount; q += FT) {
-            const int2 pr = pair[sa + q];
+            const int2 pr = fits ? make_int2(lref[q], lsrc[q]) : pair[sa + q];
             const int32_t ref = pr.x;
             const typename FB<F>::Rec qr = rec[pr.y];
             const int cl = qr.key - (int)c0;
